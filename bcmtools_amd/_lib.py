@@ -1,0 +1,180 @@
+"""ctypes binding of libbcm_hip.so (the C ABI declared in include/bcm_hip.h).
+
+The shared library holds both the host octree builder and the HIP kernels.
+Nothing here falls back to a CPU path: if the library is missing, every
+entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbcm_hip.so")
+
+# Status codes (include/bcm_hip.h).
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BALANCE, ERR_NAN, ERR_NOCONV, ERR_BUSY = range(9)
+STATUS_NAMES = {
+    0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_STATE", 5: "ERR_BALANCE",
+    6: "ERR_NAN", 7: "ERR_NOCONV", 8: "ERR_BUSY",
+}
+
+F64, F32 = 0, 1
+X_M, X_P, Y_M, Y_P, Z_M, Z_P = range(6)
+BC_INNER, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
+DIV_FLAT, DIV_SIMPLE, DIV_SHELL = range(3)
+ORDER_Z, ORDER_HILBERT = range(2)
+BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z = range(4)
+GF_FLAT, GF_F2C, GF_C2F, GF_ALL = 1, 2, 4, 7
+GF_X, GF_Y, GF_Z = 8, 16, 32
+(K_GHOST, K_DIRICHLET, K_SOR, K_JACOBI, K_SPMV, K_RESID, K_DOT, K_BLAS1, K_REDUCE,
+ K_ASSEMBLE) = range(10)
+MAX_SHELLS = 8
+
+
+class BlockInfo(ctypes.Structure):
+    _fields_ = [
+        ("pedigree", ctypes.c_uint64),
+        ("level", ctypes.c_int32),
+        ("root_id", ctypes.c_int32),
+        ("origin", ctypes.c_double * 3),
+        ("block_size", ctypes.c_double * 3),
+        ("nb_level_diff", ctypes.c_int32 * 6),
+        ("nb_id", (ctypes.c_int32 * 4) * 6),
+        ("nb_subface", ctypes.c_int32 * 6),
+        ("bc_type", ctypes.c_int32 * 6),
+        ("bc_id", ctypes.c_int32 * 6),
+    ]
+
+
+BLOCK_DTYPE = np.dtype(BlockInfo)
+
+
+class TreeParams(ctypes.Structure):
+    _fields_ = [
+        ("root_n", ctypes.c_int32 * 3),
+        ("periodic", ctypes.c_int32 * 3),
+        ("root_origin", ctypes.c_double * 3),
+        ("root_length", ctypes.c_double),
+        ("divider", ctypes.c_int32),
+        ("min_level", ctypes.c_int32),
+        ("max_level", ctypes.c_int32),
+        ("n_shells", ctypes.c_int32),
+        ("shells", (ctypes.c_double * 5) * MAX_SHELLS),
+        ("ordering", ctypes.c_int32),
+        ("balance", ctypes.c_int32),
+        ("bc_setter", ctypes.c_int32),
+        ("shell_mode", ctypes.c_int32),
+        ("shell_margin", ctypes.c_double),
+    ]
+
+
+class KrylovParams(ctypes.Structure):
+    _fields_ = [
+        ("omega", ctypes.c_double),
+        ("count_pre", ctypes.c_int32),
+        ("count_max", ctypes.c_int32),
+        ("epsilon", ctypes.c_double),
+    ]
+
+
+class IRParams(ctypes.Structure):
+    _fields_ = [
+        ("tol", ctypes.c_double),
+        ("inner_tol", ctypes.c_double),
+        ("inner_max", ctypes.c_int32),
+        ("outer_max", ctypes.c_int32),
+        ("count_pre", ctypes.c_int32),
+        ("omega", ctypes.c_double),
+        ("fp32_only", ctypes.c_int32),
+    ]
+
+
+class BCMError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where} failed: {STATUS_NAMES.get(code, code)}")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "bcm_tree_build": (_I, [ctypes.POINTER(TreeParams), ctypes.POINTER(_P)]),
+    "bcm_tree_num_blocks": (_I, [_P]),
+    "bcm_tree_get_blocks": (_I, [_P, _P, _I]),
+    "bcm_tree_destroy": (_I, [_P]),
+    "bcm_mesh_create": (_I, [_P, _I, _I, _I, ctypes.POINTER(_P)]),
+    "bcm_mesh_destroy": (_I, [_P]),
+    "bcm_mesh_task_counts": (_I, [_P, _IP]),
+    "bcm_mesh_sync": (_I, [_P]),
+    "bcm_mesh_set_timing": (_I, [_P, _I]),
+    "bcm_mesh_kernel_time": (_I, [_P, _I, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "bcm_mesh_reset_timing": (_I, [_P]),
+    "bcm_field_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "bcm_field_destroy": (_I, [_P]),
+    "bcm_field_upload": (_I, [_P, _P, ctypes.c_size_t]),
+    "bcm_field_download": (_I, [_P, _P, ctypes.c_size_t]),
+    "bcm_field_device_ptr": (_P, [_P]),
+    "bcm_field_bytes": (ctypes.c_size_t, [_P]),
+    "bcm_ghost_fill": (_I, [_P, _I]),
+    "bcm_dirichlet_mirror": (_I, [_P, _DP, _I]),
+    "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
+    "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
+    "bcm_sor_residual": (_I, [_P, _P, _DP]),
+    "bcm_jacobi7": (_I, [_P, _P, ctypes.POINTER(_P), _P, _D]),
+    "bcm_calc_ax": (_I, [_P, ctypes.POINTER(_P), _P]),
+    "bcm_calc_r": (_I, [_P, ctypes.POINTER(_P), _P, _P]),
+    "bcm_calc_r2": (_I, [ctypes.POINTER(_P), _P, _P, _DP]),
+    "bcm_dot": (_I, [_P, _P, _DP]),
+    "bcm_axpy": (_I, [_P, _P, _D]),
+    "bcm_xpay": (_I, [_P, _P, _D]),
+    "bcm_axpyz": (_I, [_P, _P, _P, _D]),
+    "bcm_axpbypz": (_I, [_P, _P, _P, _D, _D]),
+    "bcm_fill": (_I, [_P, _D]),
+    "bcm_copy": (_I, [_P, _P]),
+    "bcm_poisson_assemble": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _I]),
+    "bcm_jacobi7_poisson": (_I, [_P, _P, _P, _D]),
+    "bcm_calc_ax_poisson": (_I, [_P, _P]),
+    "bcm_calc_r_poisson": (_I, [_P, _P, _P]),
+    "bcm_jacobi_solve": (_I, [_P, ctypes.POINTER(_P), _P, _P, ctypes.POINTER(KrylovParams), _DP, _I, _IP, _DP]),
+    "bcm_bicgstab": (_I, [_P, ctypes.POINTER(_P), _P, ctypes.POINTER(_P), ctypes.POINTER(KrylovParams), _DP, _I,
+                          _IP, _DP]),
+    "bcm_solve_mixed": (_I, [_P, _P, _DP, _I, ctypes.POINTER(IRParams), _IP, _IP, _DP]),
+    "bcm_solve_fp64": (_I, [_P, _P, _DP, _I, ctypes.POINTER(IRParams), _IP, _DP]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbcm_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP extension missing: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code, where):
+    if code != OK:
+        raise BCMError(code, where)
+    return code
+
+
+def dptr(arr):
+    """ctypes double* of a C-contiguous float64 array (or None)."""
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(_DP)

@@ -1,0 +1,294 @@
+"""Host-side handles over the C ABI: Tree, Mesh, Field and the kernels.
+
+Names mirror the reference API they stand in for:
+  Tree                 BCMOctree + BlockFactory (Utils/src/BCMOctree.cpp)
+  Mesh                 BlockManager after registerBlock/endRegisterBlock and
+                       prepareForVCUpdate (src/BlockManager.cpp:28-114)
+  Field                Scalar3D<T> on every block, contiguous storage
+                       (include/BlockManager.h:253-283)
+  Mesh.update_vc*      BlockManager::updateVC* (include/BlockManager.h:311-440)
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Tree:
+    """Leaf-block table of an octree mesh (host only)."""
+
+    def __init__(self, root_n=(1, 1, 1), periodic=(0, 0, 0), root_origin=(0.0, 0.0, 0.0), root_length=1.0,
+                 divider=L.DIV_FLAT, min_level=0, max_level=0, shells=(), ordering=L.ORDER_Z, balance=1,
+                 bc_setter=L.BCSET_ALL_DIRICHLET, shell_mode=0, shell_margin=0.0):
+        lib = L.load()
+        p = L.TreeParams()
+        for a in range(3):
+            p.root_n[a] = int(root_n[a])
+            p.periodic[a] = int(periodic[a])
+            p.root_origin[a] = float(root_origin[a])
+        p.root_length = float(root_length)
+        p.divider = int(divider)
+        p.min_level = int(min_level)
+        p.max_level = int(max_level)
+        shells = list(shells)
+        if len(shells) > L.MAX_SHELLS:
+            raise ValueError("too many shells")
+        p.n_shells = len(shells)
+        for s, sh in enumerate(shells):
+            for c in range(5):
+                p.shells[s][c] = float(sh[c])
+        p.ordering = int(ordering)
+        p.balance = int(balance)
+        p.bc_setter = int(bc_setter)
+        p.shell_mode = int(shell_mode)
+        p.shell_margin = float(shell_margin)
+        self.params = p
+        h = ctypes.c_void_p()
+        L.check(lib.bcm_tree_build(ctypes.byref(p), ctypes.byref(h)), "bcm_tree_build")
+        try:
+            nb = lib.bcm_tree_num_blocks(h)
+            if nb < 0:
+                raise L.BCMError(-nb, "bcm_tree_num_blocks")
+            blocks = np.zeros(nb, dtype=L.BLOCK_DTYPE)
+            L.check(lib.bcm_tree_get_blocks(h, blocks.ctypes.data, nb), "bcm_tree_get_blocks")
+        finally:
+            lib.bcm_tree_destroy(h)
+        self.blocks = blocks
+
+    def __len__(self):
+        return len(self.blocks)
+
+
+class Field:
+    """A ghosted scalar field on every block of a mesh, resident in HBM."""
+
+    def __init__(self, mesh, dtype=np.float64):
+        self.mesh = mesh
+        self.dtype = np.dtype(dtype)
+        code = L.F64 if self.dtype == np.float64 else L.F32
+        if self.dtype not in (np.float64, np.float32):
+            raise ValueError("dtype must be float64 or float32")
+        h = ctypes.c_void_p()
+        L.check(mesh.lib.bcm_field_create(mesh.handle, code, ctypes.byref(h)), "bcm_field_create")
+        self.handle = h
+
+    @property
+    def shape(self):
+        m = self.mesh
+        return (m.nblocks, m.nx0, m.nx0, m.nx0)
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=self.dtype).reshape(self.shape)
+        L.check(self.mesh.lib.bcm_field_upload(self.handle, a.ctypes.data, a.nbytes), "bcm_field_upload")
+
+    def download(self):
+        a = np.empty(self.shape, dtype=self.dtype)
+        L.check(self.mesh.lib.bcm_field_download(self.handle, a.ctypes.data, a.nbytes), "bcm_field_download")
+        return a
+
+    def device_ptr(self):
+        return self.mesh.lib.bcm_field_device_ptr(self.handle)
+
+    def free(self):
+        if self.handle:
+            self.mesh.lib.bcm_field_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _handles(fields):
+    arr = (ctypes.c_void_p * len(fields))()
+    for i, f in enumerate(fields):
+        arr[i] = f.handle.value if isinstance(f.handle, ctypes.c_void_p) else f.handle
+    return arr
+
+
+class Mesh:
+    """Device mesh: block table, face-task lists and the mesh's HIP stream."""
+
+    def __init__(self, blocks, n, vc=1):
+        self.lib = L.load()
+        blocks = np.ascontiguousarray(blocks, dtype=L.BLOCK_DTYPE)
+        self.blocks = blocks
+        self.n = int(n)
+        self.vc = int(vc)
+        self.nblocks = len(blocks)
+        self.nx0 = self.n + 2 * self.vc
+        h = ctypes.c_void_p()
+        L.check(self.lib.bcm_mesh_create(blocks.ctypes.data, self.nblocks, self.n, self.vc, ctypes.byref(h)),
+                "bcm_mesh_create")
+        self.handle = h
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if self.handle:
+            self.lib.bcm_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field(self, dtype=np.float64, host=None):
+        f = Field(self, dtype)
+        if host is not None:
+            f.upload(host)
+        return f
+
+    def sync(self):
+        L.check(self.lib.bcm_mesh_sync(self.handle), "bcm_mesh_sync")
+
+    def task_counts(self):
+        c = (ctypes.c_int * 4)()
+        L.check(self.lib.bcm_mesh_task_counts(self.handle, c), "bcm_mesh_task_counts")
+        return {"flat": c[0], "f2c": c[1], "c2f": c[2], "boundary": c[3]}
+
+    # -- timing -----------------------------------------------------------
+    def set_timing(self, on=True):
+        L.check(self.lib.bcm_mesh_set_timing(self.handle, int(bool(on))), "bcm_mesh_set_timing")
+
+    def reset_timing(self):
+        L.check(self.lib.bcm_mesh_reset_timing(self.handle), "bcm_mesh_reset_timing")
+
+    def kernel_time(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        L.check(self.lib.bcm_mesh_kernel_time(self.handle, kernel, ctypes.byref(ms), ctypes.byref(n)),
+                "bcm_mesh_kernel_time")
+        return ms.value, n.value
+
+    # -- ghost fill (BlockManager::updateVC*) ------------------------------
+    def update_vc(self, f, mask=L.GF_ALL):
+        L.check(self.lib.bcm_ghost_fill(f.handle, int(mask)), "bcm_ghost_fill")
+
+    def update_vc_xyz(self, f, axis, kinds=L.GF_ALL):
+        """updateVC_{X,Y,Z}[_{Flat,F2C,C2F}] (include/BlockManager.h:327-343)."""
+        self.update_vc(f, kinds | (L.GF_X << axis))
+
+    def dirichlet_mirror(self, f, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        L.check(self.lib.bcm_dirichlet_mirror(f.handle, L.dptr(v), len(v)), "bcm_dirichlet_mirror")
+
+    # -- red-black SOR ------------------------------------------------------
+    def rbsor_sweep(self, f, source=None, omega=1.0):
+        L.check(self.lib.bcm_rbsor_sweep(f.handle, source.handle if source is not None else None, float(omega)),
+                "bcm_rbsor_sweep")
+
+    def sor_iterate(self, f, source, omega, bvalues, iters):
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        L.check(self.lib.bcm_sor_iterate(f.handle, source.handle if source is not None else None, float(omega),
+                                         L.dptr(v), len(v), int(iters)), "bcm_sor_iterate")
+
+    def sor_residual(self, f, source):
+        r = ctypes.c_double()
+        L.check(self.lib.bcm_sor_residual(f.handle, source.handle, ctypes.byref(r)), "bcm_sor_residual")
+        return r.value
+
+    # -- SolverTest kernels ---------------------------------------------------
+    def jacobi7(self, x1, x0, A, b, omega):
+        L.check(self.lib.bcm_jacobi7(x1.handle, x0.handle, _handles(A), b.handle, float(omega)), "bcm_jacobi7")
+
+    def jacobi7_poisson(self, x1, x0, b, omega):
+        L.check(self.lib.bcm_jacobi7_poisson(x1.handle, x0.handle, b.handle, float(omega)), "bcm_jacobi7_poisson")
+
+    def calc_ax_poisson(self, ax, x):
+        L.check(self.lib.bcm_calc_ax_poisson(ax.handle, x.handle), "bcm_calc_ax_poisson")
+
+    def calc_r_poisson(self, r, x, b):
+        L.check(self.lib.bcm_calc_r_poisson(r.handle, x.handle, b.handle), "bcm_calc_r_poisson")
+
+    def calc_ax(self, ax, A, x):
+        L.check(self.lib.bcm_calc_ax(ax.handle, _handles(A), x.handle), "bcm_calc_ax")
+
+    def calc_r(self, r, A, x, b):
+        L.check(self.lib.bcm_calc_r(r.handle, _handles(A), x.handle, b.handle), "bcm_calc_r")
+
+    def calc_r2(self, A, x, b):
+        v = ctypes.c_double()
+        L.check(self.lib.bcm_calc_r2(_handles(A), x.handle, b.handle, ctypes.byref(v)), "bcm_calc_r2")
+        return v.value
+
+    def dot(self, x, y):
+        v = ctypes.c_double()
+        L.check(self.lib.bcm_dot(x.handle, y.handle, ctypes.byref(v)), "bcm_dot")
+        return v.value
+
+    def axpy(self, y, x, a):
+        L.check(self.lib.bcm_axpy(y.handle, x.handle, float(a)), "bcm_axpy")
+
+    def xpay(self, y, x, a):
+        L.check(self.lib.bcm_xpay(y.handle, x.handle, float(a)), "bcm_xpay")
+
+    def axpyz(self, z, x, y, a):
+        L.check(self.lib.bcm_axpyz(z.handle, x.handle, y.handle, float(a)), "bcm_axpyz")
+
+    def axpbypz(self, z, x, y, a, b):
+        L.check(self.lib.bcm_axpbypz(z.handle, x.handle, y.handle, float(a), float(b)), "bcm_axpbypz")
+
+    def fill(self, x, a):
+        L.check(self.lib.bcm_fill(x.handle, float(a)), "bcm_fill")
+
+    def copy(self, y, x):
+        L.check(self.lib.bcm_copy(y.handle, x.handle), "bcm_copy")
+
+    def poisson_assemble(self, A, b, rhs, bvalues):
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        L.check(self.lib.bcm_poisson_assemble(_handles(A) if A is not None else None, b.handle, rhs.handle,
+                                              L.dptr(v), len(v)), "bcm_poisson_assemble")
+
+    # -- solvers ----------------------------------------------------------------
+    def jacobi_solve(self, x, A, b, x0, omega=1.0, count_max=100, epsilon=1e-8, bvalues=()):
+        p = L.KrylovParams(float(omega), 0, int(count_max), float(epsilon))
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        cnt = ctypes.c_int()
+        res = ctypes.c_double()
+        rc = self.lib.bcm_jacobi_solve(x.handle, _handles(A), b.handle, x0.handle, ctypes.byref(p),
+                                       L.dptr(v) if len(v) else None, len(v), ctypes.byref(cnt), ctypes.byref(res))
+        if rc not in (L.OK, L.ERR_NOCONV):
+            L.check(rc, "bcm_jacobi_solve")
+        return rc, cnt.value, res.value
+
+    def bicgstab(self, x, A, b, work, omega=0.8, count_pre=4, count_max=100, epsilon=1e-8, bvalues=()):
+        """ILS3D::BiCGSTAB; `work` = 9 fields {r, r0, p, p_, q_, s, s_, t_, x0}."""
+        p = L.KrylovParams(float(omega), int(count_pre), int(count_max), float(epsilon))
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        cnt = ctypes.c_int()
+        res = ctypes.c_double()
+        rc = self.lib.bcm_bicgstab(x.handle, _handles(A), b.handle, _handles(work), ctypes.byref(p),
+                                   L.dptr(v) if len(v) else None, len(v), ctypes.byref(cnt), ctypes.byref(res))
+        if rc not in (L.OK, L.ERR_NOCONV):
+            L.check(rc, "bcm_bicgstab")
+        return rc, cnt.value, res.value
+
+    def solve_fp64(self, x, rhs, bvalues, tol=1e-10, max_iter=2000, count_pre=2, omega=0.8):
+        p = L.IRParams(float(tol), float(tol), int(max_iter), 1, int(count_pre), float(omega), 0)
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        it = ctypes.c_int()
+        rr = ctypes.c_double()
+        rc = self.lib.bcm_solve_fp64(x.handle, rhs.handle, L.dptr(v), len(v), ctypes.byref(p), ctypes.byref(it),
+                                     ctypes.byref(rr))
+        if rc not in (L.OK, L.ERR_NOCONV):
+            L.check(rc, "bcm_solve_fp64")
+        return rc, it.value, rr.value
+
+    def solve_mixed(self, x, rhs, bvalues, tol=1e-10, inner_tol=1e-4, inner_max=500, outer_max=20, count_pre=2,
+                    omega=0.8, fp32_only=False):
+        p = L.IRParams(float(tol), float(inner_tol), int(inner_max), int(outer_max), int(count_pre), float(omega),
+                       int(bool(fp32_only)))
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        outer = ctypes.c_int()
+        inner = ctypes.c_int()
+        rr = ctypes.c_double()
+        rc = self.lib.bcm_solve_mixed(x.handle, rhs.handle, L.dptr(v), len(v), ctypes.byref(p), ctypes.byref(outer),
+                                      ctypes.byref(inner), ctypes.byref(rr))
+        if rc not in (L.OK, L.ERR_NOCONV):
+            L.check(rc, "bcm_solve_mixed")
+        return rc, outer.value, inner.value, rr.value

@@ -1,0 +1,138 @@
+// Internal definitions shared by the HIP translation units of libbcm_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "bcm_hip.h"
+
+#define BCM_HIP_TRY(expr)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      std::fprintf(stderr, "bcm: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                              \
+      return BCM_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+#define BCM_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != BCM_OK) return rc_; \
+  } while (0)
+
+namespace bcm {
+
+// Per-block constants used on the device.
+struct BlockMeta {
+  double cx, cy, cz;      // 1/h^2 per axis, h = block_size / n
+  int32_t dir_mask;       // bit f: face f carries a Dirichlet condition
+  int32_t bc_id[6];       // boundary-value index per face (-1 if none)
+};
+
+// One ghost-fill task: dst block, src block, code = face | subface<<3 | kind<<5.
+enum TaskKind { KIND_COPY = 0, KIND_F2C = 1, KIND_C2F = 2 };
+
+struct EventPair {
+  hipEvent_t start, stop;
+  int kernel;
+};
+
+}  // namespace bcm
+
+struct bcm_mesh {
+  int n = 0, vc = 0, nblocks = 0;
+  int nx0 = 0;                    // n + 2 vc
+  size_t bvol = 0;                // (n+2vc)^3 values per block
+  size_t ncell = 0;               // nblocks * n^3 interior cells
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::vector<bcm_block_info> blocks;
+  std::vector<bcm::BlockMeta> meta;
+
+  // Ghost-fill tasks ordered by [axis][kind]; off[a][k] .. off[a][k+1].
+  int4* d_tasks = nullptr;
+  int ntask_total = 0;
+  int off[3][4] = {};
+  int kind_count[3] = {};
+  // Boundary face tasks {block, face, bc_id, type}.
+  int4* d_bc_tasks = nullptr;
+  int n_bc_tasks = 0;
+  int n_dirichlet = 0;
+  bcm::BlockMeta* d_meta = nullptr;
+
+  // Reduction scratch: one double per reduction work-group, plus results.
+  double* d_partials = nullptr;
+  size_t n_partials = 0;
+  double* d_result = nullptr;     // 16 doubles
+  double* h_result = nullptr;     // pinned mirror of d_result
+
+  // Boundary-value table on the device.
+  double* d_bvalues = nullptr;
+  int n_bvalues_cap = 0;
+
+  // SOR coefficient cache (per block: c0, cx, cy, cz) for one omega/form.
+  double* d_sor_coef = nullptr;
+  double sor_omega = 0.0;
+  int sor_form = -1;
+
+  // Timing.
+  bool timing = false;
+  double kernel_ms[BCM_K_COUNT] = {};
+  long long kernel_launches[BCM_K_COUNT] = {};
+  std::vector<bcm::EventPair> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  // Solver workspace (lazily allocated, kept for the mesh's lifetime).
+  std::vector<void*> ws;
+};
+
+struct bcm_field {
+  bcm_mesh* mesh = nullptr;
+  int dtype = BCM_F64;
+  void* data = nullptr;
+  size_t count = 0;  // values
+  size_t bytes = 0;
+};
+
+namespace bcm {
+
+// Timing helpers (bcm_mesh.hip).
+int timer_begin(bcm_mesh* m, int kernel, EventPair* ev);
+int timer_end(bcm_mesh* m, EventPair* ev);
+
+// Upload the boundary-value table (grows as needed).
+int set_bvalues(bcm_mesh* m, const double* values, int n);
+
+// Reduction of the per-work-group partials into d_result[slot] and,
+// when `to_host` is set, copy back (synchronises the stream).
+int reduce_partials(bcm_mesh* m, int nparts, int slot);
+int reduce_partials_at(bcm_mesh* m, const double* parts, int nparts, int slot);
+int fetch_result(bcm_mesh* m, int nslots, double* out);
+
+// Device allocation helpers.
+int dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
+
+// Raw-pointer kernels shared between the C ABI and the solvers.
+template <typename T> int ghost_fill_raw(bcm_mesh* m, T* data, int mask);
+template <typename T> int jacobi_poisson_raw(bcm_mesh* m, T* x1, const T* x0, const T* b, double omega);
+template <typename T> int residual_poisson_raw(bcm_mesh* m, T* r, const T* x, const T* b, bool want_norm, int slot);
+template <typename T> int spmv_poisson_raw(bcm_mesh* m, T* ax, const T* x);
+template <typename T> int dot_raw(bcm_mesh* m, const T* x, const T* y, int slot);
+template <typename T> int blas_raw(bcm_mesh* m, int op, T* z, const T* x, const T* y, double a, double b);
+template <typename T> int fill_raw(bcm_mesh* m, T* x, double a);
+int assemble_poisson_b_raw(bcm_mesh* m, double* b, const double* rhs);
+int residual_d2f_raw(bcm_mesh* m, float* r32, const double* x, const double* b, int slot);
+int convert_raw(bcm_mesh* m, int op, void* dst, const void* src);
+// Dirichlet mirror with the boundary-value table already on the device.
+int dirichlet_device(bcm_mesh* m, bcm_field* f);
+
+enum BlasOp { OP_AXPY = 0, OP_XPAY = 1, OP_AXPYZ = 2, OP_AXPBYPZ = 3, OP_COPY_INTERIOR = 4 };
+enum ConvOp { CONV_D2F = 0, CONV_ADD_F2D = 1 };
+
+}  // namespace bcm

@@ -1,0 +1,575 @@
+// Device mesh, fields, ghost fill and boundary mirror.
+//
+// Ghost fill is one launch over a precomputed face-task table: each
+// work-group takes one task (a face, or one subface for F2C) and writes
+// that face's ghost layer from the neighbour's interior cells only
+// (include/Scalar3DUpdater.h:190-266, 321-347, 519-708).  Because every
+// task reads interiors and writes ghosts, the tasks are independent and
+// the result does not depend on launch order.
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "bcm_internal.h"
+
+using bcm::BlockMeta;
+
+namespace bcm {
+
+int dev_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return BCM_ERR_NOMEM;
+  }
+  return BCM_OK;
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static hipEvent_t get_event(bcm_mesh* m) {
+  if (!m->event_pool.empty()) {
+    hipEvent_t e = m->event_pool.back();
+    m->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int timer_begin(bcm_mesh* m, int kernel, EventPair* ev) {
+  ev->start = ev->stop = nullptr;
+  ev->kernel = kernel;
+  if (!m->timing) return BCM_OK;
+  ev->start = get_event(m);
+  ev->stop = get_event(m);
+  if (!ev->start || !ev->stop) return BCM_ERR_HIP;
+  BCM_HIP_TRY(hipEventRecord(ev->start, m->stream));
+  return BCM_OK;
+}
+
+int timer_end(bcm_mesh* m, EventPair* ev) {
+  if (!ev->start) return BCM_OK;
+  BCM_HIP_TRY(hipEventRecord(ev->stop, m->stream));
+  m->pending.push_back(*ev);
+  // Bound the number of outstanding pairs.
+  if (m->pending.size() > 4096) {
+    BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+    for (auto& p : m->pending) {
+      float ms = 0.f;
+      BCM_HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
+      m->kernel_ms[p.kernel] += ms;
+      m->kernel_launches[p.kernel] += 1;
+      m->event_pool.push_back(p.start);
+      m->event_pool.push_back(p.stop);
+    }
+    m->pending.clear();
+  }
+  return BCM_OK;
+}
+
+int set_bvalues(bcm_mesh* m, const double* values, int n) {
+  if (n < 0 || (n > 0 && !values)) return BCM_ERR_ARG;
+  // Every Dirichlet face must index a supplied value.
+  for (const auto& bi : m->blocks)
+    for (int f = 0; f < 6; ++f)
+      if (bi.bc_type[f] == BCM_BC_DIRICHLET && (bi.bc_id[f] < 0 || bi.bc_id[f] >= n)) return BCM_ERR_ARG;
+  if (n > m->n_bvalues_cap) {
+    dev_free(m->d_bvalues);
+    m->d_bvalues = nullptr;
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&m->d_bvalues), sizeof(double) * size_t(n)));
+    m->n_bvalues_cap = n;
+  }
+  if (n > 0)
+    BCM_HIP_TRY(hipMemcpyAsync(m->d_bvalues, values, sizeof(double) * size_t(n), hipMemcpyHostToDevice,
+                               m->stream));
+  return BCM_OK;
+}
+
+// ---------------------------------------------------------------------
+// Ghost fill
+// ---------------------------------------------------------------------
+
+struct Geo {
+  int n, vc, nx0;
+  long long bvol;
+  __device__ __forceinline__ int idx(int i, int j, int k) const {
+    return (i + vc) + nx0 * ((j + vc) + nx0 * (k + vc));
+  }
+};
+
+// 8-cell restriction, summed in the order of include/Scalar3DUpdater.h:190-198.
+template <typename T>
+__device__ __forceinline__ T f2c_value(const T* __restrict__ f, const Geo& g, int I, int J, int K) {
+  const int i = 2 * I, j = 2 * J, k = 2 * K;
+  T s = f[g.idx(i, j, k)];
+  s = s + f[g.idx(i + 1, j, k)];
+  s = s + f[g.idx(i, j + 1, k)];
+  s = s + f[g.idx(i + 1, j + 1, k)];
+  s = s + f[g.idx(i, j, k + 1)];
+  s = s + f[g.idx(i + 1, j, k + 1)];
+  s = s + f[g.idx(i, j + 1, k + 1)];
+  s = s + f[g.idx(i + 1, j + 1, k + 1)];
+  return T(0.125 * s);
+}
+
+// linearInterpolate (include/Scalar3DUpdater.h:242-266): coarse base index
+// clamped to [0, n-2] and weight -0.25 + 0.5 i - I in double.
+__device__ __forceinline__ void c2f_weight(int i, int n, int& I, double& r) {
+  I = min(max(i / 2 - 1 + i % 2, 0), n - 2);
+  r = -0.25 + 0.5 * double(i) - double(I);
+}
+
+// Trilinear prolongation in the operand order of interpolateC2F
+// (include/Scalar3DUpdater.h:221-236), evaluated in double.
+template <typename T>
+__device__ __forceinline__ T c2f_value(const T* __restrict__ c, const Geo& g, int i, int j, int k) {
+  int I, J, K;
+  double r, s, t;
+  c2f_weight(i, g.n, I, r);
+  c2f_weight(j, g.n, J, s);
+  c2f_weight(k, g.n, K, t);
+  const double r1 = 1.0 - r, s1 = 1.0 - s, t1 = 1.0 - t;
+  const double c000 = c[g.idx(I, J, K)], c100 = c[g.idx(I + 1, J, K)];
+  const double c010 = c[g.idx(I, J + 1, K)], c110 = c[g.idx(I + 1, J + 1, K)];
+  const double c001 = c[g.idx(I, J, K + 1)], c101 = c[g.idx(I + 1, J, K + 1)];
+  const double c011 = c[g.idx(I, J + 1, K + 1)], c111 = c[g.idx(I + 1, J + 1, K + 1)];
+  const double lo = s1 * (r1 * c000 + r * c100) + s * (r1 * c010 + r * c110);
+  const double hi = s1 * (r1 * c001 + r * c101) + s * (r1 * c011 + r * c111);
+  return T(t1 * lo + t * hi);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ghost_fill(T* __restrict__ d, const int4* __restrict__ tasks, Geo g) {
+  const int4 tk = tasks[blockIdx.x];
+  const int face = tk.z & 7, sub = (tk.z >> 3) & 3, kind = (tk.z >> 5) & 3;
+  const int axis = face >> 1;
+  const bool plus = face & 1;
+  T* __restrict__ dst = d + (long long)tk.x * g.bvol;
+  const T* __restrict__ src = d + (long long)tk.y * g.bvol;
+  const int n = g.n, vc = g.vc, h = n / 2;
+  const int tn = (kind == KIND_F2C) ? h : n;  // tangential extent
+  // Loop space (x fastest): depth vc along the face normal.
+  const int ni = axis == 0 ? vc : tn;
+  const int nj = axis == 1 ? vc : tn;
+  const int nk = axis == 2 ? vc : tn;
+  const int count = ni * nj * nk;
+  const int so0 = sub & 1, so1 = sub >> 1;
+  for (int e = threadIdx.x; e < count; e += blockDim.x) {
+    const int i = e % ni;
+    const int j = (e / ni) % nj;
+    const int k = e / (ni * nj);
+    int dc[3] = {i, j, k};
+    int sc[3] = {i, j, k};
+    const int dpt = dc[axis];
+    if (kind == KIND_COPY) {
+      dc[axis] = plus ? n + dpt : dpt - vc;
+      sc[axis] = plus ? dpt : n - vc + dpt;
+      dst[g.idx(dc[0], dc[1], dc[2])] = src[g.idx(sc[0], sc[1], sc[2])];
+    } else if (kind == KIND_F2C) {
+      // Coarse ghost at the subface's quarter (copyFromNeighborF2C_0).
+      const int t0 = (axis + 1) % 3, t1 = (axis + 2) % 3;  // X:(y,z) Y:(z,x) Z:(x,y)
+      dc[axis] = plus ? n + dpt : dpt - vc;
+      dc[t0] += h * so0;
+      dc[t1] += h * so1;
+      sc[axis] = plus ? dpt : dpt + h - vc;
+      dst[g.idx(dc[0], dc[1], dc[2])] = f2c_value(src, g, sc[0], sc[1], sc[2]);
+    } else {
+      // Fine ghost interpolated from the coarse neighbour (copyFromNeighborC2F_0).
+      const int t0 = (axis + 1) % 3, t1 = (axis + 2) % 3;
+      dc[axis] = plus ? n + dpt : dpt - vc;
+      sc[axis] = plus ? dpt : dpt + 2 * n - vc;
+      sc[t0] += n * so0;
+      sc[t1] += n * so1;
+      dst[g.idx(dc[0], dc[1], dc[2])] = c2f_value(src, g, sc[0], sc[1], sc[2]);
+    }
+  }
+}
+
+template <typename T>
+int ghost_fill_raw(bcm_mesh* m, T* data, int mask) {
+  const int kinds = mask & BCM_GF_ALL;
+  int axes = (mask >> 3) & 7;
+  if (axes == 0) axes = 7;
+  if (kinds == 0) return BCM_OK;
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_GHOST, &ev));
+  auto launch = [&](int lo, int hi) -> int {
+    if (hi <= lo) return BCM_OK;
+    hipLaunchKernelGGL(k_ghost_fill<T>, dim3(hi - lo), dim3(256), 0, m->stream, data, m->d_tasks + lo, g);
+    BCM_HIP_TRY(hipGetLastError());
+    return BCM_OK;
+  };
+  if (kinds == BCM_GF_ALL && axes == 7) {
+    BCM_TRY(launch(0, m->ntask_total));
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      if (!(axes & (1 << a))) continue;
+      if (kinds == BCM_GF_ALL) {
+        BCM_TRY(launch(m->off[a][0], m->off[a][3]));
+        continue;
+      }
+      for (int k = 0; k < 3; ++k)
+        if (kinds & (1 << k)) BCM_TRY(launch(m->off[a][k], m->off[a][k + 1]));
+    }
+  }
+  return timer_end(m, &ev);
+}
+
+template int ghost_fill_raw<double>(bcm_mesh*, double*, int);
+template int ghost_fill_raw<float>(bcm_mesh*, float*, int);
+
+// ---------------------------------------------------------------------
+// Dirichlet mirror: g = 2v - u0 on the first ghost layer
+// (examples/sor1/Solver.cpp:232-294).
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_dirichlet(T* __restrict__ d, const int4* __restrict__ tasks,
+                                                   const double* __restrict__ vals, Geo g) {
+  const int4 tk = tasks[blockIdx.x];
+  if (tk.w != BCM_BC_DIRICHLET) return;
+  const int face = tk.y, axis = face >> 1;
+  const bool plus = face & 1;
+  T* __restrict__ f = d + (long long)tk.x * g.bvol;
+  const double v = vals[tk.z];
+  const int n = g.n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int u = e % n, w = e / n;
+    int c[3], s[3];
+    // X faces loop (j,k), Y faces (i,k), Z faces (i,j).
+    if (axis == 0) { c[1] = u; c[2] = w; }
+    else if (axis == 1) { c[0] = u; c[2] = w; }
+    else { c[0] = u; c[1] = w; }
+    s[0] = c[0]; s[1] = c[1]; s[2] = c[2];
+    c[axis] = plus ? n : -1;
+    s[axis] = plus ? n - 1 : 0;
+    f[g.idx(c[0], c[1], c[2])] = T(2.0 * v - double(f[g.idx(s[0], s[1], s[2])]));
+  }
+}
+
+template <typename T>
+static int dirichlet_raw(bcm_mesh* m, T* data) {
+  if (m->n_bc_tasks == 0) return BCM_OK;
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_DIRICHLET, &ev));
+  hipLaunchKernelGGL(k_dirichlet<T>, dim3(m->n_bc_tasks), dim3(256), 0, m->stream, data, m->d_bc_tasks,
+                     m->d_bvalues, g);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+int dirichlet_device(bcm_mesh* m, bcm_field* f) {
+  if (f->dtype == BCM_F64) return dirichlet_raw<double>(m, static_cast<double*>(f->data));
+  return dirichlet_raw<float>(m, static_cast<float*>(f->data));
+}
+
+// ---------------------------------------------------------------------
+// Reductions: deterministic two-stage sums.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ parts, int n,
+                                                         double* __restrict__ out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += parts[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+int reduce_partials(bcm_mesh* m, int nparts, int slot) { return reduce_partials_at(m, m->d_partials, nparts, slot); }
+
+int reduce_partials_at(bcm_mesh* m, const double* parts, int nparts, int slot) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_REDUCE, &ev));
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, m->stream, parts, nparts, m->d_result + slot);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+int fetch_result(bcm_mesh* m, int nslots, double* out) {
+  BCM_HIP_TRY(hipMemcpyAsync(m->h_result, m->d_result, sizeof(double) * size_t(nslots), hipMemcpyDeviceToHost,
+                             m->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  for (int i = 0; i < nslots; ++i) out[i] = m->h_result[i];
+  return BCM_OK;
+}
+
+}  // namespace bcm
+
+using namespace bcm;
+
+// ---------------------------------------------------------------------
+// C ABI: mesh
+// ---------------------------------------------------------------------
+
+static void mesh_release(bcm_mesh* m) {
+  if (!m) return;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (auto& p : m->pending) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (auto e : m->event_pool) (void)hipEventDestroy(e);
+  dev_free(m->d_tasks);
+  dev_free(m->d_bc_tasks);
+  dev_free(m->d_meta);
+  dev_free(m->d_partials);
+  dev_free(m->d_result);
+  dev_free(m->d_bvalues);
+  dev_free(m->d_sor_coef);
+  for (void* p : m->ws) dev_free(p);
+  if (m->h_result) (void)hipHostFree(m->h_result);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc, bcm_mesh_t* out) {
+  if (!out) return BCM_ERR_ARG;
+  *out = nullptr;
+  if (!blocks || nblocks <= 0 || n < 2 || (n % 2) != 0 || vc < 1 || n / 2 < vc || n > 256) return BCM_ERR_ARG;
+  // Validate the neighbour table.
+  for (int b = 0; b < nblocks; ++b) {
+    const bcm_block_info& bi = blocks[b];
+    for (int f = 0; f < 6; ++f) {
+      const int ld = bi.nb_level_diff[f];
+      if (ld < -1 || ld > 1) return BCM_ERR_ARG;
+      for (int s = 0; s < 4; ++s)
+        if (bi.nb_id[f][s] < -1 || bi.nb_id[f][s] >= nblocks) return BCM_ERR_ARG;
+      if (ld == -1 && (bi.nb_subface[f] < 0 || bi.nb_subface[f] > 3)) return BCM_ERR_ARG;
+      if (bi.bc_type[f] < BCM_BC_INNER || bi.bc_type[f] > BCM_BC_PERIODIC) return BCM_ERR_ARG;
+    }
+  }
+  bcm_mesh* m = new (std::nothrow) bcm_mesh();
+  if (!m) return BCM_ERR_NOMEM;
+  m->n = n;
+  m->vc = vc;
+  m->nblocks = nblocks;
+  m->nx0 = n + 2 * vc;
+  m->bvol = size_t(m->nx0) * m->nx0 * m->nx0;
+  m->ncell = size_t(nblocks) * n * n * n;
+  m->blocks.assign(blocks, blocks + nblocks);
+  int rc = BCM_OK;
+  do {
+    if (hipGetDevice(&m->device) != hipSuccess) { rc = BCM_ERR_HIP; break; }
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { rc = BCM_ERR_HIP; break; }
+    // Face tasks, as BlockManager::setFaceLists (src/BlockManager.cpp:471-530)
+    // walks them: per block, per face, per subface.
+    std::vector<int4> lists[3][3];
+    std::vector<int4> bc;
+    m->meta.resize(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      const bcm_block_info& bi = blocks[b];
+      BlockMeta& me = m->meta[b];
+      const double hx = bi.block_size[0] / n, hy = bi.block_size[1] / n, hz = bi.block_size[2] / n;
+      me.cx = 1.0 / (hx * hx);
+      me.cy = 1.0 / (hy * hy);
+      me.cz = 1.0 / (hz * hz);
+      me.dir_mask = 0;
+      for (int f = 0; f < 6; ++f) {
+        me.bc_id[f] = bi.bc_id[f];
+        const int axis = f / 2;
+        const int ld = bi.nb_level_diff[f];
+        if (ld == 0 || ld == -1) {
+          const int nb = bi.nb_id[f][0];
+          if (nb >= 0) {
+            const int kind = ld == 0 ? KIND_COPY : KIND_C2F;
+            const int sub = ld == 0 ? 0 : bi.nb_subface[f];
+            lists[axis][kind].push_back(make_int4(b, nb, f | (sub << 3) | (kind << 5), 0));
+          }
+        } else {
+          for (int s = 0; s < 4; ++s) {
+            const int nb = bi.nb_id[f][s];
+            if (nb >= 0) lists[axis][KIND_F2C].push_back(make_int4(b, nb, f | (s << 3) | (KIND_F2C << 5), 0));
+          }
+        }
+        if (bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) {
+          bc.push_back(make_int4(b, f, bi.bc_id[f], bi.bc_type[f]));
+          if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
+            me.dir_mask |= 1 << f;
+            m->n_dirichlet++;
+          }
+        }
+      }
+    }
+    std::vector<int4> all;
+    for (int a = 0; a < 3; ++a) {
+      for (int k = 0; k < 3; ++k) {
+        m->off[a][k] = int(all.size());
+        all.insert(all.end(), lists[a][k].begin(), lists[a][k].end());
+        m->kind_count[k] += int(lists[a][k].size());
+      }
+      m->off[a][3] = int(all.size());
+    }
+    m->ntask_total = int(all.size());
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_tasks), sizeof(int4) * all.size())) != BCM_OK) break;
+    if (!all.empty() &&
+        hipMemcpy(m->d_tasks, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = BCM_ERR_HIP;
+      break;
+    }
+    m->n_bc_tasks = int(bc.size());
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_bc_tasks), sizeof(int4) * bc.size())) != BCM_OK) break;
+    if (!bc.empty() &&
+        hipMemcpy(m->d_bc_tasks, bc.data(), sizeof(int4) * bc.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = BCM_ERR_HIP;
+      break;
+    }
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_meta), sizeof(BlockMeta) * size_t(nblocks))) != BCM_OK)
+      break;
+    if (hipMemcpy(m->d_meta, m->meta.data(), sizeof(BlockMeta) * size_t(nblocks), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      rc = BCM_ERR_HIP;
+      break;
+    }
+    // One partial per 256-cell work-group of a cell sweep.
+    m->n_partials = (m->ncell + 255) / 256 + 1024;
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_partials), sizeof(double) * m->n_partials)) != BCM_OK)
+      break;
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_result), sizeof(double) * 16)) != BCM_OK) break;
+    if (hipHostMalloc(reinterpret_cast<void**>(&m->h_result), sizeof(double) * 16, hipHostMallocDefault) !=
+        hipSuccess) {
+      m->h_result = nullptr;
+      rc = BCM_ERR_NOMEM;
+      break;
+    }
+  } while (false);
+  if (rc != BCM_OK) {
+    mesh_release(m);
+    return rc;
+  }
+  *out = m;
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_destroy(bcm_mesh_t mesh) {
+  mesh_release(mesh);
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_task_counts(bcm_mesh_t m, int counts[4]) {
+  if (!m || !counts) return BCM_ERR_ARG;
+  counts[0] = m->kind_count[KIND_COPY];
+  counts[1] = m->kind_count[KIND_F2C];
+  counts[2] = m->kind_count[KIND_C2F];
+  counts[3] = m->n_bc_tasks;
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_sync(bcm_mesh_t m) {
+  if (!m) return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_set_timing(bcm_mesh_t m, int enable) {
+  if (!m) return BCM_ERR_ARG;
+  m->timing = enable != 0;
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_kernel_time(bcm_mesh_t m, int kernel, double* total_ms, long long* launches) {
+  if (!m || kernel < 0 || kernel >= BCM_K_COUNT) return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  for (auto& p : m->pending) {
+    float ms = 0.f;
+    BCM_HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
+    m->kernel_ms[p.kernel] += ms;
+    m->kernel_launches[p.kernel] += 1;
+    m->event_pool.push_back(p.start);
+    m->event_pool.push_back(p.stop);
+  }
+  m->pending.clear();
+  if (total_ms) *total_ms = m->kernel_ms[kernel];
+  if (launches) *launches = m->kernel_launches[kernel];
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_reset_timing(bcm_mesh_t m) {
+  if (!m) return BCM_ERR_ARG;
+  double dummy;
+  BCM_TRY(bcm_mesh_kernel_time(m, 0, &dummy, nullptr));
+  for (int k = 0; k < BCM_K_COUNT; ++k) {
+    m->kernel_ms[k] = 0.0;
+    m->kernel_launches[k] = 0;
+  }
+  return BCM_OK;
+}
+
+// ---------------------------------------------------------------------
+// C ABI: fields
+// ---------------------------------------------------------------------
+
+extern "C" int bcm_field_create(bcm_mesh_t m, int dtype, bcm_field_t* out) {
+  if (!m || !out || (dtype != BCM_F64 && dtype != BCM_F32)) return BCM_ERR_ARG;
+  *out = nullptr;
+  bcm_field* f = new (std::nothrow) bcm_field();
+  if (!f) return BCM_ERR_NOMEM;
+  f->mesh = m;
+  f->dtype = dtype;
+  f->count = size_t(m->nblocks) * m->bvol;
+  f->bytes = f->count * (dtype == BCM_F64 ? 8 : 4);
+  int rc = dev_alloc(&f->data, f->bytes);
+  if (rc != BCM_OK) {
+    delete f;
+    return rc;
+  }
+  if (hipMemsetAsync(f->data, 0, f->bytes, m->stream) != hipSuccess) {
+    dev_free(f->data);
+    delete f;
+    return BCM_ERR_HIP;
+  }
+  *out = f;
+  return BCM_OK;
+}
+
+extern "C" int bcm_field_destroy(bcm_field_t f) {
+  if (!f) return BCM_OK;
+  if (f->mesh) (void)hipStreamSynchronize(f->mesh->stream);
+  dev_free(f->data);
+  delete f;
+  return BCM_OK;
+}
+
+extern "C" int bcm_field_upload(bcm_field_t f, const void* host, size_t bytes) {
+  if (!f || !host || bytes != f->bytes) return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipMemcpyAsync(f->data, host, bytes, hipMemcpyHostToDevice, f->mesh->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(f->mesh->stream));
+  return BCM_OK;
+}
+
+extern "C" int bcm_field_download(bcm_field_t f, void* host, size_t bytes) {
+  if (!f || !host || bytes != f->bytes) return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipMemcpyAsync(host, f->data, bytes, hipMemcpyDeviceToHost, f->mesh->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(f->mesh->stream));
+  return BCM_OK;
+}
+
+extern "C" void* bcm_field_device_ptr(bcm_field_t f) { return f ? f->data : nullptr; }
+
+extern "C" size_t bcm_field_bytes(bcm_field_t f) { return f ? f->bytes : 0; }
+
+extern "C" int bcm_ghost_fill(bcm_field_t f, int mask) {
+  if (!f || (mask & ~63)) return BCM_ERR_ARG;
+  if (f->dtype == BCM_F64) return ghost_fill_raw<double>(f->mesh, static_cast<double*>(f->data), mask);
+  return ghost_fill_raw<float>(f->mesh, static_cast<float*>(f->data), mask);
+}
+
+extern "C" int bcm_dirichlet_mirror(bcm_field_t f, const double* values, int nvalues) {
+  if (!f) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  for (const auto& bi : m->blocks)
+    for (int k = 0; k < 6; ++k)
+      if (bi.bc_type[k] == BCM_BC_NEUMANN) return BCM_ERR_ARG;  // not implemented, as in sor1/sor2
+  BCM_TRY(set_bvalues(m, values, nvalues));
+  if (f->dtype == BCM_F64) return dirichlet_raw<double>(m, static_cast<double*>(f->data));
+  return dirichlet_raw<float>(m, static_cast<float*>(f->data));
+}

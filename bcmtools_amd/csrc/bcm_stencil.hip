@@ -1,0 +1,856 @@
+// Per-block stencil sweeps, BLAS-1 and reductions over all blocks.
+//
+// Data layout: block-major, each block a ghosted (n+2vc)^3 array with x
+// fastest (include/Index3D.h:35-59).  Every kernel is HBM-bound (well
+// below 1 flop/byte), so the design goal is one read of each input and
+// one write of each output per sweep: coalesced x-rows, LDS staging of
+// the z-planes for the red-black sweep, and no MFMA.
+//
+// Arithmetic follows the reference operand order with FP contraction
+// disabled at compile time, so FP64 results are bit-identical to the
+// CPU path (no fused multiply-add, IEEE division).
+#include <cmath>
+
+#include "bcm_internal.h"
+
+namespace bcm {
+
+struct CellGeo {
+  int n, vc, nx0;
+  long long bvol;
+  long long ncell;  // nblocks * n^3
+};
+
+// Cells per thread and cells per work-group of a cell sweep.
+constexpr int kCPT = 4;
+constexpr int kWG = 256;
+constexpr int kCellsPerWG = kCPT * kWG;
+
+__device__ __forceinline__ long long cell_offset(const CellGeo& g, long long c, int& b, int& i, int& j, int& k) {
+  const long long n3 = (long long)g.n * g.n * g.n;
+  b = int(c / n3);
+  const int rem = int(c - (long long)b * n3);
+  i = rem % g.n;
+  j = (rem / g.n) % g.n;
+  k = rem / (g.n * g.n);
+  return (long long)b * g.bvol + (i + g.vc) + (long long)g.nx0 * ((j + g.vc) + (long long)g.nx0 * (k + g.vc));
+}
+
+// Deterministic work-group sum of one double per thread (wave64 butterfly,
+// then a fixed-order sum of the four wave totals).
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = ((red[0] + red[1]) + red[2]) + red[3];
+  return s;
+}
+
+static int cell_grid(const bcm_mesh* m) { return int((m->ncell + kCellsPerWG - 1) / kCellsPerWG); }
+
+static CellGeo cell_geo(const bcm_mesh* m) {
+  return CellGeo{m->n, m->vc, m->nx0, (long long)m->bvol, (long long)m->ncell};
+}
+
+// ---------------------------------------------------------------------
+// Red-black SOR
+// ---------------------------------------------------------------------
+// Per-block coefficients {c0, cx, cy, cz}:
+//   sor1 (examples/sor1/Solver.cpp:98-106): c = 2(cx+cy+cz); cx = w cx / c ...; c0 = 1 - w
+//   sor2 (examples/sor2/Solver.cpp:99-102): cx = 1/dx^2 ...; c0 = 1 / (2 (cx+cy+cz))
+static int sor_coef(bcm_mesh* m, double omega, int form) {
+  if (m->d_sor_coef && m->sor_omega == omega && m->sor_form == form) return BCM_OK;
+  std::vector<double> h(size_t(m->nblocks) * 4);
+  for (int b = 0; b < m->nblocks; ++b) {
+    const bcm_block_info& bi = m->blocks[b];
+    const double dx = bi.block_size[0] / m->n, dy = bi.block_size[1] / m->n, dz = bi.block_size[2] / m->n;
+    double cx = 1.0 / (dx * dx);
+    double cy = 1.0 / (dy * dy);
+    double cz = 1.0 / (dz * dz);
+    double c0;
+    if (form == 1) {
+      const double c = 2.0 * (cx + cy + cz);
+      cx = omega * cx / c;
+      cy = omega * cy / c;
+      cz = omega * cz / c;
+      c0 = 1.0 - omega;
+    } else {
+      c0 = 1.0 / (2.0 * (cx + cy + cz));
+    }
+    h[4 * b + 0] = c0;
+    h[4 * b + 1] = cx;
+    h[4 * b + 2] = cy;
+    h[4 * b + 3] = cz;
+  }
+  if (!m->d_sor_coef) BCM_TRY(dev_alloc(reinterpret_cast<void**>(&m->d_sor_coef), h.size() * sizeof(double)));
+  BCM_HIP_TRY(hipMemcpyAsync(m->d_sor_coef, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  m->sor_omega = omega;
+  m->sor_form = form;
+  return BCM_OK;
+}
+
+// One cell update.  sor1: f = c0 f + cx(W+E) + cy(S+N) + cz(B+T);
+// sor2: fNew = c0 (cx(W+E) + cy(S+N) + cz(B+T) - s); f = w fNew + (1-w) f.
+template <typename T, bool SRC>
+__device__ __forceinline__ T sor_update(T fc, T fw, T fe, T fs, T fn, T fb, T ft, T sv, T c0, T cx, T cy, T cz,
+                                        T om, T om1) {
+  if (SRC) {
+    const T fnew = c0 * (((cx * (fw + fe) + cy * (fs + fn)) + cz * (fb + ft)) - sv);
+    return om * fnew + om1 * fc;
+  } else {
+    return ((c0 * fc + cx * (fw + fe)) + cy * (fs + fn)) + cz * (fb + ft);
+  }
+}
+
+// Fused two-colour sweep of one block per work-group, vc == 1.
+// Thread t owns the 2x2 patch (2pi..2pi+1, 2pj..2pj+1) of every z-plane,
+// so each thread updates two red and two black cells per plane.  A ring
+// of four ghosted z-planes sits in LDS; at step k the red cells of plane k
+// and then the black cells of plane k-1 are updated (black cells read the
+// red values of planes k-2..k, all final by then), and plane k-1 is
+// written back.  Plane k+2 is prefetched into registers during step k.
+template <typename T, int N, bool SRC>
+__global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, const T* __restrict__ s,
+                                                         const double* __restrict__ coef, double omega) {
+  constexpr int NX = N + 2;
+  constexpr int PL = NX * NX;
+  constexpr int NT = N * N / 4;
+  constexpr int HALF = N / 2;
+  constexpr int PER = (PL + NT - 1) / NT;
+  __shared__ T lds[4][PL];
+  const int b = blockIdx.x;
+  const long long base = (long long)b * PL * NX;
+  T* __restrict__ fb = f + base;
+  const T* __restrict__ sb = SRC ? s + base : nullptr;
+  const T c0 = T(coef[4 * b + 0]), cx = T(coef[4 * b + 1]), cy = T(coef[4 * b + 2]), cz = T(coef[4 * b + 3]);
+  const T om = T(omega), om1 = T(1.0 - omega);
+  const int t = threadIdx.x;
+  const int i0 = 2 * (t % HALF), j0 = 2 * (t / HALF);
+  T pre[PER];
+  auto fetch = [&](int p) {
+    const T* src = fb + (long long)(p + 1) * PL;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + q * NT;
+      if (e < PL) pre[q] = src[e];
+    }
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = t + q * NT;
+      if (e < PL) lds[slot][e] = pre[q];
+    }
+  };
+  auto L = [](int i, int j) { return (j + 1) * NX + (i + 1); };
+  // Update the two cells of colour `col` in plane p of my patch.
+  auto update = [&](int p, int col, const T* sv) {
+    const int P = p & 3, Pm = (p + 3) & 3, Pp = (p + 1) & 3;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int a = q & 1, bb = q >> 1;
+      if (((a + bb + p) & 1) != col) continue;
+      const int i = i0 + a, j = j0 + bb;
+      const int c = L(i, j);
+      const T v = sor_update<T, SRC>(lds[P][c], lds[P][c - 1], lds[P][c + 1], lds[P][c - NX], lds[P][c + NX],
+                                     lds[Pm][c], lds[Pp][c], SRC ? sv[q] : T(0), c0, cx, cy, cz, om, om1);
+      lds[P][c] = v;
+    }
+  };
+  auto store = [&](int p) {
+    const int P = p & 3;
+    T* dst = fb + (long long)(p + 1) * PL;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + (q & 1), j = j0 + (q >> 1);
+      dst[L(i, j)] = lds[P][L(i, j)];
+    }
+  };
+  auto load_s = [&](int p, T* sv) {
+    if (!SRC) return;
+    const T* src = sb + (long long)(p + 1) * PL;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sv[q] = src[L(i0 + (q & 1), j0 + (q >> 1))];
+  };
+
+  T s_cur[4] = {T(0), T(0), T(0), T(0)}, s_prev[4] = {T(0), T(0), T(0), T(0)};
+  fetch(-1);
+  commit(3);
+  fetch(0);
+  commit(0);
+  fetch(1);
+  for (int k = 0; k < N; ++k) {
+    commit((k + 1) & 3);
+    if (k + 2 <= N) fetch(k + 2);
+    load_s(k, s_cur);
+    __syncthreads();
+    update(k, 0, s_cur);
+    __syncthreads();
+    if (k >= 1) {
+      update(k - 1, 1, s_prev);
+      store(k - 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s_prev[q] = s_cur[q];
+  }
+  update(N - 1, 1, s_prev);
+  store(N - 1);
+}
+
+// Generic one-colour pass for any n, vc: one thread per cell of the colour.
+template <typename T, bool SRC>
+__global__ __launch_bounds__(256) void k_rbsor_colour(T* __restrict__ f, const T* __restrict__ s,
+                                                      const double* __restrict__ coef, double omega, CellGeo g,
+                                                      int colour) {
+  const long long half = g.ncell / 2;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= half) return;
+  // Enumerate (block, k, j, ih) with i = 2 ih + (colour + j + k) % 2.
+  const int n = g.n, nh = n / 2;
+  const long long per_block = (long long)n * n * nh;
+  const int b = int(c / per_block);
+  int rem = int(c - (long long)b * per_block);
+  const int ih = rem % nh;
+  const int j = (rem / nh) % n;
+  const int k = rem / (nh * n);
+  const int i = 2 * ih + ((colour + j + k) & 1);
+  const int sy = g.nx0, sz = g.nx0 * g.nx0;
+  const long long o = (long long)b * g.bvol + (i + g.vc) + (long long)sy * (j + g.vc) + (long long)sz * (k + g.vc);
+  const T c0 = T(coef[4 * b + 0]), cx = T(coef[4 * b + 1]), cy = T(coef[4 * b + 2]), cz = T(coef[4 * b + 3]);
+  f[o] = sor_update<T, SRC>(f[o], f[o - 1], f[o + 1], f[o - sy], f[o + sy], f[o - sz], f[o + sz],
+                            SRC ? s[o] : T(0), c0, cx, cy, cz, T(omega), T(1.0 - omega));
+}
+
+template <typename T>
+static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega) {
+  const int form = s ? 2 : 1;
+  BCM_TRY(sor_coef(m, omega, form));
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_SOR, &ev));
+  const int nb = m->nblocks;
+  if (m->vc == 1 && m->n == 32) {
+    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 32, true>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega);
+    else hipLaunchKernelGGL((k_rbsor_fused<T, 32, false>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega);
+  } else if (m->vc == 1 && m->n == 16) {
+    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 16, true>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega);
+    else hipLaunchKernelGGL((k_rbsor_fused<T, 16, false>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega);
+  } else {
+    const CellGeo g = cell_geo(m);
+    const long long half = g.ncell / 2;
+    const int grid = int((half + 255) / 256);
+    for (int colour = 0; colour < 2; ++colour) {
+      if (s) hipLaunchKernelGGL((k_rbsor_colour<T, true>), dim3(grid), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, g, colour);
+      else hipLaunchKernelGGL((k_rbsor_colour<T, false>), dim3(grid), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, g, colour);
+    }
+  }
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+// Relative residual of lap_h f = s: r = s - (cx(W+E-2C) + cy(..) + cz(..)).
+__global__ __launch_bounds__(kWG) void k_sor_residual(const double* __restrict__ f, const double* __restrict__ s,
+                                                      const double* __restrict__ coef_sor2, CellGeo g,
+                                                      double* __restrict__ parts) {
+  __shared__ double red[4];
+  const int sy = g.nx0, sz = g.nx0 * g.nx0;
+  double rr = 0.0, ss = 0.0;
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) break;
+    int b, i, j, k;
+    const long long o = cell_offset(g, c, b, i, j, k);
+    const double cx = coef_sor2[4 * b + 1], cy = coef_sor2[4 * b + 2], cz = coef_sor2[4 * b + 3];
+    const double fc = f[o];
+    const double lap = (cx * ((f[o - 1] + f[o + 1]) - 2.0 * fc) + cy * ((f[o - sy] + f[o + sy]) - 2.0 * fc)) +
+                       cz * ((f[o - sz] + f[o + sz]) - 2.0 * fc);
+    const double r = s[o] - lap;
+    rr += r * r;
+    ss += s[o] * s[o];
+  }
+  const double a = wg_sum(rr, red);
+  __syncthreads();
+  const double bsum = wg_sum(ss, red);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = a;
+    parts[gridDim.x + blockIdx.x] = bsum;
+  }
+}
+
+// ---------------------------------------------------------------------
+// 7-point operators.  Stored coefficients follow blas.f90; the Poisson
+// operator computes the same coefficients on the fly: l = 1/h^2 per
+// axis, Ap = -(l0+l1)-(l2+l3)-(l4+l5) (bcut.f90:2857-2859) and, on an
+// outer Dirichlet face, Ap -= Anb; Anb = 0 (bca.f90:12-42).
+// ---------------------------------------------------------------------
+template <typename T>
+struct Coef7 {
+  T p, w, e, s, n, b, t;
+};
+
+template <typename T>
+__device__ __forceinline__ Coef7<T> poisson_coef(const BlockMeta& me, int i, int j, int k, int n) {
+  const T lx = T(me.cx), ly = T(me.cy), lz = T(me.cz);
+  Coef7<T> a;
+  a.w = lx; a.e = lx; a.s = ly; a.n = ly; a.b = lz; a.t = lz;
+  a.p = ((-(a.w + a.e)) - (a.s + a.n)) - (a.b + a.t);
+  const int dm = me.dir_mask;
+  if (dm) {
+    if ((dm & 1) && i == 0) { a.p = a.p - a.w; a.w = T(0); }
+    if ((dm & 2) && i == n - 1) { a.p = a.p - a.e; a.e = T(0); }
+    if ((dm & 4) && j == 0) { a.p = a.p - a.s; a.s = T(0); }
+    if ((dm & 8) && j == n - 1) { a.p = a.p - a.n; a.n = T(0); }
+    if ((dm & 16) && k == 0) { a.p = a.p - a.b; a.b = T(0); }
+    if ((dm & 32) && k == n - 1) { a.p = a.p - a.t; a.t = T(0); }
+  }
+  return a;
+}
+
+template <typename T>
+struct StoredA {
+  const T *p, *w, *e, *s, *n, *b, *t;
+};
+
+template <typename T>
+__device__ __forceinline__ Coef7<T> stored_coef(const StoredA<T>& A, long long o) {
+  return Coef7<T>{A.p[o], A.w[o], A.e[o], A.s[o], A.n[o], A.b[o], A.t[o]};
+}
+
+// jacobi_smoother (blas.f90:485-493).
+template <typename T>
+__device__ __forceinline__ T jacobi_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz,
+                                         T bv, T om) {
+  T r = bv - a.w * x[o - 1];
+  r = r - a.e * x[o + 1];
+  r = r - a.s * x[o - sy];
+  r = r - a.n * x[o + sy];
+  r = r - a.b * x[o - sz];
+  r = r - a.t * x[o + sz];
+  const T xt = r / a.p;
+  const T xc = x[o];
+  return xc - om * (xc - xt);
+}
+
+// calc_ax (blas.f90:584-591): neighbours first, centre last.
+template <typename T>
+__device__ __forceinline__ T ax_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz) {
+  T v = a.w * x[o - 1];
+  v = v + a.e * x[o + 1];
+  v = v + a.s * x[o - sy];
+  v = v + a.n * x[o + sy];
+  v = v + a.b * x[o - sz];
+  v = v + a.t * x[o + sz];
+  return v + a.p * x[o];
+}
+
+// calc_r (blas.f90:632-639).
+template <typename T>
+__device__ __forceinline__ T r_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz, T bv) {
+  T r = bv - a.w * x[o - 1];
+  r = r - a.e * x[o + 1];
+  r = r - a.s * x[o - sy];
+  r = r - a.n * x[o + sy];
+  r = r - a.b * x[o - sz];
+  r = r - a.t * x[o + sz];
+  return r - a.p * x[o];
+}
+
+template <typename T, bool STORED>
+__global__ __launch_bounds__(kWG) void k_jacobi(T* __restrict__ x1, const T* __restrict__ x0, StoredA<T> A,
+                                                const BlockMeta* __restrict__ meta, const T* __restrict__ b,
+                                                double omega, CellGeo g) {
+  const int sy = g.nx0, sz = g.nx0 * g.nx0;
+  const T om = T(omega);
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) return;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    const Coef7<T> a = STORED ? stored_coef(A, o) : poisson_coef<T>(meta[bk], i, j, k, g.n);
+    x1[o] = jacobi_cell(a, x0, o, sy, sz, b[o], om);
+  }
+}
+
+// mode 0: Ax; mode 1: r = b - Ax; mode 2: sum r^2 only.
+template <typename T, bool STORED, int MODE>
+__global__ __launch_bounds__(kWG) void k_apply(T* __restrict__ out, const T* __restrict__ x, StoredA<T> A,
+                                               const BlockMeta* __restrict__ meta, const T* __restrict__ b,
+                                               CellGeo g, double* __restrict__ parts) {
+  __shared__ double red[4];
+  const int sy = g.nx0, sz = g.nx0 * g.nx0;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) break;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    const Coef7<T> a = STORED ? stored_coef(A, o) : poisson_coef<T>(meta[bk], i, j, k, g.n);
+    if (MODE == 0) {
+      out[o] = ax_cell(a, x, o, sy, sz);
+    } else {
+      const T r = r_cell(a, x, o, sy, sz, b[o]);
+      if (MODE == 1) out[o] = r;
+      else acc += double(r) * double(r);
+    }
+  }
+  if (MODE == 2) {
+    const double s = wg_sum(acc, red);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+  }
+}
+
+// FP64 residual r = b - A x of the Poisson operator, written to FP32
+// (the refinement step's right-hand side), with sum r^2 in FP64.
+__global__ __launch_bounds__(kWG) void k_residual_d2f(float* __restrict__ r32, const double* __restrict__ x,
+                                                      const BlockMeta* __restrict__ meta,
+                                                      const double* __restrict__ b, CellGeo g,
+                                                      double* __restrict__ parts) {
+  __shared__ double red[4];
+  const int sy = g.nx0, sz = g.nx0 * g.nx0;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) break;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    const Coef7<double> a = poisson_coef<double>(meta[bk], i, j, k, g.n);
+    const double r = r_cell(a, x, o, sy, sz, b[o]);
+    if (r32) r32[o] = float(r);
+    acc += r * r;
+  }
+  const double s = wg_sum(acc, red);
+  if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// dot (blas.f90:379-412): per-thread accumulation in T, work-group and
+// global sums in FP64 by a fixed tree.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_dot(const T* __restrict__ x, const T* __restrict__ y, CellGeo g,
+                                             double* __restrict__ parts) {
+  __shared__ double red[4];
+  T acc = T(0);
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) break;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    acc = acc + x[o] * y[o];
+  }
+  const double s = wg_sum(double(acc), red);
+  if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// blas.f90:242-377 (interior cells only).
+template <typename T, int OP>
+__global__ __launch_bounds__(kWG) void k_blas(T* __restrict__ z, const T* __restrict__ x, const T* __restrict__ y,
+                                              double a, double bcoef, CellGeo g) {
+  const T ta = T(a), tb = T(bcoef);
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) return;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    if (OP == OP_AXPY) z[o] = z[o] + ta * x[o];                       // y = y + a x
+    else if (OP == OP_XPAY) z[o] = x[o] + ta * z[o];                  // y = x + a y
+    else if (OP == OP_AXPYZ) z[o] = ta * x[o] + y[o];                 // z = a x + y
+    else if (OP == OP_AXPBYPZ) z[o] = (ta * x[o] + tb * y[o]) + z[o]; // z = a x + b y + z
+    else z[o] = x[o];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_all(T* __restrict__ x, T v, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_all(T* __restrict__ y, const T* __restrict__ x, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) y[i] = x[i];
+}
+
+// Interior conversions for the refinement loop.
+__global__ __launch_bounds__(kWG) void k_convert(void* __restrict__ dst, const void* __restrict__ src, int op,
+                                                 CellGeo g) {
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) return;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    if (op == CONV_D2F) static_cast<float*>(dst)[o] = float(static_cast<const double*>(src)[o]);
+    else static_cast<double*>(dst)[o] = static_cast<double*>(dst)[o] + double(static_cast<const float*>(src)[o]);
+  }
+}
+
+// Poisson assembly: stored A (optional) and folded b
+// (bcut.f90:2848-2865 without cut cells; bca.f90 bc_A*_d: b -= Anb xc 2).
+__global__ __launch_bounds__(kWG) void k_assemble(double* Ap, double* Aw, double* Ae, double* As, double* An,
+                                                  double* Ab, double* At, double* __restrict__ bout,
+                                                  const double* __restrict__ rhs, const BlockMeta* __restrict__ meta,
+                                                  const double* __restrict__ bval, CellGeo g) {
+#pragma unroll
+  for (int q = 0; q < kCPT; ++q) {
+    const long long c = (long long)blockIdx.x * kCellsPerWG + q * kWG + threadIdx.x;
+    if (c >= g.ncell) return;
+    int bk, i, j, k;
+    const long long o = cell_offset(g, c, bk, i, j, k);
+    const BlockMeta me = meta[bk];
+    const double l[6] = {me.cx, me.cx, me.cy, me.cy, me.cz, me.cz};
+    double a[6] = {l[0], l[1], l[2], l[3], l[4], l[5]};
+    double ap = ((-(a[0] + a[1])) - (a[2] + a[3])) - (a[4] + a[5]);
+    double bv = rhs[o];
+    const int at[6] = {i == 0, i == g.n - 1, j == 0, j == g.n - 1, k == 0, k == g.n - 1};
+    for (int f = 0; f < 6; ++f) {
+      if ((me.dir_mask >> f & 1) && at[f]) {
+        ap = ap - a[f];
+        bv = bv - a[f] * bval[me.bc_id[f]] * 2.0;
+        a[f] = 0.0;
+      }
+    }
+    bout[o] = bv;
+    if (Ap) {
+      Ap[o] = ap; Aw[o] = a[0]; Ae[o] = a[1]; As[o] = a[2]; An[o] = a[3]; Ab[o] = a[4]; At[o] = a[5];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// Raw launchers
+// ---------------------------------------------------------------------
+template <typename T>
+int jacobi_poisson_raw(bcm_mesh* m, T* x1, const T* x0, const T* b, double omega) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_JACOBI, &ev));
+  hipLaunchKernelGGL((k_jacobi<T, false>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream, x1, x0, StoredA<T>{},
+                     m->d_meta, b, omega, cell_geo(m));
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+template int jacobi_poisson_raw<double>(bcm_mesh*, double*, const double*, const double*, double);
+template int jacobi_poisson_raw<float>(bcm_mesh*, float*, const float*, const float*, double);
+
+template <typename T>
+int spmv_poisson_raw(bcm_mesh* m, T* ax, const T* x) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_SPMV, &ev));
+  hipLaunchKernelGGL((k_apply<T, false, 0>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream, ax, x, StoredA<T>{},
+                     m->d_meta, (const T*)nullptr, cell_geo(m), m->d_partials);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+template int spmv_poisson_raw<double>(bcm_mesh*, double*, const double*);
+template int spmv_poisson_raw<float>(bcm_mesh*, float*, const float*);
+
+template <typename T>
+int residual_poisson_raw(bcm_mesh* m, T* r, const T* x, const T* b, bool want_norm, int slot) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_RESID, &ev));
+  const int grid = cell_grid(m);
+  if (want_norm)
+    hipLaunchKernelGGL((k_apply<T, false, 2>), dim3(grid), dim3(kWG), 0, m->stream, r, x, StoredA<T>{}, m->d_meta, b,
+                       cell_geo(m), m->d_partials);
+  else
+    hipLaunchKernelGGL((k_apply<T, false, 1>), dim3(grid), dim3(kWG), 0, m->stream, r, x, StoredA<T>{}, m->d_meta, b,
+                       cell_geo(m), m->d_partials);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(timer_end(m, &ev));
+  if (want_norm) BCM_TRY(reduce_partials(m, grid, slot));
+  return BCM_OK;
+}
+template int residual_poisson_raw<double>(bcm_mesh*, double*, const double*, const double*, bool, int);
+template int residual_poisson_raw<float>(bcm_mesh*, float*, const float*, const float*, bool, int);
+
+int residual_d2f_raw(bcm_mesh* m, float* r32, const double* x, const double* b, int slot) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_RESID, &ev));
+  const int grid = cell_grid(m);
+  hipLaunchKernelGGL(k_residual_d2f, dim3(grid), dim3(kWG), 0, m->stream, r32, x, m->d_meta, b, cell_geo(m),
+                     m->d_partials);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(timer_end(m, &ev));
+  return reduce_partials(m, grid, slot);
+}
+
+template <typename T>
+int dot_raw(bcm_mesh* m, const T* x, const T* y, int slot) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_DOT, &ev));
+  const int grid = cell_grid(m);
+  hipLaunchKernelGGL(k_dot<T>, dim3(grid), dim3(kWG), 0, m->stream, x, y, cell_geo(m), m->d_partials);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(timer_end(m, &ev));
+  return reduce_partials(m, grid, slot);
+}
+template int dot_raw<double>(bcm_mesh*, const double*, const double*, int);
+template int dot_raw<float>(bcm_mesh*, const float*, const float*, int);
+
+template <typename T>
+int blas_raw(bcm_mesh* m, int op, T* z, const T* x, const T* y, double a, double b) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_BLAS1, &ev));
+  const dim3 grid(cell_grid(m)), blk(kWG);
+  const CellGeo g = cell_geo(m);
+  switch (op) {
+    case OP_AXPY: hipLaunchKernelGGL((k_blas<T, OP_AXPY>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
+    case OP_XPAY: hipLaunchKernelGGL((k_blas<T, OP_XPAY>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
+    case OP_AXPYZ: hipLaunchKernelGGL((k_blas<T, OP_AXPYZ>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
+    case OP_AXPBYPZ: hipLaunchKernelGGL((k_blas<T, OP_AXPBYPZ>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
+    default: hipLaunchKernelGGL((k_blas<T, OP_COPY_INTERIOR>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
+  }
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+template int blas_raw<double>(bcm_mesh*, int, double*, const double*, const double*, double, double);
+template int blas_raw<float>(bcm_mesh*, int, float*, const float*, const float*, double, double);
+
+template <typename T>
+int fill_raw(bcm_mesh* m, T* x, double a) {
+  const long long count = (long long)m->nblocks * (long long)m->bvol;
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_BLAS1, &ev));
+  hipLaunchKernelGGL(k_fill_all<T>, dim3(unsigned((count + 255) / 256)), dim3(256), 0, m->stream, x, T(a), count);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+template int fill_raw<double>(bcm_mesh*, double*, double);
+template int fill_raw<float>(bcm_mesh*, float*, double);
+
+int convert_raw(bcm_mesh* m, int op, void* dst, const void* src) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_BLAS1, &ev));
+  hipLaunchKernelGGL(k_convert, dim3(cell_grid(m)), dim3(kWG), 0, m->stream, dst, src, op, cell_geo(m));
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+int assemble_poisson_b_raw(bcm_mesh* m, double* b, const double* rhs) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_ASSEMBLE, &ev));
+  hipLaunchKernelGGL(k_assemble, dim3(cell_grid(m)), dim3(kWG), 0, m->stream, (double*)nullptr, (double*)nullptr,
+                     (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, b, rhs,
+                     m->d_meta, m->d_bvalues, cell_geo(m));
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+}  // namespace bcm
+
+using namespace bcm;
+
+// ---------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------
+namespace {
+
+bool same_mesh(bcm_field_t a, bcm_field_t b) { return a && b && a->mesh == b->mesh && a->dtype == b->dtype; }
+
+template <typename T>
+StoredA<T> stored(const bcm_field_t A[7]) {
+  return StoredA<T>{static_cast<const T*>(A[0]->data), static_cast<const T*>(A[1]->data),
+                    static_cast<const T*>(A[2]->data), static_cast<const T*>(A[3]->data),
+                    static_cast<const T*>(A[4]->data), static_cast<const T*>(A[5]->data),
+                    static_cast<const T*>(A[6]->data)};
+}
+
+bool check_A(bcm_field_t ref, const bcm_field_t A[7]) {
+  if (!A) return false;
+  for (int i = 0; i < 7; ++i)
+    if (!same_mesh(ref, A[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" int bcm_rbsor_sweep(bcm_field_t f, bcm_field_t source, double omega) {
+  if (!f || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
+  if (f->dtype == BCM_F64)
+    return rbsor_raw<double>(f->mesh, static_cast<double*>(f->data),
+                             source ? static_cast<const double*>(source->data) : nullptr, omega);
+  return rbsor_raw<float>(f->mesh, static_cast<float*>(f->data),
+                          source ? static_cast<const float*>(source->data) : nullptr, omega);
+}
+
+extern "C" int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
+                               int iters) {
+  if (!f || iters < 0 || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
+  BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));  // validates the table
+  for (int it = 0; it < iters; ++it) {
+    BCM_TRY(bcm_rbsor_sweep(f, source, omega));
+    BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));
+    BCM_TRY(bcm_ghost_fill(f, BCM_GF_ALL));
+    BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));
+  }
+  return BCM_OK;
+}
+
+extern "C" int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relres) {
+  if (!f || !source || !relres || !same_mesh(f, source) || f->dtype != BCM_F64) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  BCM_TRY(sor_coef(m, m->sor_form == 2 ? m->sor_omega : 1.0, 2));
+  const int grid = cell_grid(m);
+  if (size_t(2 * grid) > m->n_partials) return BCM_ERR_STATE;
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_RESID, &ev));
+  hipLaunchKernelGGL(k_sor_residual, dim3(grid), dim3(kWG), 0, m->stream, static_cast<const double*>(f->data),
+                     static_cast<const double*>(source->data), m->d_sor_coef, cell_geo(m), m->d_partials);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(timer_end(m, &ev));
+  BCM_TRY(reduce_partials(m, grid, 0));
+  BCM_TRY(reduce_partials_at(m, m->d_partials + grid, grid, 1));
+  double v[2];
+  BCM_TRY(fetch_result(m, 2, v));
+  *relres = v[1] > 0.0 ? std::sqrt(v[0] / v[1]) : std::sqrt(v[0]);
+  return BCM_OK;
+}
+
+extern "C" int bcm_jacobi7(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, double omega) {
+  if (!same_mesh(x1, x0) || !same_mesh(x1, b) || !check_A(x1, A) || x1 == x0) return BCM_ERR_ARG;
+  bcm_mesh* m = x1->mesh;
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_JACOBI, &ev));
+  if (x1->dtype == BCM_F64)
+    hipLaunchKernelGGL((k_jacobi<double, true>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream,
+                       static_cast<double*>(x1->data), static_cast<const double*>(x0->data), stored<double>(A),
+                       m->d_meta, static_cast<const double*>(b->data), omega, cell_geo(m));
+  else
+    hipLaunchKernelGGL((k_jacobi<float, true>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream,
+                       static_cast<float*>(x1->data), static_cast<const float*>(x0->data), stored<float>(A),
+                       m->d_meta, static_cast<const float*>(b->data), omega, cell_geo(m));
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+template <typename T, int MODE>
+static int apply_stored(bcm_field_t out, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b) {
+  bcm_mesh* m = x->mesh;
+  EventPair ev;
+  BCM_TRY(timer_begin(m, MODE == 0 ? BCM_K_SPMV : BCM_K_RESID, &ev));
+  const int grid = cell_grid(m);
+  hipLaunchKernelGGL((k_apply<T, true, MODE>), dim3(grid), dim3(kWG), 0, m->stream,
+                     out ? static_cast<T*>(out->data) : nullptr, static_cast<const T*>(x->data), stored<T>(A),
+                     m->d_meta, b ? static_cast<const T*>(b->data) : nullptr, cell_geo(m), m->d_partials);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(timer_end(m, &ev));
+  if (MODE == 2) BCM_TRY(reduce_partials(m, grid, 0));
+  return BCM_OK;
+}
+
+extern "C" int bcm_calc_ax(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x) {
+  if (!same_mesh(ax, x) || !check_A(x, A) || ax == x) return BCM_ERR_ARG;
+  return x->dtype == BCM_F64 ? apply_stored<double, 0>(ax, A, x, nullptr) : apply_stored<float, 0>(ax, A, x, nullptr);
+}
+
+extern "C" int bcm_calc_r(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b) {
+  if (!same_mesh(r, x) || !same_mesh(x, b) || !check_A(x, A) || r == x) return BCM_ERR_ARG;
+  return x->dtype == BCM_F64 ? apply_stored<double, 1>(r, A, x, b) : apply_stored<float, 1>(r, A, x, b);
+}
+
+extern "C" int bcm_calc_r2(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, double* rr) {
+  if (!same_mesh(x, b) || !check_A(x, A) || !rr) return BCM_ERR_ARG;
+  const int rc = x->dtype == BCM_F64 ? apply_stored<double, 2>(nullptr, A, x, b) : apply_stored<float, 2>(nullptr, A, x, b);
+  if (rc != BCM_OK) return rc;
+  return fetch_result(x->mesh, 1, rr);
+}
+
+extern "C" int bcm_dot(bcm_field_t x, bcm_field_t y, double* xy) {
+  if (!same_mesh(x, y) || !xy) return BCM_ERR_ARG;
+  bcm_mesh* m = x->mesh;
+  if (x->dtype == BCM_F64) BCM_TRY(dot_raw<double>(m, static_cast<const double*>(x->data), static_cast<const double*>(y->data), 0));
+  else BCM_TRY(dot_raw<float>(m, static_cast<const float*>(x->data), static_cast<const float*>(y->data), 0));
+  return fetch_result(m, 1, xy);
+}
+
+static int blas_c(int op, bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b) {
+  if (!same_mesh(z, x) || (y && !same_mesh(z, y))) return BCM_ERR_ARG;
+  bcm_mesh* m = z->mesh;
+  if (z->dtype == BCM_F64)
+    return blas_raw<double>(m, op, static_cast<double*>(z->data), static_cast<const double*>(x->data),
+                            y ? static_cast<const double*>(y->data) : nullptr, a, b);
+  return blas_raw<float>(m, op, static_cast<float*>(z->data), static_cast<const float*>(x->data),
+                         y ? static_cast<const float*>(y->data) : nullptr, a, b);
+}
+
+extern "C" int bcm_axpy(bcm_field_t y, bcm_field_t x, double a) { return blas_c(OP_AXPY, y, x, nullptr, a, 0.0); }
+extern "C" int bcm_xpay(bcm_field_t y, bcm_field_t x, double a) { return blas_c(OP_XPAY, y, x, nullptr, a, 0.0); }
+extern "C" int bcm_axpyz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a) {
+  if (!y) return BCM_ERR_ARG;
+  return blas_c(OP_AXPYZ, z, x, y, a, 0.0);
+}
+extern "C" int bcm_axpbypz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b) {
+  if (!y) return BCM_ERR_ARG;
+  return blas_c(OP_AXPBYPZ, z, x, y, a, b);
+}
+
+extern "C" int bcm_fill(bcm_field_t x, double a) {
+  if (!x) return BCM_ERR_ARG;
+  if (x->dtype == BCM_F64) return fill_raw<double>(x->mesh, static_cast<double*>(x->data), a);
+  return fill_raw<float>(x->mesh, static_cast<float*>(x->data), a);
+}
+
+extern "C" int bcm_copy(bcm_field_t y, bcm_field_t x) {
+  if (!same_mesh(y, x)) return BCM_ERR_ARG;
+  bcm_mesh* m = x->mesh;
+  const long long count = (long long)m->nblocks * (long long)m->bvol;
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_BLAS1, &ev));
+  const unsigned grid = unsigned((count + 255) / 256);
+  if (x->dtype == BCM_F64)
+    hipLaunchKernelGGL(k_copy_all<double>, dim3(grid), dim3(256), 0, m->stream, static_cast<double*>(y->data),
+                       static_cast<const double*>(x->data), count);
+  else
+    hipLaunchKernelGGL(k_copy_all<float>, dim3(grid), dim3(256), 0, m->stream, static_cast<float*>(y->data),
+                       static_cast<const float*>(x->data), count);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+extern "C" int bcm_poisson_assemble(const bcm_field_t A[7], bcm_field_t b, bcm_field_t rhs, const double* bvalues,
+                                    int nbvalues) {
+  if (!same_mesh(b, rhs) || b->dtype != BCM_F64) return BCM_ERR_ARG;
+  if (A && !check_A(b, A)) return BCM_ERR_ARG;
+  bcm_mesh* m = b->mesh;
+  BCM_TRY(set_bvalues(m, bvalues, nbvalues));
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_ASSEMBLE, &ev));
+  auto d = [&](int i) { return A ? static_cast<double*>(A[i]->data) : nullptr; };
+  hipLaunchKernelGGL(k_assemble, dim3(cell_grid(m)), dim3(kWG), 0, m->stream, d(0), d(1), d(2), d(3), d(4), d(5),
+                     d(6), static_cast<double*>(b->data), static_cast<const double*>(rhs->data), m->d_meta,
+                     m->d_bvalues, cell_geo(m));
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+extern "C" int bcm_jacobi7_poisson(bcm_field_t x1, bcm_field_t x0, bcm_field_t b, double omega) {
+  if (!same_mesh(x1, x0) || !same_mesh(x1, b) || x1 == x0) return BCM_ERR_ARG;
+  if (x1->dtype == BCM_F64)
+    return jacobi_poisson_raw<double>(x1->mesh, static_cast<double*>(x1->data), static_cast<const double*>(x0->data),
+                                      static_cast<const double*>(b->data), omega);
+  return jacobi_poisson_raw<float>(x1->mesh, static_cast<float*>(x1->data), static_cast<const float*>(x0->data),
+                                   static_cast<const float*>(b->data), omega);
+}
+
+extern "C" int bcm_calc_ax_poisson(bcm_field_t ax, bcm_field_t x) {
+  if (!same_mesh(ax, x) || ax == x) return BCM_ERR_ARG;
+  if (x->dtype == BCM_F64)
+    return spmv_poisson_raw<double>(x->mesh, static_cast<double*>(ax->data), static_cast<const double*>(x->data));
+  return spmv_poisson_raw<float>(x->mesh, static_cast<float*>(ax->data), static_cast<const float*>(x->data));
+}
+
+extern "C" int bcm_calc_r_poisson(bcm_field_t r, bcm_field_t x, bcm_field_t b) {
+  if (!same_mesh(r, x) || !same_mesh(x, b) || r == x) return BCM_ERR_ARG;
+  if (x->dtype == BCM_F64)
+    return residual_poisson_raw<double>(x->mesh, static_cast<double*>(r->data), static_cast<const double*>(x->data),
+                                        static_cast<const double*>(b->data), false, 0);
+  return residual_poisson_raw<float>(x->mesh, static_cast<float*>(r->data), static_cast<const float*>(x->data),
+                                     static_cast<const float*>(b->data), false, 0);
+}

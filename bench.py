@@ -1,0 +1,197 @@
+"""Benchmark: red-black SOR sweeps with ghost fill on the adaptive FP64 mesh.
+
+Workload (default): the adaptive shell octree refined to level 5 --
+3,816 leaf blocks of 32^3 cells (125 M interior cells), one ghost layer,
+RootGrid(2,2,2) of length 0.5, 2:1 balanced, Dirichlet u = 0 on the
+outer faces -- solving lap(u) = s with s = -(sin(pi x) sin(pi y) sin(pi z)
++ 0.01 N(0,1)) by the sor2 red-black SOR at omega = 1.6.
+
+One step = one outer iteration of examples/sor2/Solver.cpp:77-133:
+red-black SOR on every block, Dirichlet mirror, ghost fill (same-level
+copy + F2C + C2F), Dirichlet mirror.  Inputs are resident in HBM before
+the timed region.  value = interior cell updates per second (GCUPS).
+
+roofline: the dominant kernel is the fused red-black sweep; its
+algorithmic traffic is 24 B per interior cell (read u and s, write u,
+FP64), timed with HIP events on the mesh stream inside the timed region.
+
+cpu_baseline: the oracle (C, OpenMP, -O3 -ffp-contract=off) runs the same
+step on the same mesh for a bounded number of steps on the host cores.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sor|jacobi]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for ln in fp:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.json), if recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fp:
+            return json.load(fp).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def bench_sor(args):
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    omega = 1.6
+    t = W.config2_tree()
+    blocks, n, vc = t.blocks, 32, 1
+    ncell = len(blocks) * n ** 3
+    print(f"[bench] sor: {len(blocks)} blocks x {n}^3 = {ncell} cells", file=sys.stderr, flush=True)
+    f = W.rhs_field(blocks, n, vc, seed=0)
+    s = -f
+    del f
+    mesh = Mesh(blocks, n, vc)
+    gf = mesh.field(np.float64)
+    gs = mesh.field(np.float64, s)
+    vals = [0.0]
+    mesh.dirichlet_mirror(gf, vals)
+    mesh.sor_iterate(gf, gs, omega, vals, args.warmup)
+    mesh.sync()
+    mesh.reset_timing()
+    mesh.set_timing(True)
+    mesh.sync()
+    t0 = time.perf_counter()
+    mesh.sor_iterate(gf, gs, omega, vals, args.steps)
+    mesh.sync()
+    t1 = time.perf_counter()
+    mesh.set_timing(False)
+    elapsed = t1 - t0
+    sor_ms, sor_n = mesh.kernel_time(L.K_SOR)
+    gh_ms, gh_n = mesh.kernel_time(L.K_GHOST)
+    bc_ms, bc_n = mesh.kernel_time(L.K_DIRICHLET)
+    sor_avg = sor_ms / max(sor_n, 1) * 1e-3
+    algo_bytes = 24.0 * ncell
+    achieved = algo_bytes / sor_avg / 1e9
+    gcups = ncell * args.steps / elapsed / 1e9
+    print(f"[bench] sor kernel avg {sor_avg * 1e3:.3f} ms ({achieved:.0f} GB/s), ghost avg "
+          f"{gh_ms / max(gh_n, 1):.3f} ms, dirichlet avg {bc_ms / max(bc_n, 1):.3f} ms", file=sys.stderr, flush=True)
+    result = {
+        "metric": "stencil GCUPS (FP64 red-black SOR + ghost fill, adaptive octree)",
+        "value": gcups,
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "adaptive shell octree level 5, 3816 x 32^3 blocks, vc=1, sor2 RB-SOR omega=1.6 "
+                               "+ Dirichlet + ghost fill per step", "blocks": len(blocks), "block_cells": n,
+                   "interior_cells": ncell, "omega": omega},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("rbsor_fused_n32"),
+                     "kernel": "k_rbsor_fused<double,32,true>", "avg_ms": sor_avg * 1e3,
+                     "algorithmic_bytes_per_launch": algo_bytes},
+        "kernels_ms_per_step": {"rbsor": sor_ms / max(args.steps, 1), "ghost_fill": gh_ms / max(args.steps, 1),
+                                "dirichlet": bc_ms / max(args.steps, 1)},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_sor(blocks, n, vc, s, omega, vals, args.cpu_steps)
+    return result
+
+
+def cpu_sor(blocks, n, vc, s, omega, vals, steps):
+    import oracle
+
+    ncell = len(blocks) * n ** 3
+    f = np.zeros_like(s)
+    oracle.dirichlet(blocks, n, vc, f, vals)
+    oracle.sor_iterate(blocks, n, vc, f, s, omega, vals, 1)  # warm caches / threads
+    t0 = time.perf_counter()
+    oracle.sor_iterate(blocks, n, vc, f, s, omega, vals, steps)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": ncell * steps / dt / 1e9, "unit": "GCUPS", "cores": cores, "kind": "port",
+            "sample": f"same mesh and step, {steps} steps ({dt:.1f} s) on {cores} threads of {cpu_model()}"}
+
+
+def bench_jacobi(args):
+    """Config 1: 64 x 16^3, 100-sweep Jacobi with ghost fill (L2-resident)."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    t = W.config1_tree()
+    blocks, n, vc = t.blocks, 16, 1
+    ncell = len(blocks) * n ** 3
+    f = W.rhs_field(blocks, n, vc, seed=1)
+    mesh = Mesh(blocks, n, vc)
+    rhs = mesh.field(np.float64, -f)
+    gb = mesh.field(np.float64)
+    mesh.poisson_assemble(None, gb, rhs, [0.0])
+    gx, g0 = mesh.field(np.float64), mesh.field(np.float64)
+    for _ in range(args.warmup):
+        mesh.jacobi7_poisson(g0, gx, gb, 1.0)
+        gx, g0 = g0, gx
+        mesh.update_vc(gx)
+    mesh.sync()
+    mesh.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mesh.jacobi7_poisson(g0, gx, gb, 1.0)
+        gx, g0 = g0, gx
+        mesh.update_vc(gx)
+    mesh.sync()
+    el = time.perf_counter() - t0
+    jms, jn = mesh.kernel_time(L.K_JACOBI)
+    javg = jms / max(jn, 1) * 1e-3
+    achieved = 24.0 * ncell / javg / 1e9
+    return {"metric": "stencil GCUPS (FP64 Jacobi7 + ghost fill, config 1)", "value": ncell * args.steps / el / 1e9,
+            "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "uniform level-2 octree, 64 x 16^3, matrix-free Jacobi7 + ghost fill"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi"])
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    res = bench_sor(args) if args.workload == "sor" else bench_jacobi(args)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,283 @@
+/*
+ * bcm_hip.h -- C ABI of the MI355X-native BCM block data path.
+ *
+ * One GPU holds every leaf block of a mesh in HBM.  Fields are stored
+ * block-major, each block as a ghosted (n+2vc)^3 array in the reference's
+ * Index3DS order (x fastest), i.e. the contiguous layout of
+ * include/BlockManager.h:253-283 with block b at offset b*(n+2vc)^3.
+ *
+ * Every entry point returns an int status (BCM_OK == 0) unless stated.
+ * No torch types appear here: plain pointers, sizes and opaque handles.
+ * All device work of a mesh is issued on the mesh's own HIP stream (the
+ * default priority); bcm_mesh_sync() waits for it.
+ *
+ * Citations are relative to the BCMTools 1.0.5 source tree.
+ */
+#ifndef BCM_HIP_H
+#define BCM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Face / subface numbering of include/BCMTools.h:48,51. */
+enum { BCM_X_M = 0, BCM_X_P, BCM_Y_M, BCM_Y_P, BCM_Z_M, BCM_Z_P, BCM_NUM_FACE };
+enum { BCM_SF_00 = 0, BCM_SF_01, BCM_SF_10, BCM_SF_11, BCM_NUM_SUBFACE };
+
+/* Boundary types of Utils/include/BoundaryInfo.h:30-35. */
+enum { BCM_BC_INNER = 0, BCM_BC_DIRICHLET = 1, BCM_BC_NEUMANN = 2, BCM_BC_PERIODIC = 3 };
+
+/* Status codes.  The reference calls Exit() (include/BCMTools.h:36-37);
+ * this library returns instead. */
+enum {
+  BCM_OK = 0,
+  BCM_ERR_ARG = 1,      /* bad argument, size mismatch, unsupported shape */
+  BCM_ERR_HIP = 2,      /* a HIP runtime call failed */
+  BCM_ERR_NOMEM = 3,    /* device or host allocation failed */
+  BCM_ERR_STATE = 4,    /* handle used in the wrong state */
+  BCM_ERR_BALANCE = 5,  /* 2:1 constraint broken (Utils/src/BCMOctree.cpp:462-467) */
+  BCM_ERR_NAN = 6,      /* NaN/Inf residual (examples/SolverTest/ILS3D.h:53-59) */
+  BCM_ERR_NOCONV = 7,   /* iteration limit reached before the tolerance */
+  BCM_ERR_BUSY = 8      /* admission refused: HBM budget or slot table full */
+};
+
+enum { BCM_F64 = 0, BCM_F32 = 1 };
+
+/* ------------------------------------------------------------------ */
+/* Block table.  One record per leaf block, in block-ID order.         */
+/* Mirrors BlockBase (include/BlockBase.h:68-128), NeighborInfo        */
+/* (include/NeighborInfo.h:36-228) and BoundaryInfo                    */
+/* (Utils/include/BoundaryInfo.h).  nb_id[f][0] is the neighbour for   */
+/* level difference 0 / -1; nb_id[f][0..3] the four finer neighbours   */
+/* for level difference +1; -1 means no neighbour.                     */
+/* ------------------------------------------------------------------ */
+typedef struct bcm_block_info {
+  uint64_t pedigree;          /* x16|y16|z16|root12|level4, Utils/include/Pedigree.h:42-73 */
+  int32_t level;
+  int32_t root_id;
+  double origin[3];
+  double block_size[3];
+  int32_t nb_level_diff[6];   /* neighbour level minus my level: -1, 0, +1 */
+  int32_t nb_id[6][4];
+  int32_t nb_subface[6];      /* my position on the coarser neighbour's face (diff -1) */
+  int32_t bc_type[6];         /* BCM_BC_* */
+  int32_t bc_id[6];           /* index into the boundary-value table, -1 if none */
+} bcm_block_info;
+
+/* ------------------------------------------------------------------ */
+/* Host octree builder: BCMOctree + dividers + BlockFactory            */
+/* (Utils/src/BCMOctree.cpp:23-478, Utils/include/SimpleDivider.h,     */
+/* Utils/include/BlockFactory.h:89-96) plus an explicit 2:1 balancer,  */
+/* which the reference lacks.                                           */
+/* ------------------------------------------------------------------ */
+enum { BCM_DIV_FLAT = 0, BCM_DIV_SIMPLE = 1, BCM_DIV_SHELL = 2 };
+enum { BCM_ORDER_Z = 0, BCM_ORDER_HILBERT = 1 };
+/* Boundary setters: all outer faces Dirichlet id 0
+ * (examples/sor2/BoundaryConditionSetter.h), or the sor1 'x'/'y'/'z'
+ * setter (examples/sor1/BoundaryConditionSetter.h): outer faces periodic
+ * except the two faces normal to the axis, Dirichlet ids 0 and 1. */
+enum { BCM_BCSET_ALL_DIRICHLET = 0, BCM_BCSET_SOR1_X = 1, BCM_BCSET_SOR1_Y = 2, BCM_BCSET_SOR1_Z = 3 };
+
+#define BCM_MAX_SHELLS 8
+typedef struct bcm_tree_params {
+  int32_t root_n[3];          /* RootGrid(nx, ny, nz) */
+  int32_t periodic[3];        /* RootGrid::setPeriodic{X,Y,Z} */
+  double root_origin[3];      /* BlockFactory rootOrigin */
+  double root_length;         /* BlockFactory rootLength */
+  int32_t divider;            /* BCM_DIV_* */
+  int32_t min_level;
+  int32_t max_level;
+  int32_t n_shells;           /* BCM_DIV_SHELL: refine a node whose box meets any shell */
+  double shells[BCM_MAX_SHELLS][5]; /* cx, cy, cz, r_in, r_out in physical coordinates */
+  int32_t ordering;           /* BCM_ORDER_* */
+  int32_t balance;            /* 1: split leaves until face neighbours differ by <= 1 level */
+  int32_t bc_setter;          /* BCM_BCSET_* */
+  /* BCM_DIV_SHELL only. shell_mode 0: split while the node's box meets a
+   * shell. shell_mode 1: as 0 above the last split; the last split (to
+   * max_level) needs the node centre within shell_margin node edges of a
+   * shell, which refines a band about one node thick instead of every box
+   * the shell touches. */
+  int32_t shell_mode;
+  double shell_margin;
+} bcm_tree_params;
+
+typedef struct bcm_tree* bcm_tree_t;
+
+/* Replaces `new BCMOctree(rootGrid, divider, ordering)` + the
+ * BlockFactory::makeBlock loop of examples/sor1/main.cpp:88-123. */
+int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out);
+/* BCMOctree::getNumLeafNode (Utils/include/BCMOctree.h:111). Returns the count (>=0) or -BCM_ERR_ARG. */
+int bcm_tree_num_blocks(bcm_tree_t tree);
+/* Copies the block table (one record per leaf in block-ID order). */
+int bcm_tree_get_blocks(bcm_tree_t tree, bcm_block_info* out, int capacity);
+int bcm_tree_destroy(bcm_tree_t tree);
+
+/* ------------------------------------------------------------------ */
+/* Device mesh and fields                                               */
+/* ------------------------------------------------------------------ */
+typedef struct bcm_mesh* bcm_mesh_t;
+typedef struct bcm_field* bcm_field_t;
+
+/* BlockManager::registerBlock/endRegisterBlock + prepareForVCUpdate
+ * (src/BlockManager.cpp:28-114, 471-648): uploads the block table and
+ * the same-level / F2C / C2F / Dirichlet face-task lists. n = cells per
+ * block edge (even, as include/BlockBase.h:75-78 demands), vc = ghost
+ * width (n/2 >= vc, include/DataClass.h:53-56). */
+int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc, bcm_mesh_t* out);
+int bcm_mesh_destroy(bcm_mesh_t mesh);
+/* counts[0..3] = same-level, F2C, C2F and Dirichlet/Neumann face tasks. */
+int bcm_mesh_task_counts(bcm_mesh_t mesh, int counts[4]);
+int bcm_mesh_sync(bcm_mesh_t mesh);
+/* Per-kernel HIP-event timing on the mesh stream (off by default). */
+enum {
+  BCM_K_GHOST = 0, BCM_K_DIRICHLET, BCM_K_SOR, BCM_K_JACOBI, BCM_K_SPMV, BCM_K_RESID,
+  BCM_K_DOT, BCM_K_BLAS1, BCM_K_REDUCE, BCM_K_ASSEMBLE, BCM_K_COUNT
+};
+int bcm_mesh_set_timing(bcm_mesh_t mesh, int enable);
+/* Sum of event-measured durations (ms) and launch count since the last reset. */
+int bcm_mesh_kernel_time(bcm_mesh_t mesh, int kernel, double* total_ms, long long* launches);
+int bcm_mesh_reset_timing(bcm_mesh_t mesh);
+
+/* Scalar3D<T>(size, vc) allocation on every block
+ * (BlockManager::setDataClass<Scalar3D<T>,Scalar3DUpdater<T>,T>,
+ * include/BlockManager.h:253-283); zero-filled, ghosts included. */
+int bcm_field_create(bcm_mesh_t mesh, int dtype, bcm_field_t* out);
+int bcm_field_destroy(bcm_field_t field);
+/* Host <-> device copies of the whole field (nblocks*(n+2vc)^3 values). */
+int bcm_field_upload(bcm_field_t field, const void* host, size_t bytes);
+int bcm_field_download(bcm_field_t field, void* host, size_t bytes);
+void* bcm_field_device_ptr(bcm_field_t field);
+size_t bcm_field_bytes(bcm_field_t field);
+
+/* ------------------------------------------------------------------ */
+/* Ghost ("virtual cell") fill                                           */
+/* ------------------------------------------------------------------ */
+/* kind bits: same-level copy, F2C restriction, C2F prolongation.       */
+#define BCM_GF_FLAT 1
+#define BCM_GF_F2C 2
+#define BCM_GF_C2F 4
+#define BCM_GF_ALL 7
+/* axis bits (none set = all three axes) */
+#define BCM_GF_X 8
+#define BCM_GF_Y 16
+#define BCM_GF_Z 32
+/* BlockManager::updateVC / updateVC_{X,Y,Z} / updateVC_{X,Y,Z}_{Flat,F2C,C2F}
+ * (include/BlockManager.h:311-440) -> VCUpdater::copyFromNeighbor
+ * (include/VCUpdater.h:65-82) -> Scalar3DUpdater<T>::copyFromNeighbor,
+ * copyFromNeighborF2C_0, copyFromNeighborC2F_0
+ * (include/Scalar3DUpdater.h:321-347, 519-708). Face ghosts only. */
+int bcm_ghost_fill(bcm_field_t field, int mask);
+
+/* Dirichlet mirror g = 2v - u0 on every outer face whose bc_type is
+ * DIRICHLET, v = values[bc_id]: Solver::setBoundaryCondition /
+ * setDirichletBoundaryInBlock (examples/sor1/Solver.cpp:202-294). */
+int bcm_dirichlet_mirror(bcm_field_t field, const double* values, int nvalues);
+
+/* ------------------------------------------------------------------ */
+/* Red-black SOR (examples/sor1/Solver.cpp:132-153,                     */
+/* examples/sor2/Solver.cpp:136-160).  Both colours, ghosts frozen.      */
+/* source == NULL: sor1 form  f = c0 f + cx(fW+fE) + cy(fS+fN) + cz(fB+fT)  */
+/* source != NULL: sor2 form  fNew = c(cx(..)+cy(..)+cz(..) - s);          */
+/*                            f = w fNew + (1-w) f                         */
+/* ------------------------------------------------------------------ */
+int bcm_rbsor_sweep(bcm_field_t f, bcm_field_t source, double omega);
+/* One sor1/sor2 outer iteration, `iters` times: SOR on every block, the
+ * in-block Dirichlet BC, ghost fill (AtOnce), Dirichlet BC
+ * (examples/sor2/Solver.cpp:77-133 with nLoopInner == 1). */
+int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega,
+                    const double* bvalues, int nbvalues, int iters);
+/* Relative residual of the sor2 system lap_h(f) = s:
+ * ||s - lap_h f||_2 / ||s||_2 over interior cells (FP64 reduction). */
+int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relres);
+
+/* ------------------------------------------------------------------ */
+/* SolverTest per-block kernels (examples/SolverTest/blas.h:23-126),   */
+/* applied to every block.  A[7] = {Ap, Aw, Ae, As, An, Ab, At}.        */
+/* ------------------------------------------------------------------ */
+/* jacobi_smoother_ (blas.f90:452-502): x1 = x0 - w (x0 - r/Ap). */
+int bcm_jacobi7(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, double omega);
+/* calc_ax_ (blas.f90:556-600) */
+int bcm_calc_ax(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x);
+/* calc_r_ (blas.f90:602-648) */
+int bcm_calc_r(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b);
+/* calc_r2_ + CalcR2 (blas.f90:650-698, ILS3D.h:168-217): sum of r^2, FP64 out. */
+int bcm_calc_r2(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, double* rr);
+/* dot_ + ILS3D::DOT (blas.f90:379-412, ILS3D.h:272-311). Deterministic tree. */
+int bcm_dot(bcm_field_t x, bcm_field_t y, double* xy);
+/* axpy y+=a x; xpay y=x+a y; axpyz z=a x+y; axpbypz z=a x+b y+z (interior);
+ * fill/copy include ghosts (blas.f90:18-377). */
+int bcm_axpy(bcm_field_t y, bcm_field_t x, double a);
+int bcm_xpay(bcm_field_t y, bcm_field_t x, double a);
+int bcm_axpyz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a);
+int bcm_axpbypz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b);
+int bcm_fill(bcm_field_t x, double a);
+int bcm_copy(bcm_field_t y, bcm_field_t x);
+
+/* Poisson operator A = lap_h (reference sign, bcut.f90:2848-2865 with no
+ * cut cells: Aw..At = 1/h^2, Ap = -(l0+l1)-(l2+l3)-(l4+l5)) with the
+ * Dirichlet fold of bca.f90:12-202 on outer faces (Ap -= Anb;
+ * b -= 2 Anb v; Anb = 0).  Writes A[7] and b = rhs folded. */
+int bcm_poisson_assemble(const bcm_field_t A[7], bcm_field_t b, bcm_field_t rhs,
+                         const double* bvalues, int nbvalues);
+
+/* Matrix-free forms of jacobi_smoother_, calc_ax_ and calc_r_ for the
+ * Poisson operator above: the coefficients are recomputed per cell in the
+ * same operand order as the assembled arrays, so results are bit-identical
+ * to the stored-coefficient kernels on bcm_poisson_assemble's output while
+ * reading 3 arrays instead of 10 (b must be the folded right-hand side). */
+int bcm_jacobi7_poisson(bcm_field_t x1, bcm_field_t x0, bcm_field_t b, double omega);
+int bcm_calc_ax_poisson(bcm_field_t ax, bcm_field_t x);
+int bcm_calc_r_poisson(bcm_field_t r, bcm_field_t x, bcm_field_t b);
+
+/* ------------------------------------------------------------------ */
+/* Solvers                                                               */
+/* ------------------------------------------------------------------ */
+typedef struct bcm_krylov_params {
+  double omega;               /* Jacobi weight (ILS3D Jacobi_Smoother) */
+  int32_t count_pre;          /* Jacobi preconditioner sweeps */
+  int32_t count_max;          /* iteration limit */
+  double epsilon;             /* relative-residual tolerance ||r||/||b|| */
+} bcm_krylov_params;
+
+/* ILS3D::Jacobi (ILS3D.h:478-529) on device: sweeps with ghost fill +
+ * Dirichlet mirror between sweeps, FP64 residual each sweep.  x0 is work. */
+int bcm_jacobi_solve(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_field_t x0,
+                     const bcm_krylov_params* p, const double* bvalues, int nbvalues,
+                     int* count, double* residual);
+/* ILS3D::BiCGSTAB (ILS3D.h:663-844) on device, Jacobi-preconditioned,
+ * in the precision of x (all fields must share it).  work[9] =
+ * {r, r0, p, p_, q_, s, s_, t_, x0}. */
+int bcm_bicgstab(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
+                 const bcm_krylov_params* p, const double* bvalues, int nbvalues,
+                 int* count, double* residual);
+
+/* Mixed precision: FP64 outer iterative refinement around an FP32
+ * Jacobi-preconditioned BiCGSTAB.  Only the FP64 residual r = b - A u,
+ * the FP64 update u += d and the FP64 norms run in FP64.  x, b, rhs in
+ * FP64; A is the Poisson operator assembled internally in both
+ * precisions.  Reports outer/inner iteration counts and the final FP64
+ * relative residual ||b - A x|| / ||b||. */
+typedef struct bcm_ir_params {
+  double tol;                 /* FP64 relative residual target (e.g. 1e-10) */
+  double inner_tol;           /* FP32 inner relative tolerance (e.g. 1e-4) */
+  int32_t inner_max;          /* inner BiCGSTAB iteration limit */
+  int32_t outer_max;          /* refinement step limit */
+  int32_t count_pre;          /* Jacobi preconditioner sweeps */
+  double omega;               /* Jacobi weight */
+  int32_t fp32_only;          /* 1: no refinement, plain FP32 solve to tol */
+} bcm_ir_params;
+int bcm_solve_mixed(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
+                    const bcm_ir_params* p, int* outer, int* inner_total, double* relres);
+/* Plain FP64 BiCGSTAB on the assembled Poisson operator (the per-mesh
+ * FP64 GPU path the mixed solver is compared against). */
+int bcm_solve_fp64(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
+                   const bcm_ir_params* p, int* iters, double* relres);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCM_HIP_H */

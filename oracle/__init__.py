@@ -1,0 +1,165 @@
+"""ctypes binding of the CPU oracle (oracle/bcm_oracle.c -> liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product path
+(bcmtools_amd, libbcm_hip.so) never imports it.
+
+Fields are numpy arrays of shape (nblocks, n+2vc, n+2vc, n+2vc) in the
+Index3DS order of include/Index3D.h:35-59 (x fastest, i.e. C order k, j, i).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "oracle_ghost_fill_d": [_P, _I, _I, _I, _P, _I],
+        "oracle_ghost_fill_f": [_P, _I, _I, _I, _P, _I],
+        "oracle_dirichlet_d": [_P, _I, _I, _I, _P, _P],
+        "oracle_dirichlet_f": [_P, _I, _I, _I, _P, _P],
+        "oracle_sor_sweep_d": [_P, _I, _I, _I, _P, _P, _D],
+        "oracle_sor_iterate_d": [_P, _I, _I, _I, _P, _P, _D, _P, _I],
+        "oracle_jacobi7_d": [_I, _I, _I, _P, _P, _P, _P, _D],
+        "oracle_jacobi7_f": [_I, _I, _I, _P, _P, _P, _P, _D],
+        "oracle_calc_ax_d": [_I, _I, _I, _P, _P, _P],
+        "oracle_calc_ax_f": [_I, _I, _I, _P, _P, _P],
+        "oracle_calc_r_d": [_I, _I, _I, _P, _P, _P, _P],
+        "oracle_calc_r_f": [_I, _I, _I, _P, _P, _P, _P],
+        "oracle_blas_d": [_I, _I, _I, _I, _P, _P, _P, _D, _D],
+        "oracle_blas_f": [_I, _I, _I, _I, _P, _P, _P, _D, _D],
+        "oracle_poisson_assemble_d": [_P, _I, _I, _I, _P, _P, _P, _P],
+        "oracle_bicgstab_d": [_P, _I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _D, _P, _P],
+        "oracle_jacobi_solve_d": [_P, _I, _I, _I, _P, _P, _P, _P, _D, _I, _D, _P, _P],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = None
+    for name in ("oracle_calc_r2_d", "oracle_calc_r2_f", "oracle_dot_d", "oracle_dot_f"):
+        fn = getattr(lib, name)
+        fn.argtypes = [_I, _I, _I, _P, _P, _P] if "r2" in name else [_I, _I, _I, _P, _P]
+        fn.restype = _D
+    lib.oracle_bicgstab_d.restype = _I
+    lib.oracle_jacobi_solve_d.restype = _I
+    _lib = lib
+    return lib
+
+
+def _suf(a):
+    return "d" if a.dtype == np.float64 else "f"
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def _arr7(A):
+    arr = (ctypes.c_void_p * 7)()
+    for i, a in enumerate(A):
+        arr[i] = a.ctypes.data
+    return arr
+
+
+def ghost_fill(blocks, n, vc, data, mask=7):
+    """In-place ghost fill of `data` (float64 or float32)."""
+    lib = load()
+    getattr(lib, "oracle_ghost_fill_" + _suf(data))(blocks.ctypes.data, len(blocks), n, vc, data.ctypes.data, mask)
+
+
+def dirichlet(blocks, n, vc, data, values):
+    lib = load()
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    getattr(lib, "oracle_dirichlet_" + _suf(data))(blocks.ctypes.data, len(blocks), n, vc, data.ctypes.data,
+                                                   v.ctypes.data)
+
+
+def sor_sweep(blocks, n, vc, f, s, omega):
+    load().oracle_sor_sweep_d(blocks.ctypes.data, len(blocks), n, vc, f.ctypes.data, _ptr(s), float(omega))
+
+
+def sor_iterate(blocks, n, vc, f, s, omega, values, iters):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    load().oracle_sor_iterate_d(blocks.ctypes.data, len(blocks), n, vc, f.ctypes.data, _ptr(s), float(omega),
+                                v.ctypes.data, int(iters))
+
+
+def jacobi7(nb, n, vc, x1, x0, A, b, omega):
+    getattr(load(), "oracle_jacobi7_" + _suf(x1))(nb, n, vc, x1.ctypes.data, x0.ctypes.data, _arr7(A),
+                                                  b.ctypes.data, float(omega))
+
+
+def calc_ax(nb, n, vc, ax, A, x):
+    getattr(load(), "oracle_calc_ax_" + _suf(ax))(nb, n, vc, ax.ctypes.data, _arr7(A), x.ctypes.data)
+
+
+def calc_r(nb, n, vc, r, A, x, b):
+    getattr(load(), "oracle_calc_r_" + _suf(r))(nb, n, vc, r.ctypes.data, _arr7(A), x.ctypes.data, b.ctypes.data)
+
+
+def calc_r2(nb, n, vc, A, x, b):
+    return getattr(load(), "oracle_calc_r2_" + _suf(x))(nb, n, vc, _arr7(A), x.ctypes.data, b.ctypes.data)
+
+
+def dot(nb, n, vc, x, y):
+    return getattr(load(), "oracle_dot_" + _suf(x))(nb, n, vc, x.ctypes.data, y.ctypes.data)
+
+
+BLAS_OPS = {"axpy": 0, "xpay": 1, "axpyz": 2, "axpbypz": 3, "copy": 4}
+
+
+def blas(nb, n, vc, op, z, x, y=None, a=0.0, b=0.0):
+    getattr(load(), "oracle_blas_" + _suf(z))(nb, n, vc, BLAS_OPS[op], z.ctypes.data, x.ctypes.data, _ptr(y),
+                                              float(a), float(b))
+
+
+def poisson_assemble(blocks, n, vc, A, b, rhs, values):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    load().oracle_poisson_assemble_d(blocks.ctypes.data, len(blocks), n, vc, _arr7(A) if A is not None else None,
+                                     b.ctypes.data, rhs.ctypes.data, v.ctypes.data)
+
+
+def bicgstab(blocks, n, vc, x, A, b, work, omega, count_pre, count_max, eps):
+    w = (ctypes.c_void_p * 9)()
+    for i, a in enumerate(work):
+        w[i] = a.ctypes.data
+    cnt = ctypes.c_int()
+    res = ctypes.c_double()
+    st = load().oracle_bicgstab_d(blocks.ctypes.data, len(blocks), n, vc, x.ctypes.data, _arr7(A), b.ctypes.data, w,
+                                  float(omega), int(count_pre), int(count_max), float(eps), ctypes.byref(cnt),
+                                  ctypes.byref(res))
+    return st, cnt.value, res.value
+
+
+def jacobi_solve(blocks, n, vc, x, A, b, x0, omega, count_max, eps):
+    """Returns (status, count, residual, result array) -- the result may live in x or x0 after swaps."""
+    px = ctypes.c_void_p(x.ctypes.data)
+    p0 = ctypes.c_void_p(x0.ctypes.data)
+    cnt = ctypes.c_int()
+    res = ctypes.c_double()
+    st = load().oracle_jacobi_solve_d(blocks.ctypes.data, len(blocks), n, vc, ctypes.byref(px), _arr7(A),
+                                      b.ctypes.data, ctypes.byref(p0), float(omega), int(count_max), float(eps),
+                                      ctypes.byref(cnt), ctypes.byref(res))
+    out = x if px.value == x.ctypes.data else x0
+    return st, cnt.value, res.value, out

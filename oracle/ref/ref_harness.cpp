@@ -1,0 +1,286 @@
+// Fixture writer linked against the reference BCMTools sources (built by
+// oracle/ref/Makefile into oracle/_ref/; never shipped, never run on the
+// GPU box).  Builds meshes with the reference's own octree, block factory
+// and BlockManager, runs the reference ghost fill (BlockManager::updateVC*)
+// and the reference sor1 solver (examples/sor1/Solver.cpp), and dumps
+// block tables and fields in the bcm_block_info layout of include/bcm_hip.h.
+//
+// usage:
+//   ref_harness mesh  <out> <mesh args>
+//   ref_harness ghost <out> <method 0|1|2> <dtype d|f> <mesh args>
+//   ref_harness sor1  <out> <config file> <root_n> <ordering>
+// mesh args: divider(flat|simple|sphere) minLevel maxLevel ordering(Z|Hilbert)
+//            rnx rny rnz bc(dir|x|y|z) n vc [cx cy cz r]
+#include "mpi.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "BCMTools.h"
+#include "BCMOctree.h"
+#include "Block.h"
+#include "BlockFactory.h"
+#include "BlockManager.h"
+#include "BoundaryInfo.h"
+#include "Partition.h"
+#include "RootGrid.h"
+#include "Scalar3D.h"
+#include "Scalar3DUpdater.h"
+#include "SimpleDivider.h"
+#include "SphereDivider.h"
+#include "Config.h"
+#include "Solver.h"
+#include "BoundaryConditionSetter.h"  // examples/sor1 setter (type x/y/z)
+
+#include "bcm_hip.h"
+
+namespace {
+
+// Outer faces Dirichlet id 0 (the setter of examples/sor2).
+class AllDirichletSetter : public BoundaryConditionSetterBase {
+ public:
+  BoundaryInfo* operator()(const Node* node, const BCMOctree* tree) {
+    BoundaryInfo* bi = new BoundaryInfo[NUM_FACE];
+    for (int i = 0; i < NUM_FACE; i++) {
+      if (tree->checkOnOuterBoundary(node, Face(i))) {
+        bi[i].setType(BoundaryInfo::DIRICHLET);
+        bi[i].setID(0);
+      }
+    }
+    return bi;
+  }
+};
+
+// sor1-style setter without a Config object.
+class AxisSetter : public BoundaryConditionSetterBase {
+  int axis;
+ public:
+  explicit AxisSetter(int a) : axis(a) {}
+  BoundaryInfo* operator()(const Node* node, const BCMOctree* tree) {
+    BoundaryInfo* bi = new BoundaryInfo[NUM_FACE];
+    for (int i = 0; i < NUM_FACE; i++)
+      if (tree->checkOnOuterBoundary(node, Face(i))) bi[i].setType(BoundaryInfo::PERIODIC);
+    for (int s = 0; s < 2; ++s) {
+      Face f = Face(2 * axis + s);
+      if (tree->checkOnOuterBoundary(node, f)) {
+        bi[f].setType(BoundaryInfo::DIRICHLET);
+        bi[f].setID(s);
+      }
+    }
+    return bi;
+  }
+};
+
+// splitmix64 of (block, cell): deterministic values in [-1, 1).
+double cell_value(uint64_t b, uint64_t c) {
+  uint64_t z = (b << 32) ^ c ^ 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z = z ^ (z >> 31);
+  return double(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+struct MeshArgs {
+  std::string divider, ordering, bc;
+  int minLevel, maxLevel, rn[3], n, vc;
+  double sph[4];
+};
+
+int parse_mesh(char** a, int na, MeshArgs& m) {
+  if (na < 10) return 1;
+  m.divider = a[0];
+  m.minLevel = atoi(a[1]);
+  m.maxLevel = atoi(a[2]);
+  m.ordering = a[3];
+  m.rn[0] = atoi(a[4]);
+  m.rn[1] = atoi(a[5]);
+  m.rn[2] = atoi(a[6]);
+  m.bc = a[7];
+  m.n = atoi(a[8]);
+  m.vc = atoi(a[9]);
+  if (m.divider == "sphere") {
+    if (na < 14) return 1;
+    for (int i = 0; i < 4; ++i) m.sph[i] = atof(a[10 + i]);
+  }
+  return 0;
+}
+
+BCMOctree* g_tree = 0;
+
+// Build the mesh as examples/sor1/main.cpp:76-123 does, one rank.
+void build_mesh(const MeshArgs& a) {
+  RootGrid* rootGrid = new RootGrid(a.rn[0], a.rn[1], a.rn[2]);
+  int axis = -1;
+  if (a.bc == "x") axis = 0;
+  if (a.bc == "y") axis = 1;
+  if (a.bc == "z") axis = 2;
+  if (axis >= 0) {
+    if (axis != 0) rootGrid->setPeriodicX();
+    if (axis != 1) rootGrid->setPeriodicY();
+    if (axis != 2) rootGrid->setPeriodicZ();
+  }
+  Divider* divider;
+  if (a.divider == "flat") divider = new FlatDivider(rootGrid, a.maxLevel);
+  else if (a.divider == "simple") divider = new SimpleDivider(rootGrid, a.minLevel, a.maxLevel);
+  else divider = new SphereDivider(rootGrid, a.minLevel, a.maxLevel, a.sph[0], a.sph[1], a.sph[2], a.sph[3]);
+  BCMOctree::Ordering ord = a.ordering == "Hilbert" ? BCMOctree::HILBERT : BCMOctree::Z;
+  g_tree = new BCMOctree(rootGrid, divider, ord);
+  Partition* partition = new Partition(1, g_tree->getNumLeafNode());
+  Vec3i size(a.n, a.n, a.n);
+  BoundaryConditionSetterBase* setter;
+  if (axis >= 0) setter = new AxisSetter(axis);
+  else setter = new AllDirichletSetter();
+  BlockFactory factory(g_tree, partition, setter, size);
+  BlockManager& bm = BlockManager::getInstance();
+  std::vector<Node*>& leaves = g_tree->getLeafNodeArray();
+  for (int id = 0; id < g_tree->getNumLeafNode(); id++) bm.registerBlock(factory.makeBlock(leaves[id]));
+  bm.endRegisterBlock();
+}
+
+std::vector<bcm_block_info> block_table() {
+  BlockManager& bm = BlockManager::getInstance();
+  std::vector<bcm_block_info> out(bm.getNumBlock());
+  std::vector<Node*>& leaves = g_tree->getLeafNodeArray();
+  for (int b = 0; b < bm.getNumBlock(); ++b) {
+    Block* blk = dynamic_cast<Block*>(bm.getBlock(b));
+    bcm_block_info& bi = out[b];
+    std::memset(&bi, 0, sizeof(bi));
+    leaves[b]->getPedigree().serialize(&bi.pedigree);
+    bi.level = blk->getLevel();
+    bi.root_id = leaves[b]->getPedigree().getRootID();
+    for (int a = 0; a < 3; ++a) {
+      bi.origin[a] = blk->getOrigin()[a];
+      bi.block_size[a] = blk->getBlockSize()[a];
+    }
+    const NeighborInfo* ni = blk->getNeighborInfo();
+    const BoundaryInfo* bo = blk->getBoundaryInfo();
+    for (int f = 0; f < NUM_FACE; ++f) {
+      bi.nb_level_diff[f] = ni[f].getLevelDifference();
+      for (int s = 0; s < NUM_SUBFACE; ++s) bi.nb_id[f][s] = ni[f].getID(Subface(s));
+      bi.nb_subface[f] = ni[f].getNeighborSubface();
+      bi.bc_type[f] = int(bo[f].getType());
+      bi.bc_id[f] = bo[f].getID();
+    }
+  }
+  return out;
+}
+
+void write_header(FILE* fp, const std::vector<bcm_block_info>& t, int n, int vc) {
+  int32_t h[4] = {int32_t(t.size()), n, vc, int32_t(sizeof(bcm_block_info))};
+  fwrite(h, sizeof(h), 1, fp);
+  fwrite(t.data(), sizeof(bcm_block_info), t.size(), fp);
+}
+
+template <typename T>
+int run_ghost(const char* out, int method, const MeshArgs& a) {
+  build_mesh(a);
+  BlockManager& bm = BlockManager::getInstance();
+  int id = bm.setDataClass<Scalar3D<T>, Scalar3DUpdater<T> >(a.vc);
+  VCUpdateMethod::Type mt = method == 0 ? VCUpdateMethod::AtOnce
+                          : method == 1 ? VCUpdateMethod::SeparateXYZ
+                                        : VCUpdateMethod::SeparateXYZ_SeparateLevelDiff;
+  bm.prepareForVCUpdate(id, 100, mt);
+  const int nx0 = a.n + 2 * a.vc;
+  const size_t bvol = size_t(nx0) * nx0 * nx0;
+  std::vector<T> in(bvol * bm.getNumBlock()), res(in.size());
+  for (int b = 0; b < bm.getNumBlock(); ++b) {
+    Scalar3D<T>* s = dynamic_cast<Scalar3D<T>*>(bm.getBlock(b)->getDataClass(id));
+    T* d = s->getData();
+    for (size_t c = 0; c < bvol; ++c) d[c] = T(cell_value(b, c));
+    std::memcpy(&in[b * bvol], d, bvol * sizeof(T));
+  }
+  if (method == 0) {
+    bm.updateVC(id);
+  } else if (method == 1) {
+    bm.updateVC_X(id);
+    bm.updateVC_Y(id);
+    bm.updateVC_Z(id);
+  } else {
+    // Order of examples/SolverTest/LocalScalar3D.h:214-224.
+    bm.updateVC_X_F2C(id); bm.updateVC_Y_F2C(id); bm.updateVC_Z_F2C(id);
+    bm.updateVC_X_Flat(id); bm.updateVC_Y_Flat(id); bm.updateVC_Z_Flat(id);
+    bm.updateVC_X_C2F(id); bm.updateVC_Y_C2F(id); bm.updateVC_Z_C2F(id);
+  }
+  for (int b = 0; b < bm.getNumBlock(); ++b) {
+    Scalar3D<T>* s = dynamic_cast<Scalar3D<T>*>(bm.getBlock(b)->getDataClass(id));
+    std::memcpy(&res[b * bvol], s->getData(), bvol * sizeof(T));
+  }
+  FILE* fp = fopen(out, "wb");
+  if (!fp) return 2;
+  write_header(fp, block_table(), a.n, a.vc);
+  fwrite(in.data(), sizeof(T), in.size(), fp);
+  fwrite(res.data(), sizeof(T), res.size(), fp);
+  fclose(fp);
+  return 0;
+}
+
+// The reference sor1 driver run (examples/sor1/main.cpp:124-150) on a mesh
+// built from its config file; root grid and ordering come from argv so the
+// block table is reproducible (the shipped config uses random ordering).
+int run_sor1(const char* out, const char* conf_file, int rn, const char* ordering) {
+  Config conf;
+  conf.load(conf_file);
+  MeshArgs a;
+  a.divider = conf.treeType;
+  a.minLevel = conf.minLevel;
+  a.maxLevel = conf.maxLevel;
+  a.ordering = ordering;
+  a.rn[0] = a.rn[1] = a.rn[2] = rn;
+  a.bc = std::string(1, conf.type);
+  a.n = conf.size;
+  a.vc = conf.vc;
+  build_mesh(a);
+  std::vector<double> boundaryValue(2);
+  boundaryValue[0] = conf.b0;
+  boundaryValue[1] = conf.b1;
+  Solver solver(conf, boundaryValue);
+  solver.initialize();
+  solver.run();
+  solver.checkResult(conf.type, false);
+  BlockManager& bm = BlockManager::getInstance();
+  // The solver's field is the first data class registered on every block.
+  const int nx0 = a.n + 2 * a.vc;
+  const size_t bvol = size_t(nx0) * nx0 * nx0;
+  std::vector<double> res(bvol * bm.getNumBlock());
+  for (int b = 0; b < bm.getNumBlock(); ++b) {
+    Scalar3D<double>* s = dynamic_cast<Scalar3D<double>*>(bm.getBlock(b)->getDataClass(0));
+    std::memcpy(&res[b * bvol], s->getData(), bvol * sizeof(double));
+  }
+  FILE* fp = fopen(out, "wb");
+  if (!fp) return 2;
+  write_header(fp, block_table(), a.n, a.vc);
+  fwrite(res.data(), sizeof(double), res.size(), fp);
+  fclose(fp);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  MPI::Init(argc, argv);
+  int rc = 1;
+  if (argc >= 3) {
+    std::string cmd = argv[1];
+    MeshArgs a;
+    if (cmd == "mesh" && parse_mesh(argv + 3, argc - 3, a) == 0) {
+      build_mesh(a);
+      FILE* fp = fopen(argv[2], "wb");
+      if (fp) {
+        write_header(fp, block_table(), a.n, a.vc);
+        fclose(fp);
+        rc = 0;
+      }
+    } else if (cmd == "ghost" && argc >= 5 && parse_mesh(argv + 5, argc - 5, a) == 0) {
+      rc = argv[4][0] == 'f' ? run_ghost<float>(argv[2], atoi(argv[3]), a) : run_ghost<double>(argv[2], atoi(argv[3]), a);
+    } else if (cmd == "sor1" && argc >= 6) {
+      rc = run_sor1(argv[2], argv[3], atoi(argv[4]), argv[5]);
+    }
+  }
+  if (rc == 1) fprintf(stderr, "usage: see oracle/ref/ref_harness.cpp\n");
+  MPI::Finalize();
+  return rc;
+}

@@ -1,0 +1,119 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+Runs oracle/_ref/ref_harness -- the reference BCMTools library sources
+compiled by oracle/ref/Makefile with our fixture-writer main -- and stores
+its block tables and fields as .npz.  Only runs where the reference tree
+exists (this container); the fixtures are committed and the tests read
+them everywhere.
+
+usage: python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from bcmtools_amd._lib import BLOCK_DTYPE  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF = "/root/reference"
+
+MESHES = {
+    # name: mesh args (divider minLevel maxLevel ordering rnx rny rnz bc n vc [sphere])
+    "mesh_flat2_z": "flat 0 2 Z 1 1 1 dir 16 1",
+    "mesh_simple13_z": "simple 1 3 Z 1 1 1 dir 4 1",
+    "mesh_simple03_hilbert_x": "simple 0 3 Hilbert 1 1 1 x 2 1",
+    "mesh_flat1_hilbert_roots_y": "flat 0 1 Hilbert 2 2 2 y 4 1",
+    "mesh_simple24_hilbert_z": "simple 2 4 Hilbert 1 2 1 z 4 1",
+}
+
+GHOSTS = {
+    # name: (method, dtype, mesh args)
+    "ghost_sphere_atonce_d": (0, "d", "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
+    "ghost_sphere_xyz_d": (1, "d", "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
+    "ghost_sphere_leveldiff_d": (2, "d", "sphere 1 3 Hilbert 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
+    "ghost_sphere_atonce_f": (0, "f", "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
+    "ghost_simple_vc2_d": (0, "d", "simple 1 2 Z 1 1 1 dir 8 2"),
+    "ghost_periodic_x_d": (0, "d", "simple 1 3 Z 1 1 1 x 4 1"),
+}
+
+SOR1 = {
+    # name: (config text, roots, ordering)
+    "sor1_log4": (None, 1, "Z"),
+    "sor1_adaptive_y": ("minLevel = 1\nmaxLevel = 3\ntreeType = simple\nordering = Z\ntype = y\n"
+                        "nLoopOuter = 25\nnLoopInner = 1\nomega = 1.5\nsize = 8\nvc = 1\nb0 = 0.25\nb1 = 2.0\n"
+                        "output = out.dat\n", 1, "Z"),
+}
+
+
+def read_dump(path, nfields, dtype):
+    with open(path, "rb") as fp:
+        hdr = np.frombuffer(fp.read(16), dtype=np.int32)
+        nb, n, vc, rec = (int(v) for v in hdr)
+        assert rec == BLOCK_DTYPE.itemsize, "block record size mismatch"
+        blocks = np.frombuffer(fp.read(rec * nb), dtype=BLOCK_DTYPE).copy()
+        nx0 = n + 2 * vc
+        fields = []
+        for _ in range(nfields):
+            cnt = nb * nx0 ** 3
+            fields.append(np.frombuffer(fp.read(cnt * np.dtype(dtype).itemsize), dtype=dtype)
+                          .reshape(nb, nx0, nx0, nx0).copy())
+    return blocks, n, vc, fields
+
+
+def run(args, cwd):
+    out = subprocess.run([HARNESS] + args, cwd=cwd, check=True, capture_output=True, text=True)
+    return out.stdout
+
+
+def main():
+    if not os.path.isdir(REF):
+        sys.exit("reference tree not present; fixtures are only regenerated where it is")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref")], check=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, margs in MESHES.items():
+            out = os.path.join(tmp, name + ".bin")
+            run(["mesh", out] + margs.split(), tmp)
+            blocks, n, vc, _ = read_dump(out, 0, np.float64)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), blocks=blocks.view(np.uint8), n=n, vc=vc,
+                                args=margs)
+            print(name, len(blocks))
+        for name, (method, dt, margs) in GHOSTS.items():
+            out = os.path.join(tmp, name + ".bin")
+            run(["ghost", out, str(method), dt] + margs.split(), tmp)
+            dtype = np.float64 if dt == "d" else np.float32
+            blocks, n, vc, (fin, fout) = read_dump(out, 2, dtype)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), blocks=blocks.view(np.uint8), n=n, vc=vc,
+                                method=method, fin=fin, fout=fout, args=margs)
+            print(name, len(blocks), fin.shape)
+        for name, (conf, rn, ordering) in SOR1.items():
+            cfile = os.path.join(tmp, name + ".conf")
+            if conf is None:
+                with open(os.path.join(REF, "examples", "sor1", "test.conf")) as fp:
+                    conf = fp.read()
+            with open(cfile, "w") as fp:
+                fp.write(conf)
+            out = os.path.join(tmp, name + ".bin")
+            text = run(["sor1", out, cfile, str(rn), ordering], tmp)
+            err = float([ln for ln in text.splitlines() if ln.startswith("errorMax")][-1].split("=")[1])
+            blocks, n, vc, (fout,) = read_dump(out, 1, np.float64)
+            keys = {}
+            for ln in conf.splitlines():
+                if "=" in ln and not ln.strip().startswith("#"):
+                    k, v = ln.split("=", 1)
+                    keys[k.strip()] = v.strip()
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), blocks=blocks.view(np.uint8), n=n, vc=vc,
+                                fout=fout, error_max=err, omega=float(keys["omega"]),
+                                iters=int(keys["nLoopOuter"]), type=keys["type"],
+                                b0=float(keys.get("b0", 0.0)), b1=float(keys.get("b1", 1.0)))
+            print(name, len(blocks), "errorMax", err)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,285 @@
+"""HIP path vs the oracle and the reference's own fixtures, through the C ABI.
+
+Bit-exact: ghost fill (copy/F2C/C2F, FP64 and FP32), Dirichlet mirror,
+red-black SOR (sor1 and sor2 forms, fused and generic kernels), Jacobi7,
+calc_ax, calc_r, BLAS-1, Poisson assembly.  Tolerance (reduction order
+differs from the reference's OpenMP order): dot, calc_r2 (rel 1e-12),
+BiCGSTAB (solution rel 1e-9), mixed-precision solve (FP64 relative
+residual <= 1e-10).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from bcmtools_amd import Mesh
+from bcmtools_amd import _lib as L
+from bcmtools_amd import workloads as W
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu_update(mesh, fld, method):
+    if method == 0:
+        mesh.update_vc(fld)
+    elif method == 1:
+        for ax in range(3):
+            mesh.update_vc_xyz(fld, ax)
+    else:
+        for kind in (L.GF_F2C, L.GF_FLAT, L.GF_C2F):
+            for ax in range(3):
+                mesh.update_vc_xyz(fld, ax, kind)
+
+
+@pytest.mark.parametrize("name", ["ghost_sphere_atonce_d", "ghost_sphere_xyz_d", "ghost_sphere_leveldiff_d",
+                                  "ghost_sphere_atonce_f", "ghost_simple_vc2_d", "ghost_periodic_x_d"])
+def test_ghost_fill_vs_reference_fixture(name):
+    g = load_golden(name)
+    case(f"ghost fill {name} nb={len(g['blocks'])} n={g['n']} vc={g['vc']}")
+    mesh = Mesh(g["blocks"], g["n"], g["vc"])
+    fld = mesh.field(g["fin"].dtype, g["fin"])
+    gpu_update(mesh, fld, g["method"])
+    np.testing.assert_array_equal(fld.download(), g["fout"])
+
+
+@pytest.mark.parametrize("name", ["sor1_log4", "sor1_adaptive_y"])
+def test_sor1_driver_vs_reference_fixture(name):
+    g = load_golden(name)
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    case(f"sor1 {name} nb={len(blocks)} n={n} iters={g['iters']}")
+    mesh = Mesh(blocks, n, vc)
+    f = mesh.field(np.float64)
+    vals = [g["b0"], g["b1"]]
+    mesh.update_vc(f)
+    mesh.dirichlet_mirror(f, vals)
+    mesh.sor_iterate(f, None, g["omega"], vals, g["iters"])
+    out = f.download()
+    mask = W.face_ghost_mask(n, vc)
+    np.testing.assert_array_equal(out[:, mask], g["fout"][:, mask])
+
+
+@pytest.mark.parametrize("n,level,form", [(16, 3, 2), (32, 2, 2), (16, 3, 1), (32, 2, 1), (8, 3, 2), (4, 3, 1)])
+def test_rbsor_vs_oracle(n, level, form):
+    t = W.shell_tree(level)
+    blocks = t.blocks
+    case(f"rbsor n={n} level={level} form=sor{form} nb={len(blocks)}")
+    rng = np.random.default_rng(n + level)
+    nx0 = n + 2
+    f0 = rng.standard_normal((len(blocks), nx0, nx0, nx0))
+    s = W.rhs_field(blocks, n, 1, seed=3) if form == 2 else None
+    mesh = Mesh(blocks, n, 1)
+    f = mesh.field(np.float64, f0)
+    sf = mesh.field(np.float64, s) if s is not None else None
+    vals = [0.0]
+    mesh.sor_iterate(f, sf, 1.3, vals, 3)
+    ref = f0.copy()
+    oracle.dirichlet(blocks, n, 1, ref, vals)
+    oracle.sor_iterate(blocks, n, 1, ref, s, 1.3, vals, 3)
+    np.testing.assert_array_equal(f.download(), ref)
+
+
+def random_A(shape, rng, dtype):
+    A = [rng.uniform(0.1, 1.0, shape).astype(dtype) for _ in range(7)]
+    A[0] = (-rng.uniform(6.0, 8.0, shape)).astype(dtype)
+    return A
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_solvertest_kernels_vs_oracle(dtype):
+    t = W.shell_tree(3)
+    blocks, n, vc = t.blocks, 16, 1
+    nb = len(blocks)
+    case(f"jacobi7/calc_ax/calc_r/blas dtype={np.dtype(dtype).name} nb={nb}")
+    rng = np.random.default_rng(11)
+    shape = (nb, n + 2, n + 2, n + 2)
+    A = random_A(shape, rng, dtype)
+    x = rng.standard_normal(shape).astype(dtype)
+    b = rng.standard_normal(shape).astype(dtype)
+    y = rng.standard_normal(shape).astype(dtype)
+    mesh = Mesh(blocks, n, vc)
+    gA = [mesh.field(dtype, a) for a in A]
+    gx, gb, gy = mesh.field(dtype, x), mesh.field(dtype, b), mesh.field(dtype, y)
+    out = mesh.field(dtype, np.zeros(shape, dtype))
+    # jacobi_smoother
+    mesh.jacobi7(out, gx, gA, gb, 0.8)
+    ref = np.zeros(shape, dtype)
+    oracle.jacobi7(nb, n, vc, ref, x, A, b, 0.8)
+    np.testing.assert_array_equal(out.download(), ref)
+    # calc_ax
+    mesh.calc_ax(out, gA, gx)
+    oracle.calc_ax(nb, n, vc, ref, A, x)
+    np.testing.assert_array_equal(out.download(), ref)
+    # calc_r
+    mesh.calc_r(out, gA, gx, gb)
+    oracle.calc_r(nb, n, vc, ref, A, x, b)
+    np.testing.assert_array_equal(out.download(), ref)
+    # calc_r2 and dot: reduction order differs -> tolerance
+    rtol = 1e-12 if dtype == np.float64 else 2e-5
+    assert mesh.calc_r2(gA, gx, gb) == pytest.approx(oracle.calc_r2(nb, n, vc, A, x, b), rel=rtol)
+    assert mesh.dot(gx, gy) == pytest.approx(oracle.dot(nb, n, vc, x, y), rel=rtol * 10)
+    # BLAS-1
+    for op, call in (("axpy", lambda z: mesh.axpy(z, gx, 0.3)), ("xpay", lambda z: mesh.xpay(z, gx, 0.3)),
+                     ("axpyz", lambda z: mesh.axpyz(z, gx, gy, 0.3)),
+                     ("axpbypz", lambda z: mesh.axpbypz(z, gx, gy, 0.3, -1.7))):
+        z0 = rng.standard_normal(shape).astype(dtype)
+        gz = mesh.field(dtype, z0)
+        call(gz)
+        zr = z0.copy()
+        oracle.blas(nb, n, vc, op, zr, x, y, 0.3, -1.7)
+        np.testing.assert_array_equal(gz.download(), zr, err_msg=op)
+    mesh.copy(out, gx)
+    np.testing.assert_array_equal(out.download(), x)
+    mesh.fill(out, 2.5)
+    assert np.all(out.download() == dtype(2.5))
+
+
+def test_poisson_assembly_and_matrix_free_kernels_vs_oracle():
+    t = W.shell_tree(3)
+    blocks, n, vc = t.blocks, 16, 1
+    nb = len(blocks)
+    case(f"poisson assemble + matrix-free nb={nb}")
+    f = W.rhs_field(blocks, n, vc, seed=9)
+    vals = [0.25]
+    mesh = Mesh(blocks, n, vc)
+    grhs = mesh.field(np.float64, -f)
+    gA = [mesh.field(np.float64) for _ in range(7)]
+    gb = mesh.field(np.float64)
+    mesh.poisson_assemble(gA, gb, grhs, vals)
+    A = [np.zeros_like(f) for _ in range(7)]
+    b = np.zeros_like(f)
+    oracle.poisson_assemble(blocks, n, vc, A, b, -f, vals)
+    interior = np.pad(np.ones((n,) * 3, bool), vc)
+    for i in range(7):
+        np.testing.assert_array_equal(gA[i].download()[:, interior], A[i][:, interior])
+    np.testing.assert_array_equal(gb.download()[:, interior], b[:, interior])
+    # Matrix-free kernels equal the stored-coefficient kernels bit for bit.
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(f.shape)
+    gx = mesh.field(np.float64, x)
+    o1, o2 = mesh.field(np.float64), mesh.field(np.float64)
+    mesh.jacobi7(o1, gx, gA, gb, 0.9)
+    mesh.jacobi7_poisson(o2, gx, gb, 0.9)
+    np.testing.assert_array_equal(o1.download(), o2.download())
+    mesh.calc_ax(o1, gA, gx)
+    mesh.calc_ax_poisson(o2, gx)
+    np.testing.assert_array_equal(o1.download(), o2.download())
+    mesh.calc_r(o1, gA, gx, gb)
+    mesh.calc_r_poisson(o2, gx, gb)
+    np.testing.assert_array_equal(o1.download(), o2.download())
+
+
+@pytest.mark.parametrize("matrix_free", [False, True])
+def test_config1_100_jacobi_sweeps_bit_exact(matrix_free):
+    """Config 1: 64 x 16^3, Dirichlet u = 0, 100 x (Jacobi sweep + ghost fill)."""
+    t = W.config1_tree()
+    blocks, n, vc = t.blocks, 16, 1
+    nb = len(blocks)
+    case(f"config1 jacobi x100 matrix_free={matrix_free}")
+    f = W.rhs_field(blocks, n, vc, seed=1)
+    A = [np.zeros_like(f) for _ in range(7)]
+    b = np.zeros_like(f)
+    oracle.poisson_assemble(blocks, n, vc, A, b, -f, [0.0])
+    x, x0 = np.zeros_like(f), np.zeros_like(f)
+    for _ in range(100):
+        oracle.jacobi7(nb, n, vc, x0, x, A, b, 1.0)
+        x, x0 = x0, x
+        oracle.ghost_fill(blocks, n, vc, x, L.GF_ALL)
+    mesh = Mesh(blocks, n, vc)
+    gx, g0 = mesh.field(np.float64), mesh.field(np.float64)
+    gb = mesh.field(np.float64, b)
+    gA = [mesh.field(np.float64, a) for a in A]
+    for _ in range(100):
+        if matrix_free:
+            mesh.jacobi7_poisson(g0, gx, gb, 1.0)
+        else:
+            mesh.jacobi7(g0, gx, gA, gb, 1.0)
+        gx, g0 = g0, gx
+        mesh.update_vc(gx)
+    np.testing.assert_array_equal(gx.download(), x)
+
+
+def test_bicgstab_vs_oracle():
+    t = W.shell_tree(3)
+    blocks, n, vc = t.blocks, 8, 1
+    case(f"bicgstab fp64 nb={len(blocks)}")
+    f = W.rhs_field(blocks, n, vc, seed=4)
+    A = [np.zeros_like(f) for _ in range(7)]
+    b = np.zeros_like(f)
+    oracle.poisson_assemble(blocks, n, vc, A, b, -f, [0.0])
+    x = np.zeros_like(f)
+    work = [np.zeros_like(f) for _ in range(9)]
+    st, cnt, res = oracle.bicgstab(blocks, n, vc, x, A, b, work, 0.8, 2, 400, 1e-11)
+    assert st == 0
+    mesh = Mesh(blocks, n, vc)
+    gA = [mesh.field(np.float64, a) for a in A]
+    gb = mesh.field(np.float64, b)
+    gx = mesh.field(np.float64)
+    gw = [mesh.field(np.float64) for _ in range(9)]
+    rc, gcnt, gres = mesh.bicgstab(gx, gA, gb, gw, omega=0.8, count_pre=2, count_max=400, epsilon=1e-11)
+    assert rc == L.OK
+    assert abs(gcnt - cnt) <= max(3, cnt // 10)
+    xi, ri = W.interior(gx.download(), vc), W.interior(x, vc)
+    assert np.abs(xi - ri).max() <= 1e-8 * np.abs(ri).max()
+
+
+def test_jacobi_solve_vs_oracle():
+    t = W.config1_tree()
+    blocks, n, vc = t.blocks, 8, 1
+    case("jacobi solve fp64")
+    f = W.rhs_field(blocks, n, vc, seed=6)
+    A = [np.zeros_like(f) for _ in range(7)]
+    b = np.zeros_like(f)
+    oracle.poisson_assemble(blocks, n, vc, A, b, -f, [0.0])
+    x, x0 = np.zeros_like(f), np.zeros_like(f)
+    st, cnt, res, out = oracle.jacobi_solve(blocks, n, vc, x, A, b, x0, 1.0, 300, 1e-14)
+    mesh = Mesh(blocks, n, vc)
+    gA = [mesh.field(np.float64, a) for a in A]
+    gb = mesh.field(np.float64, b)
+    gx, g0 = mesh.field(np.float64), mesh.field(np.float64)
+    rc, gcnt, gres = mesh.jacobi_solve(gx, gA, gb, g0, omega=1.0, count_max=300, epsilon=1e-14)
+    assert gcnt == cnt == 300
+    # Sweeps are bit-exact; only the residual reduction order differs.
+    np.testing.assert_array_equal(gx.download(), out)
+    assert gres == pytest.approx(res, rel=1e-10)
+
+
+def test_mixed_precision_reaches_1e10_and_matches_fp64():
+    t = W.shell_tree(4)
+    blocks, n, vc = t.blocks, 16, 1
+    case(f"mixed precision nb={len(blocks)}")
+    f = W.rhs_field(blocks, n, vc, seed=8)
+    mesh = Mesh(blocks, n, vc)
+    rhs = mesh.field(np.float64, -f)
+    x1 = mesh.field(np.float64)
+    rc, outer, inner, rr = mesh.solve_mixed(x1, rhs, [0.0], tol=1e-10, inner_tol=1e-4)
+    assert rc == L.OK and rr <= 1e-10 and outer >= 2
+    x2 = mesh.field(np.float64)
+    rc2, it2, rr2 = mesh.solve_fp64(x2, rhs, [0.0], tol=1e-11)
+    assert rc2 == L.OK and rr2 <= 1e-11
+    a, b = W.interior(x1.download(), vc), W.interior(x2.download(), vc)
+    assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+
+
+def test_full_size_config2_ghost_fill_vs_oracle():
+    """Config 2 at full size (3,816 x 32^3): GPU ghost fill equals the oracle
+    bit for bit; a second fill changes nothing (idempotence)."""
+    t = W.config2_tree()
+    blocks, n, vc = t.blocks, 32, 1
+    case(f"full-size config2 ghost fill nb={len(blocks)}")
+    f = W.rhs_field(blocks, n, vc, seed=12)
+    mesh = Mesh(blocks, n, vc)
+    gf = mesh.field(np.float64, f)
+    mesh.update_vc(gf)
+    out = gf.download()
+    oracle.ghost_fill(blocks, n, vc, f, L.GF_ALL)
+    assert np.array_equal(out, f)
+    mesh.update_vc(gf)
+    assert np.array_equal(gf.download(), out)

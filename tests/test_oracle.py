@@ -1,0 +1,348 @@
+"""Pin the CPU oracle (oracle/bcm_oracle.c).
+
+Ghost fill and sor1 sweeps are compared bit for bit with fixtures the
+reference library itself wrote (tests/golden/make_golden.py).  The Fortran
+SolverTest kernels (blas.f90) cannot be compiled here, so they are pinned
+by an independent numpy restatement with the same operand order, by
+closed forms and by a direct sparse solve.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from bcmtools_amd import _lib as L
+from bcmtools_amd import workloads as W
+
+from conftest import load_golden
+
+GHOST_FIXTURES = ["ghost_sphere_atonce_d", "ghost_sphere_xyz_d", "ghost_sphere_leveldiff_d", "ghost_sphere_atonce_f",
+                  "ghost_simple_vc2_d", "ghost_periodic_x_d"]
+
+
+def oracle_update(blocks, n, vc, data, method):
+    """BlockManager::updateVC in the three VCUpdateMethod forms."""
+    if method == 0:
+        oracle.ghost_fill(blocks, n, vc, data, L.GF_ALL)
+    elif method == 1:
+        for ax in range(3):
+            oracle.ghost_fill(blocks, n, vc, data, L.GF_ALL | (L.GF_X << ax))
+    else:
+        for kind in (L.GF_F2C, L.GF_FLAT, L.GF_C2F):
+            for ax in range(3):
+                oracle.ghost_fill(blocks, n, vc, data, kind | (L.GF_X << ax))
+
+
+@pytest.mark.parametrize("name", GHOST_FIXTURES)
+def test_ghost_fill_bit_exact_vs_reference(name):
+    g = load_golden(name)
+    data = g["fin"].copy()
+    oracle_update(g["blocks"], g["n"], g["vc"], data, g["method"])
+    assert data.dtype == g["fout"].dtype
+    np.testing.assert_array_equal(data, g["fout"])
+    # The fixture exercises every kind of face task.
+    ld = g["blocks"]["nb_level_diff"]
+    if "sphere" in name:
+        assert (ld == 1).any() and (ld == -1).any()
+
+
+def test_golden_pins_f2c_sum_order():
+    """A restriction summed in another order (SolverTest's sup.f90 order,
+    nnn+pnn+npn+nnp+npp+pnp+ppn+ppp) differs from the reference's fixture."""
+    g = load_golden("ghost_sphere_atonce_d")
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    f = g["fin"]
+    diffs = 0
+    for b, bi in enumerate(blocks):
+        if bi["nb_level_diff"][L.X_P] != 1:
+            continue
+        for s in range(4):
+            nb = bi["nb_id"][L.X_P][s]
+            if nb < 0:
+                continue
+            h = n // 2
+            j0, k0 = h * (s & 1), h * (s >> 1)
+            for k in range(h):
+                for j in range(h):
+                    c = f[nb, vc + 2 * k:vc + 2 * k + 2, vc + 2 * j:vc + 2 * j + 2, vc:vc + 2]  # [k, j, i]
+                    alt = 0.125 * (c[0, 0, 0] + c[0, 0, 1] + c[0, 1, 0] + c[1, 0, 0] + c[1, 1, 0] + c[1, 0, 1]
+                                   + c[0, 1, 1] + c[1, 1, 1])
+                    ref = g["fout"][b, vc + k0 + k, vc + j0 + j, vc + n]
+                    diffs += alt != ref
+    assert diffs > 0
+
+
+def test_closed_forms_of_f2c_and_c2f():
+    """F2C of a constant is that constant; F2C/C2F reproduce a linear field
+    (C2F including its -0.25 / 1.25 extrapolation at the ends)."""
+    g = load_golden("ghost_sphere_atonce_d")
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    data = np.full(g["fin"].shape, 0.75)
+    oracle.ghost_fill(blocks, n, vc, data, L.GF_ALL)
+    mask = W.face_ghost_mask(n, vc)
+    assert np.all(data[:, mask] == 0.75)
+    # Linear field at cell centres.
+    nx0 = n + 2 * vc
+    t = np.arange(nx0) - vc + 0.5
+    lin = np.empty(g["fin"].shape)
+    for b, bi in enumerate(blocks):
+        h = bi["block_size"] / n
+        x = bi["origin"][0] + t * h[0]
+        y = bi["origin"][1] + t * h[1]
+        z = bi["origin"][2] + t * h[2]
+        lin[b] = 1.0 + 2.0 * x[None, None, :] - 3.0 * y[None, :, None] + 0.5 * z[:, None, None]
+    data = lin.copy()
+    # Scramble the face ghosts, then fill them again.
+    face = W.face_ghost_mask(n, vc) & ~np.pad(np.ones((n,) * 3, bool), vc)
+    data[:, face] = 123.0
+    oracle.ghost_fill(blocks, n, vc, data, L.GF_ALL)
+    filled = np.zeros(data.shape, bool)
+    for b, bi in enumerate(blocks):
+        for f in range(6):
+            if bi["nb_id"][f][0] >= 0 or (bi["nb_id"][f] >= 0).any():
+                sl = [slice(vc, vc + n)] * 3
+                ax = 2 - f // 2  # array axes are (k, j, i)
+                sl[ax] = slice(vc + n, vc + n + vc) if f & 1 else slice(0, vc)
+                filled[(b,) + tuple(sl)] = True
+    np.testing.assert_allclose(data[filled], lin[filled], rtol=0, atol=1e-12)
+    assert filled.sum() > 0
+
+
+def sor1_start(g):
+    """Solver::initialize (examples/sor1/Solver.cpp:48-71): zero interior,
+    ghost fill, Dirichlet mirror with b0/b1."""
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    nx0 = n + 2 * vc
+    f = np.zeros((len(blocks), nx0, nx0, nx0))
+    vals = np.array([g["b0"], g["b1"]])
+    oracle.ghost_fill(blocks, n, vc, f, L.GF_ALL)
+    oracle.dirichlet(blocks, n, vc, f, vals)
+    return f, vals
+
+
+@pytest.mark.parametrize("name", ["sor1_log4", "sor1_adaptive_y"])
+def test_sor1_bit_exact_vs_reference(name):
+    g = load_golden(name)
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    f, vals = sor1_start(g)
+    oracle.sor_iterate(blocks, n, vc, f, None, g["omega"], vals, g["iters"])
+    mask = W.face_ghost_mask(n, vc)
+    np.testing.assert_array_equal(f[:, mask], g["fout"][:, mask])
+
+
+def test_sor1_known_answer_log4():
+    """examples/sor1/log.4:54: errorMax = 0.00353051 against f0 = x."""
+    g = load_golden("sor1_log4")
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    f, vals = sor1_start(g)
+    oracle.sor_iterate(blocks, n, vc, f, None, g["omega"], vals, g["iters"])
+    t = np.arange(n) + 0.5
+    x = blocks["origin"][:, 0, None] + t[None, :] * (blocks["block_size"][:, 0, None] / n)
+    err = np.abs(W.interior(f, vc) - x[:, None, None, :]).max()
+    assert err == pytest.approx(0.00353051, rel=2e-6)
+    assert err == pytest.approx(float(g["error_max"]), rel=2e-6)
+
+
+def small_tree(level=1, n=8):
+    return W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=level), n
+
+
+def test_sor2_matches_sor1_form_without_source():
+    """With s = 0 the sor2 update is algebraically the sor1 update."""
+    t, n = small_tree(1, 8)
+    rng = np.random.default_rng(0)
+    nx0 = n + 2
+    a = rng.standard_normal((len(t.blocks), nx0, nx0, nx0))
+    b = a.copy()
+    oracle.sor_sweep(t.blocks, n, 1, a, None, 1.3)
+    oracle.sor_sweep(t.blocks, n, 1, b, np.zeros_like(b), 1.3)
+    np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+def test_sor2_converges_to_closed_form():
+    """sor2's problem: lap f = -3 a^2 sin(ax) sin(ay) sin(az), a = 2 pi, f = 0
+    on the boundary (examples/sor2/Solver.cpp:383-405); error falls as h^2."""
+    errs = []
+    for level, n in ((1, 4), (1, 8)):
+        t = W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=level)
+        blocks = t.blocks
+        s = W.rhs_field(blocks, n, 1, sigma=0.0, freq=2.0) * (-3.0 * (2 * np.pi) ** 2)
+        f = np.zeros_like(s)
+        oracle.ghost_fill(blocks, n, 1, f, L.GF_ALL)
+        oracle.dirichlet(blocks, n, 1, f, [0.0])
+        oracle.sor_iterate(blocks, n, 1, f, s, 1.2, [0.0], 800)
+        exact = W.rhs_field(blocks, n, 1, sigma=0.0, freq=2.0)
+        errs.append(np.abs(W.interior(f - exact, 1)).max())
+    assert errs[1] < errs[0] / 3.0
+    assert errs[1] < 0.05
+
+
+def test_frozen_ghost_sor_diverges_at_omega_1_6():
+    """The reference iteration (SOR inside a block, ghosts frozen until the
+    next exchange) is unstable for omega >= 1.6 even on a uniform mesh; the
+    compiled reference sor1 shows the same (errorMax ~ 6e31 after 400
+    sweeps at omega 1.6 on 64 x 16^3).  omega 1.4 converges."""
+    t = W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=1)
+    blocks, n = t.blocks, 16
+    s = W.rhs_field(blocks, n, 1, sigma=0.0, freq=2.0) * (-3.0 * (2 * np.pi) ** 2)
+    out = {}
+    for om in (1.4, 1.6):
+        f = np.zeros_like(s)
+        oracle.ghost_fill(blocks, n, 1, f, L.GF_ALL)
+        oracle.dirichlet(blocks, n, 1, f, [0.0])
+        oracle.sor_iterate(blocks, n, 1, f, s, om, [0.0], 300)
+        out[om] = np.abs(W.interior(f, 1)).max()
+    assert out[1.4] < 2.0
+    assert out[1.6] > 1e10
+
+
+# ---------------------------------------------------------------------------
+# SolverTest kernels vs a numpy restatement with the same operand order.
+# ---------------------------------------------------------------------------
+def nb_views(x, n, vc):
+    s = slice(vc, vc + n)
+    sm = slice(vc - 1, vc + n - 1)
+    spl = slice(vc + 1, vc + n + 1)
+    return (x[:, s, s, sm], x[:, s, s, spl], x[:, s, sm, s], x[:, s, spl, s], x[:, sm, s, s], x[:, spl, s, s],
+            x[:, s, s, s])
+
+
+def random_system(dtype, nb=8, n=6, vc=1, seed=1):
+    rng = np.random.default_rng(seed)
+    nx0 = n + 2 * vc
+    shape = (nb, nx0, nx0, nx0)
+    A = [rng.uniform(-1, 1, shape).astype(dtype) for _ in range(7)]
+    A[0] = (rng.uniform(2, 3, shape) * rng.choice([-1, 1], shape)).astype(dtype)
+    x = rng.standard_normal(shape).astype(dtype)
+    b = rng.standard_normal(shape).astype(dtype)
+    return A, x, b
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_jacobi_ax_r_bit_exact_vs_numpy(dtype):
+    nb, n, vc = 8, 6, 1
+    A, x, b = random_system(dtype, nb, n, vc)
+    I = W.interior
+    xW, xE, xS, xN, xB, xT, xP = nb_views(x, n, vc)
+    Ap, Aw, Ae, As, An, Ab, At = (I(a, vc) for a in A)
+    bi = I(b, vc)
+    om = dtype(0.8)
+    # jacobi_smoother (blas.f90:485-493)
+    r = bi - Aw * xW - Ae * xE - As * xS - An * xN - Ab * xB - At * xT
+    want = xP - om * (xP - r / Ap)
+    x1 = np.zeros_like(x)
+    oracle.jacobi7(nb, n, vc, x1, x, A, b, 0.8)
+    np.testing.assert_array_equal(I(x1, vc), want)
+    # calc_ax (blas.f90:584-591)
+    want = Aw * xW + Ae * xE + As * xS + An * xN + Ab * xB + At * xT + Ap * xP
+    ax = np.zeros_like(x)
+    oracle.calc_ax(nb, n, vc, ax, A, x)
+    np.testing.assert_array_equal(I(ax, vc), want)
+    # calc_r (blas.f90:632-639)
+    want = bi - Aw * xW - Ae * xE - As * xS - An * xN - Ab * xB - At * xT - Ap * xP
+    rr = np.zeros_like(x)
+    oracle.calc_r(nb, n, vc, rr, A, x, b)
+    np.testing.assert_array_equal(I(rr, vc), want)
+    r2 = oracle.calc_r2(nb, n, vc, A, x, b)
+    assert r2 == pytest.approx(float((want.astype(np.float64) ** 2).sum()), rel=1e-5 if dtype == np.float32 else 1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_blas1_bit_exact_vs_numpy(dtype):
+    nb, n, vc = 4, 6, 1
+    _, x, y = random_system(dtype, nb, n, vc, seed=3)
+    z0 = np.random.default_rng(4).standard_normal(x.shape).astype(dtype)
+    I = W.interior
+    a, bcoef = dtype(0.3), dtype(-1.7)
+    cases = {
+        "axpy": lambda z: I(z, vc) + a * I(x, vc),
+        "xpay": lambda z: I(x, vc) + a * I(z, vc),
+        "axpyz": lambda z: a * I(x, vc) + I(y, vc),
+        "axpbypz": lambda z: a * I(x, vc) + bcoef * I(y, vc) + I(z, vc),
+        "copy": lambda z: I(x, vc),
+    }
+    for op, fn in cases.items():
+        z = z0.copy()
+        want = fn(z)
+        oracle.blas(nb, n, vc, op, z, x, y, 0.3, -1.7)
+        np.testing.assert_array_equal(I(z, vc), want, err_msg=op)
+        ghost = ~np.pad(np.ones((n,) * 3, bool), vc)
+        np.testing.assert_array_equal(z[:, ghost], z0[:, ghost], err_msg=op + " touched ghosts")
+    d = oracle.dot(nb, n, vc, x, y)
+    assert d == pytest.approx(float((I(x, vc).astype(np.float64) * I(y, vc)).sum()),
+                              rel=1e-5 if dtype == np.float32 else 1e-12)
+
+
+def uniform_poisson_matrix(m, h):
+    """7-point lap_h on an m^3 grid with the Dirichlet mirror (ghost = -u)."""
+    l = 1.0 / (h * h)
+    e = np.ones(m)
+    d1 = sp.diags([e[:-1] * l, -2 * l * e, e[:-1] * l], [-1, 0, 1]).tolil()
+    d1[0, 0] -= l
+    d1[m - 1, m - 1] -= l
+    d1 = d1.tocsr()
+    I = sp.identity(m)
+    return (sp.kron(sp.kron(I, I), d1) + sp.kron(sp.kron(I, d1), I) + sp.kron(sp.kron(d1, I), I)).tocsr()
+
+
+def assemble(blocks, n, vc, rhs, vals):
+    A = [np.zeros_like(rhs) for _ in range(7)]
+    b = np.zeros_like(rhs)
+    oracle.poisson_assemble(blocks, n, vc, A, b, rhs, vals)
+    return A, b
+
+
+def test_bicgstab_matches_direct_solve():
+    """ILS3D::BiCGSTAB on the folded Poisson operator vs scipy's sparse LU."""
+    t = W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=1)
+    blocks, n, vc = t.blocks, 8, 1
+    f = W.rhs_field(blocks, n, vc, seed=5)
+    A, b = assemble(blocks, n, vc, -f, [0.0])
+    x = np.zeros_like(f)
+    work = [np.zeros_like(f) for _ in range(9)]
+    st, cnt, res = oracle.bicgstab(blocks, n, vc, x, A, b, work, 0.8, 2, 500, 1e-12)
+    assert st == 0 and cnt > 0 and res <= 1e-24
+    # Global grid: 16^3 with block (bx,by,bz) at x-fastest Z order of level 1.
+    m = 16
+    h = 1.0 / m
+    M = uniform_poisson_matrix(m, h)
+    rhs_g = np.zeros((m, m, m))
+    xg = np.zeros((m, m, m))
+    for bk, bi in enumerate(blocks):
+        o = np.rint(bi["origin"] / (n * h)).astype(int)
+        sl = (slice(o[2] * n, o[2] * n + n), slice(o[1] * n, o[1] * n + n), slice(o[0] * n, o[0] * n + n))
+        rhs_g[sl] = W.interior(-f, vc)[bk]
+        xg[sl] = W.interior(x, vc)[bk]
+    u = spla.spsolve(M, rhs_g.ravel())
+    np.testing.assert_allclose(xg.ravel(), u, rtol=0, atol=1e-9 * np.abs(u).max())
+
+
+def test_jacobi_solve_reduces_residual_and_is_a_fixed_point():
+    t = W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=1)
+    blocks, n, vc = t.blocks, 4, 1
+    f = W.rhs_field(blocks, n, vc, seed=2)
+    A, b = assemble(blocks, n, vc, -f, [0.0])
+    x = np.zeros_like(f)
+    x0 = np.zeros_like(f)
+    st, cnt, res, out = oracle.jacobi_solve(blocks, n, vc, x, A, b, x0, 1.0, 5000, 1e-10)
+    assert st == 0 and res <= 1e-20
+    # The converged solution is (nearly) a fixed point of one more sweep.
+    y = np.zeros_like(out)
+    oracle.jacobi7(len(blocks), n, vc, y, out, A, b, 1.0)
+    np.testing.assert_allclose(W.interior(y, vc), W.interior(out, vc), rtol=0, atol=1e-9)
+
+
+def test_assembly_folds_dirichlet_values():
+    """bc_Aw_d: Ap -= Aw; b -= 2 Aw xc; Aw = 0 on the outer face."""
+    t = W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=0)
+    blocks, n, vc = t.blocks, 4, 1
+    rhs = np.zeros((1, 6, 6, 6))
+    A, b = assemble(blocks, n, vc, rhs, [0.5])
+    l = 1.0 / (0.25 ** 2)
+    Ap = W.interior(A[0], vc)[0]
+    assert Ap[1, 1, 1] == -6 * l
+    assert Ap[0, 1, 1] == -7 * l and Ap[0, 0, 0] == -9 * l
+    assert W.interior(A[1], vc)[0][1, 1, 0] == 0.0 and W.interior(A[1], vc)[0][1, 1, 1] == l
+    assert W.interior(b, vc)[0][1, 1, 0] == -2 * l * 0.5
+    assert W.interior(b, vc)[0][0, 0, 0] == -6 * l * 0.5
