@@ -74,6 +74,9 @@ struct bcm_mesh {
   // Boundary-value table on the device.
   double* d_bvalues = nullptr;
   int n_bvalues_cap = 0;
+  std::vector<double> h_bvalues;  // last table uploaded (skip identical re-uploads)
+  bool bvalues_valid = false;
+  int need_bvalues = 0;           // 1 + largest Dirichlet bc_id
 
   // SOR coefficient cache (per block: c0, cx, cy, cz) for one omega/form.
   double* d_sor_coef = nullptr;

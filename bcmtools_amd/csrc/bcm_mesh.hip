@@ -77,18 +77,25 @@ int timer_end(bcm_mesh* m, EventPair* ev) {
 int set_bvalues(bcm_mesh* m, const double* values, int n) {
   if (n < 0 || (n > 0 && !values)) return BCM_ERR_ARG;
   // Every Dirichlet face must index a supplied value.
-  for (const auto& bi : m->blocks)
-    for (int f = 0; f < 6; ++f)
-      if (bi.bc_type[f] == BCM_BC_DIRICHLET && (bi.bc_id[f] < 0 || bi.bc_id[f] >= n)) return BCM_ERR_ARG;
+  if (n < m->need_bvalues) return BCM_ERR_ARG;
+  if (m->bvalues_valid && int(m->h_bvalues.size()) == n &&
+      std::equal(m->h_bvalues.begin(), m->h_bvalues.end(), values))
+    return BCM_OK;
   if (n > m->n_bvalues_cap) {
+    BCM_HIP_TRY(hipStreamSynchronize(m->stream));
     dev_free(m->d_bvalues);
     m->d_bvalues = nullptr;
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&m->d_bvalues), sizeof(double) * size_t(n)));
     m->n_bvalues_cap = n;
   }
-  if (n > 0)
-    BCM_HIP_TRY(hipMemcpyAsync(m->d_bvalues, values, sizeof(double) * size_t(n), hipMemcpyHostToDevice,
-                               m->stream));
+  m->bvalues_valid = false;
+  if (n > 0) {
+    // Kernels already queued may still read the old table: wait, then copy.
+    BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+    BCM_HIP_TRY(hipMemcpy(m->d_bvalues, values, sizeof(double) * size_t(n), hipMemcpyHostToDevice));
+  }
+  m->h_bvalues.assign(values, values + n);
+  m->bvalues_valid = true;
   return BCM_OK;
 }
 
@@ -346,6 +353,7 @@ extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n,
         if (bi.nb_id[f][s] < -1 || bi.nb_id[f][s] >= nblocks) return BCM_ERR_ARG;
       if (ld == -1 && (bi.nb_subface[f] < 0 || bi.nb_subface[f] > 3)) return BCM_ERR_ARG;
       if (bi.bc_type[f] < BCM_BC_INNER || bi.bc_type[f] > BCM_BC_PERIODIC) return BCM_ERR_ARG;
+      if (bi.bc_type[f] == BCM_BC_DIRICHLET && bi.bc_id[f] < 0) return BCM_ERR_ARG;
     }
   }
   bcm_mesh* m = new (std::nothrow) bcm_mesh();
@@ -396,6 +404,7 @@ extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n,
           if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
             me.dir_mask |= 1 << f;
             m->n_dirichlet++;
+            m->need_bvalues = std::max(m->need_bvalues, bi.bc_id[f] + 1);
           }
         }
       }

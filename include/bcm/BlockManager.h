@@ -1,0 +1,331 @@
+// BlockManager facade (include/BlockManager.h, src/BlockManager.cpp) over
+// the C ABI of include/bcm_hip.h.
+//
+// Same calls as the reference: registerBlock / endRegisterBlock,
+// setDataClass<D[,U[,T]]>(vc), prepareForVCUpdate(id, tag, method) and
+// every updateVC* / beginUpdateVC* / endUpdateVC* form.  Each data class
+// registered with an updater lives in one contiguous host arena (the
+// reference's setDataClass<D,U,T> layout) mirrored by one device field.
+//
+// Residency:
+//  * host-resident (default, drop-in): the host arena is authoritative;
+//    updateVC* uploads it, fills the face ghosts on the GPU and downloads
+//    it, so a driver that computes on getData() pointers runs unchanged.
+//  * device-resident (setDeviceResident(id, true)): the device field is
+//    authoritative, updateVC* is one kernel launch, and the driver calls
+//    the device kernels (deviceField(id) with the bcm_* entry points);
+//    syncToHost / syncToDevice move the data explicitly.
+//
+// Single GPU, single process: getStartID() is 0, ranks are 0, and there
+// is no communicator (MPI is not linked).  beginUpdateVC* performs the
+// whole exchange and endUpdateVC* returns immediately, which gives the
+// reference's results because the exchange writes face ghosts only.
+#ifndef BCM_FACADE_BLOCKMANAGER_H
+#define BCM_FACADE_BLOCKMANAGER_H
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <iostream>
+#include <map>
+#include <vector>
+
+#include "BCMTools.h"
+#include "BlockBase.h"
+#include "Scalar3D.h"
+#include "Scalar3DUpdater.h"
+#include "bcm_hip.h"
+
+class VCUpdateMethod {
+ public:
+  enum Type { AtOnce, SeparateXYZ, SeparateXYZ_SeparateLevelDiff };
+};
+
+class BlockManager {
+  struct FieldRec {
+    int vc = 0;
+    int dtype = BCM_F64;
+    bcm_field_t dev = nullptr;
+    void* host = nullptr;
+    size_t bytes = 0;
+    bool updatable = false;
+    bool prepared = false;
+    bool device_resident = false;
+    void (*free_host)(void*) = nullptr;
+  };
+
+  std::vector<BlockBase*> blockList;
+  Vec3i size;
+  int numBlock = 0;
+  std::vector<bcm_block_info> table;
+  std::map<int, bcm_mesh_t> meshes;  // by ghost width
+  std::map<int, FieldRec> fields;    // by data-class id
+
+  static void check(int rc, const char* where) {
+    if (rc != BCM_OK) {
+      std::cout << "*** error: " << where << " returned " << rc << std::endl;
+      Exit(EX_FAILURE);
+    }
+  }
+
+  template <typename T>
+  static void free_array(void* p) {
+    delete[] static_cast<T*>(p);
+  }
+
+  bcm_mesh_t mesh_for(int vc) {
+    auto it = meshes.find(vc);
+    if (it != meshes.end()) return it->second;
+    if (size.x != size.y || size.x != size.z) {
+      std::cout << "*** error: the device path needs cubic blocks" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    bcm_mesh_t m = nullptr;
+    check(bcm_mesh_create(table.data(), numBlock, size.x, vc, &m), "bcm_mesh_create");
+    meshes[vc] = m;
+    return m;
+  }
+
+  FieldRec& rec(int id) {
+    auto it = fields.find(id);
+    if (it == fields.end()) {
+      std::cout << "*** error: unknown data class id " << id << std::endl;
+      Exit(EX_FAILURE);
+    }
+    return it->second;
+  }
+
+  template <typename D, typename T>
+  int add_contiguous(int vc, bool updatable) {
+    const int nx0 = size.x + 2 * vc, ny0 = size.y + 2 * vc, nz0 = size.z + 2 * vc;
+    const size_t bvol = size_t(nx0) * ny0 * nz0;
+    T* data = new T[bvol * numBlock]();
+    int id = -1;
+    for (int b = 0; b < numBlock; ++b) {
+      BlockBase* block = blockList[b];
+      D* dc = new D(block->getSize(), vc, &data[bvol * b]);
+      if (updatable) dc->setUpdater(new Scalar3DUpdater<T>(block->getNeighborInfo()));
+      const int id0 = block->setDataClass(dc);
+      if (b == 0) id = id0;
+      else if (id0 != id) {
+        std::cout << "error: DataClass register inconsistency" << std::endl;
+        Exit(EX_FAILURE);
+      }
+    }
+    FieldRec r;
+    r.vc = vc;
+    r.dtype = sizeof(T) == 8 ? BCM_F64 : BCM_F32;
+    r.host = data;
+    r.bytes = bvol * numBlock * sizeof(T);
+    r.updatable = updatable;
+    r.free_host = &free_array<T>;
+    if (updatable) check(bcm_field_create(mesh_for(vc), r.dtype, &r.dev), "bcm_field_create");
+    fields[id] = r;
+    return id;
+  }
+
+  void exchange(int id, int mask) {
+    FieldRec& r = rec(id);
+    if (!r.updatable || !r.prepared) {
+      std::cout << "*** error: data class " << id << " is not prepared for VC update" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    if (!r.device_resident) check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
+    check(bcm_ghost_fill(r.dev, mask), "bcm_ghost_fill");
+    if (!r.device_resident) check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
+  }
+
+  static int axis_bit(int xyz) { return BCM_GF_X << xyz; }
+
+ public:
+  BlockManager() {}
+  ~BlockManager() {
+    for (auto& kv : fields) {
+      if (kv.second.dev) bcm_field_destroy(kv.second.dev);
+      // Host arenas stay valid while blocks may still reference them.
+    }
+    for (auto& kv : meshes) bcm_mesh_destroy(kv.second);
+  }
+
+  static BlockManager& getInstance() {
+    static BlockManager instance;
+    return instance;
+  }
+
+  void registerBlock(BlockBase* block) {
+    blockList.push_back(block);
+    numBlock++;
+  }
+
+  // src/BlockManager.cpp:36-59 plus the device block table.
+  void endRegisterBlock() {
+    if (numBlock == 0) {
+      std::cout << "***error: number of blocks must be > 0" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    size = blockList[0]->getSize();
+    table.assign(numBlock, bcm_block_info());
+    for (int b = 0; b < numBlock; ++b) {
+      BlockBase* blk = blockList[b];
+      if (blk->getSize() != size) {
+        std::cout << "***error: block size inconsistent" << std::endl;
+        Exit(EX_FAILURE);
+      }
+      bcm_block_info& bi = table[b];
+      std::memset(&bi, 0, sizeof(bi));
+      bi.level = blk->getLevel();
+      for (int a = 0; a < 3; ++a) {
+        bi.origin[a] = blk->getOrigin()[a];
+        bi.block_size[a] = blk->getBlockSize()[a];
+      }
+      const NeighborInfo* ni = blk->getNeighborInfo();
+      for (int f = 0; f < NUM_FACE; ++f) {
+        bi.nb_level_diff[f] = ni[f].getLevelDifference();
+        for (int s = 0; s < NUM_SUBFACE; ++s) bi.nb_id[f][s] = ni[f].getID(Subface(s));
+        bi.nb_subface[f] = ni[f].getNeighborSubface();
+        bi.bc_type[f] = blk->boundaryType(f);
+        bi.bc_id[f] = blk->boundaryID(f);
+      }
+    }
+  }
+
+  const Vec3i& getSize() const { return size; }
+  int getStartID() const { return 0; }
+  int getNumBlock() const { return numBlock; }
+  BlockBase* getBlock(int localID) {
+    if (localID < 0 || localID >= numBlock) {
+      std::cout << "***error: illegal local block ID" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    return blockList[localID];
+  }
+
+  // src/BlockManager.cpp:118-287, one rank.
+  void printBlockLayoutInfo() {
+    int lmin = INT_MAX, lmax = 0;
+    for (int b = 0; b < numBlock; ++b) {
+      lmin = std::min(lmin, blockList[b]->getLevel());
+      lmax = std::max(lmax, blockList[b]->getLevel());
+    }
+    std::cout << "Block Layout Information" << std::endl;
+    std::cout << "  Min level: " << lmin << std::endl << "  Max level: " << lmax << std::endl;
+    std::cout << "  Number of blocks" << std::endl;
+    for (int l = lmin; l <= lmax; ++l) {
+      int c = 0;
+      for (int b = 0; b < numBlock; ++b) c += blockList[b]->getLevel() == l;
+      std::cout << "    L=" << l << ": " << c << std::endl;
+    }
+    std::cout << "    total: " << numBlock << std::endl;
+    int faces[3] = {0, 0, 0};
+    for (int b = 0; b < numBlock; ++b) {
+      const NeighborInfo* ni = blockList[b]->getNeighborInfo();
+      for (int f = 0; f < NUM_FACE; ++f) {
+        const int ld = ni[f].getLevelDifference();
+        if (ld == 1) {
+          for (int s = 0; s < NUM_SUBFACE; ++s) faces[2] += ni[f].getID(Subface(s)) >= 0;
+        } else if (ni[f].getID() >= 0) {
+          faces[ld + 1] += 1;
+        }
+      }
+    }
+    std::cout << "  Number of faces (dLevel -1/0/+1): " << faces[0] << "/" << faces[1] << "/" << faces[2]
+              << std::endl;
+  }
+
+  // include/BlockManager.h:202-283.  A class without an updater is host-only.
+  template <typename D>
+  int setDataClass(int vc) {
+    return add_contiguous<D, typename D::value_type>(vc, false);
+  }
+  template <typename D, typename U>
+  int setDataClass(int vc) {
+    return add_contiguous<D, typename D::value_type>(vc, true);
+  }
+  template <typename D, typename U, typename T>
+  int setDataClass(int vc) {
+    return add_contiguous<D, T>(vc, true);
+  }
+
+  void prepareForVCUpdate(int id, int /*tag*/, VCUpdateMethod::Type /*method*/) { rec(id).prepared = true; }
+  void prepareForVCUpdate(int id, int tag, bool separate = false) {
+    prepareForVCUpdate(id, tag, separate ? VCUpdateMethod::SeparateXYZ : VCUpdateMethod::AtOnce);
+  }
+
+  // include/BlockManager.h:311-726.
+  void updateVC(int id) { exchange(id, BCM_GF_ALL); }
+  void updateVC(int id, int xyz) { exchange(id, BCM_GF_ALL | axis_bit(xyz)); }
+  void updateVC_Flat(int id, int xyz) { exchange(id, BCM_GF_FLAT | axis_bit(xyz)); }
+  void updateVC_F2C(int id, int xyz) { exchange(id, BCM_GF_F2C | axis_bit(xyz)); }
+  void updateVC_C2F(int id, int xyz) { exchange(id, BCM_GF_C2F | axis_bit(xyz)); }
+  void updateVC_X(int id) { updateVC(id, 0); }
+  void updateVC_Y(int id) { updateVC(id, 1); }
+  void updateVC_Z(int id) { updateVC(id, 2); }
+  void updateVC_X_Flat(int id) { updateVC_Flat(id, 0); }
+  void updateVC_Y_Flat(int id) { updateVC_Flat(id, 1); }
+  void updateVC_Z_Flat(int id) { updateVC_Flat(id, 2); }
+  void updateVC_X_F2C(int id) { updateVC_F2C(id, 0); }
+  void updateVC_Y_F2C(int id) { updateVC_F2C(id, 1); }
+  void updateVC_Z_F2C(int id) { updateVC_F2C(id, 2); }
+  void updateVC_X_C2F(int id) { updateVC_C2F(id, 0); }
+  void updateVC_Y_C2F(int id) { updateVC_C2F(id, 1); }
+  void updateVC_Z_C2F(int id) { updateVC_C2F(id, 2); }
+  void beginUpdateVC(int id) { updateVC(id); }
+  void beginUpdateVC(int id, int xyz) { updateVC(id, xyz); }
+  void beginUpdateVC_X(int id) { updateVC(id, 0); }
+  void beginUpdateVC_Y(int id) { updateVC(id, 1); }
+  void beginUpdateVC_Z(int id) { updateVC(id, 2); }
+  void beginUpdateVC_Flat(int id, int xyz) { updateVC_Flat(id, xyz); }
+  void beginUpdateVC_F2C(int id, int xyz) { updateVC_F2C(id, xyz); }
+  void beginUpdateVC_C2F(int id, int xyz) { updateVC_C2F(id, xyz); }
+  void beginUpdateVC_X_Flat(int id) { updateVC_Flat(id, 0); }
+  void beginUpdateVC_Y_Flat(int id) { updateVC_Flat(id, 1); }
+  void beginUpdateVC_Z_Flat(int id) { updateVC_Flat(id, 2); }
+  void beginUpdateVC_X_F2C(int id) { updateVC_F2C(id, 0); }
+  void beginUpdateVC_Y_F2C(int id) { updateVC_F2C(id, 1); }
+  void beginUpdateVC_Z_F2C(int id) { updateVC_F2C(id, 2); }
+  void beginUpdateVC_X_C2F(int id) { updateVC_C2F(id, 0); }
+  void beginUpdateVC_Y_C2F(int id) { updateVC_C2F(id, 1); }
+  void beginUpdateVC_Z_C2F(int id) { updateVC_C2F(id, 2); }
+  void endUpdateVC(int) {}
+  void endUpdateVC(int, int) {}
+  void endUpdateVC_X(int) {}
+  void endUpdateVC_Y(int) {}
+  void endUpdateVC_Z(int) {}
+  void endUpdateVC_Flat(int, int) {}
+  void endUpdateVC_F2C(int, int) {}
+  void endUpdateVC_C2F(int, int) {}
+  void endUpdateVC_X_Flat(int) {}
+  void endUpdateVC_Y_Flat(int) {}
+  void endUpdateVC_Z_Flat(int) {}
+  void endUpdateVC_X_F2C(int) {}
+  void endUpdateVC_Y_F2C(int) {}
+  void endUpdateVC_Z_F2C(int) {}
+  void endUpdateVC_X_C2F(int) {}
+  void endUpdateVC_Y_C2F(int) {}
+  void endUpdateVC_Z_C2F(int) {}
+
+  // Device path.
+  bcm_field_t deviceField(int id) { return rec(id).dev; }
+  bcm_mesh_t deviceMesh(int id) { return mesh_for(rec(id).vc); }
+  void setDeviceResident(int id, bool on) {
+    FieldRec& r = rec(id);
+    if (on && !r.device_resident) check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
+    if (!on && r.device_resident) check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
+    r.device_resident = on;
+  }
+  void syncToHost(int id) {
+    FieldRec& r = rec(id);
+    check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
+  }
+  void syncToDevice(int id) {
+    FieldRec& r = rec(id);
+    check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
+  }
+  const std::vector<bcm_block_info>& blockTable() const { return table; }
+
+ private:
+  BlockManager(const BlockManager&);
+  BlockManager& operator=(const BlockManager&);
+};
+
+#endif
