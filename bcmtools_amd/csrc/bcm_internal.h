@@ -92,7 +92,20 @@ struct bcm_mesh {
 
   // Solver workspace (lazily allocated, kept for the mesh's lifetime).
   std::vector<void*> ws;
+
+  // Halo-gathering sweep: per block, 6 faces x 8 ints
+  // {kind, nb0, nb1, nb2, nb3, subface, bc_id, 0}; usable when every face
+  // has a neighbour on each subface or a Dirichlet condition.
+  int* d_desc = nullptr;
+  bool halo_ok = false;
+  // X-face column exports (columns 0, 1, n-2, n-1 of every block), two
+  // buffers for the ping-pong of iterates; bytes sized for FP64.
+  void* d_xexp[2] = {nullptr, nullptr};
 };
+
+namespace bcm {
+enum FaceKind { FD_NONE = 0, FD_COPY = 1, FD_F2C = 2, FD_C2F = 3, FD_DIR = 4 };
+}
 
 struct bcm_field {
   bcm_mesh* mesh = nullptr;
@@ -134,6 +147,13 @@ int residual_d2f_raw(bcm_mesh* m, float* r32, const double* x, const double* b, 
 int convert_raw(bcm_mesh* m, int op, void* dst, const void* src);
 // Dirichlet mirror with the boundary-value table already on the device.
 int dirichlet_device(bcm_mesh* m, bcm_field* f);
+// Halo-gathering red-black sweep A -> B (bcm_sweep.hip) and the X-face
+// column export it reads.
+template <typename T> int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout);
+template <typename T> int xexport_raw(bcm_mesh* m, const T* A, T* xout);
+// Field-sized workspace buffer kept on the mesh (bcm_solver.hip).
+void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc);
+enum WorkspaceSlot { WS_SOR_PONG_F64 = 30, WS_SOR_PONG_F32 = 31 };
 
 enum BlasOp { OP_AXPY = 0, OP_XPAY = 1, OP_AXPYZ = 2, OP_AXPBYPZ = 3, OP_COPY_INTERIOR = 4 };
 enum ConvOp { CONV_D2F = 0, CONV_ADD_F2D = 1 };

@@ -11,6 +11,7 @@
 #include <new>
 
 #include "bcm_internal.h"
+#include "bcm_interp.h"
 
 using bcm::BlockMeta;
 
@@ -111,45 +112,14 @@ struct Geo {
   }
 };
 
-// 8-cell restriction, summed in the order of include/Scalar3DUpdater.h:190-198.
 template <typename T>
 __device__ __forceinline__ T f2c_value(const T* __restrict__ f, const Geo& g, int I, int J, int K) {
-  const int i = 2 * I, j = 2 * J, k = 2 * K;
-  T s = f[g.idx(i, j, k)];
-  s = s + f[g.idx(i + 1, j, k)];
-  s = s + f[g.idx(i, j + 1, k)];
-  s = s + f[g.idx(i + 1, j + 1, k)];
-  s = s + f[g.idx(i, j, k + 1)];
-  s = s + f[g.idx(i + 1, j, k + 1)];
-  s = s + f[g.idx(i, j + 1, k + 1)];
-  s = s + f[g.idx(i + 1, j + 1, k + 1)];
-  return T(0.125 * s);
+  return f2c_sum<T>([&](int i, int j, int k) { return f[g.idx(i, j, k)]; }, I, J, K);
 }
 
-// linearInterpolate (include/Scalar3DUpdater.h:242-266): coarse base index
-// clamped to [0, n-2] and weight -0.25 + 0.5 i - I in double.
-__device__ __forceinline__ void c2f_weight(int i, int n, int& I, double& r) {
-  I = min(max(i / 2 - 1 + i % 2, 0), n - 2);
-  r = -0.25 + 0.5 * double(i) - double(I);
-}
-
-// Trilinear prolongation in the operand order of interpolateC2F
-// (include/Scalar3DUpdater.h:221-236), evaluated in double.
 template <typename T>
 __device__ __forceinline__ T c2f_value(const T* __restrict__ c, const Geo& g, int i, int j, int k) {
-  int I, J, K;
-  double r, s, t;
-  c2f_weight(i, g.n, I, r);
-  c2f_weight(j, g.n, J, s);
-  c2f_weight(k, g.n, K, t);
-  const double r1 = 1.0 - r, s1 = 1.0 - s, t1 = 1.0 - t;
-  const double c000 = c[g.idx(I, J, K)], c100 = c[g.idx(I + 1, J, K)];
-  const double c010 = c[g.idx(I, J + 1, K)], c110 = c[g.idx(I + 1, J + 1, K)];
-  const double c001 = c[g.idx(I, J, K + 1)], c101 = c[g.idx(I + 1, J, K + 1)];
-  const double c011 = c[g.idx(I, J + 1, K + 1)], c111 = c[g.idx(I + 1, J + 1, K + 1)];
-  const double lo = s1 * (r1 * c000 + r * c100) + s * (r1 * c010 + r * c110);
-  const double hi = s1 * (r1 * c001 + r * c101) + s * (r1 * c011 + r * c111);
-  return T(t1 * lo + t * hi);
+  return c2f_interp<T>([&](int a, int b, int d) { return c[g.idx(a, b, d)]; }, g.n, i, j, k);
 }
 
 template <typename T>
@@ -333,6 +303,9 @@ static void mesh_release(bcm_mesh* m) {
   dev_free(m->d_result);
   dev_free(m->d_bvalues);
   dev_free(m->d_sor_coef);
+  dev_free(m->d_desc);
+  dev_free(m->d_xexp[0]);
+  dev_free(m->d_xexp[1]);
   for (void* p : m->ws) dev_free(p);
   if (m->h_result) (void)hipHostFree(m->h_result);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -408,6 +381,33 @@ extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n,
           }
         }
       }
+    }
+    // Face descriptors of the halo-gathering sweep.
+    std::vector<int> desc(size_t(nblocks) * 48, 0);
+    m->halo_ok = true;
+    for (int b = 0; b < nblocks; ++b) {
+      const bcm_block_info& bi = blocks[b];
+      for (int f = 0; f < 6; ++f) {
+        int* d = &desc[size_t(b) * 48 + f * 8];
+        const int ld = bi.nb_level_diff[f];
+        for (int s = 0; s < 4; ++s) d[1 + s] = bi.nb_id[f][s];
+        d[5] = bi.nb_subface[f];
+        d[6] = bi.bc_id[f];
+        if (ld == 0 && bi.nb_id[f][0] >= 0) d[0] = FD_COPY;
+        else if (ld == -1 && bi.nb_id[f][0] >= 0) d[0] = FD_C2F;
+        else if (ld == 1 && bi.nb_id[f][0] >= 0 && bi.nb_id[f][1] >= 0 && bi.nb_id[f][2] >= 0 &&
+                 bi.nb_id[f][3] >= 0) d[0] = FD_F2C;
+        else if (bi.bc_type[f] == BCM_BC_DIRICHLET) d[0] = FD_DIR;
+        else {
+          d[0] = FD_NONE;
+          m->halo_ok = false;
+        }
+      }
+    }
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_desc), sizeof(int) * desc.size())) != BCM_OK) break;
+    if (hipMemcpy(m->d_desc, desc.data(), sizeof(int) * desc.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = BCM_ERR_HIP;
+      break;
     }
     std::vector<int4> all;
     for (int a = 0; a < 3; ++a) {

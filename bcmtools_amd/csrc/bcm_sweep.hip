@@ -1,0 +1,462 @@
+// Halo-gathering red-black SOR sweep.
+//
+// The reference's outer iteration is: SOR on every block with frozen
+// ghosts, then the ghost exchange (copy / F2C / C2F) and the Dirichlet
+// mirror (examples/sor2/Solver.cpp:77-133).  The ghosts seen by sweep k+1
+// are therefore a pure function of the interiors after sweep k
+// (include/Scalar3DUpdater.h:190-266, 321-347; examples/sor1/Solver.cpp:232-294).
+// This kernel runs sweep k+1 reading the iterate of sweep k from buffer A
+// and writing buffer B, and builds each block's ghost frame on the fly from
+// A with the same arithmetic as the exchange, so no separate ghost pass and
+// no ghost writes are needed between sweeps.  The X-face sources (columns
+// 0, 1, n-2, n-1 of every block) come from a compact export written by the
+// previous sweep, which keeps every read coalesced.
+#include <type_traits>
+
+#include "bcm_internal.h"
+#include "bcm_interp.h"
+
+namespace bcm {
+
+namespace {
+
+template <typename T, int N>
+struct HaloSrc {
+  const T* __restrict__ A;   // previous iterate
+  const T* __restrict__ xe;  // its X-face export [blk][4][k][j]
+  const int* desc;   // face descriptors of this work-group's block (LDS copy)
+  const double* __restrict__ bval;
+  const double* bface;  // Dirichlet value per face of this block (LDS copy)
+  static constexpr int NX = N + 2;
+  __device__ __forceinline__ T a_at(int blk, int i, int j, int k) const {
+    return A[(long long)blk * NX * NX * NX + (i + 1) + NX * ((j + 1) + NX * (k + 1))];
+  }
+  // i must be 0, 1, N-2 or N-1.
+  __device__ __forceinline__ T x_at(int blk, int i, int j, int k) const {
+    const int col = i < 2 ? i : i - (N - 4);
+    return xe[((long long)(blk * 4 + col) * N + k) * N + j];
+  }
+  __device__ __forceinline__ T at(int a, int blk, int i, int j, int k) const {
+    return a == 0 ? x_at(blk, i, j, k) : a_at(blk, i, j, k);
+  }
+  __device__ __forceinline__ const T* a_ptr(int blk, int i, int j, int k) const {
+    return A + (long long)blk * NX * NX * NX + (i + 1) + NX * ((j + 1) + NX * (k + 1));
+  }
+  __device__ __forceinline__ const T* x_ptr(int blk, int i, int j, int k) const {
+    const int col = i < 2 ? i : i - (N - 4);
+    return xe + ((long long)(blk * 4 + col) * N + k) * N + j;
+  }
+  __device__ __forceinline__ const T* ptr(int a, int blk, int i, int j, int k) const {
+    return a == 0 ? x_ptr(blk, i, j, k) : a_ptr(blk, i, j, k);
+  }
+  // Phase 1 of a ghost value of block b at frame cell c (one coordinate -1
+  // or N) on face F: the source addresses of the face kind (a single
+  // source repeated for copy / Dirichlet faces), then eight loads in
+  // straight-line code so the outstanding-load count stays exact.
+  template <int F>
+  __device__ __forceinline__ int gather(int b, int ci, int cj, int ck, T* v) const {
+    constexpr int a = F >> 1;
+    constexpr bool plus = F & 1;
+    constexpr int t0 = (a + 1) % 3, t1 = (a + 2) % 3;
+    const int* d = desc + F * 8;
+    const int kind = d[0];
+    int c[3] = {ci, cj, ck};
+    const T* base;
+    int o[8];
+    if (kind == FD_F2C) {
+      constexpr int h = N / 2;
+      const int s0 = c[t0] >= h, s1 = c[t1] >= h;
+      const int fb = d[1 + s0 + 2 * s1];
+      int I[3];
+      I[t0] = c[t0] - h * s0;
+      I[t1] = c[t1] - h * s1;
+      I[a] = plus ? 0 : h - 1;
+      base = ptr(a, fb, 2 * I[0], 2 * I[1], 2 * I[2]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        o[q] = int(ptr(a, fb, 2 * I[0] + (q & 1), 2 * I[1] + ((q >> 1) & 1), 2 * I[2] + (q >> 2)) - base);
+    } else if (kind == FD_C2F) {
+      const int cb = d[1], sf = d[5];
+      int p[3], I[3];
+      double w;
+      p[t0] = c[t0] + N * (sf & 1);
+      p[t1] = c[t1] + N * (sf >> 1);
+      p[a] = plus ? 0 : 2 * N - 1;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) c2f_weight(p[e], N, I[e], w);
+      base = ptr(a, cb, I[0], I[1], I[2]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        o[q] = int(ptr(a, cb, I[0] + (q & 1), I[1] + ((q >> 1) & 1), I[2] + (q >> 2)) - base);
+    } else {
+      if (kind == FD_COPY) {
+        c[a] = plus ? 0 : N - 1;
+        base = ptr(a, d[1], c[0], c[1], c[2]);
+      } else if (kind == FD_DIR) {
+        c[a] = plus ? N - 1 : 0;
+        base = ptr(a, b, c[0], c[1], c[2]);
+      } else {
+        base = a_ptr(b, ci, cj, ck);  // no neighbour, no condition: the stored ghost
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[q] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = base[o[q]];
+    return kind;
+  }
+  // Phase 2: the exchange / mirror arithmetic on the gathered values.
+  template <int F>
+  __device__ __forceinline__ T combine(int b, int ci, int cj, int ck, int kind, const T* v) const {
+    constexpr int a = F >> 1;
+    constexpr bool plus = F & 1;
+    constexpr int t0 = (a + 1) % 3, t1 = (a + 2) % 3;
+    if (kind == FD_DIR) return T(2.0 * bface[F] - double(v[0]));
+    if (kind == FD_F2C) return f2c_combine<T>(v);
+    if (kind == FD_C2F) {
+      const int sf = desc[F * 8 + 5];
+      const int c[3] = {ci, cj, ck};
+      int p[3], I[3];
+      double w[3], vv[8];
+      p[t0] = c[t0] + N * (sf & 1);
+      p[t1] = c[t1] + N * (sf >> 1);
+      p[a] = plus ? 0 : 2 * N - 1;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) c2f_weight(p[e], N, I[e], w[e]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) vv[q] = double(v[q]);
+      return T(c2f_combine(vv, w[0], w[1], w[2]));
+    }
+    return v[0];
+  }
+  // Branch-free forms for the X/Y frame edges (face chosen per lane at run
+  // time): addresses and weights by selects, so the eight loads are issued
+  // where they are written and consumed one z-step later.
+  __device__ __forceinline__ void place(int a, int nv, int u0, int u1, int& x, int& y, int& z) const {
+    // (normal value, coordinate along t0 = (a+1)%3, along t1 = (a+2)%3) -> (x, y, z)
+    x = a == 0 ? nv : (a == 1 ? u1 : u0);
+    y = a == 0 ? u0 : (a == 1 ? nv : u1);
+    z = a == 0 ? u1 : (a == 1 ? u0 : nv);
+  }
+  __device__ __forceinline__ int gather_rt(int face, int b, int ci, int cj, int ck, T* v) const {
+    const int a = face >> 1;
+    const bool plus = face & 1;
+    const int* d = desc + face * 8;
+    const int kind = d[0];
+    const int ct0 = a == 0 ? cj : (a == 1 ? ck : ci);
+    const int ct1 = a == 0 ? ck : (a == 1 ? ci : cj);
+    constexpr int h = N / 2;
+    int blk, x, y, z, ox[8];
+    // F2C: fine block of the subface, base cell 2I.
+    const int s0 = ct0 >= h, s1 = ct1 >= h;
+    // C2F: coarse base cell.
+    const int sf = d[5];
+    int Ix, Iy, Iz;
+    double w;
+    int px, py, pz;
+    place(a, plus ? 0 : 2 * N - 1, ct0 + N * (sf & 1), ct1 + N * (sf >> 1), px, py, pz);
+    c2f_weight(px, N, Ix, w);
+    c2f_weight(py, N, Iy, w);
+    c2f_weight(pz, N, Iz, w);
+    int fx, fy, fz;
+    place(a, plus ? 0 : h - 1, ct0 - h * s0, ct1 - h * s1, fx, fy, fz);
+    const bool f2c = kind == FD_F2C, c2f = kind == FD_C2F;
+    if (f2c) {
+      blk = d[1 + s0 + 2 * s1];
+      x = 2 * fx; y = 2 * fy; z = 2 * fz;
+    } else if (c2f) {
+      blk = d[1];
+      x = Ix; y = Iy; z = Iz;
+    } else if (kind == FD_COPY) {
+      blk = d[1];
+      place(a, plus ? 0 : N - 1, ct0, ct1, x, y, z);
+    } else if (kind == FD_DIR) {
+      blk = b;
+      place(a, plus ? N - 1 : 0, ct0, ct1, x, y, z);
+    } else {
+      blk = b;
+      x = ci; y = cj; z = ck;
+    }
+    const bool many = f2c || c2f;
+    const bool inner = kind == FD_NONE;  // the stored ghost lies outside the export columns
+    const T* base = (a == 0 && !inner) ? x_ptr(blk, x, y, z) : a_ptr(blk, x, y, z);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int dx = many ? (q & 1) : 0, dy = many ? ((q >> 1) & 1) : 0, dz = many ? (q >> 2) : 0;
+      const T* pq = (a == 0 && !inner) ? x_ptr(blk, x + dx, y + dy, z + dz) : a_ptr(blk, x + dx, y + dy, z + dz);
+      ox[q] = int(pq - base);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = base[ox[q]];
+    return kind;
+  }
+  __device__ __forceinline__ T combine_rt(int face, int ci, int cj, int ck, int kind, const T* v) const {
+    const int a = face >> 1;
+    const bool plus = face & 1;
+    const int* d = desc + face * 8;
+    const int ct0 = a == 0 ? cj : (a == 1 ? ck : ci);
+    const int ct1 = a == 0 ? ck : (a == 1 ? ci : cj);
+    const int sf = d[5];
+    int px, py, pz, I;
+    double wx, wy, wz, vv[8];
+    place(a, plus ? 0 : 2 * N - 1, ct0 + N * (sf & 1), ct1 + N * (sf >> 1), px, py, pz);
+    c2f_weight(px, N, I, wx);
+    c2f_weight(py, N, I, wy);
+    c2f_weight(pz, N, I, wz);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) vv[q] = double(v[q]);
+    const T vc2f = T(c2f_combine(vv, wx, wy, wz));
+    const T vf2c = f2c_combine<T>(v);
+    const T vdir = T(2.0 * bface[face] - double(v[0]));
+    return kind == FD_F2C ? vf2c : (kind == FD_C2F ? vc2f : (kind == FD_DIR ? vdir : v[0]));
+  }
+};
+
+template <typename T, bool SRC>
+__device__ __forceinline__ T sor_cell(T fc, T fw, T fe, T fs, T fn, T fb, T ft, T sv, T c0, T cx, T cy, T cz, T om,
+                                      T om1) {
+  if (SRC) {
+    const T fnew = c0 * (((cx * (fw + fe) + cy * (fs + fn)) + cz * (fb + ft)) - sv);
+    return om * fnew + om1 * fc;
+  } else {
+    return ((c0 * fc + cx * (fw + fe)) + cy * (fs + fn)) + cz * (fb + ft);
+  }
+}
+
+}  // namespace
+
+// One block per work-group, N^2/2 threads; thread t owns the cell pair
+// (2pi..2pi+1, pj) of every z-plane (one red, one black) and threads t < 4N own one cell
+// of the X/Y ghost frame.  The thread's own cells of planes k-2..k+2 sit in
+// registers (vertical neighbours never go through LDS); LDS holds three
+// ghosted planes for the in-plane neighbours, so one barrier per z-step
+// suffices:
+//   step k: combine + commit plane k (own old values, frame); issue the
+//   loads of plane k+2 (own cells), of plane k+1's frame and source;
+//   barrier; red cells of plane k (in-plane from LDS, vertical from
+//   registers; red results also to LDS); black cells of plane k-1 (in-plane
+//   red values from LDS); write plane k-1 to B (and its X-face columns).
+// Every LDS cell is written by one thread; slot k%3 is rewritten at step
+// k+3, after the barrier of step k+1 and k+2.
+template <typename T, int N, bool SRC>
+__global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_rbsor_halo(
+    const T* __restrict__ A, T* __restrict__ B, const T* __restrict__ s, const double* __restrict__ coef,
+    double omega, const int* __restrict__ desc, const T* __restrict__ xin, T* __restrict__ xout,
+    const double* __restrict__ bval) {
+  constexpr int NX = N + 2;
+  constexpr int PL = NX * NX;
+  constexpr int HALF = N / 2;
+  __shared__ T lds[3][PL];
+  __shared__ int bdesc[48];
+  __shared__ double bface[6];
+  const int b = blockIdx.x;
+  if (threadIdx.x < 48) bdesc[threadIdx.x] = desc[b * 48 + threadIdx.x];
+  if (threadIdx.x < 6) {
+    const int id = desc[b * 48 + threadIdx.x * 8 + 6];
+    bface[threadIdx.x] = desc[b * 48 + threadIdx.x * 8] == FD_DIR ? bval[id] : 0.0;
+  }
+  __syncthreads();
+  const HaloSrc<T, N> src{A, xin, bdesc, bval, bface};
+  const long long base = (long long)b * PL * NX;
+  const T* __restrict__ Ab = A + base;
+  T* __restrict__ Bb = B + base;
+  const T* __restrict__ sb = SRC ? s + base : nullptr;
+  const T c0 = T(coef[4 * b + 0]), cx = T(coef[4 * b + 1]), cy = T(coef[4 * b + 2]), cz = T(coef[4 * b + 3]);
+  const T om = T(omega), om1 = T(1.0 - omega);
+  const int t = threadIdx.x;
+  const int i0 = 2 * (t % HALF), j0 = t / HALF;
+  const bool framer = t < 4 * N;
+  const int edge = t / N, u = t % N;
+  const int fci = edge == 0 ? -1 : (edge == 1 ? N : u);
+  const int fcj = edge < 2 ? u : (edge == 2 ? -1 : N);
+  auto L = [](int i, int j) __attribute__((always_inline)) { return (j + 1) * NX + (i + 1); };
+  auto LQ = [&](int q) __attribute__((always_inline)) { return (j0 + 1) * NX + (i0 + q + 1); };
+
+  // Own cells of plane p: interior rows of A, or the gathered z ghost plane.
+  auto load_own = [&](int p, T* o) __attribute__((always_inline)) {
+    if (p >= 0 && p < N) {
+      const T* row = Ab + (long long)(p + 1) * PL;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) o[q] = row[LQ(q)];
+    } else {
+#pragma unroll 1
+      for (int q = 0; q < 2; ++q) {
+        T v[8], w;
+        const int i = i0 + q, j = j0;
+        if (p < 0) {
+          const int kd = src.template gather<BCM_Z_M>(b, i, j, p, v);
+          w = src.template combine<BCM_Z_M>(b, i, j, p, kd, v);
+        } else {
+          const int kd = src.template gather<BCM_Z_P>(b, i, j, p, v);
+          w = src.template combine<BCM_Z_P>(b, i, j, p, kd, v);
+        }
+        o[0] = q == 0 ? w : o[0];
+        o[1] = q == 1 ? w : o[1];
+      }
+    }
+  };
+  auto load_s = [&](int p, T* sv) __attribute__((always_inline)) {
+    if (!SRC) return;
+    const T* row = sb + (long long)(p + 1) * PL;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) sv[q] = row[LQ(q)];
+  };
+  T fv[8];
+  int fkind = 0;
+  auto gather_frame = [&](int p) __attribute__((always_inline)) {
+    if (framer) fkind = src.gather_rt(edge, b, fci, fcj, p, fv);
+  };
+  auto put_frame = [&](int p) __attribute__((always_inline)) {
+    if (framer) lds[p % 3][L(fci, fcj)] = src.combine_rt(edge, fci, fcj, p, fkind, fv);
+  };
+  auto commit = [&](int p, const T* o) __attribute__((always_inline)) {
+    const int S = p % 3;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) lds[S][LQ(q)] = o[q];
+  };
+  // Red cells of plane p: in-plane black values from LDS, vertical from the
+  // own cells of planes p-1 (dn) and p+1 (up).
+  auto red = [&](auto par, int p, T* o, const T* dn, const T* up, const T* sv) __attribute__((always_inline)) {
+    constexpr int PAR = decltype(par)::value;
+    const int S = p % 3;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (((q + j0 + PAR) & 1) != 0) continue;
+      const int c = LQ(q);
+      o[q] = sor_cell<T, SRC>(o[q], lds[S][c - 1], lds[S][c + 1], lds[S][c - NX], lds[S][c + NX], dn[q], up[q],
+                              SRC ? sv[q] : T(0), c0, cx, cy, cz, om, om1);
+      lds[S][c] = o[q];
+    }
+  };
+  auto black = [&](auto par, int p, T* o, const T* dn, const T* up, const T* sv) __attribute__((always_inline)) {
+    constexpr int PAR = decltype(par)::value;
+    const int S = p % 3;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (((q + j0 + PAR) & 1) != 1) continue;
+      const int c = LQ(q);
+      o[q] = sor_cell<T, SRC>(o[q], lds[S][c - 1], lds[S][c + 1], lds[S][c - NX], lds[S][c + NX], dn[q], up[q],
+                              SRC ? sv[q] : T(0), c0, cx, cy, cz, om, om1);
+    }
+  };
+  auto store = [&](int p, const T* o) __attribute__((always_inline)) {
+    T* dst = Bb + (long long)(p + 1) * PL;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      dst[LQ(q)] = o[q];
+      if (i0 == 0 || i0 == N - 2) {
+        const int i = i0 + q, j = j0;
+        const int colx = i < 2 ? i : i - (N - 4);
+        xout[((long long)(b * 4 + colx) * N + p) * N + j] = o[q];
+      }
+    }
+  };
+
+  // Register window: own cells of planes k-2 .. k+3 (loads of plane k+3
+  // issued at step k), source of planes k-1 .. k+2 (k+2 issued at step k).
+  // The frame of plane k+1 is combined and written after step k's barrier
+  // from loads issued after step k-1's barrier.  The loop is unrolled by two
+  // so each cell's colour is known at compile time.
+  T um2[2] = {T(0), T(0)}, um1[2], u0[2], up1[2], up2[2], up3[2];
+  T sp[2] = {T(0), T(0)}, sc[2] = {T(0), T(0)}, sn[2] = {T(0), T(0)}, snn[2] = {T(0), T(0)};
+  T zp[2];
+  load_own(N, zp);  // z ghost planes (own positions) up front
+  load_own(-1, um1);
+  load_own(0, u0);
+  load_own(1, up1);
+  load_own(2, up2);
+  load_s(0, sc);
+  load_s(1, sn);
+  gather_frame(0);
+  put_frame(0);
+  gather_frame(1);
+  auto step = [&](auto par, int k) __attribute__((always_inline)) {
+    constexpr int PAR = decltype(par)::value;  // k % 2
+    commit(k, u0);
+    if (k + 3 < N) {
+      const T* row = Ab + (long long)(k + 4) * PL;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) up3[q] = row[LQ(q)];
+    } else if (k + 3 == N) {
+      up3[0] = zp[0];
+      up3[1] = zp[1];
+    }
+    if (k + 2 < N) load_s(k + 2, snn);
+    __syncthreads();
+    if (k + 1 < N) put_frame(k + 1);
+    if (k + 2 < N) gather_frame(k + 2);
+    red(std::integral_constant<int, PAR>(), k, u0, um1, up1, sc);
+    if (k >= 1) {
+      black(std::integral_constant<int, 1 - PAR>(), k - 1, um1, um2, u0, sp);
+      store(k - 1, um1);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      um2[q] = um1[q];
+      um1[q] = u0[q];
+      u0[q] = up1[q];
+      up1[q] = up2[q];
+      up2[q] = up3[q];
+      sp[q] = sc[q];
+      sc[q] = sn[q];
+      sn[q] = snn[q];
+    }
+  };
+  // Fully unrolled: the window rotation becomes register renaming, so no
+  // copy waits for a load that is still in flight.
+#pragma unroll
+  for (int k = 0; k < N; k += 2) {
+    step(std::integral_constant<int, 0>(), k);
+    step(std::integral_constant<int, 1>(), k + 1);
+  }
+  // um1 = plane N-1, um2 = plane N-2, u0 = plane N (ghost); sp = s(N-1).
+  __syncthreads();
+  black(std::integral_constant<int, (N - 1) & 1>(), N - 1, um1, um2, u0, sp);
+  store(N - 1, um1);
+}
+
+// X-face columns 0, 1, N-2, N-1 of every block of `A` into `xout`.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_xexport(const T* __restrict__ A, T* __restrict__ xout) {
+  constexpr int NX = N + 2;
+  const int b = blockIdx.x;
+  const T* Ab = A + (long long)b * NX * NX * NX;
+  for (int e = threadIdx.x; e < 4 * N * N; e += blockDim.x) {
+    const int j = e % N, k = (e / N) % N, col = e / (N * N);
+    const int i = col < 2 ? col : col + (N - 4);
+    xout[((long long)(b * 4 + col) * N + k) * N + j] = Ab[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
+  }
+}
+
+template <typename T>
+int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout) {
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_SOR, &ev));
+  const int nb = m->nblocks;
+  if (m->n == 32) {
+    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 32, true>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
+    else hipLaunchKernelGGL((k_rbsor_halo<T, 32, false>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
+  } else if (m->n == 16) {
+    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 16, true>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
+    else hipLaunchKernelGGL((k_rbsor_halo<T, 16, false>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
+  } else {
+    return BCM_ERR_ARG;
+  }
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+template int halo_sweep_raw<double>(bcm_mesh*, const double*, double*, const double*, double, const double*, double*);
+template int halo_sweep_raw<float>(bcm_mesh*, const float*, float*, const float*, double, const float*, float*);
+
+template <typename T>
+int xexport_raw(bcm_mesh* m, const T* A, T* xout) {
+  if (m->n == 32) hipLaunchKernelGGL((k_xexport<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, A, xout);
+  else if (m->n == 16) hipLaunchKernelGGL((k_xexport<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, A, xout);
+  else return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+template int xexport_raw<double>(bcm_mesh*, const double*, double*);
+template int xexport_raw<float>(bcm_mesh*, const float*, float*);
+
+}  // namespace bcm

@@ -176,12 +176,39 @@ def bench_jacobi(args):
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
 
 
+def bench_copy(args):
+    """Plain device copy of one config-2-sized FP64 field (calibrates HBM counters)."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    t = W.config2_tree()
+    mesh = Mesh(t.blocks, 32, 1)
+    a, b = mesh.field(np.float64), mesh.field(np.float64)
+    nbytes = len(t.blocks) * 34 ** 3 * 8
+    for _ in range(args.warmup):
+        mesh.copy(b, a)
+    mesh.sync()
+    mesh.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mesh.copy(b, a)
+    mesh.sync()
+    el = time.perf_counter() - t0
+    ms, nl = mesh.kernel_time(L.K_BLAS1)
+    gbs = 2.0 * nbytes / (ms / max(nl, 1) * 1e-3) / 1e9
+    return {"metric": "device copy bandwidth (read + write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "copy of one 3816 x 34^3 FP64 field", "bytes_per_copy": nbytes}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi"])
+    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi", "copy"])
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -189,7 +216,7 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = bench_sor(args) if args.workload == "sor" else bench_jacobi(args)
+    res = {"sor": bench_sor, "jacobi": bench_jacobi, "copy": bench_copy}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -283,3 +283,53 @@ def test_full_size_config2_ghost_fill_vs_oracle():
     assert np.array_equal(out, f)
     mesh.update_vc(gf)
     assert np.array_equal(gf.download(), out)
+
+
+@pytest.mark.parametrize("n,maker,form", [
+    (16, lambda: W.shell_tree(3), 2),
+    (32, lambda: W.shell_tree(2), 1),
+    (16, lambda: W.Tree(root_n=(1, 1, 1), periodic=(0, 1, 1), divider=L.DIV_SIMPLE, min_level=1, max_level=3,
+                        bc_setter=L.BCSET_SOR1_X), 1),
+    (16, lambda: W.random_tree(5), 2),
+])
+def test_halo_gathering_sor_iterations_vs_oracle(n, maker, form):
+    """Iterations after the first run the halo-gathering sweep (ghosts built
+    in the kernel from the previous iterate); 6 iterations must equal the
+    reference-order oracle bit for bit, every cell."""
+    t = maker()
+    blocks = t.blocks
+    vals = [0.25, 1.5] if blocks["bc_id"].max() >= 1 else [0.25]
+    case(f"halo sor n={n} form=sor{form} nb={len(blocks)}")
+    rng = np.random.default_rng(n + len(blocks))
+    f0 = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2))
+    s = W.rhs_field(blocks, n, 1, seed=4) if form == 2 else None
+    mesh = Mesh(blocks, n, 1)
+    f = mesh.field(np.float64, f0)
+    sf = mesh.field(np.float64, s) if s is not None else None
+    mesh.sor_iterate(f, sf, 1.25, vals, 6)
+    ref = f0.copy()
+    oracle.dirichlet(blocks, n, 1, ref, vals)
+    oracle.sor_iterate(blocks, n, 1, ref, s, 1.25, vals, 6)
+    np.testing.assert_array_equal(f.download(), ref)
+    # A second call continues from the stored state.
+    mesh.sor_iterate(f, sf, 1.25, vals, 3)
+    oracle.sor_iterate(blocks, n, 1, ref, s, 1.25, vals, 3)
+    np.testing.assert_array_equal(f.download(), ref)
+
+
+def test_full_size_config2_sor_vs_oracle():
+    """Config 2 at full size: 4 outer iterations (1 classic + 3 halo-gathering) bit-exact vs the oracle."""
+    t = W.config2_tree()
+    blocks, n = t.blocks, 32
+    case(f"full-size config2 sor nb={len(blocks)}")
+    s = -W.rhs_field(blocks, n, 1, seed=0)
+    mesh = Mesh(blocks, n, 1)
+    f = mesh.field(np.float64)
+    sf = mesh.field(np.float64, s)
+    mesh.dirichlet_mirror(f, [0.0])
+    mesh.sor_iterate(f, sf, 1.6, [0.0], 4)
+    out = f.download()
+    ref = np.zeros_like(s)
+    oracle.dirichlet(blocks, n, 1, ref, [0.0])
+    oracle.sor_iterate(blocks, n, 1, ref, s, 1.6, [0.0], 4)
+    assert np.array_equal(out, ref)
