@@ -129,86 +129,129 @@ struct HaloSrc {
     }
     return v[0];
   }
-  // Branch-free forms for the X/Y frame edges (face chosen per lane at run
-  // time): addresses and weights by selects, so the eight loads are issued
-  // where they are written and consumed one z-step later.
-  __device__ __forceinline__ void place(int a, int nv, int u0, int u1, int& x, int& y, int& z) const {
-    // (normal value, coordinate along t0 = (a+1)%3, along t1 = (a+2)%3) -> (x, y, z)
-    x = a == 0 ? nv : (a == 1 ? u1 : u0);
-    y = a == 0 ? u0 : (a == 1 ? nv : u1);
-    z = a == 0 ? u1 : (a == 1 ? u0 : nv);
+  // X/Y frame edges.  A frame cell's sources move along z in a fixed
+  // pattern, so each thread resolves its face once (block, base cell,
+  // strides) and every plane then costs an index select and one address
+  // add per value.
+  //
+  // z-index of plane p inside the source block: the plane itself for copy /
+  // Dirichlet / stored ghosts, 2(p - N/2 [p >= N/2]) in the fine block of an
+  // F2C face, the clamped coarse index of fine row p + N*sfz for C2F.
+  static __device__ __forceinline__ int z_index(int kind, int zoff, int p) {
+    constexpr int h = N / 2;
+    const int zf = 2 * (p - h * (p >= h));
+    const int zc0 = min(max((p + 1) / 2 - 1, 0), N - 2);
+    const int zc1 = min(max((p + N + 1) / 2 - 1, 0), N - 2);
+    return kind == FD_F2C ? zf : (kind == FD_C2F ? (zoff ? zc1 : zc0) : p);
   }
-  __device__ __forceinline__ int gather_rt(int face, int b, int ci, int cj, int ck, T* v) const {
-    const int a = face >> 1;
+  template <int VPT>
+  struct FrameLoad {
+    const T* base[2];  // source of z-index 0, planes p < N/2 and p >= N/2
+    int off[VPT];      // offsets of this thread's values from the base cell
+    int zs;            // stride of one z-index
+    int kind;
+    int zoff;          // C2F: N * (subface bit along z)
+  };
+  template <int VPT>
+  __device__ __forceinline__ FrameLoad<VPT> frame_setup(int face, int b, int ci, int cj, int q0) const {
+    constexpr int h = N / 2;
+    FrameLoad<VPT> fl;
+    const int a = face >> 1;  // 0: X face (tangents y, z), 1: Y face (tangents z, x)
     const bool plus = face & 1;
     const int* d = desc + face * 8;
     const int kind = d[0];
-    const int ct0 = a == 0 ? cj : (a == 1 ? ck : ci);
-    const int ct1 = a == 0 ? ck : (a == 1 ? ci : cj);
-    constexpr int h = N / 2;
-    int blk, x, y, z, ox[8];
-    // F2C: fine block of the subface, base cell 2I.
-    const int s0 = ct0 >= h, s1 = ct1 >= h;
-    // C2F: coarse base cell.
+    const int cin = a == 0 ? cj : ci;  // in-plane tangential coordinate
     const int sf = d[5];
-    int Ix, Iy, Iz;
+    const int sin_ = cin >= h;
+    const int sfin = a == 0 ? (sf & 1) : (sf >> 1);
+    const int sfz = a == 0 ? (sf >> 1) : (sf & 1);
+    int nrm, inp, blk0, blk1;
     double w;
-    int px, py, pz;
-    place(a, plus ? 0 : 2 * N - 1, ct0 + N * (sf & 1), ct1 + N * (sf >> 1), px, py, pz);
-    c2f_weight(px, N, Ix, w);
-    c2f_weight(py, N, Iy, w);
-    c2f_weight(pz, N, Iz, w);
-    int fx, fy, fz;
-    place(a, plus ? 0 : h - 1, ct0 - h * s0, ct1 - h * s1, fx, fy, fz);
-    const bool f2c = kind == FD_F2C, c2f = kind == FD_C2F;
-    if (f2c) {
-      blk = d[1 + s0 + 2 * s1];
-      x = 2 * fx; y = 2 * fy; z = 2 * fz;
-    } else if (c2f) {
-      blk = d[1];
-      x = Ix; y = Iy; z = Iz;
+    if (kind == FD_F2C) {
+      nrm = plus ? 0 : N - 2;
+      inp = 2 * (cin - h * sin_);
+      // subface (s0, s1): X faces s0 along y, s1 along z; Y faces s0 along z, s1 along x
+      blk0 = a == 0 ? d[1 + sin_] : d[1 + 2 * sin_];
+      blk1 = a == 0 ? d[3 + sin_] : d[2 + 2 * sin_];
+    } else if (kind == FD_C2F) {
+      c2f_weight(plus ? 0 : 2 * N - 1, N, nrm, w);
+      c2f_weight(cin + N * sfin, N, inp, w);
+      blk0 = blk1 = d[1];
     } else if (kind == FD_COPY) {
-      blk = d[1];
-      place(a, plus ? 0 : N - 1, ct0, ct1, x, y, z);
+      nrm = plus ? 0 : N - 1;
+      inp = cin;
+      blk0 = blk1 = d[1];
     } else if (kind == FD_DIR) {
-      blk = b;
-      place(a, plus ? N - 1 : 0, ct0, ct1, x, y, z);
+      nrm = plus ? N - 1 : 0;
+      inp = cin;
+      blk0 = blk1 = b;
     } else {
-      blk = b;
-      x = ci; y = cj; z = ck;
+      nrm = a == 0 ? ci : cj;  // the stored ghost itself
+      inp = cin;
+      blk0 = blk1 = b;
     }
-    const bool many = f2c || c2f;
-    const bool inner = kind == FD_NONE;  // the stored ghost lies outside the export columns
-    const T* base = (a == 0 && !inner) ? x_ptr(blk, x, y, z) : a_ptr(blk, x, y, z);
+    const int x = a == 0 ? nrm : inp, y = a == 0 ? inp : nrm;
+    const bool viax = a == 0 && kind != FD_NONE;  // X faces read the column export
+    fl.base[0] = viax ? x_ptr(blk0, x, y, 0) : a_ptr(blk0, x, y, 0);
+    fl.base[1] = viax ? x_ptr(blk1, x, y, 0) : a_ptr(blk1, x, y, 0);
+    fl.zs = viax ? N : NX * NX;
+    const int xs = viax ? N * N : 1, ys = viax ? 1 : NX;
+    const bool many = kind == FD_F2C || kind == FD_C2F;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int dx = many ? (q & 1) : 0, dy = many ? ((q >> 1) & 1) : 0, dz = many ? (q >> 2) : 0;
-      const T* pq = (a == 0 && !inner) ? x_ptr(blk, x + dx, y + dy, z + dz) : a_ptr(blk, x + dx, y + dy, z + dz);
-      ox[q] = int(pq - base);
+    for (int e = 0; e < VPT; ++e) {
+      const int q = q0 + e;
+      fl.off[e] = many ? (q & 1) * xs + ((q >> 1) & 1) * ys + (q >> 2) * fl.zs : 0;
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = base[ox[q]];
-    return kind;
+    fl.kind = kind;
+    fl.zoff = kind == FD_C2F ? N * sfz : 0;
+    return fl;
   }
-  __device__ __forceinline__ T combine_rt(int face, int ci, int cj, int ck, int kind, const T* v) const {
+  template <int VPT>
+  static __device__ __forceinline__ void frame_load(const FrameLoad<VPT>& fl, int p, T* v) {
+    const T* bp = fl.base[p >= N / 2];
+    const int zo = z_index(fl.kind, fl.zoff, p) * fl.zs;
+#pragma unroll
+    for (int e = 0; e < VPT; ++e) v[e] = bp[fl.off[e] + zo];
+  }
+  // Combiner of one frame cell: kind, the C2F weights along x and y, the
+  // Dirichlet term 2 * value.
+  struct FrameComb {
+    int kind, zoff;
+    double wx, wy, dir2;
+  };
+  __device__ __forceinline__ FrameComb comb_setup(int face, int ci, int cj) const {
+    FrameComb fc;
     const int a = face >> 1;
     const bool plus = face & 1;
     const int* d = desc + face * 8;
-    const int ct0 = a == 0 ? cj : (a == 1 ? ck : ci);
-    const int ct1 = a == 0 ? ck : (a == 1 ? ci : cj);
     const int sf = d[5];
-    int px, py, pz, I;
-    double wx, wy, wz, vv[8];
-    place(a, plus ? 0 : 2 * N - 1, ct0 + N * (sf & 1), ct1 + N * (sf >> 1), px, py, pz);
-    c2f_weight(px, N, I, wx);
-    c2f_weight(py, N, I, wy);
-    c2f_weight(pz, N, I, wz);
+    const int cin = a == 0 ? cj : ci;
+    const int sfin = a == 0 ? (sf & 1) : (sf >> 1);
+    const int sfz = a == 0 ? (sf >> 1) : (sf & 1);
+    int I;
+    double wn, win;
+    c2f_weight(plus ? 0 : 2 * N - 1, N, I, wn);
+    c2f_weight(cin + N * sfin, N, I, win);
+    fc.kind = d[0];
+    fc.zoff = N * sfz;
+    fc.wx = a == 0 ? wn : win;
+    fc.wy = a == 0 ? win : wn;
+    fc.dir2 = 2.0 * bface[face];
+    return fc;
+  }
+  static __device__ __forceinline__ T frame_combine(const FrameComb& fc, int p, const T* v) {
+    if (fc.kind == FD_F2C) return f2c_combine<T>(v);
+    if (fc.kind == FD_C2F) {
+      int I;
+      double w0, w1, vv[8];
+      c2f_weight(p, N, I, w0);
+      c2f_weight(p + N, N, I, w1);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) vv[q] = double(v[q]);
-    const T vc2f = T(c2f_combine(vv, wx, wy, wz));
-    const T vf2c = f2c_combine<T>(v);
-    const T vdir = T(2.0 * bface[face] - double(v[0]));
-    return kind == FD_F2C ? vf2c : (kind == FD_C2F ? vc2f : (kind == FD_DIR ? vdir : v[0]));
+      for (int q = 0; q < 8; ++q) vv[q] = double(v[q]);
+      return T(c2f_combine(vv, fc.wx, fc.wy, fc.zoff ? w1 : w0));
+    }
+    if (fc.kind == FD_DIR) return T(fc.dir2 - double(v[0]));
+    return v[0];
   }
 };
 
@@ -226,18 +269,20 @@ __device__ __forceinline__ T sor_cell(T fc, T fw, T fe, T fs, T fn, T fb, T ft, 
 }  // namespace
 
 // One block per work-group, N^2/2 threads; thread t owns the cell pair
-// (2pi..2pi+1, pj) of every z-plane (one red, one black) and threads t < 4N own one cell
-// of the X/Y ghost frame.  The thread's own cells of planes k-2..k+2 sit in
-// registers (vertical neighbours never go through LDS); LDS holds three
-// ghosted planes for the in-plane neighbours, so one barrier per z-step
+// (2pi..2pi+1, pj) of every z-plane (one red, one black).  The thread's own
+// cells of planes k-2..k+3 sit in registers (vertical neighbours never go
+// through LDS); LDS holds three ghosted planes for the in-plane neighbours
+// and two staging planes for the X/Y ghost frame, so one barrier per z-step
 // suffices:
-//   step k: combine + commit plane k (own old values, frame); issue the
-//   loads of plane k+2 (own cells), of plane k+1's frame and source;
-//   barrier; red cells of plane k (in-plane from LDS, vertical from
-//   registers; red results also to LDS); black cells of plane k-1 (in-plane
-//   red values from LDS); write plane k-1 to B (and its X-face columns).
+//   step k: commit plane k (own old values); park the frame sources of plane
+//   k+1; issue the loads of plane k+3 (own cells and frame sources) and of
+//   the source term of plane k+2; barrier; combine the frame of plane k+1;
+//   red cells of plane k (in-plane from LDS, vertical from registers; red
+//   results also to LDS); black cells of plane k-1 (in-plane red values from
+//   LDS); write plane k-1 to B (and its X-face columns).
 // Every LDS cell is written by one thread; slot k%3 is rewritten at step
-// k+3, after the barrier of step k+1 and k+2.
+// k+3, after the barrier of step k+1 and k+2; staging slot p&1 is rewritten
+// two steps after it was read.
 template <typename T, int N, bool SRC>
 __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_rbsor_halo(
     const T* __restrict__ A, T* __restrict__ B, const T* __restrict__ s, const double* __restrict__ coef,
@@ -247,6 +292,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int PL = NX * NX;
   constexpr int HALF = N / 2;
   __shared__ T lds[3][PL];
+  __shared__ T stage[2][8][4 * N];
   __shared__ int bdesc[48];
   __shared__ double bface[6];
   const int b = blockIdx.x;
@@ -301,13 +347,30 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
     for (int q = 0; q < 2; ++q) sv[q] = row[LQ(q)];
   };
-  T fv[8];
-  int fkind = 0;
-  auto gather_frame = [&](int p) __attribute__((always_inline)) {
-    if (framer) fkind = src.gather_rt(edge, b, fci, fcj, p, fv);
+  // X/Y ghost frame: 4N cells per plane, eight source values each, spread
+  // over all threads: thread t loads values VPT*gq .. VPT*gq+VPT-1 of frame
+  // cell gc (lanes of a wave take consecutive cells of one edge).  A plane's
+  // loads are issued three steps before its red cells need it, parked in
+  // LDS one step before, and combined by threads t < 4N.
+  constexpr int VPT = 64 / N;
+  const int gq = t / (4 * N), gc = t % (4 * N);
+  const int gedge = gc / N, gu = gc % N;
+  const int gci = gedge == 0 ? -1 : (gedge == 1 ? N : gu);
+  const int gcj = gedge < 2 ? gu : (gedge == 2 ? -1 : N);
+  const auto fload = src.template frame_setup<VPT>(gedge, b, gci, gcj, VPT * gq);
+  const auto fcomb = src.comb_setup(framer ? edge : 0, fci, fcj);
+  auto gather_frame = [&](int p, T* v) __attribute__((always_inline)) { src.frame_load(fload, p, v); };
+  auto park_frame = [&](int p, const T* v) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < VPT; ++e) stage[p & 1][VPT * gq + e][gc] = v[e];
   };
   auto put_frame = [&](int p) __attribute__((always_inline)) {
-    if (framer) lds[p % 3][L(fci, fcj)] = src.combine_rt(edge, fci, fcj, p, fkind, fv);
+    if (framer) {
+      T v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = stage[p & 1][q][t];
+      lds[p % 3][L(fci, fcj)] = src.frame_combine(fcomb, p, v);
+    }
   };
   auto commit = [&](int p, const T* o) __attribute__((always_inline)) {
     const int S = p % 3;
@@ -339,8 +402,13 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
                               SRC ? sv[q] : T(0), c0, cx, cy, cz, om, om1);
     }
   };
+  // Plane p to B: own cells, their X-face columns to the export, and the
+  // frame (the ghost exchange of A, which only unconditioned faces read
+  // back: there it is the stored ghost itself) so that whole rows, and with
+  // them whole cache lines, are written.
   auto store = [&](int p, const T* o) __attribute__((always_inline)) {
     T* dst = Bb + (long long)(p + 1) * PL;
+    if (framer) dst[L(fci, fcj)] = lds[p % 3][L(fci, fcj)];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       dst[LQ(q)] = o[q];
@@ -353,10 +421,9 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   };
 
   // Register window: own cells of planes k-2 .. k+3 (loads of plane k+3
-  // issued at step k), source of planes k-1 .. k+2 (k+2 issued at step k).
-  // The frame of plane k+1 is combined and written after step k's barrier
-  // from loads issued after step k-1's barrier.  The loop is unrolled by two
-  // so each cell's colour is known at compile time.
+  // issued at step k), source of planes k-1 .. k+2 (k+2 issued at step k),
+  // frame sources of planes k+1 .. k+3 (fr1 .. fr3).  The loop is unrolled
+  // so each cell's colour and each plane's frame index are compile-time.
   T um2[2] = {T(0), T(0)}, um1[2], u0[2], up1[2], up2[2], up3[2];
   T sp[2] = {T(0), T(0)}, sc[2] = {T(0), T(0)}, sn[2] = {T(0), T(0)}, snn[2] = {T(0), T(0)};
   T zp[2];
@@ -367,12 +434,17 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   load_own(2, up2);
   load_s(0, sc);
   load_s(1, sn);
-  gather_frame(0);
+  T fr1[VPT], fr2[VPT], fr3[VPT];
+  gather_frame(0, fr1);
+  park_frame(0, fr1);
+  gather_frame(1, fr1);
+  gather_frame(2, fr2);
+  __syncthreads();
   put_frame(0);
-  gather_frame(1);
   auto step = [&](auto par, int k) __attribute__((always_inline)) {
     constexpr int PAR = decltype(par)::value;  // k % 2
     commit(k, u0);
+    if (k + 1 < N) park_frame(k + 1, fr1);
     if (k + 3 < N) {
       const T* row = Ab + (long long)(k + 4) * PL;
 #pragma unroll
@@ -382,9 +454,9 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
       up3[1] = zp[1];
     }
     if (k + 2 < N) load_s(k + 2, snn);
+    if (k + 3 < N) gather_frame(k + 3, fr3);
     __syncthreads();
     if (k + 1 < N) put_frame(k + 1);
-    if (k + 2 < N) gather_frame(k + 2);
     red(std::integral_constant<int, PAR>(), k, u0, um1, up1, sc);
     if (k >= 1) {
       black(std::integral_constant<int, 1 - PAR>(), k - 1, um1, um2, u0, sp);
@@ -400,6 +472,11 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
       sp[q] = sc[q];
       sc[q] = sn[q];
       sn[q] = snn[q];
+    }
+#pragma unroll
+    for (int e = 0; e < VPT; ++e) {
+      fr1[e] = fr2[e];
+      fr2[e] = fr3[e];
     }
   };
   // Fully unrolled: the window rotation becomes register renaming, so no
