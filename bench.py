@@ -11,9 +11,12 @@ red-black SOR on every block, Dirichlet mirror, ghost fill (same-level
 copy + F2C + C2F), Dirichlet mirror.  Inputs are resident in HBM before
 the timed region.  value = interior cell updates per second (GCUPS).
 
-roofline: the dominant kernel is the fused red-black sweep; its
-algorithmic traffic is 24 B per interior cell (read u and s, write u,
-FP64), timed with HIP events on the mesh stream inside the timed region.
+roofline: the dominant kernel is the halo-gathering red-black sweep
+(k_rbsor_halo: one sweep reading iterate A, writing iterate B, ghost frame
+built from the neighbours' interiors); its algorithmic traffic is 24 B per
+interior cell (read u and s, write u, FP64), timed with HIP events on the
+mesh stream inside the timed region (the first of the K steps is the
+stored-ghost sweep k_rbsor_fused, which the same timer includes).
 
 cpu_baseline: the oracle (C, OpenMP, -O3 -ffp-contract=off) runs the same
 step on the same mesh for a bounded number of steps on the host cores.
@@ -110,8 +113,8 @@ def bench_sor(args):
                                "+ Dirichlet + ghost fill per step", "blocks": len(blocks), "block_cells": n,
                    "interior_cells": ncell, "omega": omega},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("rbsor_fused_n32"),
-                     "kernel": "k_rbsor_fused<double,32,true>", "avg_ms": sor_avg * 1e3,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("rbsor_halo_n32"),
+                     "kernel": "k_rbsor_halo<double,32,true>", "avg_ms": sor_avg * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes},
         "kernels_ms_per_step": {"rbsor": sor_ms / max(args.steps, 1), "ghost_fill": gh_ms / max(args.steps, 1),
                                 "dirichlet": bc_ms / max(args.steps, 1)},
