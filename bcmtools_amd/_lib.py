@@ -27,6 +27,9 @@ ORDER_Z, ORDER_HILBERT = range(2)
 BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z = range(4)
 GF_FLAT, GF_F2C, GF_C2F, GF_ALL = 1, 2, 4, 7
 GF_X, GF_Y, GF_Z = 8, 16, 32
+GF_BLOCKWISE = 64
+PREC_FP64, PREC_MIXED, PREC_FP32 = range(3)
+SOLVE_ACTIVE, SOLVE_CONVERGED, SOLVE_NOCONV, SOLVE_FAILED = range(4)
 (K_GHOST, K_DIRICHLET, K_SOR, K_JACOBI, K_SPMV, K_RESID, K_DOT, K_BLAS1, K_REDUCE,
  K_ASSEMBLE) = range(10)
 MAX_SHELLS = 8
@@ -90,6 +93,32 @@ class IRParams(ctypes.Structure):
     ]
 
 
+class SolveParams(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int32),
+        ("count_pre", ctypes.c_int32),
+        ("omega", ctypes.c_double),
+        ("refresh_max", ctypes.c_int32),
+        ("refresh_delta", ctypes.c_double),
+        ("check_every", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("restart_every", ctypes.c_int32),
+        ("stall_window", ctypes.c_int32),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("iters", ctypes.c_int32),
+        ("refreshes", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("relres", ctypes.c_double),
+    ]
+
+
 class BCMError(RuntimeError):
     def __init__(self, code, where):
         super().__init__(f"{where} failed: {STATUS_NAMES.get(code, code)}")
@@ -146,6 +175,10 @@ SIGNATURES = {
                           _IP, _DP]),
     "bcm_solve_mixed": (_I, [_P, _P, _DP, _I, ctypes.POINTER(IRParams), _IP, _IP, _DP]),
     "bcm_solve_fp64": (_I, [_P, _P, _DP, _I, ctypes.POINTER(IRParams), _IP, _DP]),
+    "bcm_mesh_create_batch": (_I, [_P, _IP, _I, _I, _I, ctypes.POINTER(_P)]),
+    "bcm_mesh_num_segments": (_I, [_P]),
+    "bcm_mesh_segment": (_I, [_P, _I, _IP, _IP]),
+    "bcm_solve": (_I, [_P, _P, _DP, _I, ctypes.POINTER(SolveParams), _DP, ctypes.POINTER(SolveStats)]),
 }
 
 _lib = None

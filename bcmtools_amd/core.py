@@ -112,7 +112,9 @@ def _handles(fields):
 class Mesh:
     """Device mesh: block table, face-task lists and the mesh's HIP stream."""
 
-    def __init__(self, blocks, n, vc=1):
+    def __init__(self, blocks, n, vc=1, segments=None):
+        """`segments`: block counts of independent meshes concatenated in
+        `blocks` (each with neighbour ids local to its own table)."""
         self.lib = L.load()
         blocks = np.ascontiguousarray(blocks, dtype=L.BLOCK_DTYPE)
         self.blocks = blocks
@@ -121,9 +123,27 @@ class Mesh:
         self.nblocks = len(blocks)
         self.nx0 = self.n + 2 * self.vc
         h = ctypes.c_void_p()
-        L.check(self.lib.bcm_mesh_create(blocks.ctypes.data, self.nblocks, self.n, self.vc, ctypes.byref(h)),
-                "bcm_mesh_create")
+        if segments is None:
+            L.check(self.lib.bcm_mesh_create(blocks.ctypes.data, self.nblocks, self.n, self.vc, ctypes.byref(h)),
+                    "bcm_mesh_create")
+        else:
+            cnt = np.ascontiguousarray(segments, dtype=np.int32)
+            L.check(self.lib.bcm_mesh_create_batch(blocks.ctypes.data, cnt.ctypes.data_as(L._IP), len(cnt), self.n,
+                                                   self.vc, ctypes.byref(h)), "bcm_mesh_create_batch")
         self.handle = h
+        self.nseg = self.lib.bcm_mesh_num_segments(h)
+
+    @classmethod
+    def batch(cls, block_tables, n, vc=1):
+        """One device mesh holding several independent meshes (segments)."""
+        tables = [np.ascontiguousarray(t, dtype=L.BLOCK_DTYPE) for t in block_tables]
+        return cls(np.concatenate(tables), n, vc, segments=[len(t) for t in tables])
+
+    def segment(self, s):
+        """(first block, block count) of segment s."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        L.check(self.lib.bcm_mesh_segment(self.handle, int(s), ctypes.byref(a), ctypes.byref(b)), "bcm_mesh_segment")
+        return a.value, b.value
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -278,6 +298,28 @@ class Mesh:
         if rc not in (L.OK, L.ERR_NOCONV):
             L.check(rc, "bcm_solve_fp64")
         return rc, it.value, rr.value
+
+    def solve(self, x, rhs, bvalues, precision="mixed", tol=1e-10, max_iter=5000, count_pre=2, omega=0.8,
+              refresh_max=0, refresh_delta=0.0, check_every=8, use_graph=True, seg_tol=None, restart_every=0,
+              stall_window=0):
+        """Fused device-resident BiCGSTAB on every segment (bcm_solve).
+        Returns one dict per segment: status, iters, refreshes, relres."""
+        prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "fp32": L.PREC_FP32}[precision]
+        p = L.SolveParams(prec, float(tol), int(max_iter), int(count_pre), float(omega), int(refresh_max),
+                          float(refresh_delta), int(check_every), int(bool(use_graph)), int(restart_every),
+                          int(stall_window))
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        st = (L.SolveStats * self.nseg)()
+        tl = None
+        if seg_tol is not None:
+            tl = np.ascontiguousarray(seg_tol, dtype=np.float64)
+            if len(tl) != self.nseg:
+                raise ValueError("seg_tol needs one tolerance per segment")
+        L.check(self.lib.bcm_solve(x.handle, rhs.handle, L.dptr(v), len(v), ctypes.byref(p), L.dptr(tl), st),
+                "bcm_solve")
+        names = {L.SOLVE_CONVERGED: "converged", L.SOLVE_NOCONV: "noconv", L.SOLVE_FAILED: "failed"}
+        return [{"status": names.get(e.status, e.status), "iters": e.iters, "refreshes": e.refreshes,
+                 "relres": e.relres} for e in st]
 
     def solve_mixed(self, x, rhs, bvalues, tol=1e-10, inner_tol=1e-4, inner_max=500, outer_max=20, count_pre=2,
                     omega=0.8, fp32_only=False):

@@ -42,6 +42,8 @@ struct EventPair {
   int kernel;
 };
 
+struct Krylov;  // fused solver state (bcm_krylov.hip)
+
 }  // namespace bcm
 
 struct bcm_mesh {
@@ -101,6 +103,16 @@ struct bcm_mesh {
   // X-face column exports (columns 0, 1, n-2, n-1 of every block), two
   // buffers for the ping-pong of iterates; bytes sized for FP64.
   void* d_xexp[2] = {nullptr, nullptr};
+
+  // Segments: independent meshes sharing the arena, segment s owning blocks
+  // seg_first[s] .. seg_first[s+1]-1 (one segment for a plain mesh).
+  int nseg = 1;
+  std::vector<int> seg_first;
+  int* d_seg_first = nullptr;
+  int* d_blk_seg = nullptr;
+
+  // Fused Krylov engine (vectors, per-segment states, captured graph).
+  bcm::Krylov* kry = nullptr;
 };
 
 namespace bcm {
@@ -151,6 +163,10 @@ int dirichlet_device(bcm_mesh* m, bcm_field* f);
 // column export it reads.
 template <typename T> int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout);
 template <typename T> int xexport_raw(bcm_mesh* m, const T* A, T* xout);
+// Block-wise ghost fill (all faces, all kinds; n in {8, 16, 32}, vc = 1).
+template <typename T> int ghost_gather_raw(bcm_mesh* m, T* data);
+// Frees the fused solver's state (bcm_krylov.hip).
+void krylov_release(bcm_mesh* m);
 // Field-sized workspace buffer kept on the mesh (bcm_solver.hip).
 void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc);
 enum WorkspaceSlot { WS_SOR_PONG_F64 = 30, WS_SOR_PONG_F32 = 31 };

@@ -1,0 +1,942 @@
+// Fused, device-resident BiCGSTAB over one mesh or a batch of meshes.
+//
+// Algorithm: ILS3D::BiCGSTAB (examples/SolverTest/ILS3D.h:663-844) with
+// the Jacobi preconditioner (ILS3D.h:452-475, blas.f90:452-502) on the
+// Poisson operator with the Dirichlet fold (bcut.f90:2848-2866,
+// bca.f90:12-202), restated for HBM:
+//   * every scalar (rho, alpha, gamma, beta, norms) lives in a per-segment
+//     state record on the device, so an iteration never waits for the
+//     host and a block of iterations can be captured in one hipGraph;
+//   * vector updates are fused with the first preconditioner sweep and
+//     the dot products with the kernels that produce their operands;
+//   * Krylov vectors that no stencil reads are stored compact (interior
+//     cells only, n^3 per block); the three stencil inputs (the two
+//     preconditioner iterates and the sweep scratch) keep the ghosted
+//     block layout and get their faces from a block-wise ghost gather;
+//   * a segment is one independent mesh of a batch: kernels run over all
+//     blocks of all segments, reductions and scalars stay per segment and
+//     a converged segment's blocks drop out.
+// Precision: FP64 runs the whole iteration in FP64.  MIXED stores the
+// Krylov vectors r, r0, p, q, s, t and the sweep scratch in FP32 and runs
+// the Jacobi sweeps and vector updates in FP32, but keeps in FP64 what an
+// FP32 rounding would spoil: a smooth FP32 vector v carries a relative
+// error eps32 that A multiplies by ~(12/h^2)/lambda_min (1e-2..1e-1 on a
+// level-6/7 shell mesh), so the SpMV inputs p_ = M^-1 p and s_ = M^-1 s,
+// the SpMV arithmetic, the solution correction dx and every dot product are
+// FP64.  An FP64 update (x += dx; r = b - A x, rounded to FP32) replaces
+// the recurrence residual when it has fallen by refresh_delta, every
+// refresh_max iterations and when the recurrence claims convergence;
+// convergence is only ever declared on the true FP64 residual
+// ||b - A x|| / ||b|| <= tol.
+#include <cfloat>
+#include <cmath>
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "bcm_internal.h"
+#include "bcm_interp.h"
+#include "bcm_poisson.h"
+
+namespace bcm {
+
+struct SegState {
+  double bb;        // (b, b)
+  double rho;       // (r, r0) of the current r
+  double rho_prev;  // rho of the previous iteration
+  double alpha, gamma;
+  double rr;        // recurrence (r, r)
+  double rr_ref;    // (r, r) right after the last reliable update
+  double rr_true;   // true FP64 ||b - A x||^2 at the last reliable update
+  double tol2;      // tol^2
+  double delta2;    // refresh_delta^2 (0: off)
+  double rr_min;    // smallest recurrence (r, r) since the last reliable update
+  int32_t status;   // BCM_SOLVE_*
+  int32_t first;    // next iteration starts with p = r
+  int32_t refresh;  // reliable update after this iteration
+  int32_t restart;  // that update also restarts the Krylov space (r0 = r)
+  int32_t reset_r0; // the next iteration's p update also sets r0 = r
+  int32_t trig;     // why the pending reliable update was requested (TRIG_*)
+  int32_t iters, since, max_iter, refresh_max, nref, zero_x;
+  int32_t restart_every;  // restart at every k-th reliable update (0: never)
+  int32_t stall;          // iterations without a new rr_min that force an update (0: off)
+  int32_t since_min;      // iterations since rr_min last fell by 1 %
+};
+
+enum { TRIG_CONV = 1, TRIG_SCHEDULE = 2, TRIG_LIMIT = 4, TRIG_BREAKDOWN = 8 };
+
+namespace {
+
+constexpr int KWG = 256;
+
+template <int N>
+struct Blk {
+  static constexpr int NX = N + 2;
+  static constexpr int N3 = N * N * N;
+  static constexpr long long GV = (long long)NX * NX * NX;
+  static constexpr int PER = N3 / KWG;  // cells per thread of the iteration kernels (N >= 8)
+  __device__ static __forceinline__ void ijk(int c, int& i, int& j, int& k) {
+    i = c % N;
+    j = (c / N) % N;
+    k = c / (N * N);
+  }
+  __device__ static __forceinline__ int g(int i, int j, int k) { return (i + 1) + NX * ((j + 1) + NX * (k + 1)); }
+};
+
+// Deterministic work-group sums of two doubles (wave64 butterfly, then a
+// fixed-order sum of the four wave totals); the result is in thread 0.
+__device__ __forceinline__ void wg_sum2(double& a, double& b, double* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[4 + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = ((red[0] + red[1]) + red[2]) + red[3];
+    b = ((red[4] + red[5]) + red[6]) + red[7];
+  }
+}
+
+// Gate of a block-wise kernel: 0 every block, 1 blocks of active
+// segments, 2 blocks of active segments with a reliable update pending.
+template <int GATE>
+__device__ __forceinline__ bool gated_out(const int* __restrict__ blk_seg, const SegState* __restrict__ st, int b) {
+  if (GATE == 0) return false;
+  const SegState* sp = st + blk_seg[b];
+  if (sp->status != BCM_SOLVE_ACTIVE) return true;
+  return GATE == 2 && !sp->refresh;
+}
+
+// ---------------------------------------------------------------------
+// Block-wise ghost gather: one work-group per destination block fills its
+// six faces from the neighbours' interiors (copy, F2C, C2F as
+// include/Scalar3DUpdater.h:190-266, 321-347, 519-708), the same values as
+// the per-face task kernel.  Dirichlet / unconnected faces are left as
+// they are.
+// ---------------------------------------------------------------------
+template <typename T, int N, int GATE>
+__global__ __launch_bounds__(KWG) void k_gather(T* __restrict__ a, const int* __restrict__ desc,
+                                                const int* __restrict__ blk_seg, const SegState* __restrict__ st) {
+  using B = Blk<N>;
+  const int b = blockIdx.x;
+  if (gated_out<GATE>(blk_seg, st, b)) return;
+  __shared__ int d[48];
+  if (threadIdx.x < 48) d[threadIdx.x] = desc[b * 48 + threadIdx.x];
+  __syncthreads();
+  T* __restrict__ dst = a + b * B::GV;
+  constexpr int h = N / 2;
+  for (int e = threadIdx.x; e < 6 * N * N; e += KWG) {
+    const int f = e / (N * N), uv = e % (N * N), u = uv % N, v = uv / N;
+    const int* df = d + f * 8;
+    const int kind = df[0];
+    if (kind != FD_COPY && kind != FD_F2C && kind != FD_C2F) continue;
+    const int ax = f >> 1;
+    const bool plus = f & 1;
+    const int t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
+    int c[3];
+    c[ax] = plus ? N : -1;
+    c[t0] = u;
+    c[t1] = v;
+    T val;
+    if (kind == FD_COPY) {
+      const T* __restrict__ src = a + df[1] * B::GV;
+      int sc[3] = {c[0], c[1], c[2]};
+      sc[ax] = plus ? 0 : N - 1;
+      val = src[B::g(sc[0], sc[1], sc[2])];
+    } else if (kind == FD_F2C) {
+      const int s0 = u >= h, s1 = v >= h;
+      const T* __restrict__ src = a + df[1 + s0 + 2 * s1] * B::GV;
+      int I[3];
+      I[ax] = plus ? 0 : h - 1;
+      I[t0] = u - h * s0;
+      I[t1] = v - h * s1;
+      val = f2c_sum<T>([&](int i, int j, int k) { return src[B::g(i, j, k)]; }, I[0], I[1], I[2]);
+    } else {
+      const int sf = df[5];
+      const T* __restrict__ src = a + df[1] * B::GV;
+      int p[3];
+      p[ax] = plus ? 0 : 2 * N - 1;
+      p[t0] = u + N * (sf & 1);
+      p[t1] = v + N * (sf >> 1);
+      val = c2f_interp<T>([&](int i, int j, int k) { return src[B::g(i, j, k)]; }, N, p[0], p[1], p[2]);
+    }
+    dst[B::g(c[0], c[1], c[2])] = val;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Iteration kernels (one work-group per block, cell c = tid + 256 m)
+// ---------------------------------------------------------------------
+// Sweep 1 of the Jacobi preconditioner from a zero start, written as
+// jacobi_smoother does it: r = b (the neighbours are zero), x = 0 - w (0 - r/Ap).
+template <typename T>
+__device__ __forceinline__ T first_sweep(T bv, T ap, T om) {
+  const T xt = bv / ap;
+  const T xc = T(0);
+  return xc - om * (xc - xt);
+}
+
+// p = r (first) or p = r + beta (p - gamma q)  [AXPY(p, q, -gamma); XPAY(p, r, beta)],
+// then the first preconditioner sweep of p into g (PRE) or g = p.  After a
+// restart the same pass sets the shadow residual r0 = r.
+template <typename T, int N, int PRE, typename TG>
+__global__ __launch_bounds__(KWG) void k_kry_p(T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ q,
+                                               T* __restrict__ r0, TG* __restrict__ g,
+                                               const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
+                                               const SegState* __restrict__ st, double omega) {
+  using B = Blk<N>;
+  const int b = blockIdx.x;
+  const SegState* sp = st + blk_seg[b];
+  if (sp->status != BCM_SOLVE_ACTIVE) return;
+  const bool first = sp->first != 0;
+  const bool reset = sp->reset_r0 != 0;
+  const double beta = first ? 0.0 : sp->rho / sp->rho_prev * sp->alpha / sp->gamma;
+  const T tb = T(beta), tg = T(-sp->gamma), om = T(omega);
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    const long long o = cb + c;
+    const T rv = r[o];
+    const T pv = first ? rv : rv + tb * (p[o] + tg * q[o]);
+    p[o] = pv;
+    if (reset) r0[o] = rv;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    if (PRE) {
+      const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+      g[gb + B::g(i, j, k)] = TG(first_sweep(pv, a.p, om));
+    } else {
+      g[gb + B::g(i, j, k)] = TG(pv);
+    }
+  }
+}
+
+// s = r - alpha q  [AXPYZ(s, q, r, -alpha)], then the first sweep of s into g.
+template <typename T, int N, int PRE, typename TG>
+__global__ __launch_bounds__(KWG) void k_kry_s(T* __restrict__ s, const T* __restrict__ r, const T* __restrict__ q,
+                                               TG* __restrict__ g, const BlockMeta* __restrict__ meta,
+                                               const int* __restrict__ blk_seg, const SegState* __restrict__ st,
+                                               double omega) {
+  using B = Blk<N>;
+  const int b = blockIdx.x;
+  const SegState* sp = st + blk_seg[b];
+  if (sp->status != BCM_SOLVE_ACTIVE) return;
+  const T ta = T(-sp->alpha), om = T(omega);
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    const long long o = cb + c;
+    const T sv = ta * q[o] + r[o];
+    s[o] = sv;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    if (PRE) {
+      const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+      g[gb + B::g(i, j, k)] = TG(first_sweep(sv, a.p, om));
+    } else {
+      g[gb + B::g(i, j, k)] = TG(sv);
+    }
+  }
+}
+
+// Later preconditioner sweeps: gout = jacobi_smoother(gin; b = rhs),
+// arithmetic in T (the sweep precision), input / output stored as TI / TO.
+template <typename T, typename TI, typename TO>
+__device__ __forceinline__ T jacobi_mixed(const Coef7<T>& a, const TI* __restrict__ x, long long o, int sy, int sz,
+                                          T bv, T om) {
+  T r = bv - a.w * T(x[o - 1]);
+  r = r - a.e * T(x[o + 1]);
+  r = r - a.s * T(x[o - sy]);
+  r = r - a.n * T(x[o + sy]);
+  r = r - a.b * T(x[o - sz]);
+  r = r - a.t * T(x[o + sz]);
+  const T xt = r / a.p;
+  const T xc = T(x[o]);
+  return xc - om * (xc - xt);
+}
+
+template <typename T, int N, typename TI, typename TO>
+__global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, const TI* __restrict__ gin,
+                                                    const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
+                                                    const int* __restrict__ blk_seg,
+                                                    const SegState* __restrict__ st, double omega) {
+  using B = Blk<N>;
+  const int b = blockIdx.x;
+  if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
+  const T om = T(omega);
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    const long long go = gb + B::g(i, j, k);
+    const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+    gout[go] = TO(jacobi_mixed<T, TI, TO>(a, gin, go, B::NX, B::NX * B::NX, rhs[cb + c], om));
+  }
+}
+
+// out = A g (calc_ax) in FP64 from the FP64 preconditioned vector g, stored
+// in T, with the per-block partial dots (out, y) and, for NDOT == 2,
+// (out, out), formed and summed in FP64.
+template <typename T, int N, int NDOT>
+__global__ __launch_bounds__(KWG) void k_kry_spmv(T* __restrict__ out, const double* __restrict__ g,
+                                                  const T* __restrict__ y, const BlockMeta* __restrict__ meta,
+                                                  const int* __restrict__ blk_seg, const SegState* __restrict__ st,
+                                                  double* __restrict__ parts) {
+  using B = Blk<N>;
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  double d0 = 0.0, d1 = 0.0;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    const Coef7<double> a = poisson_coef<double>(me, i, j, k, N);
+    const T v = T(ax_cell(a, g, gb + B::g(i, j, k), B::NX, B::NX * B::NX));
+    out[cb + c] = v;
+    d0 += double(v) * double(y[cb + c]);
+    if (NDOT == 2) d1 += double(v) * double(v);
+  }
+  wg_sum2(d0, d1, red);
+  if (threadIdx.x == 0) {
+    parts[2 * b] = d0;
+    parts[2 * b + 1] = d1;
+  }
+}
+
+// dx += alpha p_ + gamma s_  [AXPBYPZ(x, p_, s_, alpha, gamma)] in FP64;
+// r = s - gamma t  [AXPYZ(r, t_, s, -gamma)] in T; partial (r, r), (r, r0).
+template <typename T, int N>
+__global__ __launch_bounds__(KWG) void k_kry_xr(double* __restrict__ dx, const double* __restrict__ gp,
+                                                const double* __restrict__ gs, T* __restrict__ r,
+                                                const T* __restrict__ s, const T* __restrict__ t,
+                                                const T* __restrict__ r0, const int* __restrict__ blk_seg,
+                                                const SegState* __restrict__ st, double* __restrict__ parts) {
+  using B = Blk<N>;
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const SegState* sp = st + blk_seg[b];
+  if (sp->status != BCM_SOLVE_ACTIVE) return;
+  const double al = sp->alpha, ga = sp->gamma;
+  const T tmg = T(-sp->gamma);
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  double d0 = 0.0, d1 = 0.0;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    const long long o = cb + c;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    const long long go = gb + B::g(i, j, k);
+    dx[o] = (al * gp[go] + ga * gs[go]) + dx[o];
+    const T rv = tmg * t[o] + s[o];
+    r[o] = rv;
+    d0 += double(rv) * double(rv);
+    d1 += double(rv) * double(r0[o]);
+  }
+  wg_sum2(d0, d1, red);
+  if (threadIdx.x == 0) {
+    parts[2 * b] = d0;
+    parts[2 * b + 1] = d1;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Reliable update (FP64) and set-up kernels
+// ---------------------------------------------------------------------
+// x += dx (interior), dx = 0.
+template <int N>
+__global__ __launch_bounds__(KWG) void k_ref_x(double* __restrict__ x, double* __restrict__ dx,
+                                               const int* __restrict__ blk_seg, const SegState* __restrict__ st) {
+  using B = Blk<N>;
+  const int b = blockIdx.x;
+  if (gated_out<2>(blk_seg, st, b)) return;
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    const long long go = gb + B::g(i, j, k);
+    x[go] = x[go] + dx[cb + c];
+    dx[cb + c] = 0.0;
+  }
+}
+
+// r = T(b - A x) in FP64 (calc_r).  Partials: FP64 (r, r) of the
+// unrounded residual and (r, r0) of the rounded one.
+template <typename T, int N>
+__global__ __launch_bounds__(KWG) void k_ref_r(T* __restrict__ r, const T* __restrict__ r0,
+                                               const double* __restrict__ x, const double* __restrict__ b64,
+                                               const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
+                                               const SegState* __restrict__ st, double* __restrict__ parts) {
+  using B = Blk<N>;
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const SegState* sp = st + blk_seg[b];
+  if (sp->status != BCM_SOLVE_ACTIVE || !sp->refresh) return;
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  double acc = 0.0, d1 = 0.0;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    const long long o = cb + c;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    const Coef7<double> a = poisson_coef<double>(me, i, j, k, N);
+    const double rv = r_cell(a, x, gb + B::g(i, j, k), B::NX, B::NX * B::NX, b64[o]);
+    const T rt = T(rv);
+    r[o] = rt;
+    acc += rv * rv;
+    d1 += double(rt) * double(r0[o]);
+  }
+  wg_sum2(acc, d1, red);
+  if (threadIdx.x == 0) {
+    parts[2 * b] = acc;
+    parts[2 * b + 1] = d1;
+  }
+}
+
+// Folded right-hand side, compact (k_assemble's b: bca.f90 b -= Anb v 2),
+// and the per-block (b, b).
+template <int N>
+__global__ __launch_bounds__(KWG) void k_kry_rhs(double* __restrict__ b64, const double* __restrict__ rhs,
+                                                 const BlockMeta* __restrict__ meta, const double* __restrict__ bval,
+                                                 double* __restrict__ parts) {
+  using B = Blk<N>;
+  __shared__ double red[8];
+  const int b = blockIdx.x;
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  double acc = 0.0;
+  for (int m = 0; m < B::PER; ++m) {
+    const int c = threadIdx.x + KWG * m;
+    int i, j, k;
+    B::ijk(c, i, j, k);
+    const double l[6] = {me.cx, me.cx, me.cy, me.cy, me.cz, me.cz};
+    double bv = rhs[gb + B::g(i, j, k)];
+    const int at[6] = {i == 0, i == N - 1, j == 0, j == N - 1, k == 0, k == N - 1};
+    for (int f = 0; f < 6; ++f)
+      if ((me.dir_mask >> f & 1) && at[f]) bv = bv - l[f] * bval[me.bc_id[f]] * 2.0;
+    b64[cb + c] = bv;
+    acc += bv * bv;
+  }
+  double z = 0.0;
+  wg_sum2(acc, z, red);
+  if (threadIdx.x == 0) {
+    parts[2 * b] = acc;
+    parts[2 * b + 1] = 0.0;
+  }
+}
+
+// A segment whose right-hand side is zero has the solution zero.
+__global__ __launch_bounds__(KWG) void k_zero_x(double* __restrict__ x, long long bvol,
+                                                const int* __restrict__ blk_seg, const SegState* __restrict__ st) {
+  const int b = blockIdx.x;
+  if (!st[blk_seg[b]].zero_x) return;
+  for (long long e = threadIdx.x; e < bvol; e += KWG) x[b * bvol + e] = 0.0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_kry_zero(T* __restrict__ x, long long count) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < count) x[i] = T(0);
+}
+
+// ---------------------------------------------------------------------
+// Per-segment reductions and scalar updates (one work-group per segment,
+// blocks of the segment summed in a fixed order, so a segment's scalars
+// do not depend on the other segments of the batch).
+// ---------------------------------------------------------------------
+enum Phase { PH_BB = 0, PH_Q, PH_T, PH_R, PH_F };
+
+__device__ __forceinline__ bool conv_test(double rr, const SegState& s) {
+  return s.bb > double(FLT_MIN) ? rr <= s.tol2 * s.bb : rr <= s.tol2;
+}
+
+template <int PH>
+__global__ __launch_bounds__(KWG) void k_seg(SegState* __restrict__ st, const double* __restrict__ parts,
+                                             const int* __restrict__ seg_first) {
+  __shared__ double red[8];
+  const int sg = blockIdx.x;
+  SegState* sp = st + sg;
+  if (sp->status != BCM_SOLVE_ACTIVE) return;
+  if (PH == PH_F && !sp->refresh) return;
+  double a = 0.0, c = 0.0;
+  for (int b = seg_first[sg] + threadIdx.x; b < seg_first[sg + 1]; b += KWG) {
+    a += parts[2 * b];
+    c += parts[2 * b + 1];
+  }
+  wg_sum2(a, c, red);
+  if (threadIdx.x != 0) return;
+  SegState s = *sp;
+  if (PH == PH_BB) {
+    s.bb = a;
+    if (a == 0.0) {
+      s.status = BCM_SOLVE_CONVERGED;
+      s.zero_x = 1;
+    }
+  } else if (PH == PH_Q) {
+    // alpha = rho / (q_, r0) (ILS3D.h:756); a breakdown restarts.
+    s.reset_r0 = 0;
+    double al = s.rho / a;
+    if (!std::isfinite(al)) {
+      al = 0.0;
+      s.restart = 1;
+      s.refresh = 1;
+      s.trig |= TRIG_BREAKDOWN;
+    }
+    s.alpha = al;
+  } else if (PH == PH_T) {
+    // gamma = (t_, s) / (t_, t_) (ILS3D.h:790-793).
+    double ga = a / c;
+    if (!std::isfinite(ga) || ga == 0.0) {
+      ga = std::isfinite(ga) ? ga : 0.0;
+      s.restart = 1;
+      s.refresh = 1;
+      s.trig |= TRIG_BREAKDOWN;
+    }
+    s.gamma = ga;
+  } else if (PH == PH_R) {
+    s.iters += 1;
+    s.since += 1;
+    if (!std::isfinite(a) || !std::isfinite(c)) {
+      s.status = BCM_SOLVE_FAILED;
+      s.refresh = 0;
+    } else {
+      s.rr = a;
+      s.rho_prev = s.rho;
+      s.rho = c;
+      s.first = 0;
+      if (std::fabs(c) < double(FLT_MIN)) {
+        s.restart = 1;
+        s.refresh = 1;
+        s.trig |= TRIG_BREAKDOWN;
+      }
+      // The recurrence is asked for half the tolerance before the true
+      // residual is checked, as the host-driven FP64 path does.
+      if (conv_test(4.0 * a, s)) s.trig |= TRIG_CONV;
+      // A stalled low-precision recurrence (no new minimum for `stall`
+      // iterations) gains nothing more before an FP64 update.
+      if (a < 0.98 * s.rr_min) {
+        s.rr_min = a;
+        s.since_min = 0;
+      } else {
+        s.since_min += 1;
+      }
+      if (s.since >= s.refresh_max || a <= s.delta2 * s.rr_ref || (s.stall > 0 && s.since_min >= s.stall))
+        s.trig |= TRIG_SCHEDULE;
+      if (s.iters >= s.max_iter) s.trig |= TRIG_LIMIT;
+      if (s.trig) s.refresh = 1;
+    }
+  } else {  // PH_F: after the reliable update
+    if (!std::isfinite(c)) s.restart = 1;  // (r, r0) unusable: start a new Krylov space
+    if (!std::isfinite(a)) {
+      s.status = BCM_SOLVE_FAILED;
+    } else {
+      s.rr_true = a;
+      s.rr = a;
+      s.rr_ref = a;
+      s.rr_min = a;
+      s.since_min = 0;
+      s.since = 0;
+      s.nref += 1;
+      if (conv_test(a, s)) {
+        s.status = BCM_SOLVE_CONVERGED;
+      } else if (s.iters >= s.max_iter) {
+        s.status = BCM_SOLVE_NOCONV;
+      } else if (s.restart || (s.trig & TRIG_CONV) || (s.restart_every > 0 && s.nref % s.restart_every == 0)) {
+        // Breakdown, or the recurrence met the tolerance while the true
+        // residual did not: restart from the current solution (r0 = r,
+        // p = r on the next pass), as the host-driven solver restarts.
+        s.first = 1;
+        s.reset_r0 = 1;
+        s.rho = a;
+      } else {
+        // Reliable update: keep the Krylov space, replace the residual.
+        s.rho = c;
+      }
+    }
+    s.restart = 0;
+    s.trig = 0;
+    s.refresh = 0;
+  }
+  *sp = s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------
+struct Krylov {
+  int esize = 0;            // bytes per value of the sweep-precision vectors (4 or 8)
+  void* v[6] = {};          // compact, sweep precision: r, r0, p, q, s, t
+  double* dx = nullptr;     // compact FP64 correction since the last reliable update
+  void* g1 = nullptr;       // ghosted sweep scratch, sweep precision
+  double* g[2] = {};        // ghosted FP64 preconditioned vectors p_, s_ (SpMV inputs)
+  double* b64 = nullptr;    // folded right-hand side, compact
+  double* parts = nullptr;  // 2 per block
+  SegState* st = nullptr;
+  SegState* h_st = nullptr;  // pinned mirror
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // What the captured graph was built for.
+  int key_esize = 0, key_pre = -1, key_iters = 0;
+  double key_omega = 0.0;
+  const void* key_x = nullptr;
+};
+
+static void krylov_free_vectors(Krylov* k) {
+  for (auto& p : k->v) { dev_free(p); p = nullptr; }
+  dev_free(k->g1);
+  k->g1 = nullptr;
+  k->esize = 0;
+}
+
+static void krylov_drop_graph(Krylov* k) {
+  if (k->exec) (void)hipGraphExecDestroy(k->exec);
+  if (k->graph) (void)hipGraphDestroy(k->graph);
+  k->exec = nullptr;
+  k->graph = nullptr;
+  k->key_esize = 0;
+}
+
+void krylov_release(bcm_mesh* m) {
+  Krylov* k = m->kry;
+  if (!k) return;
+  krylov_drop_graph(k);
+  krylov_free_vectors(k);
+  dev_free(k->dx);
+  for (auto& p : k->g) dev_free(p);
+  dev_free(k->b64);
+  dev_free(k->parts);
+  dev_free(k->st);
+  if (k->h_st) (void)hipHostFree(k->h_st);
+  delete k;
+  m->kry = nullptr;
+}
+
+template <typename T>
+static int zero_async(bcm_mesh* m, T* p, size_t count) {
+  hipLaunchKernelGGL(k_kry_zero<T>, dim3(unsigned((count + 255) / 256)), dim3(256), 0, m->stream, p,
+                     (long long)count);
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+
+static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
+  const size_t nc = m->ncell, ng = size_t(m->nblocks) * m->bvol;
+  if (!m->kry) {
+    Krylov* k = new (std::nothrow) Krylov();
+    if (!k) return BCM_ERR_NOMEM;
+    m->kry = k;
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->b64), sizeof(double) * nc));
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->dx), sizeof(double) * nc));
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->parts), sizeof(double) * 2 * size_t(m->nblocks)));
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->st), sizeof(SegState) * size_t(m->nseg)));
+    BCM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&k->h_st), sizeof(SegState) * size_t(m->nseg)));
+    // Ghost cells of outer faces are never written: start at zero.
+    for (auto& p : k->g) {
+      BCM_TRY(dev_alloc(reinterpret_cast<void**>(&p), sizeof(double) * ng));
+      BCM_TRY(zero_async(m, p, ng));
+    }
+  }
+  Krylov* k = m->kry;
+  if (k->esize != esize) {
+    BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+    krylov_drop_graph(k);
+    krylov_free_vectors(k);
+    for (auto& p : k->v) {
+      BCM_TRY(dev_alloc(&p, nc * esize));
+      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nc));
+      else BCM_TRY(zero_async(m, static_cast<float*>(p), nc));
+    }
+    BCM_TRY(dev_alloc(&k->g1, ng * esize));
+    if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(k->g1), ng));
+    else BCM_TRY(zero_async(m, static_cast<float*>(k->g1), ng));
+    k->esize = esize;
+  }
+  *out = k;
+  return BCM_OK;
+}
+
+template <typename T, int N>
+static int gather(bcm_mesh* m, T* a, int gate) {
+  const dim3 grid(m->nblocks), blk(KWG);
+  const SegState* st = m->kry ? m->kry->st : nullptr;
+  if (gate == 0) hipLaunchKernelGGL((k_gather<T, N, 0>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st);
+  else if (gate == 1) hipLaunchKernelGGL((k_gather<T, N, 1>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st);
+  else hipLaunchKernelGGL((k_gather<T, N, 2>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st);
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+
+template <typename T>
+int ghost_gather_raw(bcm_mesh* m, T* data) {
+  if (m->vc != 1) return BCM_ERR_ARG;
+  switch (m->n) {
+    case 4: return gather<T, 4>(m, data, 0);
+    case 8: return gather<T, 8>(m, data, 0);
+    case 16: return gather<T, 16>(m, data, 0);
+    case 32: return gather<T, 32>(m, data, 0);
+    default: return BCM_ERR_ARG;
+  }
+}
+template int ghost_gather_raw<double>(bcm_mesh*, double*);
+template int ghost_gather_raw<float>(bcm_mesh*, float*);
+
+template <int PH>
+static int seg_reduce(bcm_mesh* m) {
+  hipLaunchKernelGGL(k_seg<PH>, dim3(m->nseg), dim3(KWG), 0, m->stream, m->kry->st, m->kry->parts, m->d_seg_first);
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+
+// Jacobi preconditioner chain: the first sweep is fused into the update
+// kernel; sweeps 2..pre alternate between the sweep scratch (precision T)
+// and the FP64 result so that the last one lands in `result`.
+template <typename T, int N>
+static int precondition(bcm_mesh* m, T* scratch, double* result, const T* rhs, int pre, double omega) {
+  bool in_result = (pre % 2 == 1);
+  const dim3 grid(m->nblocks), blk(KWG);
+  const int* bs = m->d_blk_seg;
+  const SegState* st = m->kry->st;
+  for (int sweep = 2; sweep <= pre; ++sweep) {
+    if (in_result) {
+      BCM_TRY((gather<double, N>(m, result, 1)));
+      hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, result, rhs, m->d_meta,
+                         bs, st, omega);
+    } else {
+      BCM_TRY((gather<T, N>(m, scratch, 1)));
+      hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, scratch, rhs, m->d_meta,
+                         bs, st, omega);
+    }
+    BCM_HIP_TRY(hipGetLastError());
+    in_result = !in_result;
+  }
+  return BCM_OK;
+}
+
+// Update kernel (p or s) with the fused first sweep into the right target.
+template <typename T, int N, bool IS_P>
+static int update_and_first_sweep(bcm_mesh* m, T* v, const T* r, const T* q, T* r0, T* scratch, double* result,
+                                  int pre, double omega) {
+  const dim3 grid(m->nblocks), blk(KWG);
+  const int* bs = m->d_blk_seg;
+  const SegState* st = m->kry->st;
+  const bool to_result = pre == 0 || pre % 2 == 1;
+  if (IS_P) {
+    if (pre == 0)
+      hipLaunchKernelGGL((k_kry_p<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, r0, result, m->d_meta, bs, st,
+                         omega);
+    else if (to_result)
+      hipLaunchKernelGGL((k_kry_p<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, r0, result, m->d_meta, bs, st,
+                         omega);
+    else
+      hipLaunchKernelGGL((k_kry_p<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, r0, scratch, m->d_meta, bs, st,
+                         omega);
+  } else {
+    if (pre == 0)
+      hipLaunchKernelGGL((k_kry_s<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, result, m->d_meta, bs, st,
+                         omega);
+    else if (to_result)
+      hipLaunchKernelGGL((k_kry_s<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, result, m->d_meta, bs, st,
+                         omega);
+    else
+      hipLaunchKernelGGL((k_kry_s<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, scratch, m->d_meta, bs, st, omega);
+  }
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+
+// One BiCGSTAB iteration plus the (gated) reliable update.
+template <typename T, int N>
+static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
+  Krylov* K = m->kry;
+  T *r = static_cast<T*>(K->v[0]), *r0 = static_cast<T*>(K->v[1]), *p = static_cast<T*>(K->v[2]),
+    *q = static_cast<T*>(K->v[3]), *s = static_cast<T*>(K->v[4]), *t = static_cast<T*>(K->v[5]);
+  T* g1 = static_cast<T*>(K->g1);
+  double *gp = K->g[0], *gs = K->g[1];
+  const dim3 grid(m->nblocks), blk(KWG);
+  const int* bs = m->d_blk_seg;
+  SegState* st = K->st;
+  BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, gp, pre, omega)));
+  BCM_TRY((precondition<T, N>(m, g1, gp, p, pre, omega)));
+  BCM_TRY((gather<double, N>(m, gp, 1)));
+  hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, r0, m->d_meta, bs, st, K->parts);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_Q>(m));
+  BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, gs, pre, omega)));
+  BCM_TRY((precondition<T, N>(m, g1, gs, s, pre, omega)));
+  BCM_TRY((gather<double, N>(m, gs, 1)));
+  hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, s, m->d_meta, bs, st, K->parts);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_T>(m));
+  hipLaunchKernelGGL((k_kry_xr<T, N>), grid, blk, 0, m->stream, K->dx, gp, gs, r, s, t, r0, bs, st, K->parts);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_R>(m));
+  // Reliable update where requested.
+  hipLaunchKernelGGL((k_ref_x<N>), grid, blk, 0, m->stream, x, K->dx, bs, st);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY((gather<double, N>(m, x, 2)));
+  hipLaunchKernelGGL((k_ref_r<T, N>), grid, blk, 0, m->stream, r, r0, x, K->b64, m->d_meta, bs, st, K->parts);
+  BCM_HIP_TRY(hipGetLastError());
+  return seg_reduce<PH_F>(m);
+}
+
+static bool any_active(const bcm_mesh* m) {
+  for (int s = 0; s < m->nseg; ++s)
+    if (m->kry->h_st[s].status == BCM_SOLVE_ACTIVE) return true;
+  return false;
+}
+
+static int pull_state(bcm_mesh* m) {
+  BCM_HIP_TRY(hipMemcpyAsync(m->kry->h_st, m->kry->st, sizeof(SegState) * size_t(m->nseg), hipMemcpyDeviceToHost,
+                             m->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  return BCM_OK;
+}
+
+template <typename T, int N>
+static int solve_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol,
+                      bcm_solve_stats* stats) {
+  static_assert(Blk<N>::N3 % KWG == 0, "block volume must be a multiple of the work-group");
+  Krylov* K = nullptr;
+  BCM_TRY(krylov_get(m, int(sizeof(T)), &K));
+  const dim3 grid(m->nblocks), blk(KWG);
+  // Segment states.
+  // Defaults measured on the shell meshes of levels 4-6 (16^3 blocks): an
+  // FP64 update each time the FP32 recurrence has gained three digits,
+  // restarting the Krylov space there, needs the fewest iterations.
+  const double delta = P.precision == BCM_PREC_MIXED ? (P.refresh_delta > 0.0 ? P.refresh_delta : 1e-3) : 0.0;
+  const int rmax = P.refresh_max > 0 ? P.refresh_max : (P.precision == BCM_PREC_MIXED ? 1000 : (1 << 30));
+  const int rs_every = P.precision == BCM_PREC_FP64 ? 0 : (P.restart_every == 0 ? 1 : std::max(P.restart_every, 0));
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  for (int s = 0; s < m->nseg; ++s) {
+    SegState z;
+    std::memset(&z, 0, sizeof(z));
+    const double tol = seg_tol ? seg_tol[s] : P.tol;
+    z.tol2 = tol * tol;
+    z.delta2 = delta * delta;
+    z.status = BCM_SOLVE_ACTIVE;
+    z.first = 1;
+    z.refresh = 1;
+    z.restart = 1;  // the first pass sets r0 = r
+    z.max_iter = P.max_iter;
+    z.refresh_max = rmax;
+    z.restart_every = rs_every;
+    z.stall = P.precision == BCM_PREC_FP64 ? 0 : (P.stall_window > 0 ? P.stall_window : (P.stall_window == 0 ? 40 : 0));
+    z.rho_prev = 1.0;
+    z.gamma = 1.0;
+    K->h_st[s] = z;
+  }
+  BCM_HIP_TRY(hipMemcpyAsync(K->st, K->h_st, sizeof(SegState) * size_t(m->nseg), hipMemcpyHostToDevice, m->stream));
+  // b, (b, b), zero solutions of zero right-hand sides.
+  hipLaunchKernelGGL(k_kry_rhs<N>, grid, blk, 0, m->stream, K->b64, rhs, m->d_meta, m->d_bvalues, K->parts);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_BB>(m));
+  hipLaunchKernelGGL(k_zero_x, grid, blk, 0, m->stream, x, (long long)m->bvol, m->d_blk_seg, K->st);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(zero_async(m, K->dx, m->ncell));
+  // Initial residual r = b - A x, r0 = r.
+  BCM_TRY((gather<double, N>(m, x, 2)));
+  hipLaunchKernelGGL((k_ref_r<T, N>), grid, blk, 0, m->stream, static_cast<T*>(K->v[0]),
+                     static_cast<const T*>(K->v[1]), x, K->b64, m->d_meta, m->d_blk_seg, K->st, K->parts);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_F>(m));
+  BCM_TRY(pull_state(m));
+  // Iterate in blocks of `chunk` iterations; segments that are done drop
+  // out inside the block, the host looks at the states between blocks.
+  const int chunk = P.check_every > 0 ? P.check_every : 8;
+  const int pre = P.count_pre;
+  const long long max_blocks = (long long)P.max_iter / chunk + 4;
+  if (P.use_graph) {
+    if (!(K->exec && K->key_esize == int(sizeof(T)) && K->key_pre == pre && K->key_iters == chunk &&
+          K->key_omega == P.omega && K->key_x == x)) {
+      krylov_drop_graph(K);
+      BCM_HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+      int rc = BCM_OK;
+      for (int it = 0; it < chunk && rc == BCM_OK; ++it) rc = iteration<T, N>(m, x, pre, P.omega);
+      hipGraph_t gph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(m->stream, &gph);
+      if (rc != BCM_OK) {
+        if (gph) (void)hipGraphDestroy(gph);
+        return rc;
+      }
+      BCM_HIP_TRY(ec);
+      K->graph = gph;
+      BCM_HIP_TRY(hipGraphInstantiate(&K->exec, K->graph, nullptr, nullptr, 0));
+      K->key_esize = int(sizeof(T));
+      K->key_pre = pre;
+      K->key_iters = chunk;
+      K->key_omega = P.omega;
+      K->key_x = x;
+    }
+  }
+  for (long long blkno = 0; blkno < max_blocks && any_active(m); ++blkno) {
+    if (P.use_graph) {
+      BCM_HIP_TRY(hipGraphLaunch(K->exec, m->stream));
+    } else {
+      for (int it = 0; it < chunk; ++it) BCM_TRY((iteration<T, N>(m, x, pre, P.omega)));
+    }
+    BCM_TRY(pull_state(m));
+  }
+  // Ghosts of the solution, as ImposeBoundaryCondition leaves them.
+  BCM_TRY((gather<double, N>(m, x, 0)));
+  BCM_TRY(pull_state(m));
+  for (int s = 0; s < m->nseg; ++s) {
+    const SegState& z = K->h_st[s];
+    bcm_solve_stats& o = stats[s];
+    o.status = z.status == BCM_SOLVE_ACTIVE ? BCM_SOLVE_NOCONV : z.status;
+    o.iters = z.iters;
+    o.refreshes = z.nref > 0 ? z.nref - 1 : 0;
+    o.pad = 0;
+    o.relres = z.bb > double(FLT_MIN) ? std::sqrt(z.rr_true / z.bb) : std::sqrt(z.rr_true);
+  }
+  return BCM_OK;
+}
+
+template <typename T>
+static int solve_n(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol,
+                   bcm_solve_stats* stats) {
+  switch (m->n) {
+    case 8: return solve_impl<T, 8>(m, x, rhs, P, seg_tol, stats);
+    case 16: return solve_impl<T, 16>(m, x, rhs, P, seg_tol, stats);
+    case 32: return solve_impl<T, 32>(m, x, rhs, P, seg_tol, stats);
+    default: return BCM_ERR_ARG;
+  }
+}
+
+}  // namespace bcm
+
+using namespace bcm;
+
+extern "C" int bcm_solve(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
+                         const bcm_solve_params* p, const double* seg_tol, bcm_solve_stats* stats) {
+  if (!x || !rhs || !p || !stats || x->mesh != rhs->mesh || x->dtype != BCM_F64 || rhs->dtype != BCM_F64)
+    return BCM_ERR_ARG;
+  bcm_mesh* m = x->mesh;
+  if (m->vc != 1 || (m->n != 8 && m->n != 16 && m->n != 32)) return BCM_ERR_ARG;
+  if (p->precision < BCM_PREC_FP64 || p->precision > BCM_PREC_FP32 || !(p->tol > 0.0) || p->max_iter < 1 ||
+      p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0 || p->refresh_max < 0 || p->refresh_delta < 0.0)
+    return BCM_ERR_ARG;
+  if (seg_tol)
+    for (int s = 0; s < m->nseg; ++s)
+      if (!(seg_tol[s] > 0.0)) return BCM_ERR_ARG;
+  BCM_TRY(set_bvalues(m, bvalues, nbvalues));
+  double* xd = static_cast<double*>(x->data);
+  const double* rd = static_cast<const double*>(rhs->data);
+  if (p->precision == BCM_PREC_FP64) return solve_n<double>(m, xd, rd, *p, seg_tol, stats);
+  return solve_n<float>(m, xd, rd, *p, seg_tol, stats);
+}

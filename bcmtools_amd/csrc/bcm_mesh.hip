@@ -291,6 +291,9 @@ using namespace bcm;
 static void mesh_release(bcm_mesh* m) {
   if (!m) return;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  krylov_release(m);
+  dev_free(m->d_seg_first);
+  dev_free(m->d_blk_seg);
   for (auto& p : m->pending) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
@@ -312,7 +315,8 @@ static void mesh_release(bcm_mesh* m) {
   delete m;
 }
 
-extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc, bcm_mesh_t* out) {
+static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc, const int* seg_counts, int nseg,
+                       bcm_mesh_t* out) {
   if (!out) return BCM_ERR_ARG;
   *out = nullptr;
   if (!blocks || nblocks <= 0 || n < 2 || (n % 2) != 0 || vc < 1 || n / 2 < vc || n > 256) return BCM_ERR_ARG;
@@ -450,12 +454,72 @@ extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n,
       rc = BCM_ERR_NOMEM;
       break;
     }
+    // Segment tables.
+    m->nseg = nseg;
+    m->seg_first.assign(size_t(nseg) + 1, 0);
+    std::vector<int> blk_seg(size_t(nblocks), 0);
+    for (int s = 0, b = 0; s < nseg; ++s) {
+      m->seg_first[s] = b;
+      for (int e = 0; e < seg_counts[s]; ++e) blk_seg[size_t(b++)] = s;
+      m->seg_first[s + 1] = b;
+    }
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_seg_first), sizeof(int) * (size_t(nseg) + 1))) != BCM_OK)
+      break;
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_blk_seg), sizeof(int) * size_t(nblocks))) != BCM_OK) break;
+    if (hipMemcpy(m->d_seg_first, m->seg_first.data(), sizeof(int) * (size_t(nseg) + 1), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(m->d_blk_seg, blk_seg.data(), sizeof(int) * size_t(nblocks), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = BCM_ERR_HIP;
+      break;
+    }
   } while (false);
   if (rc != BCM_OK) {
     mesh_release(m);
     return rc;
   }
   *out = m;
+  return BCM_OK;
+}
+
+extern "C" int bcm_mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc, bcm_mesh_t* out) {
+  const int one[1] = {nblocks};
+  return mesh_create(blocks, nblocks, n, vc, one, 1, out);
+}
+
+extern "C" int bcm_mesh_create_batch(const bcm_block_info* blocks, const int* counts, int nseg, int n, int vc,
+                                     bcm_mesh_t* out) {
+  if (!out) return BCM_ERR_ARG;
+  *out = nullptr;
+  if (!blocks || !counts || nseg <= 0) return BCM_ERR_ARG;
+  long long total = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (counts[s] <= 0) return BCM_ERR_ARG;
+    total += counts[s];
+  }
+  if (total > (1LL << 30)) return BCM_ERR_ARG;
+  // Shift each segment's neighbour ids (local to the segment) into the
+  // concatenated table.
+  std::vector<bcm_block_info> all(blocks, blocks + total);
+  for (int s = 0, b0 = 0; s < nseg; b0 += counts[s], ++s) {
+    for (int e = 0; e < counts[s]; ++e) {
+      bcm_block_info& bi = all[size_t(b0 + e)];
+      for (int f = 0; f < 6; ++f)
+        for (int q = 0; q < 4; ++q) {
+          const int nb = bi.nb_id[f][q];
+          if (nb < -1 || nb >= counts[s]) return BCM_ERR_ARG;
+          if (nb >= 0) bi.nb_id[f][q] = nb + b0;
+        }
+    }
+  }
+  return mesh_create(all.data(), int(total), n, vc, counts, nseg, out);
+}
+
+extern "C" int bcm_mesh_num_segments(bcm_mesh_t m) { return m ? m->nseg : BCM_ERR_ARG; }
+
+extern "C" int bcm_mesh_segment(bcm_mesh_t m, int seg, int* first_block, int* nblocks) {
+  if (!m || seg < 0 || seg >= m->nseg || !first_block || !nblocks) return BCM_ERR_ARG;
+  *first_block = m->seg_first[size_t(seg)];
+  *nblocks = m->seg_first[size_t(seg) + 1] - m->seg_first[size_t(seg)];
   return BCM_OK;
 }
 
@@ -567,7 +631,13 @@ extern "C" void* bcm_field_device_ptr(bcm_field_t f) { return f ? f->data : null
 extern "C" size_t bcm_field_bytes(bcm_field_t f) { return f ? f->bytes : 0; }
 
 extern "C" int bcm_ghost_fill(bcm_field_t f, int mask) {
-  if (!f || (mask & ~63)) return BCM_ERR_ARG;
+  if (!f || (mask & ~127)) return BCM_ERR_ARG;
+  if (mask & BCM_GF_BLOCKWISE) {
+    // All faces of every block in one pass; only the complete fill.
+    if ((mask & 63) != BCM_GF_ALL) return BCM_ERR_ARG;
+    if (f->dtype == BCM_F64) return ghost_gather_raw<double>(f->mesh, static_cast<double*>(f->data));
+    return ghost_gather_raw<float>(f->mesh, static_cast<float*>(f->data));
+  }
   if (f->dtype == BCM_F64) return ghost_fill_raw<double>(f->mesh, static_cast<double*>(f->data), mask);
   return ghost_fill_raw<float>(f->mesh, static_cast<float*>(f->data), mask);
 }

@@ -13,6 +13,7 @@
 #include <utility>
 
 #include "bcm_internal.h"
+#include "bcm_poisson.h"
 
 namespace bcm {
 
@@ -288,29 +289,6 @@ __global__ __launch_bounds__(kWG) void k_sor_residual(const double* __restrict__
 // outer Dirichlet face, Ap -= Anb; Anb = 0 (bca.f90:12-42).
 // ---------------------------------------------------------------------
 template <typename T>
-struct Coef7 {
-  T p, w, e, s, n, b, t;
-};
-
-template <typename T>
-__device__ __forceinline__ Coef7<T> poisson_coef(const BlockMeta& me, int i, int j, int k, int n) {
-  const T lx = T(me.cx), ly = T(me.cy), lz = T(me.cz);
-  Coef7<T> a;
-  a.w = lx; a.e = lx; a.s = ly; a.n = ly; a.b = lz; a.t = lz;
-  a.p = ((-(a.w + a.e)) - (a.s + a.n)) - (a.b + a.t);
-  const int dm = me.dir_mask;
-  if (dm) {
-    if ((dm & 1) && i == 0) { a.p = a.p - a.w; a.w = T(0); }
-    if ((dm & 2) && i == n - 1) { a.p = a.p - a.e; a.e = T(0); }
-    if ((dm & 4) && j == 0) { a.p = a.p - a.s; a.s = T(0); }
-    if ((dm & 8) && j == n - 1) { a.p = a.p - a.n; a.n = T(0); }
-    if ((dm & 16) && k == 0) { a.p = a.p - a.b; a.b = T(0); }
-    if ((dm & 32) && k == n - 1) { a.p = a.p - a.t; a.t = T(0); }
-  }
-  return a;
-}
-
-template <typename T>
 struct StoredA {
   const T *p, *w, *e, *s, *n, *b, *t;
 };
@@ -318,45 +296,6 @@ struct StoredA {
 template <typename T>
 __device__ __forceinline__ Coef7<T> stored_coef(const StoredA<T>& A, long long o) {
   return Coef7<T>{A.p[o], A.w[o], A.e[o], A.s[o], A.n[o], A.b[o], A.t[o]};
-}
-
-// jacobi_smoother (blas.f90:485-493).
-template <typename T>
-__device__ __forceinline__ T jacobi_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz,
-                                         T bv, T om) {
-  T r = bv - a.w * x[o - 1];
-  r = r - a.e * x[o + 1];
-  r = r - a.s * x[o - sy];
-  r = r - a.n * x[o + sy];
-  r = r - a.b * x[o - sz];
-  r = r - a.t * x[o + sz];
-  const T xt = r / a.p;
-  const T xc = x[o];
-  return xc - om * (xc - xt);
-}
-
-// calc_ax (blas.f90:584-591): neighbours first, centre last.
-template <typename T>
-__device__ __forceinline__ T ax_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz) {
-  T v = a.w * x[o - 1];
-  v = v + a.e * x[o + 1];
-  v = v + a.s * x[o - sy];
-  v = v + a.n * x[o + sy];
-  v = v + a.b * x[o - sz];
-  v = v + a.t * x[o + sz];
-  return v + a.p * x[o];
-}
-
-// calc_r (blas.f90:632-639).
-template <typename T>
-__device__ __forceinline__ T r_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz, T bv) {
-  T r = bv - a.w * x[o - 1];
-  r = r - a.e * x[o + 1];
-  r = r - a.s * x[o - sy];
-  r = r - a.n * x[o + sy];
-  r = r - a.b * x[o - sz];
-  r = r - a.t * x[o + sz];
-  return r - a.p * x[o];
 }
 
 template <typename T, bool STORED>

@@ -21,7 +21,11 @@ stored-ghost sweep k_rbsor_fused, which the same timer includes).
 cpu_baseline: the oracle (C, OpenMP, -O3 -ffp-contract=off) runs the same
 step on the same mesh for a bounded number of steps on the host cores.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sor|jacobi]
+Other workloads (--workload): jacobi (64 x 16^3, Jacobi7 + ghost fill),
+bicgstab (level-7 shell, 26,888 x 16^3: mixed FP32/FP64 BiCGSTAB to 1e-10
+against the FP64 paths), copy (HBM calibration).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sor|jacobi|bicgstab|copy]
 """
 import argparse
 import json
@@ -179,6 +183,110 @@ def bench_jacobi(args):
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
 
 
+def true_relres(mesh, x, rhs_field, bvalues):
+    """FP64 ||b - A x|| / ||b|| with the calc_r / assembly kernels (bit-exact with the oracle's)."""
+    b = mesh.field(np.float64)
+    mesh.poisson_assemble(None, b, rhs_field, bvalues)
+    r = mesh.field(np.float64)
+    mesh.calc_r_poisson(r, x, b)
+    rr, bb = mesh.dot(r, r), mesh.dot(b, b)
+    return float(np.sqrt(rr / bb))
+
+
+def bench_bicgstab(args):
+    """Mixed FP32/FP64 BiCGSTAB on the level-7 shell mesh (26,888 x 16^3) to 1e-10."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    tol, level, n = 1e-10, args.level, 16
+    t = W.config3_tree(level)
+    blocks = t.blocks
+    ncell = len(blocks) * n ** 3
+    print(f"[bench] bicgstab: level {level}, {len(blocks)} blocks x {n}^3 = {ncell} cells", file=sys.stderr, flush=True)
+    f = W.rhs_field(blocks, n, 1, seed=3)
+    rhs_host = -f
+    del f
+    mesh = Mesh(blocks, n, 1)
+    rhs = mesh.field(np.float64, rhs_host)
+    vals = [0.0]
+
+    def timed(fn, x):
+        mesh.fill(x, 0.0)
+        mesh.sync()
+        t0 = time.perf_counter()
+        out = fn(x)
+        mesh.sync()
+        return out, time.perf_counter() - t0
+
+    x = mesh.field(np.float64)
+    mixed = lambda xx: mesh.solve(xx, rhs, vals, precision="mixed", tol=tol, max_iter=20000)[0]
+    fp64 = lambda xx: mesh.solve(xx, rhs, vals, precision="fp64", tol=tol, max_iter=20000)[0]
+    for _ in range(args.warmup):
+        timed(mixed, x)
+    times, st = [], None
+    for _ in range(args.steps):
+        st, dt = timed(mixed, x)
+        times.append(dt)
+        print(f"[bench] mixed solve: {st} {dt:.3f} s", file=sys.stderr, flush=True)
+    xm = x.download()
+    relres_mixed = true_relres(mesh, x, rhs, vals)
+    x64 = mesh.field(np.float64)
+    timed(fp64, x64)
+    st64, t64 = timed(fp64, x64)
+    print(f"[bench] fp64 engine solve: {st64} {t64:.3f} s", file=sys.stderr, flush=True)
+    xh = mesh.field(np.float64)
+    (rc, ith, rrh), th = timed(lambda xx: mesh.solve_fp64(xx, rhs, vals, tol=tol, max_iter=20000), xh)
+    print(f"[bench] host-driven fp64 solve: rc={rc} iters={ith} relres={rrh:.3e} {th:.3f} s", file=sys.stderr,
+          flush=True)
+    a, b = W.interior(xm, 1), W.interior(x64.download(), 1)
+    err = float(np.abs(a - b).max() / np.abs(b).max())
+    elapsed = sum(times)
+    solves_per_s = len(times) / elapsed
+    res = {
+        "metric": "solves/s (BiCGSTAB to FP64 relative residual 1e-10, mixed FP32/FP64)",
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / len(times) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"adaptive shell octree level {level} (|x-c| = 0.25, margin 0.25), "
+                               f"{len(blocks)} x 16^3 blocks, Jacobi(2 sweeps, 0.8)-preconditioned BiCGSTAB, "
+                               "FP32 Krylov vectors / FP64 SpMV inputs, FP64 refinement to 1e-10",
+                   "blocks": len(blocks), "interior_cells": ncell, "tol": tol},
+        "mixed": {"iters": st["iters"], "refreshes": st["refreshes"], "relres": relres_mixed,
+                  "err_vs_fp64_solution": err},
+        "fp64_engine": {"iters": st64["iters"], "seconds": t64, "solves_per_s": 1.0 / t64,
+                        "relres": st64["relres"]},
+        "fp64_host_driven": {"iters": ith, "seconds": th, "solves_per_s": 1.0 / th, "relres": rrh},
+        "speedup_vs_fp64_host_driven": th / (elapsed / len(times)),
+        "speedup_vs_fp64_engine": t64 / (elapsed / len(times)),
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_bicgstab(blocks, n, rhs_host, ith, args.cpu_steps)
+    return res
+
+
+def cpu_bicgstab(blocks, n, rhs, gpu_iters, iters):
+    """The oracle's ILS3D BiCGSTAB (FP64, OpenMP) for a few iterations on the
+    same mesh; time-to-solution extrapolated to the GPU FP64 iteration count."""
+    import oracle
+
+    A = [np.zeros_like(rhs) for _ in range(7)]
+    b = np.zeros_like(rhs)
+    oracle.poisson_assemble(blocks, n, 1, A, b, rhs, [0.0])
+    x = np.zeros_like(rhs)
+    work = [np.zeros_like(rhs) for _ in range(9)]
+    t0 = time.perf_counter()
+    oracle.bicgstab(blocks, n, 1, x, A, b, work, 0.8, 2, iters, 1e-30)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    per_it = dt / iters
+    return {"value": 1.0 / (per_it * gpu_iters), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"same mesh, {iters} FP64 BiCGSTAB iterations ({dt:.1f} s, {per_it:.2f} s/iteration) on "
+                      f"{cores} threads of {cpu_model()}, extrapolated to the GPU's {gpu_iters} iterations"}
+
+
 def bench_copy(args):
     """Plain device copy of one config-2-sized FP64 field (calibrates HBM counters)."""
     from bcmtools_amd import Mesh
@@ -211,7 +319,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi", "copy"])
+    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi", "bicgstab", "copy"])
+    ap.add_argument("--level", type=int, default=7, help="bicgstab: shell refinement level")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -219,7 +328,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = {"sor": bench_sor, "jacobi": bench_jacobi, "copy": bench_copy}[args.workload](args)
+    res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab,
+           "copy": bench_copy}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -276,6 +276,81 @@ int bcm_solve_mixed(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int n
 int bcm_solve_fp64(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
                    const bcm_ir_params* p, int* iters, double* relres);
 
+/* ------------------------------------------------------------------ */
+/* Batched meshes and the fused device-resident BiCGSTAB                */
+/* ------------------------------------------------------------------ */
+/* Several independent meshes in one device mesh: the block tables are
+ * concatenated (segment s holds counts[s] blocks whose neighbour ids are
+ * local to the segment; they are shifted here), so every kernel covers all
+ * meshes in one launch while reductions and solver scalars stay per
+ * segment.  The reference runs one BlockManager per process
+ * (include/BlockManager.h:153-156); each segment is one such mesh.
+ * Boundary values index one table shared by all segments. */
+int bcm_mesh_create_batch(const bcm_block_info* blocks, const int* counts, int nseg, int n, int vc,
+                          bcm_mesh_t* out);
+int bcm_mesh_num_segments(bcm_mesh_t mesh);
+int bcm_mesh_segment(bcm_mesh_t mesh, int seg, int* first_block, int* nblocks);
+
+/* Ghost-fill method bit for bcm_ghost_fill, with the complete mask only
+ * (BCM_GF_ALL, all axes): one work-group per destination block gathers
+ * its six faces from the neighbours' interiors.  Same values as the
+ * per-face tasks of updateVC (include/Scalar3DUpdater.h:190-266, 321-347,
+ * 519-708); n in {4, 8, 16, 32}, vc = 1. */
+#define BCM_GF_BLOCKWISE 64
+
+/* ILS3D::BiCGSTAB (examples/SolverTest/ILS3D.h:663-844) with the Jacobi
+ * preconditioner (ILS3D.h:452-475) on the folded Poisson operator
+ * (bcut.f90:2848-2866, bca.f90:12-202), run entirely on the device: all
+ * scalars stay in per-segment device records, the dot products are fused
+ * into the kernels that produce their operands and blocks of
+ * check_every iterations replay as one hipGraph.  Every segment of the
+ * mesh is solved at once; a segment leaves the iteration when its true
+ * FP64 relative residual ||b - A x|| / ||b|| reaches its tolerance.
+ *   FP64:  the iteration in FP64.
+ *   MIXED: FP32 vectors (r, r0, p, q, s, t), FP32 Jacobi sweeps and
+ *          vector updates; FP64 for the two SpMV inputs (the
+ *          preconditioned p_ and s_), the SpMV arithmetic, every dot
+ *          product and the solution correction dx.  FP64 updates
+ *          x += dx, r = b - A x whenever the recurrence residual has
+ *          dropped by refresh_delta, every refresh_max iterations and when
+ *          the recurrence reports convergence (iterative refinement; with
+ *          restart_every = 1 each update also restarts the FP32 Krylov
+ *          space).
+ *   FP32:  as MIXED with FP64 updates only on reported convergence.
+ * x: FP64 field holding the initial guess, overwritten with the solution
+ * (ghosts filled); rhs: FP64 field whose interior is the right-hand side
+ * of A x = rhs before the Dirichlet fold (A = lap_h as assembled by
+ * bcm_poisson_assemble).  n in {8, 16, 32}, vc = 1.
+ * seg_tol (nullable): per-segment tolerances overriding p->tol.
+ * stats[nseg]: outcome per segment. */
+enum { BCM_PREC_FP64 = 0, BCM_PREC_MIXED = 1, BCM_PREC_FP32 = 2 };
+enum { BCM_SOLVE_ACTIVE = 0, BCM_SOLVE_CONVERGED = 1, BCM_SOLVE_NOCONV = 2, BCM_SOLVE_FAILED = 3 };
+typedef struct bcm_solve_params {
+  int32_t precision;          /* BCM_PREC_* */
+  double tol;                 /* true FP64 relative residual target */
+  int32_t max_iter;           /* BiCGSTAB iterations per segment */
+  int32_t count_pre;          /* Jacobi preconditioner sweeps (0..16) */
+  double omega;               /* Jacobi weight */
+  int32_t refresh_max;        /* MIXED: reliable update at least this often (0: 1000) */
+  double refresh_delta;       /* MIXED: update when ||r|| fell by this factor (0: 1e-3) */
+  int32_t check_every;        /* iterations per replay between host checks (0: 8) */
+  int32_t use_graph;          /* 1: capture the iteration block in a hipGraph */
+  int32_t restart_every;      /* MIXED/FP32: also restart the Krylov space (r0 = r, p = r)
+                                 at every k-th reliable update; 0: every update (k = 1);
+                                 < 0: only on breakdown or a failed convergence check */
+  int32_t stall_window;       /* MIXED/FP32: FP64 update when the recurrence residual set no
+                                 new minimum for this many iterations (0: 40, < 0: off) */
+} bcm_solve_params;
+typedef struct bcm_solve_stats {
+  int32_t status;             /* BCM_SOLVE_CONVERGED / NOCONV / FAILED */
+  int32_t iters;              /* BiCGSTAB iterations */
+  int32_t refreshes;          /* reliable updates after the initial residual */
+  int32_t pad;
+  double relres;              /* true FP64 relative residual at the last update */
+} bcm_solve_stats;
+int bcm_solve(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
+              const bcm_solve_params* p, const double* seg_tol, bcm_solve_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,148 @@
+"""Fused device-resident BiCGSTAB (bcm_solve) and the block-wise ghost gather.
+
+Ghost gather: bit-exact against the compiled reference's ghost-fill fixtures
+and against the per-face task kernel.  Solver: the true FP64 relative
+residual ||b - A x|| / ||b|| of every returned solution is recomputed on
+the CPU by the oracle (calc_r on the folded Poisson system) and must meet
+the requested tolerance; FP64 and mixed solutions agree with each other and
+with the host-driven FP64 path to 1e-8 relative; a mesh solved inside a batch
+gives bit-identical results to the same mesh solved alone.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from bcmtools_amd import Mesh
+from bcmtools_amd import _lib as L
+from bcmtools_amd import workloads as W
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def oracle_relres(blocks, n, x, rhs, bvalues=(0.0,)):
+    """||b - A x|| / ||b|| of the folded Poisson system, computed by the oracle."""
+    A = [np.zeros_like(rhs) for _ in range(7)]
+    b = np.zeros_like(rhs)
+    oracle.poisson_assemble(blocks, n, 1, A, b, rhs, list(bvalues))
+    r = np.zeros_like(rhs)
+    oracle.calc_r(len(blocks), n, 1, r, A, x, b)
+    ri, bi = W.interior(r, 1), W.interior(b, 1)
+    return float(np.sqrt(np.sum(ri * ri)) / np.sqrt(np.sum(bi * bi)))
+
+
+@pytest.mark.parametrize("name", ["ghost_sphere_atonce_d", "ghost_periodic_x_d", "ghost_sphere_atonce_f"])
+def test_blockwise_ghost_fill_vs_reference_fixture(name):
+    g = load_golden(name)
+    case(f"blockwise ghost fill {name} nb={len(g['blocks'])} n={g['n']}")
+    mesh = Mesh(g["blocks"], g["n"], g["vc"])
+    fld = mesh.field(g["fin"].dtype, g["fin"])
+    mesh.update_vc(fld, L.GF_ALL | L.GF_BLOCKWISE)
+    np.testing.assert_array_equal(fld.download(), g["fout"])
+
+
+@pytest.mark.parametrize("n,level,dtype", [(8, 4, np.float64), (16, 3, np.float64), (32, 3, np.float32),
+                                           (16, 4, np.float32)])
+def test_blockwise_ghost_fill_matches_tasks(n, level, dtype):
+    t = W.shell_tree(level)
+    blocks = t.blocks
+    case(f"blockwise vs tasks n={n} nb={len(blocks)} {np.dtype(dtype).name}")
+    rng = np.random.default_rng(level * 100 + n)
+    data = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2)).astype(dtype)
+    mesh = Mesh(blocks, n, 1)
+    a, b = mesh.field(dtype, data), mesh.field(dtype, data)
+    mesh.update_vc(a)
+    mesh.update_vc(b, L.GF_ALL | L.GF_BLOCKWISE)
+    np.testing.assert_array_equal(a.download(), b.download())
+
+
+def _problem(level, n, seed):
+    t = W.shell_tree(level, r_in=0.25, r_out=0.25, margin=0.25)
+    blocks = t.blocks
+    rhs = -W.rhs_field(blocks, n, 1, seed=seed)
+    return blocks, rhs
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-10), ("fp32", 1e-6)])
+def test_engine_reaches_tolerance(precision, tol):
+    blocks, rhs = _problem(4, 16, 11)
+    case(f"engine {precision} tol={tol} nb={len(blocks)}")
+    mesh = Mesh(blocks, 16, 1)
+    grhs = mesh.field(np.float64, rhs)
+    gx = mesh.field(np.float64)
+    st = mesh.solve(gx, grhs, [0.0], precision=precision, tol=tol, max_iter=3000)[0]
+    assert st["status"] == "converged", st
+    assert st["relres"] <= tol
+    x = gx.download()
+    rr = oracle_relres(blocks, 16, x, rhs)
+    assert rr <= tol * (1 + 1e-6), (rr, st)
+    if precision == "mixed":
+        assert st["refreshes"] >= 1
+
+
+def test_engine_mixed_matches_fp64_and_host_path():
+    blocks, rhs = _problem(4, 16, 12)
+    case(f"engine mixed vs fp64 nb={len(blocks)}")
+    mesh = Mesh(blocks, 16, 1)
+    grhs = mesh.field(np.float64, rhs)
+    x64, xmx, xhost = mesh.field(np.float64), mesh.field(np.float64), mesh.field(np.float64)
+    s64 = mesh.solve(x64, grhs, [0.0], precision="fp64", tol=1e-11)[0]
+    smx = mesh.solve(xmx, grhs, [0.0], precision="mixed", tol=1e-11)[0]
+    rc, it, rr = mesh.solve_fp64(xhost, grhs, [0.0], tol=1e-11)
+    assert s64["status"] == smx["status"] == "converged" and rc == L.OK
+    a, b, c = (W.interior(f.download(), 1) for f in (x64, xmx, xhost))
+    scale = np.abs(c).max()
+    assert np.abs(a - c).max() <= 1e-8 * scale
+    assert np.abs(b - c).max() <= 1e-8 * scale
+    # The fused FP64 iteration follows the host-driven one closely.
+    assert s64["iters"] <= it + max(10, it // 5)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_engine_batch_identical_to_alone(use_graph):
+    n = 16
+    trees = [W.random_tree(s, max_level=lv) for s, lv in ((1, 3), (2, 4), (3, 2))]
+    tables = [t.blocks for t in trees]
+    rhss = [-W.rhs_field(tb, n, 1, seed=20 + i) for i, tb in enumerate(tables)]
+    tols = [1e-10, 1e-8, 1e-10]
+    case(f"engine batch graph={use_graph} segments={[len(tb) for tb in tables]}")
+    mesh = Mesh.batch(tables, n)
+    assert mesh.nseg == 3
+    gx = mesh.field(np.float64)
+    grhs = mesh.field(np.float64, np.concatenate(rhss))
+    stats = mesh.solve(gx, grhs, [0.0], precision="mixed", tol=1e-10, seg_tol=tols, use_graph=use_graph,
+                       check_every=4)
+    xb = gx.download()
+    for s, (tb, rhs, tol) in enumerate(zip(tables, rhss, tols)):
+        first, cnt = mesh.segment(s)
+        assert cnt == len(tb)
+        alone = Mesh(tb, n)
+        ax = alone.field(np.float64)
+        st = alone.solve(ax, alone.field(np.float64, rhs), [0.0], precision="mixed", tol=tol,
+                         use_graph=not use_graph, check_every=8)[0]
+        assert stats[s] == st, (s, stats[s], st)
+        assert stats[s]["status"] == "converged" and stats[s]["relres"] <= tol
+        np.testing.assert_array_equal(xb[first:first + cnt], ax.download())
+        assert oracle_relres(tb, n, ax.download(), rhs) <= tol * (1 + 1e-6)
+
+
+def test_engine_zero_rhs_and_iteration_limit():
+    blocks, rhs = _problem(3, 8, 13)
+    case(f"engine edge cases nb={len(blocks)}")
+    mesh = Mesh.batch([blocks, blocks], 8)
+    rr = np.concatenate([np.zeros_like(rhs), rhs])
+    gx = mesh.field(np.float64, np.ones((2 * len(blocks), 10, 10, 10)))
+    st = mesh.solve(gx, mesh.field(np.float64, rr), [0.0], precision="mixed", tol=1e-10, max_iter=3)
+    assert st[0]["status"] == "converged" and st[0]["iters"] == 0
+    assert st[1]["status"] == "noconv" and st[1]["iters"] == 3
+    x = gx.download()
+    assert np.all(x[:len(blocks)] == 0.0)
+    assert np.all(np.isfinite(x))
