@@ -9,7 +9,7 @@ BlockManager / Scalar3D / Scalar3DUpdater API over that ABI.
 """
 from . import _lib
 from ._lib import BCMError, BLOCK_DTYPE
-from .core import Field, Mesh, Tree
+from .core import Field, Mesh, Server, Tree
 
 __all__ = ["BCMError", "BLOCK_DTYPE", "Field", "Mesh", "Tree", "build"]
 

@@ -28,6 +28,7 @@ BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z = range(4)
 GF_FLAT, GF_F2C, GF_C2F, GF_ALL = 1, 2, 4, 7
 GF_X, GF_Y, GF_Z = 8, 16, 32
 GF_BLOCKWISE = 64
+SOR_AUTO, SOR_FUSED_TASKS, SOR_FUSED_GATHER, SOR_HALO, SOR_COLOUR_TASKS, SOR_NUM_VARIANTS = range(6)
 PREC_FP64, PREC_MIXED, PREC_FP32 = range(3)
 SOLVE_ACTIVE, SOLVE_CONVERGED, SOLVE_NOCONV, SOLVE_FAILED = range(4)
 (K_GHOST, K_DIRICHLET, K_SOR, K_JACOBI, K_SPMV, K_RESID, K_DOT, K_BLAS1, K_REDUCE,
@@ -119,6 +120,32 @@ class SolveStats(ctypes.Structure):
     ]
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_double),
+        ("max_active", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32),
+        ("count_pre", ctypes.c_int32),
+        ("omega", ctypes.c_double),
+        ("check_every", ctypes.c_int32),
+    ]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [
+        ("tag", ctypes.c_int64),
+        ("status", ctypes.c_int32),
+        ("iters", ctypes.c_int32),
+        ("relres", ctypes.c_double),
+        ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+        ("nblocks", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+    ]
+
+
 class BCMError(RuntimeError):
     def __init__(self, code, where):
         super().__init__(f"{where} failed: {STATUS_NAMES.get(code, code)}")
@@ -148,6 +175,8 @@ SIGNATURES = {
     "bcm_field_destroy": (_I, [_P]),
     "bcm_field_upload": (_I, [_P, _P, ctypes.c_size_t]),
     "bcm_field_download": (_I, [_P, _P, ctypes.c_size_t]),
+    "bcm_field_upload_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),
+    "bcm_field_download_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),
     "bcm_field_device_ptr": (_P, [_P]),
     "bcm_field_bytes": (ctypes.c_size_t, [_P]),
     "bcm_ghost_fill": (_I, [_P, _I]),
@@ -155,6 +184,9 @@ SIGNATURES = {
     "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
+    "bcm_sor_set_variant": (_I, [_P, _I]),
+    "bcm_sor_get_variant": (_I, [_P]),
+    "bcm_sor_autotune": (_I, [_P, _P, _D, _DP, _I, _I, _IP, _DP]),
     "bcm_jacobi7": (_I, [_P, _P, ctypes.POINTER(_P), _P, _D]),
     "bcm_calc_ax": (_I, [_P, ctypes.POINTER(_P), _P]),
     "bcm_calc_r": (_I, [_P, ctypes.POINTER(_P), _P, _P]),
@@ -178,6 +210,14 @@ SIGNATURES = {
     "bcm_mesh_create_batch": (_I, [_P, _IP, _I, _I, _I, ctypes.POINTER(_P)]),
     "bcm_mesh_num_segments": (_I, [_P]),
     "bcm_mesh_segment": (_I, [_P, _I, _IP, _IP]),
+    "bcm_mesh_set_progress": (_I, [_P, _D]),
+    "bcm_server_create": (_I, [ctypes.POINTER(ServerParams), ctypes.POINTER(_P)]),
+    "bcm_server_destroy": (_I, [_P]),
+    "bcm_server_clock": (_D, [_P]),
+    "bcm_server_submit": (_I, [_P, _P, _I, _P, ctypes.c_uint64, _D, _I, _P, ctypes.c_int64]),
+    "bcm_server_progress": (_I, [_P, _D, _IP, _IP]),
+    "bcm_server_poll": (_I, [_P, ctypes.POINTER(ServerResult), _I]),
+    "bcm_server_stats": (_I, [_P, _DP, _IP, _IP]),
     "bcm_solve": (_I, [_P, _P, _DP, _I, ctypes.POINTER(SolveParams), _DP, ctypes.POINTER(SolveStats)]),
 }
 

@@ -87,6 +87,20 @@ class Field:
         L.check(self.mesh.lib.bcm_field_download(self.handle, a.ctypes.data, a.nbytes), "bcm_field_download")
         return a
 
+    def upload_blocks(self, first, host):
+        """Upload blocks first .. first+len(host)-1 (e.g. one segment of a batch)."""
+        m = self.mesh
+        a = np.ascontiguousarray(host, dtype=self.dtype).reshape((-1, m.nx0, m.nx0, m.nx0))
+        L.check(m.lib.bcm_field_upload_blocks(self.handle, int(first), len(a), a.ctypes.data, a.nbytes),
+                "bcm_field_upload_blocks")
+
+    def download_blocks(self, first, count):
+        m = self.mesh
+        a = np.empty((int(count), m.nx0, m.nx0, m.nx0), dtype=self.dtype)
+        L.check(m.lib.bcm_field_download_blocks(self.handle, int(first), int(count), a.ctypes.data, a.nbytes),
+                "bcm_field_download_blocks")
+        return a
+
     def device_ptr(self):
         return self.mesh.lib.bcm_field_device_ptr(self.handle)
 
@@ -138,6 +152,10 @@ class Mesh:
         """One device mesh holding several independent meshes (segments)."""
         tables = [np.ascontiguousarray(t, dtype=L.BLOCK_DTYPE) for t in block_tables]
         return cls(np.concatenate(tables), n, vc, segments=[len(t) for t in tables])
+
+    def set_progress(self, seconds):
+        """Progress lines on stderr from long solves (0 = off)."""
+        L.check(self.lib.bcm_mesh_set_progress(self.handle, float(seconds)), "bcm_mesh_set_progress")
 
     def segment(self, s):
         """(first block, block count) of segment s."""
@@ -206,6 +224,22 @@ class Mesh:
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
         L.check(self.lib.bcm_sor_iterate(f.handle, source.handle if source is not None else None, float(omega),
                                          L.dptr(v), len(v), int(iters)), "bcm_sor_iterate")
+
+    def set_sor_variant(self, variant):
+        L.check(self.lib.bcm_sor_set_variant(self.handle, int(variant)), "bcm_sor_set_variant")
+
+    def sor_variant(self):
+        return self.lib.bcm_sor_get_variant(self.handle)
+
+    def sor_autotune(self, f, source, omega, bvalues, iters=3):
+        """Time every supported outer-iteration schedule on a scratch copy of f,
+        keep the fastest; returns (best, {variant: ms per iteration})."""
+        v = np.ascontiguousarray(bvalues, dtype=np.float64)
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * L.SOR_NUM_VARIANTS)()
+        L.check(self.lib.bcm_sor_autotune(f.handle, None if source is None else source.handle, float(omega),
+                                          L.dptr(v), len(v), int(iters), ctypes.byref(best), ms), "bcm_sor_autotune")
+        return best.value, {i: ms[i] for i in range(1, L.SOR_NUM_VARIANTS) if ms[i] >= 0}
 
     def sor_residual(self, f, source):
         r = ctypes.c_double()
@@ -334,3 +368,74 @@ class Mesh:
         if rc not in (L.OK, L.ERR_NOCONV):
             L.check(rc, "bcm_solve_mixed")
         return rc, outer.value, inner.value, rr.value
+
+
+class Server:
+    """Request server: queued Poisson solves admitted by an HBM budget and
+    run with continuous batching (bcm_server_*)."""
+
+    PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "fp32": L.PREC_FP32}
+
+    def __init__(self, hbm_budget, max_active=8, n=16, max_iter=20000, count_pre=2, omega=0.8, check_every=8):
+        self.lib = L.load()
+        p = L.ServerParams(float(hbm_budget), int(max_active), int(n), int(max_iter), int(count_pre), float(omega),
+                           int(check_every))
+        h = ctypes.c_void_p()
+        L.check(self.lib.bcm_server_create(ctypes.byref(p), ctypes.byref(h)), "bcm_server_create")
+        self.handle = h
+        self.n = int(n)
+        self._keep = {}  # tag -> host arrays the server reads or writes later
+
+    def close(self):
+        if self.handle:
+            self.lib.bcm_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self):
+        return self.lib.bcm_server_clock(self.handle)
+
+    def submit(self, blocks, tol, precision="mixed", rhs=None, seed=0, tag=0, want_x=False):
+        blocks = np.ascontiguousarray(blocks, dtype=L.BLOCK_DTYPE)
+        nx = self.n + 2
+        r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).reshape(len(blocks), nx, nx, nx)
+        x = np.zeros((len(blocks), nx, nx, nx)) if want_x else None
+        L.check(self.lib.bcm_server_submit(self.handle, blocks.ctypes.data, len(blocks),
+                                           None if r is None else r.ctypes.data, int(seed), float(tol),
+                                           self.PRECISIONS[precision], None if x is None else x.ctypes.data,
+                                           int(tag)), "bcm_server_submit")
+        self._keep[int(tag)] = (blocks, r, x)
+
+    def progress(self, seconds):
+        a, q = ctypes.c_int(), ctypes.c_int()
+        L.check(self.lib.bcm_server_progress(self.handle, float(seconds), ctypes.byref(a), ctypes.byref(q)),
+                "bcm_server_progress")
+        return a.value, q.value
+
+    def poll(self):
+        """Finished requests: dicts with tag, status, iters, relres, times and the solution (if asked for)."""
+        buf = (L.ServerResult * 64)()
+        out = []
+        while True:
+            k = self.lib.bcm_server_poll(self.handle, buf, 64)
+            if k < 0:
+                L.check(-k, "bcm_server_poll")
+            for e in buf[:k]:
+                kept = self._keep.pop(int(e.tag), (None, None, None))
+                out.append({"tag": e.tag, "status": e.status, "iters": e.iters, "relres": e.relres,
+                            "t_submit": e.t_submit, "t_admit": e.t_admit, "t_done": e.t_done, "nblocks": e.nblocks,
+                            "precision": e.precision, "x": kept[2]})
+            if k < 64:
+                return out
+
+    def stats(self):
+        b, a, q = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        L.check(self.lib.bcm_server_stats(self.handle, ctypes.byref(b), ctypes.byref(a), ctypes.byref(q)),
+                "bcm_server_stats")
+        return {"bytes_in_use": b.value, "active": a.value, "queued": q.value}

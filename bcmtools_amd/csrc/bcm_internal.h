@@ -113,6 +113,10 @@ struct bcm_mesh {
 
   // Fused Krylov engine (vectors, per-segment states, captured graph).
   bcm::Krylov* kry = nullptr;
+  // Progress lines of long solves on stderr (seconds between lines, 0: off).
+  double progress_s = 0.0;
+  // Scheduling of bcm_sor_iterate (BCM_SOR_*), set by hand or by the autotuner.
+  int sor_variant = 0;
 };
 
 namespace bcm {
@@ -167,6 +171,16 @@ template <typename T> int xexport_raw(bcm_mesh* m, const T* A, T* xout);
 template <typename T> int ghost_gather_raw(bcm_mesh* m, T* data);
 // Frees the fused solver's state (bcm_krylov.hip).
 void krylov_release(bcm_mesh* m);
+// Phases of a fused solve on the mesh stream (bcm_krylov.hip): begin
+// (asynchronous set-up), advance (launch the next block of iterations once
+// the previous one is done; wait = block until it is; *active = false when
+// every segment has finished), finish (solution ghosts, outcome; syncs).
+int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg_tol);
+int krylov_begin(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& p, const double* seg_tol);
+int krylov_advance(bcm_mesh* m, bool wait, bool* active);
+int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats);
+// Device bytes the fused solver allocates for a mesh of nblocks n^3 blocks.
+size_t krylov_bytes(int nblocks, int n, int precision);
 // Field-sized workspace buffer kept on the mesh (bcm_solver.hip).
 void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc);
 enum WorkspaceSlot { WS_SOR_PONG_F64 = 30, WS_SOR_PONG_F32 = 31 };

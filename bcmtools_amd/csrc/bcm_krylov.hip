@@ -31,6 +31,8 @@
 #include <cfloat>
 #include <cmath>
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -122,9 +124,10 @@ __device__ __forceinline__ bool gated_out(const int* __restrict__ blk_seg, const
 // ---------------------------------------------------------------------
 template <typename T, int N, int GATE>
 __global__ __launch_bounds__(KWG) void k_gather(T* __restrict__ a, const int* __restrict__ desc,
-                                                const int* __restrict__ blk_seg, const SegState* __restrict__ st) {
+                                                const int* __restrict__ blk_seg, const SegState* __restrict__ st,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
-  const int b = blockIdx.x;
+  const int b = blist ? blist[blockIdx.x] : int(blockIdx.x);
   if (gated_out<GATE>(blk_seg, st, b)) return;
   __shared__ int d[48];
   if (threadIdx.x < 48) d[threadIdx.x] = desc[b * 48 + threadIdx.x];
@@ -189,9 +192,10 @@ template <typename T, int N, int PRE, typename TG>
 __global__ __launch_bounds__(KWG) void k_kry_p(T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ q,
                                                T* __restrict__ r0, TG* __restrict__ g,
                                                const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
-                                               const SegState* __restrict__ st, double omega) {
+                                               const SegState* __restrict__ st, double omega,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE) return;
   const bool first = sp->first != 0;
@@ -223,9 +227,10 @@ template <typename T, int N, int PRE, typename TG>
 __global__ __launch_bounds__(KWG) void k_kry_s(T* __restrict__ s, const T* __restrict__ r, const T* __restrict__ q,
                                                TG* __restrict__ g, const BlockMeta* __restrict__ meta,
                                                const int* __restrict__ blk_seg, const SegState* __restrict__ st,
-                                               double omega) {
+                                               double omega,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE) return;
   const T ta = T(-sp->alpha), om = T(omega);
@@ -267,9 +272,10 @@ template <typename T, int N, typename TI, typename TO>
 __global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, const TI* __restrict__ gin,
                                                     const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
                                                     const int* __restrict__ blk_seg,
-                                                    const SegState* __restrict__ st, double omega) {
+                                                    const SegState* __restrict__ st, double omega,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
   const T om = T(omega);
   const BlockMeta me = meta[b];
@@ -291,10 +297,11 @@ template <typename T, int N, int NDOT>
 __global__ __launch_bounds__(KWG) void k_kry_spmv(T* __restrict__ out, const double* __restrict__ g,
                                                   const T* __restrict__ y, const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blk_seg, const SegState* __restrict__ st,
-                                                  double* __restrict__ parts) {
+                                                  double* __restrict__ parts,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
   __shared__ double red[8];
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
   const BlockMeta me = meta[b];
   const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
@@ -323,10 +330,11 @@ __global__ __launch_bounds__(KWG) void k_kry_xr(double* __restrict__ dx, const d
                                                 const double* __restrict__ gs, T* __restrict__ r,
                                                 const T* __restrict__ s, const T* __restrict__ t,
                                                 const T* __restrict__ r0, const int* __restrict__ blk_seg,
-                                                const SegState* __restrict__ st, double* __restrict__ parts) {
+                                                const SegState* __restrict__ st, double* __restrict__ parts,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
   __shared__ double red[8];
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE) return;
   const double al = sp->alpha, ga = sp->gamma;
@@ -358,9 +366,10 @@ __global__ __launch_bounds__(KWG) void k_kry_xr(double* __restrict__ dx, const d
 // x += dx (interior), dx = 0.
 template <int N>
 __global__ __launch_bounds__(KWG) void k_ref_x(double* __restrict__ x, double* __restrict__ dx,
-                                               const int* __restrict__ blk_seg, const SegState* __restrict__ st) {
+                                               const int* __restrict__ blk_seg, const SegState* __restrict__ st,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   if (gated_out<2>(blk_seg, st, b)) return;
   const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
   for (int m = 0; m < B::PER; ++m) {
@@ -379,10 +388,11 @@ template <typename T, int N>
 __global__ __launch_bounds__(KWG) void k_ref_r(T* __restrict__ r, const T* __restrict__ r0,
                                                const double* __restrict__ x, const double* __restrict__ b64,
                                                const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
-                                               const SegState* __restrict__ st, double* __restrict__ parts) {
+                                               const SegState* __restrict__ st, double* __restrict__ parts,
+    const int* __restrict__ blist) {
   using B = Blk<N>;
   __shared__ double red[8];
-  const int b = blockIdx.x;
+  const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE || !sp->refresh) return;
   const BlockMeta me = meta[b];
@@ -579,7 +589,18 @@ __global__ __launch_bounds__(KWG) void k_seg(SegState* __restrict__ st, const do
 // ---------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------
+// Position of an ongoing solve (between the phases).
+struct Run {
+  double* x = nullptr;
+  int chunk = 8, pre = 2;
+  double omega = 0.8;
+  bool use_graph = true;
+  long long max_blocks = 0, blocks = 0;
+};
+
 struct Krylov {
+  Run run;
+  hipEvent_t ev = nullptr;  // the state mirror is valid once this completes
   int esize = 0;            // bytes per value of the sweep-precision vectors (4 or 8)
   void* v[6] = {};          // compact, sweep precision: r, r0, p, q, s, t
   double* dx = nullptr;     // compact FP64 correction since the last reliable update
@@ -589,10 +610,15 @@ struct Krylov {
   double* parts = nullptr;  // 2 per block
   SegState* st = nullptr;
   SegState* h_st = nullptr;  // pinned mirror
+  // Blocks the iteration kernels visit: those of the segments still active
+  // (work-group i takes block list[i]); shrunk between graph replays.
+  int* list = nullptr;
+  int n_list = 0;
+  std::vector<int> h_list;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // What the captured graph was built for.
-  int key_esize = 0, key_pre = -1, key_iters = 0;
+  int key_esize = 0, key_pre = -1, key_iters = 0, key_nlist = -1;
   double key_omega = 0.0;
   const void* key_x = nullptr;
 };
@@ -621,8 +647,10 @@ void krylov_release(bcm_mesh* m) {
   for (auto& p : k->g) dev_free(p);
   dev_free(k->b64);
   dev_free(k->parts);
+  dev_free(k->list);
   dev_free(k->st);
   if (k->h_st) (void)hipHostFree(k->h_st);
+  if (k->ev) (void)hipEventDestroy(k->ev);
   delete k;
   m->kry = nullptr;
 }
@@ -644,6 +672,7 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->b64), sizeof(double) * nc));
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->dx), sizeof(double) * nc));
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->parts), sizeof(double) * 2 * size_t(m->nblocks)));
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->list), sizeof(int) * size_t(m->nblocks)));
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->st), sizeof(SegState) * size_t(m->nseg)));
     BCM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&k->h_st), sizeof(SegState) * size_t(m->nseg)));
     // Ghost cells of outer faces are never written: start at zero.
@@ -671,14 +700,42 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
   return BCM_OK;
 }
 
+// Block-wise gather.  gate 0: every block of the mesh; 1 / 2: the blocks of
+// the active list whose segment is active (2: with an update pending).
 template <typename T, int N>
 static int gather(bcm_mesh* m, T* a, int gate) {
-  const dim3 grid(m->nblocks), blk(KWG);
+  const dim3 blk(KWG);
   const SegState* st = m->kry ? m->kry->st : nullptr;
-  if (gate == 0) hipLaunchKernelGGL((k_gather<T, N, 0>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st);
-  else if (gate == 1) hipLaunchKernelGGL((k_gather<T, N, 1>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st);
-  else hipLaunchKernelGGL((k_gather<T, N, 2>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st);
+  if (gate == 0) {
+    hipLaunchKernelGGL((k_gather<T, N, 0>), dim3(m->nblocks), blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st,
+                       (const int*)nullptr);
+  } else {
+    const dim3 grid(m->kry->n_list);
+    const int* bl = m->kry->list;
+    if (gate == 1)
+      hipLaunchKernelGGL((k_gather<T, N, 1>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st, bl);
+    else
+      hipLaunchKernelGGL((k_gather<T, N, 2>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st, bl);
+  }
   BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+
+// Rebuild the active list from the segment states in the host mirror
+// (blocks of active segments, in block order).
+static int set_list(bcm_mesh* m, bool all) {
+  Krylov* K = m->kry;
+  K->h_list.clear();
+  for (int s = 0; s < m->nseg; ++s) {
+    if (!all && K->h_st[s].status != BCM_SOLVE_ACTIVE) continue;
+    for (int b = m->seg_first[size_t(s)]; b < m->seg_first[size_t(s) + 1]; ++b) K->h_list.push_back(b);
+  }
+  K->n_list = int(K->h_list.size());
+  if (K->n_list > 0)
+    BCM_HIP_TRY(hipMemcpyAsync(K->list, K->h_list.data(), sizeof(int) * K->h_list.size(), hipMemcpyHostToDevice,
+                               m->stream));
+  // The copy reads the host vector: finish it before the vector can change.
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
   return BCM_OK;
 }
 
@@ -709,18 +766,19 @@ static int seg_reduce(bcm_mesh* m) {
 template <typename T, int N>
 static int precondition(bcm_mesh* m, T* scratch, double* result, const T* rhs, int pre, double omega) {
   bool in_result = (pre % 2 == 1);
-  const dim3 grid(m->nblocks), blk(KWG);
+  const dim3 grid(m->kry->n_list), blk(KWG);
   const int* bs = m->d_blk_seg;
+  const int* bl = m->kry->list;
   const SegState* st = m->kry->st;
   for (int sweep = 2; sweep <= pre; ++sweep) {
     if (in_result) {
       BCM_TRY((gather<double, N>(m, result, 1)));
       hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, result, rhs, m->d_meta,
-                         bs, st, omega);
+                         bs, st, omega, bl);
     } else {
       BCM_TRY((gather<T, N>(m, scratch, 1)));
       hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, scratch, rhs, m->d_meta,
-                         bs, st, omega);
+                         bs, st, omega, bl);
     }
     BCM_HIP_TRY(hipGetLastError());
     in_result = !in_result;
@@ -732,29 +790,31 @@ static int precondition(bcm_mesh* m, T* scratch, double* result, const T* rhs, i
 template <typename T, int N, bool IS_P>
 static int update_and_first_sweep(bcm_mesh* m, T* v, const T* r, const T* q, T* r0, T* scratch, double* result,
                                   int pre, double omega) {
-  const dim3 grid(m->nblocks), blk(KWG);
+  const dim3 grid(m->kry->n_list), blk(KWG);
   const int* bs = m->d_blk_seg;
+  const int* bl = m->kry->list;
   const SegState* st = m->kry->st;
   const bool to_result = pre == 0 || pre % 2 == 1;
   if (IS_P) {
     if (pre == 0)
       hipLaunchKernelGGL((k_kry_p<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, r0, result, m->d_meta, bs, st,
-                         omega);
+                         omega, bl);
     else if (to_result)
       hipLaunchKernelGGL((k_kry_p<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, r0, result, m->d_meta, bs, st,
-                         omega);
+                         omega, bl);
     else
       hipLaunchKernelGGL((k_kry_p<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, r0, scratch, m->d_meta, bs, st,
-                         omega);
+                         omega, bl);
   } else {
     if (pre == 0)
       hipLaunchKernelGGL((k_kry_s<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, result, m->d_meta, bs, st,
-                         omega);
+                         omega, bl);
     else if (to_result)
       hipLaunchKernelGGL((k_kry_s<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, result, m->d_meta, bs, st,
-                         omega);
+                         omega, bl);
     else
-      hipLaunchKernelGGL((k_kry_s<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, scratch, m->d_meta, bs, st, omega);
+      hipLaunchKernelGGL((k_kry_s<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, scratch, m->d_meta, bs, st, omega,
+                         bl);
   }
   BCM_HIP_TRY(hipGetLastError());
   return BCM_OK;
@@ -768,29 +828,30 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
     *q = static_cast<T*>(K->v[3]), *s = static_cast<T*>(K->v[4]), *t = static_cast<T*>(K->v[5]);
   T* g1 = static_cast<T*>(K->g1);
   double *gp = K->g[0], *gs = K->g[1];
-  const dim3 grid(m->nblocks), blk(KWG);
+  const dim3 grid(K->n_list), blk(KWG);
   const int* bs = m->d_blk_seg;
+  const int* bl = K->list;
   SegState* st = K->st;
   BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, gp, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, gp, p, pre, omega)));
   BCM_TRY((gather<double, N>(m, gp, 1)));
-  hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, r0, m->d_meta, bs, st, K->parts);
+  hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, r0, m->d_meta, bs, st, K->parts, bl);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_Q>(m));
   BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, gs, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, gs, s, pre, omega)));
   BCM_TRY((gather<double, N>(m, gs, 1)));
-  hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, s, m->d_meta, bs, st, K->parts);
+  hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, s, m->d_meta, bs, st, K->parts, bl);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_T>(m));
-  hipLaunchKernelGGL((k_kry_xr<T, N>), grid, blk, 0, m->stream, K->dx, gp, gs, r, s, t, r0, bs, st, K->parts);
+  hipLaunchKernelGGL((k_kry_xr<T, N>), grid, blk, 0, m->stream, K->dx, gp, gs, r, s, t, r0, bs, st, K->parts, bl);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_R>(m));
   // Reliable update where requested.
-  hipLaunchKernelGGL((k_ref_x<N>), grid, blk, 0, m->stream, x, K->dx, bs, st);
+  hipLaunchKernelGGL((k_ref_x<N>), grid, blk, 0, m->stream, x, K->dx, bs, st, bl);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY((gather<double, N>(m, x, 2)));
-  hipLaunchKernelGGL((k_ref_r<T, N>), grid, blk, 0, m->stream, r, r0, x, K->b64, m->d_meta, bs, st, K->parts);
+  hipLaunchKernelGGL((k_ref_r<T, N>), grid, blk, 0, m->stream, r, r0, x, K->b64, m->d_meta, bs, st, K->parts, bl);
   BCM_HIP_TRY(hipGetLastError());
   return seg_reduce<PH_F>(m);
 }
@@ -801,27 +862,32 @@ static bool any_active(const bcm_mesh* m) {
   return false;
 }
 
-static int pull_state(bcm_mesh* m) {
-  BCM_HIP_TRY(hipMemcpyAsync(m->kry->h_st, m->kry->st, sizeof(SegState) * size_t(m->nseg), hipMemcpyDeviceToHost,
-                             m->stream));
-  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+// Copy the segment states to the pinned mirror and mark the point in the
+// stream after which the mirror is valid.
+static int post_state(bcm_mesh* m) {
+  Krylov* K = m->kry;
+  BCM_HIP_TRY(hipMemcpyAsync(K->h_st, K->st, sizeof(SegState) * size_t(m->nseg), hipMemcpyDeviceToHost, m->stream));
+  if (!K->ev) BCM_HIP_TRY(hipEventCreateWithFlags(&K->ev, hipEventDisableTiming));
+  BCM_HIP_TRY(hipEventRecord(K->ev, m->stream));
   return BCM_OK;
 }
 
+// Phase 1: segment states, right-hand side, initial residual and (with
+// use_graph) the captured block of `chunk` iterations; all asynchronous.
 template <typename T, int N>
-static int solve_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol,
-                      bcm_solve_stats* stats) {
+static int begin_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol) {
   static_assert(Blk<N>::N3 % KWG == 0, "block volume must be a multiple of the work-group");
   Krylov* K = nullptr;
   BCM_TRY(krylov_get(m, int(sizeof(T)), &K));
   const dim3 grid(m->nblocks), blk(KWG);
-  // Segment states.
-  // Defaults measured on the shell meshes of levels 4-6 (16^3 blocks): an
-  // FP64 update each time the FP32 recurrence has gained three digits,
-  // restarting the Krylov space there, needs the fewest iterations.
+  // Defaults measured on the shell meshes of levels 4-7 (16^3 blocks): an
+  // FP64 update each time the FP32 recurrence has gained three digits or
+  // stalled for 40 iterations, restarting the Krylov space there.
   const double delta = P.precision == BCM_PREC_MIXED ? (P.refresh_delta > 0.0 ? P.refresh_delta : 1e-3) : 0.0;
   const int rmax = P.refresh_max > 0 ? P.refresh_max : (P.precision == BCM_PREC_MIXED ? 1000 : (1 << 30));
   const int rs_every = P.precision == BCM_PREC_FP64 ? 0 : (P.restart_every == 0 ? 1 : std::max(P.restart_every, 0));
+  const int stall = P.precision == BCM_PREC_FP64 ? 0 : (P.stall_window > 0 ? P.stall_window : (P.stall_window == 0 ? 40 : 0));
+  // The pinned mirror may still be the target of an earlier copy.
   BCM_HIP_TRY(hipStreamSynchronize(m->stream));
   for (int s = 0; s < m->nseg; ++s) {
     SegState z;
@@ -836,7 +902,7 @@ static int solve_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve
     z.max_iter = P.max_iter;
     z.refresh_max = rmax;
     z.restart_every = rs_every;
-    z.stall = P.precision == BCM_PREC_FP64 ? 0 : (P.stall_window > 0 ? P.stall_window : (P.stall_window == 0 ? 40 : 0));
+    z.stall = stall;
     z.rho_prev = 1.0;
     z.gamma = 1.0;
     K->h_st[s] = z;
@@ -849,52 +915,84 @@ static int solve_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve
   hipLaunchKernelGGL(k_zero_x, grid, blk, 0, m->stream, x, (long long)m->bvol, m->d_blk_seg, K->st);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(zero_async(m, K->dx, m->ncell));
-  // Initial residual r = b - A x, r0 = r.
+  BCM_TRY(set_list(m, true));
+  // Initial residual r = b - A x; the first pass sets r0 = r.
   BCM_TRY((gather<double, N>(m, x, 2)));
-  hipLaunchKernelGGL((k_ref_r<T, N>), grid, blk, 0, m->stream, static_cast<T*>(K->v[0]),
-                     static_cast<const T*>(K->v[1]), x, K->b64, m->d_meta, m->d_blk_seg, K->st, K->parts);
+  hipLaunchKernelGGL((k_ref_r<T, N>), dim3(K->n_list), blk, 0, m->stream, static_cast<T*>(K->v[0]),
+                     static_cast<const T*>(K->v[1]), x, K->b64, m->d_meta, m->d_blk_seg, K->st, K->parts, K->list);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_F>(m));
-  BCM_TRY(pull_state(m));
-  // Iterate in blocks of `chunk` iterations; segments that are done drop
-  // out inside the block, the host looks at the states between blocks.
-  const int chunk = P.check_every > 0 ? P.check_every : 8;
-  const int pre = P.count_pre;
-  const long long max_blocks = (long long)P.max_iter / chunk + 4;
-  if (P.use_graph) {
-    if (!(K->exec && K->key_esize == int(sizeof(T)) && K->key_pre == pre && K->key_iters == chunk &&
-          K->key_omega == P.omega && K->key_x == x)) {
-      krylov_drop_graph(K);
-      BCM_HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-      int rc = BCM_OK;
-      for (int it = 0; it < chunk && rc == BCM_OK; ++it) rc = iteration<T, N>(m, x, pre, P.omega);
-      hipGraph_t gph = nullptr;
-      const hipError_t ec = hipStreamEndCapture(m->stream, &gph);
-      if (rc != BCM_OK) {
-        if (gph) (void)hipGraphDestroy(gph);
-        return rc;
-      }
-      BCM_HIP_TRY(ec);
-      K->graph = gph;
-      BCM_HIP_TRY(hipGraphInstantiate(&K->exec, K->graph, nullptr, nullptr, 0));
-      K->key_esize = int(sizeof(T));
-      K->key_pre = pre;
-      K->key_iters = chunk;
-      K->key_omega = P.omega;
-      K->key_x = x;
-    }
+  Run& R = K->run;
+  R.x = x;
+  R.chunk = P.check_every > 0 ? P.check_every : 8;
+  R.pre = P.count_pre;
+  R.omega = P.omega;
+  R.use_graph = P.use_graph != 0;
+  R.max_blocks = (long long)P.max_iter / R.chunk + 4;
+  R.blocks = 0;
+  return post_state(m);
+}
+
+// The captured block of `chunk` iterations for the current list.
+template <typename T, int N>
+static int ensure_graph(bcm_mesh* m) {
+  Krylov* K = m->kry;
+  Run& R = K->run;
+  if (K->exec && K->key_esize == int(sizeof(T)) && K->key_pre == R.pre && K->key_iters == R.chunk &&
+      K->key_omega == R.omega && K->key_x == R.x && K->key_nlist == K->n_list)
+    return BCM_OK;
+  krylov_drop_graph(K);
+  BCM_HIP_TRY(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  int rc = BCM_OK;
+  for (int it = 0; it < R.chunk && rc == BCM_OK; ++it) rc = iteration<T, N>(m, R.x, R.pre, R.omega);
+  hipGraph_t gph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(m->stream, &gph);
+  if (rc != BCM_OK) {
+    if (gph) (void)hipGraphDestroy(gph);
+    return rc;
   }
-  for (long long blkno = 0; blkno < max_blocks && any_active(m); ++blkno) {
-    if (P.use_graph) {
-      BCM_HIP_TRY(hipGraphLaunch(K->exec, m->stream));
-    } else {
-      for (int it = 0; it < chunk; ++it) BCM_TRY((iteration<T, N>(m, x, pre, P.omega)));
-    }
-    BCM_TRY(pull_state(m));
+  BCM_HIP_TRY(ec);
+  K->graph = gph;
+  BCM_HIP_TRY(hipGraphInstantiate(&K->exec, K->graph, nullptr, nullptr, 0));
+  K->key_esize = int(sizeof(T));
+  K->key_pre = R.pre;
+  K->key_iters = R.chunk;
+  K->key_omega = R.omega;
+  K->key_x = R.x;
+  K->key_nlist = K->n_list;
+  return BCM_OK;
+}
+
+// Phase 2: one block of `chunk` iterations.
+template <typename T, int N>
+static int chunk_impl(bcm_mesh* m) {
+  Krylov* K = m->kry;
+  Run& R = K->run;
+  // Drop the blocks of finished segments once they are 30 % of the list:
+  // their work-groups would only return at once, but at 17 launches an
+  // iteration that costs milliseconds on a large batch.
+  long long active_blocks = 0;
+  for (int s = 0; s < m->nseg; ++s)
+    if (K->h_st[s].status == BCM_SOLVE_ACTIVE) active_blocks += m->seg_first[size_t(s) + 1] - m->seg_first[size_t(s)];
+  if (active_blocks > 0 && double(active_blocks) < 0.7 * K->n_list) BCM_TRY(set_list(m, false));
+  if (R.use_graph) {
+    BCM_TRY((ensure_graph<T, N>(m)));
+    BCM_HIP_TRY(hipGraphLaunch(K->exec, m->stream));
+  } else {
+    for (int it = 0; it < R.chunk; ++it) BCM_TRY((iteration<T, N>(m, R.x, R.pre, R.omega)));
   }
-  // Ghosts of the solution, as ImposeBoundaryCondition leaves them.
-  BCM_TRY((gather<double, N>(m, x, 0)));
-  BCM_TRY(pull_state(m));
+  R.blocks += 1;
+  return post_state(m);
+}
+
+// Phase 3: ghosts of the solution (as ImposeBoundaryCondition leaves them)
+// and the per-segment outcome.
+template <int N>
+static int finish_impl(bcm_mesh* m, bcm_solve_stats* stats) {
+  Krylov* K = m->kry;
+  BCM_TRY((gather<double, N>(m, K->run.x, 0)));
+  BCM_TRY(post_state(m));
+  BCM_HIP_TRY(hipEventSynchronize(K->ev));
   for (int s = 0; s < m->nseg; ++s) {
     const SegState& z = K->h_st[s];
     bcm_solve_stats& o = stats[s];
@@ -907,26 +1005,50 @@ static int solve_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve
   return BCM_OK;
 }
 
-template <typename T>
-static int solve_n(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol,
-                   bcm_solve_stats* stats) {
+int krylov_begin(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol) {
+  const bool f64 = P.precision == BCM_PREC_FP64;
   switch (m->n) {
-    case 8: return solve_impl<T, 8>(m, x, rhs, P, seg_tol, stats);
-    case 16: return solve_impl<T, 16>(m, x, rhs, P, seg_tol, stats);
-    case 32: return solve_impl<T, 32>(m, x, rhs, P, seg_tol, stats);
+    case 8: return f64 ? begin_impl<double, 8>(m, x, rhs, P, seg_tol) : begin_impl<float, 8>(m, x, rhs, P, seg_tol);
+    case 16: return f64 ? begin_impl<double, 16>(m, x, rhs, P, seg_tol) : begin_impl<float, 16>(m, x, rhs, P, seg_tol);
+    case 32: return f64 ? begin_impl<double, 32>(m, x, rhs, P, seg_tol) : begin_impl<float, 32>(m, x, rhs, P, seg_tol);
     default: return BCM_ERR_ARG;
   }
 }
 
-}  // namespace bcm
+int krylov_advance(bcm_mesh* m, bool wait, bool* active) {
+  Krylov* K = m->kry;
+  if (!K || !K->ev) return BCM_ERR_STATE;
+  *active = true;
+  if (wait) {
+    BCM_HIP_TRY(hipEventSynchronize(K->ev));
+  } else {
+    const hipError_t q = hipEventQuery(K->ev);
+    if (q == hipErrorNotReady) return BCM_OK;
+    BCM_HIP_TRY(q);
+  }
+  if (!any_active(m) || K->run.blocks >= K->run.max_blocks) {
+    *active = false;
+    return BCM_OK;
+  }
+  const bool f64 = K->esize == 8;
+  switch (m->n) {
+    case 8: return f64 ? chunk_impl<double, 8>(m) : chunk_impl<float, 8>(m);
+    case 16: return f64 ? chunk_impl<double, 16>(m) : chunk_impl<float, 16>(m);
+    case 32: return f64 ? chunk_impl<double, 32>(m) : chunk_impl<float, 32>(m);
+    default: return BCM_ERR_ARG;
+  }
+}
 
-using namespace bcm;
+int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats) {
+  switch (m->n) {
+    case 8: return finish_impl<8>(m, stats);
+    case 16: return finish_impl<16>(m, stats);
+    case 32: return finish_impl<32>(m, stats);
+    default: return BCM_ERR_ARG;
+  }
+}
 
-extern "C" int bcm_solve(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
-                         const bcm_solve_params* p, const double* seg_tol, bcm_solve_stats* stats) {
-  if (!x || !rhs || !p || !stats || x->mesh != rhs->mesh || x->dtype != BCM_F64 || rhs->dtype != BCM_F64)
-    return BCM_ERR_ARG;
-  bcm_mesh* m = x->mesh;
+int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg_tol) {
   if (m->vc != 1 || (m->n != 8 && m->n != 16 && m->n != 32)) return BCM_ERR_ARG;
   if (p->precision < BCM_PREC_FP64 || p->precision > BCM_PREC_FP32 || !(p->tol > 0.0) || p->max_iter < 1 ||
       p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0 || p->refresh_max < 0 || p->refresh_delta < 0.0)
@@ -934,9 +1056,54 @@ extern "C" int bcm_solve(bcm_field_t x, bcm_field_t rhs, const double* bvalues, 
   if (seg_tol)
     for (int s = 0; s < m->nseg; ++s)
       if (!(seg_tol[s] > 0.0)) return BCM_ERR_ARG;
+  return BCM_OK;
+}
+
+size_t krylov_bytes(int nblocks, int n, int precision) {
+  const size_t nc = size_t(nblocks) * n * n * n, ng = size_t(nblocks) * (n + 2) * (n + 2) * (n + 2);
+  const size_t es = precision == BCM_PREC_FP64 ? 8 : 4;
+  // b, dx (FP64 compact); p_, s_ (FP64 ghosted); r, r0, p, q, s, t (compact); sweep scratch (ghosted).
+  return 2 * nc * 8 + 2 * ng * 8 + 6 * nc * es + ng * es + size_t(nblocks) * 16;
+}
+
+}  // namespace bcm
+
+using namespace bcm;
+
+extern "C" int bcm_mesh_set_progress(bcm_mesh_t m, double seconds) {
+  if (!m || !(seconds >= 0.0)) return BCM_ERR_ARG;
+  m->progress_s = seconds;
+  return BCM_OK;
+}
+
+extern "C" int bcm_solve(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
+                         const bcm_solve_params* p, const double* seg_tol, bcm_solve_stats* stats) {
+  if (!x || !rhs || !p || !stats || x->mesh != rhs->mesh || x->dtype != BCM_F64 || rhs->dtype != BCM_F64)
+    return BCM_ERR_ARG;
+  bcm_mesh* m = x->mesh;
+  BCM_TRY(krylov_check(m, p, seg_tol));
   BCM_TRY(set_bvalues(m, bvalues, nbvalues));
-  double* xd = static_cast<double*>(x->data);
-  const double* rd = static_cast<const double*>(rhs->data);
-  if (p->precision == BCM_PREC_FP64) return solve_n<double>(m, xd, rd, *p, seg_tol, stats);
-  return solve_n<float>(m, xd, rd, *p, seg_tol, stats);
+  BCM_TRY(krylov_begin(m, static_cast<double*>(x->data), static_cast<const double*>(rhs->data), *p, seg_tol));
+  using clock = std::chrono::steady_clock;
+  const auto t_start = clock::now();
+  auto t_last = t_start;
+  bool active = true;
+  while (active) {
+    BCM_TRY(krylov_advance(m, true, &active));
+    if (m->progress_s > 0.0) {
+      const auto now = clock::now();
+      if (std::chrono::duration<double>(now - t_last).count() >= m->progress_s) {
+        t_last = now;
+        int done = 0, most = 0;
+        for (int s = 0; s < m->nseg; ++s) {
+          done += m->kry->h_st[s].status != BCM_SOLVE_ACTIVE;
+          most = std::max(most, int(m->kry->h_st[s].iters));
+        }
+        std::fprintf(stderr, "[bcm_solve] %.1f s: %d/%d segments done, %d iterations (most)\n",
+                     std::chrono::duration<double>(now - t_start).count(), done, m->nseg, most);
+        std::fflush(stderr);
+      }
+    }
+  }
+  return krylov_finish(m, stats);
 }

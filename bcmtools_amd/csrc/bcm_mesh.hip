@@ -626,6 +626,33 @@ extern "C" int bcm_field_download(bcm_field_t f, void* host, size_t bytes) {
   return BCM_OK;
 }
 
+// Blocks first_block .. first_block+nblocks-1 (e.g. one segment of a batch).
+static int block_range(bcm_field_t f, int first_block, int nblocks, size_t bytes, size_t* off, size_t* len) {
+  if (!f || first_block < 0 || nblocks < 0 || first_block + nblocks > f->mesh->nblocks) return BCM_ERR_ARG;
+  const size_t es = f->dtype == BCM_F64 ? 8 : 4;
+  *off = size_t(first_block) * f->mesh->bvol * es;
+  *len = size_t(nblocks) * f->mesh->bvol * es;
+  return bytes == *len ? BCM_OK : BCM_ERR_ARG;
+}
+
+extern "C" int bcm_field_upload_blocks(bcm_field_t f, int first_block, int nblocks, const void* host, size_t bytes) {
+  size_t off = 0, len = 0;
+  BCM_TRY(block_range(f, first_block, nblocks, bytes, &off, &len));
+  if (!host && len) return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipMemcpyAsync(static_cast<char*>(f->data) + off, host, len, hipMemcpyHostToDevice, f->mesh->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(f->mesh->stream));
+  return BCM_OK;
+}
+
+extern "C" int bcm_field_download_blocks(bcm_field_t f, int first_block, int nblocks, void* host, size_t bytes) {
+  size_t off = 0, len = 0;
+  BCM_TRY(block_range(f, first_block, nblocks, bytes, &off, &len));
+  if (!host && len) return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipMemcpyAsync(host, static_cast<const char*>(f->data) + off, len, hipMemcpyDeviceToHost, f->mesh->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(f->mesh->stream));
+  return BCM_OK;
+}
+
 extern "C" void* bcm_field_device_ptr(bcm_field_t f) { return f ? f->data : nullptr; }
 
 extern "C" size_t bcm_field_bytes(bcm_field_t f) { return f ? f->bytes : 0; }

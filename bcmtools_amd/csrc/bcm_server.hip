@@ -1,0 +1,313 @@
+// Request server: Poisson solves of independent meshes arriving over time,
+// run with continuous batching on one GPU.
+//
+// Every admitted request owns a device mesh (its own HIP stream), its
+// fields and its fused-solver state; the server loop keeps every admitted
+// request's stream fed with hipGraph-replayed blocks of iterations and
+// polls completion events without blocking, so the work of all admitted
+// requests overlaps on the device.  Requests wait in FIFO order and are
+// admitted (first fit) while the device bytes they need fit in the HBM
+// budget and a slot is free; a finished request's memory is released at
+// once, which admits the next ones.  Each request is computed exactly as it
+// would be alone (its kernels read only its own mesh), so its result does
+// not depend on what else is running.
+//
+// The reference has no counterpart (one BlockManager per process,
+// include/BlockManager.h:153-156); a request's solve is bcm_solve's fused
+// BiCGSTAB (examples/SolverTest/ILS3D.h:663-844).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "bcm_internal.h"
+
+namespace bcm {
+namespace {
+
+// Synthetic right-hand side of a request (rhs == NULL at submission):
+// f = sin(pi x) sin(pi y) sin(pi z) + sigma N(0, 1) at cell centres, the
+// noise from a counter-based generator keyed by (seed, block pedigree,
+// cell), so a request's data depends on nothing but the request.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+struct BlockGeo {
+  double ox, oy, oz, hx, hy, hz;
+  unsigned long long ped;
+};
+
+__global__ __launch_bounds__(256) void k_rhs_synth(double* __restrict__ rhs, const BlockGeo* __restrict__ geo, int n,
+                                                   unsigned long long seed, double sigma) {
+  const int b = blockIdx.x;
+  const BlockGeo g = geo[b];
+  const int nx = n + 2;
+  const long long base = (long long)b * nx * nx * nx;
+  const unsigned long long key = mix64(seed * 0xD1B54A32D192ED03ULL + mix64(g.ped));
+  const double pi = 3.14159265358979323846;
+  for (int c = threadIdx.x; c < n * n * n; c += 256) {
+    const int i = c % n, j = (c / n) % n, k = c / (n * n);
+    const double x = g.ox + (i + 0.5) * g.hx, y = g.oy + (j + 0.5) * g.hy, z = g.oz + (k + 0.5) * g.hz;
+    const unsigned long long h1 = mix64(key + (unsigned long long)c);
+    const unsigned long long h2 = mix64(h1 ^ 0x632BE59BD9B4E019ULL);
+    const double u1 = 1.0 - double(h1 >> 11) * (1.0 / 9007199254740992.0);
+    const double u2 = double(h2 >> 11) * (1.0 / 9007199254740992.0);
+    const double nz = sqrt(-2.0 * log(u1)) * cos(2.0 * pi * u2);
+    const double f = sin(pi * z) * sin(pi * y) * sin(pi * x) + sigma * nz;
+    // Right-hand side of A u = rhs with A = lap_h: rhs = -f.
+    rhs[base + (i + 1) + nx * ((j + 1) + nx * (k + 1))] = -f;
+  }
+}
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t tag = 0;
+  std::vector<bcm_block_info> blocks;
+  const double* rhs_host = nullptr;
+  unsigned long long seed = 0;
+  double tol = 1e-10;
+  int precision = BCM_PREC_MIXED;
+  double* x_out = nullptr;
+  double t_submit = 0.0, t_admit = 0.0;
+  size_t bytes = 0;
+  bool counted = false;  // bytes charged to the budget
+  bcm_mesh_t mesh = nullptr;
+  bcm_field_t x = nullptr, rhs = nullptr;
+};
+
+}  // namespace
+}  // namespace bcm
+
+struct bcm_server {
+  bcm_server_params p{};
+  std::deque<bcm::Request*> queue;
+  std::vector<bcm::Request*> active;
+  std::deque<bcm_server_result> done;
+  size_t in_use = 0;
+  bcm::Clock::time_point t0;
+  double now() const { return std::chrono::duration<double>(bcm::Clock::now() - t0).count(); }
+};
+
+namespace bcm {
+namespace {
+
+// Device bytes of one request: mesh tables, x and rhs, the solver state.
+size_t request_bytes(const bcm_server* sv, int nblocks, int precision) {
+  const int n = sv->p.n;
+  const size_t ng = size_t(nblocks) * (n + 2) * (n + 2) * (n + 2);
+  return size_t(nblocks) * 640 + 2 * ng * 8 + krylov_bytes(nblocks, n, precision);
+}
+
+void release(bcm_server* sv, Request* r) {
+  if (r->x) bcm_field_destroy(r->x);
+  if (r->rhs) bcm_field_destroy(r->rhs);
+  if (r->mesh) bcm_mesh_destroy(r->mesh);
+  r->x = r->rhs = nullptr;
+  r->mesh = nullptr;
+  if (r->counted) sv->in_use -= r->bytes;
+  delete r;
+}
+
+void finish_result(bcm_server* sv, Request* r, int status, const bcm_solve_stats* st) {
+  bcm_server_result o;
+  std::memset(&o, 0, sizeof(o));
+  o.tag = r->tag;
+  o.status = status;
+  o.nblocks = int(r->blocks.size());
+  o.precision = r->precision;
+  o.t_submit = r->t_submit;
+  o.t_admit = r->t_admit;
+  o.t_done = sv->now();
+  if (st) {
+    o.iters = st->iters;
+    o.relres = st->relres;
+  }
+  sv->done.push_back(o);
+}
+
+int admit(bcm_server* sv, Request* r) {
+  const int n = sv->p.n;
+  BCM_TRY(bcm_mesh_create(r->blocks.data(), int(r->blocks.size()), n, 1, &r->mesh));
+  BCM_TRY(bcm_field_create(r->mesh, BCM_F64, &r->x));
+  BCM_TRY(bcm_field_create(r->mesh, BCM_F64, &r->rhs));
+  bcm_mesh* m = r->mesh;
+  if (r->rhs_host) {
+    BCM_TRY(bcm_field_upload(r->rhs, r->rhs_host, r->rhs->bytes));
+  } else {
+    std::vector<BlockGeo> geo(r->blocks.size());
+    for (size_t b = 0; b < geo.size(); ++b) {
+      const bcm_block_info& bi = r->blocks[b];
+      geo[b] = BlockGeo{bi.origin[0], bi.origin[1], bi.origin[2], bi.block_size[0] / n, bi.block_size[1] / n,
+                        bi.block_size[2] / n, (unsigned long long)bi.pedigree};
+    }
+    void* d = nullptr;
+    BCM_TRY(dev_alloc(&d, sizeof(BlockGeo) * geo.size()));
+    hipError_t e = hipMemcpyAsync(d, geo.data(), sizeof(BlockGeo) * geo.size(), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_rhs_synth, dim3(unsigned(geo.size())), dim3(256), 0, m->stream,
+                         static_cast<double*>(r->rhs->data), static_cast<const BlockGeo*>(d), n, r->seed, 0.01);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    dev_free(d);
+    if (e != hipSuccess) return BCM_ERR_HIP;
+  }
+  const double zero = 0.0;
+  BCM_TRY(set_bvalues(m, &zero, 1));
+  bcm_solve_params P;
+  std::memset(&P, 0, sizeof(P));
+  P.precision = r->precision;
+  P.tol = r->tol;
+  P.max_iter = sv->p.max_iter;
+  P.count_pre = sv->p.count_pre;
+  P.omega = sv->p.omega;
+  P.check_every = sv->p.check_every;
+  P.use_graph = 1;
+  BCM_TRY(krylov_check(m, &P, nullptr));
+  return krylov_begin(m, static_cast<double*>(r->x->data), static_cast<const double*>(r->rhs->data), P, nullptr);
+}
+
+// Finish a request whose solve is over: outcome, solution to the caller.
+int complete(bcm_server* sv, Request* r) {
+  bcm_solve_stats st;
+  BCM_TRY(krylov_finish(r->mesh, &st));
+  if (r->x_out) BCM_TRY(bcm_field_download(r->x, r->x_out, r->x->bytes));
+  finish_result(sv, r, st.status, &st);
+  return BCM_OK;
+}
+
+// Admit queued requests (first fit in arrival order) while slots and the
+// HBM budget allow.  A request larger than the whole budget is refused.
+void admit_ready(bcm_server* sv) {
+  for (auto it = sv->queue.begin(); it != sv->queue.end() && int(sv->active.size()) < sv->p.max_active;) {
+    Request* r = *it;
+    if (double(r->bytes) > sv->p.hbm_budget) {
+      finish_result(sv, r, -BCM_ERR_BUSY, nullptr);
+      it = sv->queue.erase(it);
+      delete r;
+      continue;
+    }
+    if (double(sv->in_use + r->bytes) > sv->p.hbm_budget) {
+      ++it;
+      continue;
+    }
+    it = sv->queue.erase(it);
+    r->t_admit = sv->now();
+    sv->in_use += r->bytes;
+    r->counted = true;
+    const int rc = admit(sv, r);
+    if (rc != BCM_OK) {
+      finish_result(sv, r, -rc, nullptr);
+      release(sv, r);
+      continue;
+    }
+    sv->active.push_back(r);
+  }
+}
+
+}  // namespace
+}  // namespace bcm
+
+using namespace bcm;
+
+extern "C" int bcm_server_create(const bcm_server_params* p, bcm_server_t* out) {
+  if (!p || !out) return BCM_ERR_ARG;
+  *out = nullptr;
+  if (!(p->hbm_budget > 0.0) || p->max_active < 1 || (p->n != 8 && p->n != 16 && p->n != 32) || p->max_iter < 1 ||
+      p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0)
+    return BCM_ERR_ARG;
+  bcm_server* sv = new (std::nothrow) bcm_server();
+  if (!sv) return BCM_ERR_NOMEM;
+  sv->p = *p;
+  sv->t0 = Clock::now();
+  *out = sv;
+  return BCM_OK;
+}
+
+extern "C" int bcm_server_destroy(bcm_server_t sv) {
+  if (!sv) return BCM_ERR_ARG;
+  for (Request* r : sv->active) release(sv, r);
+  for (Request* r : sv->queue) delete r;
+  delete sv;
+  return BCM_OK;
+}
+
+extern "C" double bcm_server_clock(bcm_server_t sv) { return sv ? sv->now() : -1.0; }
+
+extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, int nblocks, const double* rhs,
+                                 uint64_t seed, double tol, int precision, double* x_out, int64_t tag) {
+  if (!sv || !blocks || nblocks <= 0 || !(tol > 0.0) || precision < BCM_PREC_FP64 || precision > BCM_PREC_FP32)
+    return BCM_ERR_ARG;
+  for (int b = 0; b < nblocks; ++b)
+    for (int f = 0; f < 6; ++f)
+      for (int q = 0; q < 4; ++q)
+        if (blocks[b].nb_id[f][q] < -1 || blocks[b].nb_id[f][q] >= nblocks) return BCM_ERR_ARG;
+  Request* r = new (std::nothrow) Request();
+  if (!r) return BCM_ERR_NOMEM;
+  r->tag = tag;
+  r->blocks.assign(blocks, blocks + nblocks);
+  r->rhs_host = rhs;
+  r->seed = seed;
+  r->tol = tol;
+  r->precision = precision;
+  r->x_out = x_out;
+  r->t_submit = sv->now();
+  r->bytes = request_bytes(sv, nblocks, precision);
+  sv->queue.push_back(r);
+  admit_ready(sv);
+  return BCM_OK;
+}
+
+extern "C" int bcm_server_progress(bcm_server_t sv, double max_seconds, int* n_active, int* n_queued) {
+  if (!sv || !(max_seconds >= 0.0)) return BCM_ERR_ARG;
+  const double t_end = sv->now() + max_seconds;
+  do {
+    bool moved = false;
+    for (size_t i = 0; i < sv->active.size();) {
+      Request* r = sv->active[i];
+      bool act = true;
+      int rc = krylov_advance(r->mesh, false, &act);
+      if (rc == BCM_OK && act) {
+        ++i;
+        continue;
+      }
+      moved = true;
+      if (rc == BCM_OK) rc = complete(sv, r);
+      if (rc != BCM_OK) finish_result(sv, r, -rc, nullptr);
+      sv->active.erase(sv->active.begin() + long(i));
+      release(sv, r);
+    }
+    if (moved) admit_ready(sv);
+    if (sv->active.empty() && sv->queue.empty()) break;
+    if (!moved) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  } while (sv->now() < t_end);
+  if (n_active) *n_active = int(sv->active.size());
+  if (n_queued) *n_queued = int(sv->queue.size());
+  return BCM_OK;
+}
+
+extern "C" int bcm_server_poll(bcm_server_t sv, bcm_server_result* out, int capacity) {
+  if (!sv || (capacity > 0 && !out) || capacity < 0) return -BCM_ERR_ARG;
+  int k = 0;
+  while (k < capacity && !sv->done.empty()) {
+    out[k++] = sv->done.front();
+    sv->done.pop_front();
+  }
+  return k;
+}
+
+extern "C" int bcm_server_stats(bcm_server_t sv, double* bytes_in_use, int* n_active, int* n_queued) {
+  if (!sv) return BCM_ERR_ARG;
+  if (bytes_in_use) *bytes_in_use = double(sv->in_use);
+  if (n_active) *n_active = int(sv->active.size());
+  if (n_queued) *n_queued = int(sv->queue.size());
+  return BCM_OK;
+}

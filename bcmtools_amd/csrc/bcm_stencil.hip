@@ -227,19 +227,22 @@ __global__ __launch_bounds__(256) void k_rbsor_colour(T* __restrict__ f, const T
 }
 
 template <typename T>
-static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega) {
+static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_passes = false) {
   const int form = s ? 2 : 1;
   BCM_TRY(sor_coef(m, omega, form));
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_SOR, &ev));
   const int nb = m->nblocks;
-  if (m->vc == 1 && m->n == 32) {
+  if (colour_passes) {
+    // fall through to the generic kernel below
+  } else if (m->vc == 1 && m->n == 32) {
     if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 32, true>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega);
     else hipLaunchKernelGGL((k_rbsor_fused<T, 32, false>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega);
   } else if (m->vc == 1 && m->n == 16) {
     if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 16, true>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega);
     else hipLaunchKernelGGL((k_rbsor_fused<T, 16, false>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega);
-  } else {
+  }
+  if (colour_passes || !(m->vc == 1 && (m->n == 32 || m->n == 16))) {
     const CellGeo g = cell_geo(m);
     const long long half = g.ncell / 2;
     const int grid = int((half + 255) / 256);
@@ -656,23 +659,107 @@ static int sor_iterate_halo(bcm_field_t f, bcm_field_t source, double omega, int
   return dirichlet_device(m, f);
 }
 
-extern "C" int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
-                               int iters) {
-  if (!f || iters < 0 || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
-  BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));  // validates the table
+static bool variant_supported(const bcm_mesh* m, int v) {
+  switch (v) {
+    case BCM_SOR_AUTO:
+    case BCM_SOR_FUSED_TASKS:
+    case BCM_SOR_COLOUR_TASKS: return true;
+    case BCM_SOR_FUSED_GATHER: return m->vc == 1 && (m->n == 4 || m->n == 8 || m->n == 16 || m->n == 32);
+    case BCM_SOR_HALO: return m->vc == 1 && (m->n == 16 || m->n == 32) && m->halo_ok;
+    default: return false;
+  }
+}
+
+template <typename T>
+static int sor_iterate_t(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
+                         int iters, int variant) {
   bcm_mesh* m = f->mesh;
-  const bool halo = m->vc == 1 && (m->n == 16 || m->n == 32) && m->halo_ok && iters >= 2;
+  T* d = static_cast<T*>(f->data);
+  const T* s = source ? static_cast<const T*>(source->data) : nullptr;
+  if (variant == BCM_SOR_AUTO) variant = variant_supported(m, BCM_SOR_HALO) ? BCM_SOR_HALO : BCM_SOR_FUSED_TASKS;
+  const bool halo = variant == BCM_SOR_HALO && iters >= 2;
   // The first iteration uses the ghosts stored in f, as the reference does.
   const int classic = halo ? 1 : iters;
   for (int it = 0; it < classic; ++it) {
-    BCM_TRY(bcm_rbsor_sweep(f, source, omega));
-    BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));
-    BCM_TRY(bcm_ghost_fill(f, BCM_GF_ALL));
-    BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));
+    d = static_cast<T*>(f->data);
+    BCM_TRY(rbsor_raw<T>(m, d, s, omega, variant == BCM_SOR_COLOUR_TASKS));
+    BCM_TRY(dirichlet_device(m, f));
+    if (variant == BCM_SOR_FUSED_GATHER) BCM_TRY(ghost_gather_raw<T>(m, d));
+    else BCM_TRY(ghost_fill_raw<T>(m, d, BCM_GF_ALL));
+    BCM_TRY(dirichlet_device(m, f));
   }
   if (!halo) return BCM_OK;
-  return f->dtype == BCM_F64 ? sor_iterate_halo<double>(f, source, omega, iters - 1)
-                             : sor_iterate_halo<float>(f, source, omega, iters - 1);
+  return sor_iterate_halo<T>(f, source, omega, iters - 1);
+}
+
+extern "C" int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
+                               int iters) {
+  if (!f || iters < 0 || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
+  BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));  // validates and uploads the table
+  bcm_mesh* m = f->mesh;
+  const int v = variant_supported(m, m->sor_variant) ? m->sor_variant : BCM_SOR_AUTO;
+  return f->dtype == BCM_F64 ? sor_iterate_t<double>(f, source, omega, bvalues, nbvalues, iters, v)
+                             : sor_iterate_t<float>(f, source, omega, bvalues, nbvalues, iters, v);
+}
+
+extern "C" int bcm_sor_set_variant(bcm_mesh_t m, int variant) {
+  if (!m || variant < 0 || variant >= BCM_SOR_NUM_VARIANTS || !variant_supported(m, variant)) return BCM_ERR_ARG;
+  m->sor_variant = variant;
+  return BCM_OK;
+}
+
+extern "C" int bcm_sor_get_variant(bcm_mesh_t m) { return m ? m->sor_variant : -BCM_ERR_ARG; }
+
+extern "C" int bcm_sor_autotune(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
+                                int iters, int* best, double* ms_per_iter) {
+  if (!f || !best || iters < 2 || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  for (const auto& bi : m->blocks)
+    for (int k = 0; k < 6; ++k)
+      if (bi.bc_type[k] == BCM_BC_NEUMANN) return BCM_ERR_ARG;  // as bcm_dirichlet_mirror
+  BCM_TRY(set_bvalues(m, bvalues, nbvalues));  // upload only: f itself stays untouched
+  // Scratch copy of f: the candidates run on it, f keeps its values.
+  bcm_field scratch = *f;
+  void* buf = nullptr;
+  BCM_TRY(dev_alloc(&buf, f->bytes));
+  scratch.data = buf;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = BCM_OK;
+  double best_ms = 0.0;
+  *best = BCM_SOR_AUTO;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = BCM_ERR_HIP;
+  for (int v = BCM_SOR_FUSED_TASKS; v < BCM_SOR_NUM_VARIANTS && rc == BCM_OK; ++v) {
+    if (ms_per_iter) ms_per_iter[v] = -1.0;
+    if (!variant_supported(m, v)) continue;
+    float ms = 0.f;
+    for (int rep = 0; rep < 2 && rc == BCM_OK; ++rep) {  // the first run warms up (workspaces, caches)
+      if (hipMemcpyAsync(scratch.data, f->data, f->bytes, hipMemcpyDeviceToDevice, m->stream) != hipSuccess ||
+          hipEventRecord(e0, m->stream) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+      rc = f->dtype == BCM_F64 ? sor_iterate_t<double>(&scratch, source, omega, bvalues, nbvalues, iters, v)
+                               : sor_iterate_t<float>(&scratch, source, omega, bvalues, nbvalues, iters, v);
+      if (rc == BCM_OK && (hipEventRecord(e1, m->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = BCM_ERR_HIP;
+    }
+    if (rc != BCM_OK) break;
+    const double per = double(ms) / iters;
+    if (ms_per_iter) ms_per_iter[v] = per;
+    if (*best == BCM_SOR_AUTO || per < best_ms) {
+      *best = v;
+      best_ms = per;
+    }
+  }
+  if (ms_per_iter) ms_per_iter[BCM_SOR_AUTO] = -1.0;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(m->stream);
+  // The halo variant may have swapped the scratch buffer with the workspace.
+  dev_free(scratch.data);
+  if (rc == BCM_OK) m->sor_variant = *best;
+  return rc;
 }
 
 extern "C" int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relres) {

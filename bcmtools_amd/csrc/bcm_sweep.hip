@@ -255,6 +255,14 @@ struct HaloSrc {
   }
 };
 
+// One aligned two-cell store (16 B for FP64, 8 B for FP32).
+__device__ __forceinline__ void store_pair(double* p, double a, double b) {
+  *reinterpret_cast<double2*>(p) = make_double2(a, b);
+}
+__device__ __forceinline__ void store_pair(float* p, float a, float b) {
+  *reinterpret_cast<float2*>(p) = make_float2(a, b);
+}
+
 template <typename T, bool SRC>
 __device__ __forceinline__ T sor_cell(T fc, T fw, T fe, T fs, T fn, T fb, T ft, T sv, T c0, T cx, T cy, T cz, T om,
                                       T om1) {
@@ -272,14 +280,14 @@ __device__ __forceinline__ T sor_cell(T fc, T fw, T fe, T fs, T fn, T fb, T ft, 
 // (2pi..2pi+1, pj) of every z-plane (one red, one black).  The thread's own
 // cells of planes k-2..k+3 sit in registers (vertical neighbours never go
 // through LDS); LDS holds three ghosted planes for the in-plane neighbours
-// and two staging planes for the X/Y ghost frame, so one barrier per z-step
-// suffices:
+// and two staging planes for the X/Y ghost frame; two barriers per z-step:
 //   step k: commit plane k (own old values); park the frame sources of plane
 //   k+1; issue the loads of plane k+3 (own cells and frame sources) and of
 //   the source term of plane k+2; barrier; combine the frame of plane k+1;
 //   red cells of plane k (in-plane from LDS, vertical from registers; red
 //   results also to LDS); black cells of plane k-1 (in-plane red values from
-//   LDS); write plane k-1 to B (and its X-face columns).
+//   LDS, black results too); barrier; write plane k-1 from LDS to B in
+//   aligned whole rows (and its X-face columns to the export).
 // Every LDS cell is written by one thread; slot k%3 is rewritten at step
 // k+3, after the barrier of step k+1 and k+2; staging slot p&1 is rewritten
 // two steps after it was read.
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int NX = N + 2;
   constexpr int PL = NX * NX;
   constexpr int HALF = N / 2;
-  __shared__ T lds[3][PL];
+  __shared__ __align__(16) T lds[3][PL];
   __shared__ T stage[2][8][4 * N];
   __shared__ int bdesc[48];
   __shared__ double bface[6];
@@ -400,23 +408,35 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
       const int c = LQ(q);
       o[q] = sor_cell<T, SRC>(o[q], lds[S][c - 1], lds[S][c + 1], lds[S][c - NX], lds[S][c + NX], dn[q], up[q],
                               SRC ? sv[q] : T(0), c0, cx, cy, cz, om, om1);
+      lds[S][c] = o[q];
     }
   };
-  // Plane p to B: own cells, their X-face columns to the export, and the
-  // frame (the ghost exchange of A, which only unconditioned faces read
-  // back: there it is the stored ghost itself) so that whole rows, and with
-  // them whole cache lines, are written.
-  auto store = [&](int p, const T* o) __attribute__((always_inline)) {
+  // Plane p to B from LDS slot p % 3, complete once the barrier after
+  // black(p) has passed: every row in aligned cell pairs (-1,0), (1,2), ...,
+  // (N-1,N) -- the X frame included -- so each wave writes whole cache
+  // lines with 16-byte stores; the Y frame rows cell by cell; the X-face
+  // columns to the export, one coalesced row per column.  The frame is the
+  // ghost exchange of A, which only unconditioned faces read back, and
+  // there it is the stored ghost itself.
+  constexpr int NPAIR = N * (HALF + 1);
+  constexpr int NT = N * N / 2;
+  auto store_plane = [&](int p) __attribute__((always_inline)) {
+    const int S = p % 3;
     T* dst = Bb + (long long)(p + 1) * PL;
-    if (framer) dst[L(fci, fcj)] = lds[p % 3][L(fci, fcj)];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      dst[LQ(q)] = o[q];
-      if (i0 == 0 || i0 == N - 2) {
-        const int i = i0 + q, j = j0;
-        const int colx = i < 2 ? i : i - (N - 4);
-        xout[((long long)(b * 4 + colx) * N + p) * N + j] = o[q];
+    for (int e0 = 0; e0 < NPAIR; e0 += NT) {
+      const int e = e0 + t;
+      if (e < NPAIR) {
+        const int j = e / (HALF + 1), m = e % (HALF + 1);
+        const int off = (j + 1) * NX + 2 * m;
+        store_pair(dst + off, lds[S][off], lds[S][off + 1]);
       }
+    }
+    if (framer && edge >= 2) dst[L(fci, fcj)] = lds[S][L(fci, fcj)];
+    if (t < 4 * N) {
+      const int col = t / N, j = t % N;
+      const int i = col < 2 ? col : col + N - 4;
+      xout[((long long)(b * 4 + col) * N + p) * N + j] = lds[S][L(i, j)];
     }
   };
 
@@ -460,7 +480,8 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
     red(std::integral_constant<int, PAR>(), k, u0, um1, up1, sc);
     if (k >= 1) {
       black(std::integral_constant<int, 1 - PAR>(), k - 1, um1, um2, u0, sp);
-      store(k - 1, um1);
+      __syncthreads();
+      store_plane(k - 1);
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -489,7 +510,8 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   // um1 = plane N-1, um2 = plane N-2, u0 = plane N (ghost); sp = s(N-1).
   __syncthreads();
   black(std::integral_constant<int, (N - 1) & 1>(), N - 1, um1, um2, u0, sp);
-  store(N - 1, um1);
+  __syncthreads();
+  store_plane(N - 1);
 }
 
 // X-face columns 0, 1, N-2, N-1 of every block of `A` into `xout`.

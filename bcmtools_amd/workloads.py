@@ -70,6 +70,27 @@ def rhs_field(blocks, n, vc, seed=0, sigma=SIGMA, freq=1.0, dtype=np.float64, ch
     return out
 
 
+def rhs_field_fast(blocks, n, vc, seed=0, sigma=SIGMA, dtype=np.float64):
+    """As rhs_field with the noise drawn from numpy's PCG64 stream seeded per
+    mesh (block-order dependent, ~10x faster): the batched and served
+    workloads generate millions of blocks."""
+    nb = len(blocks)
+    nx0 = n + 2 * vc
+    out = np.zeros((nb, nx0, nx0, nx0), dtype=dtype)
+    t = (np.arange(n) + 0.5)
+    org = blocks["origin"]
+    h = blocks["block_size"] / n
+    x = org[:, 0, None] + t[None, :] * h[:, 0, None]
+    y = org[:, 1, None] + t[None, :] * h[:, 1, None]
+    z = org[:, 2, None] + t[None, :] * h[:, 2, None]
+    sx, sy, sz = np.sin(math.pi * x), np.sin(math.pi * y), np.sin(math.pi * z)
+    f = sz[:, :, None, None] * sy[:, None, :, None] * sx[:, None, None, :]
+    if sigma:
+        f += sigma * np.random.default_rng(seed).standard_normal(f.shape)
+    out[:, vc:vc + n, vc:vc + n, vc:vc + n] = f
+    return out
+
+
 def config1_tree():
     """Uniform level-2 octree on the unit cube: 4x4x4 = 64 blocks, Z order."""
     return Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=2, ordering=L.ORDER_Z)
@@ -110,6 +131,26 @@ def random_tree(seed, max_level=None, nspheres=None):
         shells.append((c[0], c[1], c[2], r, r))
     return Tree(root_n=(2, 2, 2), root_length=0.5, divider=L.DIV_SHELL, min_level=0, max_level=max_level - 1,
                 shells=shells, ordering=L.ORDER_Z, balance=1, shell_mode=1, shell_margin=0.25)
+
+
+def serving_tree(seed, target_blocks):
+    """A request mesh of the served workload: 1-3 random sphere surfaces
+    (as random_tree), refined to the smallest max level (2..8) whose block
+    count reaches 0.7 x target_blocks."""
+    rng = np.random.default_rng(seed)
+    nspheres = int(rng.integers(1, 4))
+    shells = []
+    for _ in range(nspheres):
+        c = rng.uniform(0.2, 0.8, size=3)
+        r = float(rng.uniform(0.05, 0.3))
+        shells.append((c[0], c[1], c[2], r, r))
+    t = None
+    for level in range(2, 9):
+        t = Tree(root_n=(2, 2, 2), root_length=0.5, divider=L.DIV_SHELL, min_level=0, max_level=level - 1,
+                 shells=shells, ordering=L.ORDER_Z, balance=1, shell_mode=1, shell_margin=0.25)
+        if len(t.blocks) >= 0.7 * target_blocks:
+            break
+    return t
 
 
 def interior(a, vc):

@@ -23,9 +23,12 @@ step on the same mesh for a bounded number of steps on the host cores.
 
 Other workloads (--workload): jacobi (64 x 16^3, Jacobi7 + ghost fill),
 bicgstab (level-7 shell, 26,888 x 16^3: mixed FP32/FP64 BiCGSTAB to 1e-10
-against the FP64 paths), copy (HBM calibration).
+against the FP64 paths), batched (256 random meshes solved as one batch,
+with a per-mesh comparison on every 8th mesh), serving (Poisson-arrival
+request stream through the continuous-batching server), copy (HBM
+calibration).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sor|jacobi|bicgstab|copy]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sor|jacobi|bicgstab|batched|serving|copy]
 """
 import argparse
 import json
@@ -207,6 +210,7 @@ def bench_bicgstab(args):
     rhs_host = -f
     del f
     mesh = Mesh(blocks, n, 1)
+    mesh.set_progress(20.0)
     rhs = mesh.field(np.float64, rhs_host)
     vals = [0.0]
 
@@ -287,6 +291,184 @@ def cpu_bicgstab(blocks, n, rhs, gpu_iters, iters):
                       f"{cores} threads of {cpu_model()}, extrapolated to the GPU's {gpu_iters} iterations"}
 
 
+def bench_batched(args):
+    """256 independent random octree meshes (16^3 blocks) solved together to 1e-10."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    tol, n, M = 1e-10, 16, args.meshes
+    trees = [W.random_tree(1000 + i) for i in range(M)]
+    tables = [t.blocks for t in trees]
+    counts = [len(tb) for tb in tables]
+    levels = [int(tb["level"].max()) + 1 for tb in tables]
+    total = sum(counts)
+    print(f"[bench] batched: {M} meshes, {total} blocks ({min(counts)}..{max(counts)} per mesh)", file=sys.stderr,
+          flush=True)
+
+    def build(idx):
+        mesh = Mesh.batch([tables[i] for i in idx], n)
+        mesh.set_progress(20.0)
+        x, rhs = mesh.field(np.float64), mesh.field(np.float64)
+        for s, i in enumerate(idx):
+            first, cnt = mesh.segment(s)
+            rhs.upload_blocks(first, -W.rhs_field_fast(tables[i], n, 1, seed=5000 + i))
+        return mesh, x, rhs
+
+    def solve(mesh, x, rhs, precision):
+        mesh.fill(x, 0.0)
+        mesh.sync()
+        t0 = time.perf_counter()
+        st = mesh.solve(x, rhs, [0.0], precision=precision, tol=tol, max_iter=5000)
+        mesh.sync()
+        return st, time.perf_counter() - t0
+
+    def summary(st):
+        ok = sum(1 for e in st if e["status"] == "converged")
+        return {"converged": ok, "segments": len(st), "max_relres": max(e["relres"] for e in st),
+                "max_iters": max(e["iters"] for e in st), "sum_iters": sum(e["iters"] for e in st)}
+
+    # Full batch, mixed precision (the headline).
+    mesh, x, rhs = build(range(M))
+    print("[bench] batched: fields uploaded", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        mesh.solve(x, rhs, [0.0], precision="mixed", tol=tol, max_iter=2)
+    times, st = [], None
+    for _ in range(args.steps):
+        st, dt = solve(mesh, x, rhs, "mixed")
+        times.append(dt)
+        print(f"[bench] batched mixed solve of {M} meshes: {dt:.2f} s {summary(st)}", file=sys.stderr, flush=True)
+    full = summary(st)
+    mesh.close()
+    del x, rhs
+
+    # Comparison on a fixed subset (every 8th mesh): batched mixed / FP64
+    # against solving the same meshes one by one.
+    idx = list(range(0, M, 8))
+    mesh, x, rhs = build(idx)
+    mesh.solve(x, rhs, [0.0], precision="mixed", tol=tol, max_iter=2)
+    st_m, t_m = solve(mesh, x, rhs, "mixed")
+    mesh.solve(x, rhs, [0.0], precision="fp64", tol=tol, max_iter=2)
+    st_d, t_d = solve(mesh, x, rhs, "fp64")
+    print(f"[bench] subset of {len(idx)}: batched mixed {t_m:.2f} s, batched fp64 {t_d:.2f} s", file=sys.stderr,
+          flush=True)
+    mesh.close()
+    del x, rhs
+    t_e = t_h = 0.0
+    host_ok = 0
+    for k, i in enumerate(idx):
+        m1 = Mesh(tables[i], n)
+        r1 = m1.field(np.float64, -W.rhs_field_fast(tables[i], n, 1, seed=5000 + i))
+        xe, xh = m1.field(np.float64), m1.field(np.float64)
+        m1.sync()
+        t0 = time.perf_counter()
+        m1.solve(xe, r1, [0.0], precision="fp64", tol=tol, max_iter=5000)
+        m1.sync()
+        t1 = time.perf_counter()
+        rc, it, rr = m1.solve_fp64(xh, r1, [0.0], tol=tol, max_iter=5000)
+        m1.sync()
+        t2 = time.perf_counter()
+        host_ok += rr <= tol
+        t_e += t1 - t0
+        t_h += t2 - t1
+        m1.close()
+        if k % 8 == 7:
+            print(f"[bench] per-mesh solves: {k + 1}/{len(idx)} (fused fp64 {t_e:.2f} s, host-driven fp64 {t_h:.2f} s)",
+                  file=sys.stderr, flush=True)
+    elapsed = sum(times)
+    ns = len(idx)
+    return {
+        "metric": "batched solves/s (256 meshes, BiCGSTAB to FP64 relative residual 1e-10, mixed FP32/FP64)",
+        "value": M * len(times) / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / len(times) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{M} random octree meshes (max level U{{2..6}}, 1-3 sphere surfaces r~U(0.05,0.3), "
+                               "2:1 balanced, 16^3 blocks) in one batch, Jacobi-preconditioned BiCGSTAB, "
+                               "FP64 refinement to 1e-10, hipGraph-replayed iteration blocks",
+                   "meshes": M, "blocks": total, "blocks_per_mesh": [min(counts), int(np.median(counts)), max(counts)],
+                   "levels": [min(levels), max(levels)], "interior_cells": total * n ** 3},
+        "batch": full,
+        "subset": {"meshes": ns, "every": 8,
+                   "batched_mixed_solves_per_s": ns / t_m, "batched_fp64_solves_per_s": ns / t_d,
+                   "per_mesh_fused_fp64_solves_per_s": ns / t_e, "per_mesh_host_fp64_solves_per_s": ns / t_h,
+                   "speedup_batched_mixed_vs_per_mesh_host_fp64": t_h / t_m,
+                   "speedup_batched_mixed_vs_batched_fp64": t_d / t_m,
+                   "batched_mixed": summary(st_m), "batched_fp64": summary(st_d),
+                   "host_fp64_converged": int(host_ok)},
+    }
+
+
+def bench_serving(args):
+    """Poisson-arrival stream of solves (block counts log-uniform in [64, 50k],
+    tol 1e-6 -> FP32 path, 1e-10 -> FP32 + FP64 refinement), HBM budget 200 GB."""
+    from bcmtools_amd import Server
+    from bcmtools_amd import workloads as W
+
+    R, lam, n = args.requests, args.lam, 16
+    rng = np.random.default_rng(args.seed)
+    targets = np.exp(rng.uniform(np.log(64), np.log(50000), size=R)).astype(int)
+    tols = rng.choice([1e-6, 1e-10], size=R)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=R))
+    trees = [W.serving_tree(7000 + i, int(targets[i])) for i in range(R)]
+    counts = [len(t.blocks) for t in trees]
+    print(f"[bench] serving: {R} requests at {lam}/s, {sum(counts)} blocks ({min(counts)}..{max(counts)})",
+          file=sys.stderr, flush=True)
+    sv = Server(200e9, max_active=args.slots, n=n, max_iter=20000, check_every=8)
+    done, k = {}, 0
+    t0 = sv.clock()
+    last_print = t0
+    while len(done) < R:
+        now = sv.clock() - t0
+        while k < R and arrivals[k] <= now:
+            prec = "fp32" if tols[k] > 1e-8 else "mixed"
+            sv.submit(trees[k].blocks, float(tols[k]), prec, rhs=None, seed=9000 + k, tag=k)
+            k += 1
+        wait = (arrivals[k] - now) if k < R else 0.05
+        sv.progress(max(0.0, min(0.01, wait)))
+        for r in sv.poll():
+            done[r["tag"]] = r
+        if sv.clock() - last_print > 15.0:
+            last_print = sv.clock()
+            st = sv.stats()
+            print(f"[bench] serving: {len(done)}/{R} done, {st['active']} active, {st['queued']} queued, "
+                  f"{st['bytes_in_use'] / 1e9:.1f} GB held", file=sys.stderr, flush=True)
+    t_end = max(r["t_done"] for r in done.values()) - t0
+    lat = np.array([done[i]["t_done"] - t0 - arrivals[i] for i in range(R)])
+    wait_q = np.array([done[i]["t_admit"] - t0 - arrivals[i] for i in range(R)])
+    ok = [done[i]["status"] == 1 and done[i]["relres"] <= tols[i] for i in range(R)]
+    sv.close()
+    by_tol = {}
+    for tv in (1e-6, 1e-10):
+        sel = [i for i in range(R) if tols[i] == tv]
+        if sel:
+            by_tol[f"{tv:g}"] = {"requests": len(sel), "p50_s": float(np.percentile(lat[sel], 50)),
+                                 "p99_s": float(np.percentile(lat[sel], 99)),
+                                 "mean_iters": float(np.mean([done[i]["iters"] for i in sel]))}
+    return {
+        "metric": "served Poisson solves/s with p50/p99 latency (continuous batching, HBM budget 200 GB)",
+        "value": R / (t_end - arrivals[0]),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": R, "warmup": 0,
+        "ms_per_step": (t_end - arrivals[0]) / R * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{R} requests, Poisson arrivals at {lam}/s, meshes of 1-3 random sphere surfaces "
+                               "with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 (FP32 path) or "
+                               "1e-10 (FP32 + FP64 refinement), up to "
+                               f"{args.slots} requests on the device, HBM budget 200 GB",
+                   "requests": R, "lambda_per_s": lam, "blocks_total": int(sum(counts)),
+                   "blocks_per_request": [int(min(counts)), int(np.median(counts)), int(max(counts))]},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean()), "max": float(lat.max())},
+        "queue_wait_s": {"p50": float(np.percentile(wait_q, 50)), "p99": float(np.percentile(wait_q, 99))},
+        "by_tolerance": by_tol,
+        "all_met_tolerance": bool(all(ok)),
+        "makespan_s": float(t_end),
+    }
+
+
 def bench_copy(args):
     """Plain device copy of one config-2-sized FP64 field (calibrates HBM counters)."""
     from bcmtools_amd import Mesh
@@ -319,7 +501,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi", "bicgstab", "copy"])
+    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy"])
+    ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
+    ap.add_argument("--lam", type=float, default=50.0, help="serving: arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=16, help="serving: requests on the device at once")
+    ap.add_argument("--seed", type=int, default=1, help="serving: request stream seed")
+    ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--level", type=int, default=7, help="bicgstab: shell refinement level")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -328,8 +515,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab,
-           "copy": bench_copy}[args.workload](args)
+    res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
+           "serving": bench_serving, "copy": bench_copy}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -149,6 +149,10 @@ int bcm_field_destroy(bcm_field_t field);
 /* Host <-> device copies of the whole field (nblocks*(n+2vc)^3 values). */
 int bcm_field_upload(bcm_field_t field, const void* host, size_t bytes);
 int bcm_field_download(bcm_field_t field, void* host, size_t bytes);
+/* The same for blocks first_block .. first_block+nblocks-1 only (one
+ * segment of a batch, a block range of one mesh); bytes must match. */
+int bcm_field_upload_blocks(bcm_field_t field, int first_block, int nblocks, const void* host, size_t bytes);
+int bcm_field_download_blocks(bcm_field_t field, int first_block, int nblocks, void* host, size_t bytes);
 void* bcm_field_device_ptr(bcm_field_t field);
 size_t bcm_field_bytes(bcm_field_t field);
 
@@ -189,6 +193,26 @@ int bcm_rbsor_sweep(bcm_field_t f, bcm_field_t source, double omega);
  * (examples/sor2/Solver.cpp:77-133 with nLoopInner == 1). */
 int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega,
                     const double* bvalues, int nbvalues, int iters);
+/* Scheduling of bcm_sor_iterate's outer iteration (all give the same bits):
+ *   AUTO          halo-gathering sweeps where the mesh allows, else FUSED_TASKS
+ *   FUSED_TASKS   one-pass red-black sweep per block (2x2 cell columns per
+ *                 thread, z-planes through LDS) + per-face-task ghost fill
+ *   FUSED_GATHER  the same sweep + block-wise ghost gather
+ *   HALO          ghost frame built inside the sweep from the previous
+ *                 iterate, ping-pong buffers, no separate fill (2x1 cell
+ *                 patches per thread; vc = 1, n in {16, 32}, every face
+ *                 connected or Dirichlet)
+ *   COLOUR_TASKS  two one-colour passes (one thread per cell) + task fill */
+enum { BCM_SOR_AUTO = 0, BCM_SOR_FUSED_TASKS, BCM_SOR_FUSED_GATHER, BCM_SOR_HALO, BCM_SOR_COLOUR_TASKS,
+       BCM_SOR_NUM_VARIANTS };
+int bcm_sor_set_variant(bcm_mesh_t mesh, int variant);
+int bcm_sor_get_variant(bcm_mesh_t mesh);
+/* Autotuner: times every variant the mesh supports for `iters` outer
+ * iterations on a scratch copy of f (f is not modified), keeps the
+ * fastest for the mesh and reports it with the time per iteration of
+ * each variant (ms_per_iter[BCM_SOR_NUM_VARIANTS]; unsupported: -1). */
+int bcm_sor_autotune(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
+                     int iters, int* best, double* ms_per_iter);
 /* Relative residual of the sor2 system lap_h(f) = s:
  * ||s - lap_h f||_2 / ||s||_2 over interior cells (FP64 reduction). */
 int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relres);
@@ -350,6 +374,61 @@ typedef struct bcm_solve_stats {
 } bcm_solve_stats;
 int bcm_solve(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
               const bcm_solve_params* p, const double* seg_tol, bcm_solve_stats* stats);
+/* bcm_solve prints a progress line to stderr at most every `seconds`
+ * (segments done, iterations) while it runs; 0 (default) keeps it quiet. */
+int bcm_mesh_set_progress(bcm_mesh_t mesh, double seconds);
+
+/* ------------------------------------------------------------------ */
+/* Request server (continuous batching of independent Poisson solves)  */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (one BlockManager per process,
+ * include/BlockManager.h:153-156).  Requests (a block table, a right-hand
+ * side, a tolerance and a precision) queue in arrival order and are
+ * admitted first-fit while their device bytes fit in hbm_budget and fewer
+ * than max_active run; each admitted request gets its own device mesh,
+ * HIP stream and fused-solver state (bcm_solve's BiCGSTAB, hipGraph
+ * replayed), and bcm_server_progress keeps every admitted stream fed
+ * without blocking, so admitted requests overlap on the device.  Memory is
+ * released as soon as a request finishes.  A request's result does not
+ * depend on the other requests.  Single-threaded: one host thread calls
+ * every bcm_server_* function of a server. */
+typedef struct bcm_server* bcm_server_t;
+typedef struct bcm_server_params {
+  double hbm_budget;          /* device bytes all admitted requests may hold */
+  int32_t max_active;         /* requests on the device at once */
+  int32_t n;                  /* cells per block edge of every request: 8, 16 or 32 */
+  int32_t max_iter;           /* BiCGSTAB iteration limit per request */
+  int32_t count_pre;          /* Jacobi preconditioner sweeps */
+  double omega;               /* Jacobi weight */
+  int32_t check_every;        /* iterations per graph replay (0: 8) */
+} bcm_server_params;
+typedef struct bcm_server_result {
+  int64_t tag;
+  int32_t status;             /* BCM_SOLVE_* or -(BCM_ERR_*) (-BCM_ERR_BUSY: exceeds the budget) */
+  int32_t iters;
+  double relres;              /* true FP64 relative residual */
+  double t_submit, t_admit, t_done;  /* seconds on the server clock */
+  int32_t nblocks;
+  int32_t precision;
+} bcm_server_result;
+int bcm_server_create(const bcm_server_params* params, bcm_server_t* out);
+int bcm_server_destroy(bcm_server_t server);
+/* Seconds since the server was created. */
+double bcm_server_clock(bcm_server_t server);
+/* Queue a request.  blocks: its block table (copied); rhs: host FP64
+ * ghosted field (nblocks*(n+2)^3 values, read when the request is
+ * admitted, so it must stay valid until then) or NULL for the synthetic
+ * right-hand side sin(pi x) sin(pi y) sin(pi z) + 0.01 N(0,1) generated on
+ * the device from `seed`; x_out: host buffer of the same size for the
+ * solution (NULL: not returned); all outer faces Dirichlet u = 0. */
+int bcm_server_submit(bcm_server_t server, const bcm_block_info* blocks, int nblocks, const double* rhs,
+                      uint64_t seed, double tol, int precision, double* x_out, int64_t tag);
+/* Drive the admitted requests for up to max_seconds (returns early when
+ * nothing is left); finished requests move to the result list. */
+int bcm_server_progress(bcm_server_t server, double max_seconds, int* n_active, int* n_queued);
+/* Move up to capacity finished results into out; returns how many. */
+int bcm_server_poll(bcm_server_t server, bcm_server_result* out, int capacity);
+int bcm_server_stats(bcm_server_t server, double* bytes_in_use, int* n_active, int* n_queued);
 
 #ifdef __cplusplus
 }

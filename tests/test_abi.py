@@ -15,7 +15,7 @@ from conftest import ROOT
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "bcm_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void\*|size_t)\s+\*?\s*(bcm_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void\*|size_t|double)\s+\*?\s*(bcm_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_the_abi():

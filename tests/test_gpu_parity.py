@@ -333,3 +333,41 @@ def test_full_size_config2_sor_vs_oracle():
     oracle.dirichlet(blocks, n, 1, ref, [0.0])
     oracle.sor_iterate(blocks, n, 1, ref, s, 1.6, [0.0], 4)
     assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("n,maker", [(16, lambda: W.shell_tree(3)), (32, lambda: W.shell_tree(2)),
+                                     (8, lambda: W.random_tree(6, max_level=3))])
+def test_every_sor_schedule_matches_the_oracle_and_autotune(n, maker):
+    """All outer-iteration schedules (fused + task fill, fused + block-wise
+    gather, halo-gathering, two colour passes) give the reference-order
+    result bit for bit; the autotuner leaves f untouched, picks a supported
+    schedule, and the mesh then iterates with it."""
+    blocks = maker().blocks
+    vals = [0.5]
+    case(f"sor schedules n={n} nb={len(blocks)}")
+    rng = np.random.default_rng(3 * n + len(blocks))
+    f0 = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2))
+    s = W.rhs_field(blocks, n, 1, seed=8)
+    ref = f0.copy()
+    oracle.dirichlet(blocks, n, 1, ref, vals)
+    oracle.sor_iterate(blocks, n, 1, ref, s, 1.3, vals, 5)
+    mesh = Mesh(blocks, n, 1)
+    sf = mesh.field(np.float64, s)
+    ran = []
+    for v in range(1, L.SOR_NUM_VARIANTS):
+        try:
+            mesh.set_sor_variant(v)
+        except L.BCMError:
+            continue  # not supported on this mesh
+        f = mesh.field(np.float64, f0)
+        mesh.sor_iterate(f, sf, 1.3, vals, 5)
+        np.testing.assert_array_equal(f.download(), ref, err_msg=f"variant {v}")
+        ran.append(v)
+    assert L.SOR_FUSED_TASKS in ran and L.SOR_COLOUR_TASKS in ran
+    f = mesh.field(np.float64, f0)
+    best, times = mesh.sor_autotune(f, sf, 1.3, vals, iters=3)
+    assert best in ran and set(times) == set(ran)
+    np.testing.assert_array_equal(f.download(), f0)
+    assert mesh.sor_variant() == best
+    mesh.sor_iterate(f, sf, 1.3, vals, 5)
+    np.testing.assert_array_equal(f.download(), ref)
