@@ -174,6 +174,103 @@ __global__ __launch_bounds__(KWG) void k_gather(T* __restrict__ a, const int* __
 }
 
 // ---------------------------------------------------------------------
+// Stencil vectors of the iteration (the preconditioned p_, s_ and the
+// sweep scratch) are compact (n^3 per block, x fastest) with an export of
+// columns 0, 1, n-2, n-1 of every block ([blk][4][k][j]) that the kernel
+// producing the vector writes.  A kernel applying the stencil builds its
+// block's ghost frame in LDS from the neighbours' rows (Y/Z faces) and
+// exports (X faces) -- coalesced reads only -- with the exchange's own
+// arithmetic (copy, F2C, C2F as include/Scalar3DUpdater.h:190-266,
+// 321-347, 519-708), so it sees exactly the values a ghost fill would
+// have written.  Frame cells of Dirichlet / unconnected faces are zero.
+// ---------------------------------------------------------------------
+template <typename T, int N>
+struct Compact {
+  const T* __restrict__ v;   // compact vector
+  const T* __restrict__ xe;  // its column export
+  __device__ __forceinline__ T at(int blk, int i, int j, int k) const {
+    return v[(long long)blk * (N * N * N) + i + N * (j + N * k)];
+  }
+  // i must be 0, 1, N-2 or N-1.
+  __device__ __forceinline__ T x_at(int blk, int i, int j, int k) const {
+    const int col = i < 2 ? i : i - (N - 4);
+    return xe[((long long)(blk * 4 + col) * N + k) * N + j];
+  }
+};
+
+// fr[f][p + N q]: X faces p = j, q = k; Y faces p = i, q = k; Z faces p = i, q = j.
+template <typename T, int N>
+__device__ __forceinline__ void build_frame(const Compact<T, N>& src, const int* d, T (*fr)[N * N]) {
+  constexpr int h = N / 2;
+  for (int e = threadIdx.x; e < 6 * N * N; e += KWG) {
+    const int f = e / (N * N), pq = e % (N * N), p = pq % N, q = pq / N;
+    const int ax = f >> 1;
+    const bool plus = f & 1;
+    const int* df = d + f * 8;
+    const int kind = df[0];
+    int c[3];
+    c[ax] = plus ? N : -1;
+    c[ax == 0 ? 1 : 0] = p;
+    c[ax == 2 ? 1 : 2] = q;
+    const int t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
+    T val = T(0);
+    if (kind == FD_COPY) {
+      int sc[3] = {c[0], c[1], c[2]};
+      sc[ax] = plus ? 0 : N - 1;
+      val = ax == 0 ? src.x_at(df[1], sc[0], sc[1], sc[2]) : src.at(df[1], sc[0], sc[1], sc[2]);
+    } else if (kind == FD_F2C) {
+      const int s0 = c[t0] >= h, s1 = c[t1] >= h;
+      const int fb = df[1 + s0 + 2 * s1];
+      int I[3];
+      I[ax] = plus ? 0 : h - 1;
+      I[t0] = c[t0] - h * s0;
+      I[t1] = c[t1] - h * s1;
+      if (ax == 0)
+        val = f2c_sum<T>([&](int i, int j, int k) { return src.x_at(fb, i, j, k); }, I[0], I[1], I[2]);
+      else
+        val = f2c_sum<T>([&](int i, int j, int k) { return src.at(fb, i, j, k); }, I[0], I[1], I[2]);
+    } else if (kind == FD_C2F) {
+      const int sf = df[5], cb = df[1];
+      int pp[3];
+      pp[ax] = plus ? 0 : 2 * N - 1;
+      pp[t0] = c[t0] + N * (sf & 1);
+      pp[t1] = c[t1] + N * (sf >> 1);
+      if (ax == 0)
+        val = c2f_interp<T>([&](int i, int j, int k) { return src.x_at(cb, i, j, k); }, N, pp[0], pp[1], pp[2]);
+      else
+        val = c2f_interp<T>([&](int i, int j, int k) { return src.at(cb, i, j, k); }, N, pp[0], pp[1], pp[2]);
+    }
+    fr[f][pq] = val;
+  }
+}
+
+// The six neighbours of cell (i, j, k) of block b: interior ones from the
+// compact vector, the others from the frame.
+template <typename T, int N, typename TV>
+__device__ __forceinline__ void neighbours(const TV* __restrict__ g, long long o, int i, int j, int k,
+                                           const TV (*fr)[N * N], TV& w, TV& e, TV& s, TV& n, TV& bo, TV& t) {
+  w = i > 0 ? g[o - 1] : fr[0][j + N * k];
+  e = i < N - 1 ? g[o + 1] : fr[1][j + N * k];
+  s = j > 0 ? g[o - N] : fr[2][i + N * k];
+  n = j < N - 1 ? g[o + N] : fr[3][i + N * k];
+  bo = k > 0 ? g[o - N * N] : fr[4][i + N * j];
+  t = k < N - 1 ? g[o + N * N] : fr[5][i + N * j];
+}
+
+// Export staging: a producer puts columns 0, 1, N-2, N-1 of its block into
+// LDS as it computes them and writes them out in one coalesced pass.
+template <int N>
+__device__ __forceinline__ int export_col(int i) {
+  return i < 2 ? i : (i >= N - 2 ? i - (N - 4) : -1);
+}
+template <typename T, int N>
+__device__ __forceinline__ void export_flush(const T* xs, T* __restrict__ xe, int b) {
+  __syncthreads();
+  T* dst = xe + (long long)b * 4 * N * N;
+  for (int e = threadIdx.x; e < 4 * N * N; e += KWG) dst[e] = xs[e];
+}
+
+// ---------------------------------------------------------------------
 // Iteration kernels (one work-group per block, cell c = tid + 256 m)
 // ---------------------------------------------------------------------
 // Sweep 1 of the Jacobi preconditioner from a zero start, written as
@@ -186,15 +283,16 @@ __device__ __forceinline__ T first_sweep(T bv, T ap, T om) {
 }
 
 // p = r (first) or p = r + beta (p - gamma q)  [AXPY(p, q, -gamma); XPAY(p, r, beta)],
-// then the first preconditioner sweep of p into g (PRE) or g = p.  After a
-// restart the same pass sets the shadow residual r0 = r.
+// then the first preconditioner sweep of p into g (PRE) or g = p, with g's
+// column export.  After a restart the same pass sets the shadow residual r0 = r.
 template <typename T, int N, int PRE, typename TG>
 __global__ __launch_bounds__(KWG) void k_kry_p(T* __restrict__ p, const T* __restrict__ r, const T* __restrict__ q,
-                                               T* __restrict__ r0, TG* __restrict__ g,
+                                               T* __restrict__ r0, TG* __restrict__ g, TG* __restrict__ gx,
                                                const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
                                                const SegState* __restrict__ st, double omega,
-    const int* __restrict__ blist) {
+                                               const int* __restrict__ blist) {
   using B = Blk<N>;
+  __shared__ TG xs[4 * N * N];
   const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE) return;
@@ -203,7 +301,7 @@ __global__ __launch_bounds__(KWG) void k_kry_p(T* __restrict__ p, const T* __res
   const double beta = first ? 0.0 : sp->rho / sp->rho_prev * sp->alpha / sp->gamma;
   const T tb = T(beta), tg = T(-sp->gamma), om = T(omega);
   const BlockMeta me = meta[b];
-  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  const long long cb = (long long)b * B::N3;
   for (int m = 0; m < B::PER; ++m) {
     const int c = threadIdx.x + KWG * m;
     const long long o = cb + c;
@@ -213,29 +311,35 @@ __global__ __launch_bounds__(KWG) void k_kry_p(T* __restrict__ p, const T* __res
     if (reset) r0[o] = rv;
     int i, j, k;
     B::ijk(c, i, j, k);
+    TG gv;
     if (PRE) {
       const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
-      g[gb + B::g(i, j, k)] = TG(first_sweep(pv, a.p, om));
+      gv = TG(first_sweep(pv, a.p, om));
     } else {
-      g[gb + B::g(i, j, k)] = TG(pv);
+      gv = TG(pv);
     }
+    g[o] = gv;
+    const int col = export_col<N>(i);
+    if (col >= 0) xs[(col * N + k) * N + j] = gv;
   }
+  export_flush<TG, N>(xs, gx, b);
 }
 
 // s = r - alpha q  [AXPYZ(s, q, r, -alpha)], then the first sweep of s into g.
 template <typename T, int N, int PRE, typename TG>
 __global__ __launch_bounds__(KWG) void k_kry_s(T* __restrict__ s, const T* __restrict__ r, const T* __restrict__ q,
-                                               TG* __restrict__ g, const BlockMeta* __restrict__ meta,
-                                               const int* __restrict__ blk_seg, const SegState* __restrict__ st,
-                                               double omega,
-    const int* __restrict__ blist) {
+                                               TG* __restrict__ g, TG* __restrict__ gx,
+                                               const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
+                                               const SegState* __restrict__ st, double omega,
+                                               const int* __restrict__ blist) {
   using B = Blk<N>;
+  __shared__ TG xs[4 * N * N];
   const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE) return;
   const T ta = T(-sp->alpha), om = T(omega);
   const BlockMeta me = meta[b];
-  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  const long long cb = (long long)b * B::N3;
   for (int m = 0; m < B::PER; ++m) {
     const int c = threadIdx.x + KWG * m;
     const long long o = cb + c;
@@ -243,51 +347,56 @@ __global__ __launch_bounds__(KWG) void k_kry_s(T* __restrict__ s, const T* __res
     s[o] = sv;
     int i, j, k;
     B::ijk(c, i, j, k);
+    TG gv;
     if (PRE) {
       const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
-      g[gb + B::g(i, j, k)] = TG(first_sweep(sv, a.p, om));
+      gv = TG(first_sweep(sv, a.p, om));
     } else {
-      g[gb + B::g(i, j, k)] = TG(sv);
+      gv = TG(sv);
     }
+    g[o] = gv;
+    const int col = export_col<N>(i);
+    if (col >= 0) xs[(col * N + k) * N + j] = gv;
   }
+  export_flush<TG, N>(xs, gx, b);
 }
 
 // Later preconditioner sweeps: gout = jacobi_smoother(gin; b = rhs),
 // arithmetic in T (the sweep precision), input / output stored as TI / TO.
-template <typename T, typename TI, typename TO>
-__device__ __forceinline__ T jacobi_mixed(const Coef7<T>& a, const TI* __restrict__ x, long long o, int sy, int sz,
-                                          T bv, T om) {
-  T r = bv - a.w * T(x[o - 1]);
-  r = r - a.e * T(x[o + 1]);
-  r = r - a.s * T(x[o - sy]);
-  r = r - a.n * T(x[o + sy]);
-  r = r - a.b * T(x[o - sz]);
-  r = r - a.t * T(x[o + sz]);
-  const T xt = r / a.p;
-  const T xc = T(x[o]);
-  return xc - om * (xc - xt);
-}
-
 template <typename T, int N, typename TI, typename TO>
-__global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, const TI* __restrict__ gin,
+__global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, TO* __restrict__ gox,
+                                                    const TI* __restrict__ gin, const TI* __restrict__ gix,
                                                     const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
                                                     const int* __restrict__ blk_seg,
                                                     const SegState* __restrict__ st, double omega,
-    const int* __restrict__ blist) {
+                                                    const int* __restrict__ blist, const int* __restrict__ desc) {
   using B = Blk<N>;
+  __shared__ TI fr[6][N * N];
+  __shared__ TO xs[4 * N * N];
+  __shared__ int d[48];
   const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
+  if (threadIdx.x < 48) d[threadIdx.x] = desc[b * 48 + threadIdx.x];
+  __syncthreads();
+  build_frame<TI, N>(Compact<TI, N>{gin, gix}, d, fr);
+  __syncthreads();
   const T om = T(omega);
   const BlockMeta me = meta[b];
-  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  const long long cb = (long long)b * B::N3;
   for (int m = 0; m < B::PER; ++m) {
     const int c = threadIdx.x + KWG * m;
+    const long long o = cb + c;
     int i, j, k;
     B::ijk(c, i, j, k);
-    const long long go = gb + B::g(i, j, k);
+    TI w, e, s, n, bo, t;
+    neighbours<T, N, TI>(gin, o, i, j, k, fr, w, e, s, n, bo, t);
     const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
-    gout[go] = TO(jacobi_mixed<T, TI, TO>(a, gin, go, B::NX, B::NX * B::NX, rhs[cb + c], om));
+    const TO v = TO(jacobi_vals(a, T(w), T(e), T(s), T(n), T(bo), T(t), T(gin[o]), rhs[o], om));
+    gout[o] = v;
+    const int col = export_col<N>(i);
+    if (col >= 0) xs[(col * N + k) * N + j] = v;
   }
+  export_flush<TO, N>(xs, gox, b);
 }
 
 // out = A g (calc_ax) in FP64 from the FP64 preconditioned vector g, stored
@@ -295,25 +404,35 @@ __global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, const
 // (out, out), formed and summed in FP64.
 template <typename T, int N, int NDOT>
 __global__ __launch_bounds__(KWG) void k_kry_spmv(T* __restrict__ out, const double* __restrict__ g,
-                                                  const T* __restrict__ y, const BlockMeta* __restrict__ meta,
+                                                  const double* __restrict__ gx, const T* __restrict__ y,
+                                                  const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blk_seg, const SegState* __restrict__ st,
-                                                  double* __restrict__ parts,
-    const int* __restrict__ blist) {
+                                                  double* __restrict__ parts, const int* __restrict__ blist,
+                                                  const int* __restrict__ desc) {
   using B = Blk<N>;
+  __shared__ double fr[6][N * N];
   __shared__ double red[8];
+  __shared__ int d[48];
   const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
+  if (threadIdx.x < 48) d[threadIdx.x] = desc[b * 48 + threadIdx.x];
+  __syncthreads();
+  build_frame<double, N>(Compact<double, N>{g, gx}, d, fr);
+  __syncthreads();
   const BlockMeta me = meta[b];
-  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  const long long cb = (long long)b * B::N3;
   double d0 = 0.0, d1 = 0.0;
   for (int m = 0; m < B::PER; ++m) {
     const int c = threadIdx.x + KWG * m;
+    const long long o = cb + c;
     int i, j, k;
     B::ijk(c, i, j, k);
+    double w, e, s, n, bo, t;
+    neighbours<double, N, double>(g, o, i, j, k, fr, w, e, s, n, bo, t);
     const Coef7<double> a = poisson_coef<double>(me, i, j, k, N);
-    const T v = T(ax_cell(a, g, gb + B::g(i, j, k), B::NX, B::NX * B::NX));
-    out[cb + c] = v;
-    d0 += double(v) * double(y[cb + c]);
+    const T v = T(ax_vals(a, w, e, s, n, bo, t, g[o]));
+    out[o] = v;
+    d0 += double(v) * double(y[o]);
     if (NDOT == 2) d1 += double(v) * double(v);
   }
   wg_sum2(d0, d1, red);
@@ -339,15 +458,12 @@ __global__ __launch_bounds__(KWG) void k_kry_xr(double* __restrict__ dx, const d
   if (sp->status != BCM_SOLVE_ACTIVE) return;
   const double al = sp->alpha, ga = sp->gamma;
   const T tmg = T(-sp->gamma);
-  const long long cb = (long long)b * B::N3, gb = (long long)b * B::GV;
+  const long long cb = (long long)b * B::N3;
   double d0 = 0.0, d1 = 0.0;
   for (int m = 0; m < B::PER; ++m) {
     const int c = threadIdx.x + KWG * m;
     const long long o = cb + c;
-    int i, j, k;
-    B::ijk(c, i, j, k);
-    const long long go = gb + B::g(i, j, k);
-    dx[o] = (al * gp[go] + ga * gs[go]) + dx[o];
+    dx[o] = (al * gp[o] + ga * gs[o]) + dx[o];
     const T rv = tmg * t[o] + s[o];
     r[o] = rv;
     d0 += double(rv) * double(rv);
@@ -604,8 +720,10 @@ struct Krylov {
   int esize = 0;            // bytes per value of the sweep-precision vectors (4 or 8)
   void* v[6] = {};          // compact, sweep precision: r, r0, p, q, s, t
   double* dx = nullptr;     // compact FP64 correction since the last reliable update
-  void* g1 = nullptr;       // ghosted sweep scratch, sweep precision
-  double* g[2] = {};        // ghosted FP64 preconditioned vectors p_, s_ (SpMV inputs)
+  void* g1 = nullptr;       // compact sweep scratch, sweep precision
+  void* x1 = nullptr;       // its column export
+  double* g[2] = {};        // compact FP64 preconditioned vectors p_, s_ (SpMV inputs)
+  double* xg[2] = {};       // their column exports
   double* b64 = nullptr;    // folded right-hand side, compact
   double* parts = nullptr;  // 2 per block
   SegState* st = nullptr;
@@ -626,7 +744,8 @@ struct Krylov {
 static void krylov_free_vectors(Krylov* k) {
   for (auto& p : k->v) { dev_free(p); p = nullptr; }
   dev_free(k->g1);
-  k->g1 = nullptr;
+  dev_free(k->x1);
+  k->g1 = k->x1 = nullptr;
   k->esize = 0;
 }
 
@@ -645,6 +764,7 @@ void krylov_release(bcm_mesh* m) {
   krylov_free_vectors(k);
   dev_free(k->dx);
   for (auto& p : k->g) dev_free(p);
+  for (auto& p : k->xg) dev_free(p);
   dev_free(k->b64);
   dev_free(k->parts);
   dev_free(k->list);
@@ -664,7 +784,7 @@ static int zero_async(bcm_mesh* m, T* p, size_t count) {
 }
 
 static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
-  const size_t nc = m->ncell, ng = size_t(m->nblocks) * m->bvol;
+  const size_t nc = m->ncell, nx = size_t(m->nblocks) * 4 * m->n * m->n;
   if (!m->kry) {
     Krylov* k = new (std::nothrow) Krylov();
     if (!k) return BCM_ERR_NOMEM;
@@ -675,10 +795,11 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->list), sizeof(int) * size_t(m->nblocks)));
     BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->st), sizeof(SegState) * size_t(m->nseg)));
     BCM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&k->h_st), sizeof(SegState) * size_t(m->nseg)));
-    // Ghost cells of outer faces are never written: start at zero.
-    for (auto& p : k->g) {
-      BCM_TRY(dev_alloc(reinterpret_cast<void**>(&p), sizeof(double) * ng));
-      BCM_TRY(zero_async(m, p, ng));
+    for (int i = 0; i < 2; ++i) {
+      BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->g[i]), sizeof(double) * nc));
+      BCM_TRY(dev_alloc(reinterpret_cast<void**>(&k->xg[i]), sizeof(double) * nx));
+      BCM_TRY(zero_async(m, k->g[i], nc));
+      BCM_TRY(zero_async(m, k->xg[i], nx));
     }
   }
   Krylov* k = m->kry;
@@ -691,9 +812,15 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
       if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nc));
       else BCM_TRY(zero_async(m, static_cast<float*>(p), nc));
     }
-    BCM_TRY(dev_alloc(&k->g1, ng * esize));
-    if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(k->g1), ng));
-    else BCM_TRY(zero_async(m, static_cast<float*>(k->g1), ng));
+    BCM_TRY(dev_alloc(&k->g1, nc * esize));
+    BCM_TRY(dev_alloc(&k->x1, nx * esize));
+    if (esize == 8) {
+      BCM_TRY(zero_async(m, static_cast<double*>(k->g1), nc));
+      BCM_TRY(zero_async(m, static_cast<double*>(k->x1), nx));
+    } else {
+      BCM_TRY(zero_async(m, static_cast<float*>(k->g1), nc));
+      BCM_TRY(zero_async(m, static_cast<float*>(k->x1), nx));
+    }
     k->esize = esize;
   }
   *out = k;
@@ -764,22 +891,20 @@ static int seg_reduce(bcm_mesh* m) {
 // kernel; sweeps 2..pre alternate between the sweep scratch (precision T)
 // and the FP64 result so that the last one lands in `result`.
 template <typename T, int N>
-static int precondition(bcm_mesh* m, T* scratch, double* result, const T* rhs, int pre, double omega) {
+static int precondition(bcm_mesh* m, T* scratch, T* sx, double* result, double* rx, const T* rhs, int pre,
+                        double omega) {
   bool in_result = (pre % 2 == 1);
   const dim3 grid(m->kry->n_list), blk(KWG);
   const int* bs = m->d_blk_seg;
   const int* bl = m->kry->list;
   const SegState* st = m->kry->st;
   for (int sweep = 2; sweep <= pre; ++sweep) {
-    if (in_result) {
-      BCM_TRY((gather<double, N>(m, result, 1)));
-      hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, result, rhs, m->d_meta,
-                         bs, st, omega, bl);
-    } else {
-      BCM_TRY((gather<T, N>(m, scratch, 1)));
-      hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, scratch, rhs, m->d_meta,
-                         bs, st, omega, bl);
-    }
+    if (in_result)
+      hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
+                         m->d_meta, bs, st, omega, bl, m->d_desc);
+    else
+      hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
+                         m->d_meta, bs, st, omega, bl, m->d_desc);
     BCM_HIP_TRY(hipGetLastError());
     in_result = !in_result;
   }
@@ -788,8 +913,8 @@ static int precondition(bcm_mesh* m, T* scratch, double* result, const T* rhs, i
 
 // Update kernel (p or s) with the fused first sweep into the right target.
 template <typename T, int N, bool IS_P>
-static int update_and_first_sweep(bcm_mesh* m, T* v, const T* r, const T* q, T* r0, T* scratch, double* result,
-                                  int pre, double omega) {
+static int update_and_first_sweep(bcm_mesh* m, T* v, const T* r, const T* q, T* r0, T* scratch, T* sx,
+                                  double* result, double* rx, int pre, double omega) {
   const dim3 grid(m->kry->n_list), blk(KWG);
   const int* bs = m->d_blk_seg;
   const int* bl = m->kry->list;
@@ -797,24 +922,24 @@ static int update_and_first_sweep(bcm_mesh* m, T* v, const T* r, const T* q, T* 
   const bool to_result = pre == 0 || pre % 2 == 1;
   if (IS_P) {
     if (pre == 0)
-      hipLaunchKernelGGL((k_kry_p<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, r0, result, m->d_meta, bs, st,
-                         omega, bl);
+      hipLaunchKernelGGL((k_kry_p<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, r0, result, rx, m->d_meta,
+                         bs, st, omega, bl);
     else if (to_result)
-      hipLaunchKernelGGL((k_kry_p<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, r0, result, m->d_meta, bs, st,
-                         omega, bl);
+      hipLaunchKernelGGL((k_kry_p<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, r0, result, rx, m->d_meta,
+                         bs, st, omega, bl);
     else
-      hipLaunchKernelGGL((k_kry_p<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, r0, scratch, m->d_meta, bs, st,
-                         omega, bl);
+      hipLaunchKernelGGL((k_kry_p<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, r0, scratch, sx, m->d_meta, bs,
+                         st, omega, bl);
   } else {
     if (pre == 0)
-      hipLaunchKernelGGL((k_kry_s<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, result, m->d_meta, bs, st,
+      hipLaunchKernelGGL((k_kry_s<T, N, 0, double>), grid, blk, 0, m->stream, v, r, q, result, rx, m->d_meta, bs, st,
                          omega, bl);
     else if (to_result)
-      hipLaunchKernelGGL((k_kry_s<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, result, m->d_meta, bs, st,
+      hipLaunchKernelGGL((k_kry_s<T, N, 1, double>), grid, blk, 0, m->stream, v, r, q, result, rx, m->d_meta, bs, st,
                          omega, bl);
     else
-      hipLaunchKernelGGL((k_kry_s<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, scratch, m->d_meta, bs, st, omega,
-                         bl);
+      hipLaunchKernelGGL((k_kry_s<T, N, 1, T>), grid, blk, 0, m->stream, v, r, q, scratch, sx, m->d_meta, bs, st,
+                         omega, bl);
   }
   BCM_HIP_TRY(hipGetLastError());
   return BCM_OK;
@@ -826,22 +951,22 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
   Krylov* K = m->kry;
   T *r = static_cast<T*>(K->v[0]), *r0 = static_cast<T*>(K->v[1]), *p = static_cast<T*>(K->v[2]),
     *q = static_cast<T*>(K->v[3]), *s = static_cast<T*>(K->v[4]), *t = static_cast<T*>(K->v[5]);
-  T* g1 = static_cast<T*>(K->g1);
-  double *gp = K->g[0], *gs = K->g[1];
+  T *g1 = static_cast<T*>(K->g1), *x1 = static_cast<T*>(K->x1);
+  double *gp = K->g[0], *gs = K->g[1], *xp = K->xg[0], *xs = K->xg[1];
   const dim3 grid(K->n_list), blk(KWG);
   const int* bs = m->d_blk_seg;
   const int* bl = K->list;
   SegState* st = K->st;
-  BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, gp, pre, omega)));
-  BCM_TRY((precondition<T, N>(m, g1, gp, p, pre, omega)));
-  BCM_TRY((gather<double, N>(m, gp, 1)));
-  hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, r0, m->d_meta, bs, st, K->parts, bl);
+  BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, x1, gp, xp, pre, omega)));
+  BCM_TRY((precondition<T, N>(m, g1, x1, gp, xp, p, pre, omega)));
+  hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts, bl,
+                     m->d_desc);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_Q>(m));
-  BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, gs, pre, omega)));
-  BCM_TRY((precondition<T, N>(m, g1, gs, s, pre, omega)));
-  BCM_TRY((gather<double, N>(m, gs, 1)));
-  hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, s, m->d_meta, bs, st, K->parts, bl);
+  BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, x1, gs, xs, pre, omega)));
+  BCM_TRY((precondition<T, N>(m, g1, x1, gs, xs, s, pre, omega)));
+  hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts, bl,
+                     m->d_desc);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_T>(m));
   hipLaunchKernelGGL((k_kry_xr<T, N>), grid, blk, 0, m->stream, K->dx, gp, gs, r, s, t, r0, bs, st, K->parts, bl);
@@ -1060,10 +1185,11 @@ int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg
 }
 
 size_t krylov_bytes(int nblocks, int n, int precision) {
-  const size_t nc = size_t(nblocks) * n * n * n, ng = size_t(nblocks) * (n + 2) * (n + 2) * (n + 2);
+  const size_t nc = size_t(nblocks) * n * n * n, nx = size_t(nblocks) * 4 * n * n;
   const size_t es = precision == BCM_PREC_FP64 ? 8 : 4;
-  // b, dx (FP64 compact); p_, s_ (FP64 ghosted); r, r0, p, q, s, t (compact); sweep scratch (ghosted).
-  return 2 * nc * 8 + 2 * ng * 8 + 6 * nc * es + ng * es + size_t(nblocks) * 16;
+  // b, dx, p_, s_ (FP64 compact) + the exports of p_, s_; r, r0, p, q, s, t
+  // and the sweep scratch (compact) + its export; partials, list.
+  return 4 * nc * 8 + 2 * nx * 8 + 7 * nc * es + nx * es + size_t(nblocks) * 20;
 }
 
 }  // namespace bcm

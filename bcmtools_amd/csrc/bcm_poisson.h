@@ -75,3 +75,33 @@ __device__ __forceinline__ T r_cell(const Coef7<T>& a, const T* __restrict__ x, 
 }
 
 }  // namespace bcm
+
+namespace bcm {
+
+// The same cell operators on gathered neighbour values (W, E, S, N, B, T
+// and the centre), for kernels whose neighbours come partly from a ghost
+// frame in LDS; the operation order is that of the pointer forms above.
+template <typename T>
+__device__ __forceinline__ T jacobi_vals(const Coef7<T>& a, T w, T e, T s, T n, T b, T t, T c, T bv, T om) {
+  T r = bv - a.w * w;
+  r = r - a.e * e;
+  r = r - a.s * s;
+  r = r - a.n * n;
+  r = r - a.b * b;
+  r = r - a.t * t;
+  const T xt = r / a.p;
+  return c - om * (c - xt);
+}
+
+template <typename T>
+__device__ __forceinline__ T ax_vals(const Coef7<T>& a, T w, T e, T s, T n, T b, T t, T c) {
+  T v = a.w * w;
+  v = v + a.e * e;
+  v = v + a.s * s;
+  v = v + a.n * n;
+  v = v + a.b * b;
+  v = v + a.t * t;
+  return v + a.p * c;
+}
+
+}  // namespace bcm

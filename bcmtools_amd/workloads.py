@@ -135,8 +135,8 @@ def random_tree(seed, max_level=None, nspheres=None):
 
 def serving_tree(seed, target_blocks):
     """A request mesh of the served workload: 1-3 random sphere surfaces
-    (as random_tree), refined to the smallest max level (2..8) whose block
-    count reaches 0.7 x target_blocks."""
+    (as random_tree), refined to the max level (2..8) whose block count is
+    closest to target_blocks in ratio."""
     rng = np.random.default_rng(seed)
     nspheres = int(rng.integers(1, 4))
     shells = []
@@ -144,13 +144,15 @@ def serving_tree(seed, target_blocks):
         c = rng.uniform(0.2, 0.8, size=3)
         r = float(rng.uniform(0.05, 0.3))
         shells.append((c[0], c[1], c[2], r, r))
-    t = None
+    best = None
     for level in range(2, 9):
         t = Tree(root_n=(2, 2, 2), root_length=0.5, divider=L.DIV_SHELL, min_level=0, max_level=level - 1,
                  shells=shells, ordering=L.ORDER_Z, balance=1, shell_mode=1, shell_margin=0.25)
-        if len(t.blocks) >= 0.7 * target_blocks:
+        if best is None or abs(math.log(len(t.blocks) / target_blocks)) < abs(math.log(len(best.blocks) / target_blocks)):
+            best = t
+        if len(t.blocks) >= target_blocks:
             break
-    return t
+    return best
 
 
 def interior(a, vc):
