@@ -107,6 +107,7 @@ class SolveParams(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("restart_every", ctypes.c_int32),
         ("stall_window", ctypes.c_int32),
+        ("stagnation", ctypes.c_int32),
     ]
 
 

@@ -335,13 +335,13 @@ class Mesh:
 
     def solve(self, x, rhs, bvalues, precision="mixed", tol=1e-10, max_iter=5000, count_pre=2, omega=0.8,
               refresh_max=0, refresh_delta=0.0, check_every=8, use_graph=True, seg_tol=None, restart_every=0,
-              stall_window=0):
+              stall_window=0, stagnation=0):
         """Fused device-resident BiCGSTAB on every segment (bcm_solve).
         Returns one dict per segment: status, iters, refreshes, relres."""
         prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "fp32": L.PREC_FP32}[precision]
         p = L.SolveParams(prec, float(tol), int(max_iter), int(count_pre), float(omega), int(refresh_max),
                           float(refresh_delta), int(check_every), int(bool(use_graph)), int(restart_every),
-                          int(stall_window))
+                          int(stall_window), int(stagnation))
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
         st = (L.SolveStats * self.nseg)()
         tl = None

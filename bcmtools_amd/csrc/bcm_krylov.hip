@@ -54,6 +54,7 @@ struct SegState {
   double tol2;      // tol^2
   double delta2;    // refresh_delta^2 (0: off)
   double rr_min;    // smallest recurrence (r, r) since the last reliable update
+  double rr_best;   // smallest true (r, r) so far
   int32_t status;   // BCM_SOLVE_*
   int32_t first;    // next iteration starts with p = r
   int32_t refresh;  // reliable update after this iteration
@@ -64,6 +65,8 @@ struct SegState {
   int32_t restart_every;  // restart at every k-th reliable update (0: never)
   int32_t stall;          // iterations without a new rr_min that force an update (0: off)
   int32_t since_min;      // iterations since rr_min last fell by 1 %
+  int32_t stag_max;       // consecutive true checks without halving ||r|| that end the solve (0: off)
+  int32_t n_stag;         // current run of such checks
 };
 
 enum { TRIG_CONV = 1, TRIG_SCHEDULE = 2, TRIG_LIMIT = 4, TRIG_BREAKDOWN = 8 };
@@ -677,9 +680,18 @@ __global__ __launch_bounds__(KWG) void k_seg(SegState* __restrict__ st, const do
       s.since_min = 0;
       s.since = 0;
       s.nref += 1;
+      // Stagnation: the true residual has reached the accuracy the
+      // arithmetic allows (about eps * cond(A) of ||b||) when it has not
+      // halved over stag_max consecutive checks.
+      if (a < 0.25 * s.rr_best) {
+        s.rr_best = a;
+        s.n_stag = 0;
+      } else {
+        s.n_stag += 1;
+      }
       if (conv_test(a, s)) {
         s.status = BCM_SOLVE_CONVERGED;
-      } else if (s.iters >= s.max_iter) {
+      } else if (s.iters >= s.max_iter || (s.stag_max > 0 && s.n_stag >= s.stag_max)) {
         s.status = BCM_SOLVE_NOCONV;
       } else if (s.restart || (s.trig & TRIG_CONV) || (s.restart_every > 0 && s.nref % s.restart_every == 0)) {
         // Breakdown, or the recurrence met the tolerance while the true
@@ -1028,6 +1040,8 @@ static int begin_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve
     z.refresh_max = rmax;
     z.restart_every = rs_every;
     z.stall = stall;
+    z.stag_max = P.stagnation > 0 ? P.stagnation : (P.stagnation == 0 ? 8 : 0);
+    z.rr_best = DBL_MAX;
     z.rho_prev = 1.0;
     z.gamma = 1.0;
     K->h_st[s] = z;

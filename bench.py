@@ -426,7 +426,10 @@ def bench_serving(args):
     while len(done) < R:
         now = sv.clock() - t0
         while k < R and arrivals[k] <= now:
-            prec = "fp32" if tols[k] > 1e-8 else "mixed"
+            if args.policy == "spec":
+                prec = "fp32" if tols[k] > 1e-8 else "mixed"
+            else:
+                prec = "fp64"
             sv.submit(trees[k].blocks, float(tols[k]), prec, rhs=None, seed=9000 + k, tag=k)
             k += 1
         wait = (arrivals[k] - now) if k < R else 0.05
@@ -442,6 +445,8 @@ def bench_serving(args):
     lat = np.array([done[i]["t_done"] - t0 - arrivals[i] for i in range(R)])
     wait_q = np.array([done[i]["t_admit"] - t0 - arrivals[i] for i in range(R)])
     ok = [done[i]["status"] == 1 and done[i]["relres"] <= tols[i] for i in range(R)]
+    failed = [{"tag": i, "status": done[i]["status"], "relres": done[i]["relres"], "iters": done[i]["iters"],
+               "nblocks": done[i]["nblocks"]} for i in range(R) if not ok[i]]
     sv.close()
     by_tol = {}
     for tv in (1e-6, 1e-10):
@@ -457,11 +462,13 @@ def bench_serving(args):
         "n_gpus": 1, "steps": R, "warmup": 0,
         "ms_per_step": (t_end - arrivals[0]) / R * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic",
+        "dtype": "f64" if args.policy == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"{R} requests, Poisson arrivals at {lam}/s, meshes of 1-3 random sphere surfaces "
-                               "with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 (FP32 path) or "
-                               "1e-10 (FP32 + FP64 refinement), up to "
-                               f"{args.slots} requests on the device, HBM budget 200 GB",
+                               "with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 or 1e-10, "
+                               + ("FP32 path for 1e-6 and FP32 + FP64 refinement for 1e-10"
+                                  if args.policy == "spec" else "fused FP64 BiCGSTAB for every request")
+                               + f", up to {args.slots} requests on the device, HBM budget 200 GB",
+                   "precision_policy": args.policy,
                    "requests": R, "lambda_per_s": lam, "blocks_total": int(sum(counts)),
                    "blocks_per_request": [int(min(counts)), int(np.median(counts)), int(max(counts))]},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -469,6 +476,7 @@ def bench_serving(args):
         "queue_wait_s": {"p50": float(np.percentile(wait_q, 50)), "p99": float(np.percentile(wait_q, 99))},
         "by_tolerance": by_tol,
         "all_met_tolerance": bool(all(ok)),
+        "not_met": failed,
         "makespan_s": float(t_end),
     }
 
@@ -510,6 +518,8 @@ def main():
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=16, help="serving: requests on the device at once")
     ap.add_argument("--seed", type=int, default=1, help="serving: request stream seed")
+    ap.add_argument("--policy", default="fp64", choices=["fp64", "spec"],
+                    help="serving: fp64 for every request, or FP32 (tol 1e-6) / mixed (1e-10)")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="batched: precision of the full batch (auto: the faster on the subset)")

@@ -364,6 +364,10 @@ typedef struct bcm_solve_params {
                                  < 0: only on breakdown or a failed convergence check */
   int32_t stall_window;       /* MIXED/FP32: FP64 update when the recurrence residual set no
                                  new minimum for this many iterations (0: 40, < 0: off) */
+  int32_t stagnation;         /* end a segment as BCM_SOLVE_NOCONV when this many consecutive
+                                 true-residual checks did not halve the best ||b - A x||:
+                                 the tolerance is below what the arithmetic attains
+                                 (0: 8, < 0: off, run to max_iter) */
 } bcm_solve_params;
 typedef struct bcm_solve_stats {
   int32_t status;             /* BCM_SOLVE_CONVERGED / NOCONV / FAILED */

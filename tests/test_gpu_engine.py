@@ -146,3 +146,21 @@ def test_engine_zero_rhs_and_iteration_limit():
     x = gx.download()
     assert np.all(x[:len(blocks)] == 0.0)
     assert np.all(np.isfinite(x))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_engine_stops_at_the_attainable_accuracy(precision):
+    """A tolerance below what FP64 attains (about eps * cond(A)) ends as
+    noconv once the true residual stops halving, long before max_iter; the
+    reported residual is the true one of the returned x."""
+    blocks, rhs = _problem(4, 16, 14)
+    case(f"engine stagnation {precision} nb={len(blocks)}")
+    mesh = Mesh(blocks, 16, 1)
+    grhs = mesh.field(np.float64, rhs)
+    gx = mesh.field(np.float64)
+    st = mesh.solve(gx, grhs, [0.0], precision=precision, tol=1e-18, max_iter=20000)[0]
+    assert st["status"] == "noconv", st
+    assert st["iters"] < 5000, st
+    rr = oracle_relres(blocks, 16, gx.download(), rhs)
+    assert rr <= 1e-10, (rr, st)
+    assert abs(rr - st["relres"]) <= 0.1 * rr, (rr, st)
