@@ -24,7 +24,7 @@ X_M, X_P, Y_M, Y_P, Z_M, Z_P = range(6)
 BC_INNER, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
 DIV_FLAT, DIV_SIMPLE, DIV_SHELL = range(3)
 ORDER_Z, ORDER_HILBERT = range(2)
-BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z = range(4)
+BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z, BCSET_PER_FACE = range(5)
 GF_FLAT, GF_F2C, GF_C2F, GF_ALL = 1, 2, 4, 7
 GF_X, GF_Y, GF_Z = 8, 16, 32
 GF_BLOCKWISE = 64
@@ -70,6 +70,7 @@ class TreeParams(ctypes.Structure):
         ("bc_setter", ctypes.c_int32),
         ("shell_mode", ctypes.c_int32),
         ("shell_margin", ctypes.c_double),
+        ("face_bc_type", ctypes.c_int32 * 6),
     ]
 
 
@@ -182,6 +183,7 @@ SIGNATURES = {
     "bcm_field_bytes": (ctypes.c_size_t, [_P]),
     "bcm_ghost_fill": (_I, [_P, _I]),
     "bcm_dirichlet_mirror": (_I, [_P, _DP, _I]),
+    "bcm_impose_bc": (_I, [_P, _DP, _I]),
     "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),

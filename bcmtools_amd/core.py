@@ -20,7 +20,7 @@ class Tree:
 
     def __init__(self, root_n=(1, 1, 1), periodic=(0, 0, 0), root_origin=(0.0, 0.0, 0.0), root_length=1.0,
                  divider=L.DIV_FLAT, min_level=0, max_level=0, shells=(), ordering=L.ORDER_Z, balance=1,
-                 bc_setter=L.BCSET_ALL_DIRICHLET, shell_mode=0, shell_margin=0.0):
+                 bc_setter=L.BCSET_ALL_DIRICHLET, shell_mode=0, shell_margin=0.0, face_bc_type=None):
         lib = L.load()
         p = L.TreeParams()
         for a in range(3):
@@ -43,6 +43,11 @@ class Tree:
         p.bc_setter = int(bc_setter)
         p.shell_mode = int(shell_mode)
         p.shell_margin = float(shell_margin)
+        if face_bc_type is not None:
+            if bc_setter != L.BCSET_PER_FACE or len(face_bc_type) != 6:
+                raise ValueError("face_bc_type needs bc_setter=BCSET_PER_FACE and six entries")
+            for f in range(6):
+                p.face_bc_type[f] = int(face_bc_type[f])
         self.params = p
         h = ctypes.c_void_p()
         L.check(lib.bcm_tree_build(ctypes.byref(p), ctypes.byref(h)), "bcm_tree_build")
@@ -214,6 +219,12 @@ class Mesh:
     def dirichlet_mirror(self, f, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
         L.check(self.lib.bcm_dirichlet_mirror(f.handle, L.dptr(v), len(v)), "bcm_dirichlet_mirror")
+
+    def impose_bc(self, f, values):
+        """BlockScalar3D::ImposeBoundaryCondition (examples/SolverTest/BlockScalar3D.h:199-206):
+        Dirichlet and Neumann ghosts on every outer face."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        L.check(self.lib.bcm_impose_bc(f.handle, L.dptr(v), len(v)), "bcm_impose_bc")
 
     # -- red-black SOR ------------------------------------------------------
     def rbsor_sweep(self, f, source=None, omega=1.0):

@@ -30,7 +30,9 @@ namespace bcm {
 // Per-block constants used on the device.
 struct BlockMeta {
   double cx, cy, cz;      // 1/h^2 per axis, h = block_size / n
+  double hx;              // cell size along x, the scale of a Neumann value
   int32_t dir_mask;       // bit f: face f carries a Dirichlet condition
+  int32_t neu_mask;       // bit f: face f carries a Neumann condition
   int32_t bc_id[6];       // boundary-value index per face (-1 if none)
 };
 
@@ -65,6 +67,7 @@ struct bcm_mesh {
   int4* d_bc_tasks = nullptr;
   int n_bc_tasks = 0;
   int n_dirichlet = 0;
+  int n_neumann = 0;
   bcm::BlockMeta* d_meta = nullptr;
 
   // Reduction scratch: one double per reduction work-group, plus results.

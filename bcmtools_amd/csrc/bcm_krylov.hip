@@ -536,7 +536,7 @@ __global__ __launch_bounds__(KWG) void k_ref_r(T* __restrict__ r, const T* __res
   }
 }
 
-// Folded right-hand side, compact (k_assemble's b: bca.f90 b -= Anb v 2),
+// Folded right-hand side, compact (k_assemble's b, bca.f90 folds),
 // and the per-block (b, b).
 template <int N>
 __global__ __launch_bounds__(KWG) void k_kry_rhs(double* __restrict__ b64, const double* __restrict__ rhs,
@@ -552,11 +552,7 @@ __global__ __launch_bounds__(KWG) void k_kry_rhs(double* __restrict__ b64, const
     const int c = threadIdx.x + KWG * m;
     int i, j, k;
     B::ijk(c, i, j, k);
-    const double l[6] = {me.cx, me.cx, me.cy, me.cy, me.cz, me.cz};
-    double bv = rhs[gb + B::g(i, j, k)];
-    const int at[6] = {i == 0, i == N - 1, j == 0, j == N - 1, k == 0, k == N - 1};
-    for (int f = 0; f < 6; ++f)
-      if ((me.dir_mask >> f & 1) && at[f]) bv = bv - l[f] * bval[me.bc_id[f]] * 2.0;
+    const double bv = fold_rhs(me, i, j, k, N, rhs[gb + B::g(i, j, k)], bval);
     b64[cb + c] = bv;
     acc += bv * bv;
   }

@@ -77,7 +77,7 @@ int timer_end(bcm_mesh* m, EventPair* ev) {
 
 int set_bvalues(bcm_mesh* m, const double* values, int n) {
   if (n < 0 || (n > 0 && !values)) return BCM_ERR_ARG;
-  // Every Dirichlet face must index a supplied value.
+  // Every Dirichlet or Neumann face must index a supplied value.
   if (n < m->need_bvalues) return BCM_ERR_ARG;
   if (m->bvalues_valid && int(m->h_bvalues.size()) == n &&
       std::equal(m->h_bvalues.begin(), m->h_bvalues.end(), values))
@@ -243,6 +243,68 @@ static int dirichlet_raw(bcm_mesh* m, T* data) {
   return timer_end(m, &ev);
 }
 
+// ---------------------------------------------------------------------
+// Outer-face conditions of BlockScalar3D::ImposeBoundaryCondition
+// (examples/SolverTest/BlockScalar3D.h:199-206 with bcx.f90:18-376), per
+// outer face and adjacent interior cell u:
+//   Dirichlet  g1 = 2 v - u,   g2 = v             (bc_x*_d)
+//   Neumann    g1 = u + xc,    g2 = u + 2 xc      on a plus face
+//              g1 = u - xc,    g2 = u - 2 xc      on a minus face (bc_x*_n)
+// with xc = v * cellSize.x (BlockScalar3D.h:98-100).  g2 is written only
+// when the field has a second ghost layer (the Fortran writes it regardless).
+// The arithmetic is FP64, rounded once to the field type.
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_impose_bc(T* __restrict__ d, const int4* __restrict__ tasks,
+                                                   const double* __restrict__ vals,
+                                                   const BlockMeta* __restrict__ meta, Geo g) {
+  const int4 tk = tasks[blockIdx.x];
+  const int face = tk.y, axis = face >> 1;
+  const bool plus = face & 1;
+  const bool neu = tk.w == BCM_BC_NEUMANN;
+  T* __restrict__ f = d + (long long)tk.x * g.bvol;
+  const double v = vals[tk.z];
+  const double xc = neu ? v * meta[tk.x].hx : v;
+  const int n = g.n;
+  const bool two = g.vc >= 2;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int p = e % n, q = e / n;
+    int c[3];
+    // X faces loop (j,k), Y faces (i,k), Z faces (i,j).
+    if (axis == 0) { c[1] = p; c[2] = q; }
+    else if (axis == 1) { c[0] = p; c[2] = q; }
+    else { c[0] = p; c[1] = q; }
+    c[axis] = plus ? n - 1 : 0;
+    const double u = double(f[g.idx(c[0], c[1], c[2])]);
+    double g1, g2;
+    if (neu) {
+      g1 = plus ? u + xc : u - xc;
+      g2 = plus ? u + xc * 2.0 : u - xc * 2.0;
+    } else {
+      g1 = 2.0 * xc - u;
+      g2 = xc;
+    }
+    c[axis] = plus ? n : -1;
+    f[g.idx(c[0], c[1], c[2])] = T(g1);
+    if (two) {
+      c[axis] = plus ? n + 1 : -2;
+      f[g.idx(c[0], c[1], c[2])] = T(g2);
+    }
+  }
+}
+
+template <typename T>
+static int impose_bc_raw(bcm_mesh* m, T* data) {
+  if (m->n_bc_tasks == 0) return BCM_OK;
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_DIRICHLET, &ev));
+  hipLaunchKernelGGL(k_impose_bc<T>, dim3(m->n_bc_tasks), dim3(256), 0, m->stream, data, m->d_bc_tasks,
+                     m->d_bvalues, m->d_meta, g);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
 int dirichlet_device(bcm_mesh* m, bcm_field* f) {
   if (f->dtype == BCM_F64) return dirichlet_raw<double>(m, static_cast<double*>(f->data));
   return dirichlet_raw<float>(m, static_cast<float*>(f->data));
@@ -330,7 +392,8 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
         if (bi.nb_id[f][s] < -1 || bi.nb_id[f][s] >= nblocks) return BCM_ERR_ARG;
       if (ld == -1 && (bi.nb_subface[f] < 0 || bi.nb_subface[f] > 3)) return BCM_ERR_ARG;
       if (bi.bc_type[f] < BCM_BC_INNER || bi.bc_type[f] > BCM_BC_PERIODIC) return BCM_ERR_ARG;
-      if (bi.bc_type[f] == BCM_BC_DIRICHLET && bi.bc_id[f] < 0) return BCM_ERR_ARG;
+      if ((bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) && bi.bc_id[f] < 0)
+        return BCM_ERR_ARG;
     }
   }
   bcm_mesh* m = new (std::nothrow) bcm_mesh();
@@ -358,7 +421,9 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
       me.cx = 1.0 / (hx * hx);
       me.cy = 1.0 / (hy * hy);
       me.cz = 1.0 / (hz * hz);
+      me.hx = hx;
       me.dir_mask = 0;
+      me.neu_mask = 0;
       for (int f = 0; f < 6; ++f) {
         me.bc_id[f] = bi.bc_id[f];
         const int axis = f / 2;
@@ -381,8 +446,11 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
           if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
             me.dir_mask |= 1 << f;
             m->n_dirichlet++;
-            m->need_bvalues = std::max(m->need_bvalues, bi.bc_id[f] + 1);
+          } else {
+            me.neu_mask |= 1 << f;
+            m->n_neumann++;
           }
+          m->need_bvalues = std::max(m->need_bvalues, bi.bc_id[f] + 1);
         }
       }
     }
@@ -667,6 +735,14 @@ extern "C" int bcm_ghost_fill(bcm_field_t f, int mask) {
   }
   if (f->dtype == BCM_F64) return ghost_fill_raw<double>(f->mesh, static_cast<double*>(f->data), mask);
   return ghost_fill_raw<float>(f->mesh, static_cast<float*>(f->data), mask);
+}
+
+extern "C" int bcm_impose_bc(bcm_field_t f, const double* values, int nvalues) {
+  if (!f) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  BCM_TRY(set_bvalues(m, values, nvalues));
+  if (f->dtype == BCM_F64) return impose_bc_raw<double>(m, static_cast<double*>(f->data));
+  return impose_bc_raw<float>(m, static_cast<float*>(f->data));
 }
 
 extern "C" int bcm_dirichlet_mirror(bcm_field_t f, const double* values, int nvalues) {

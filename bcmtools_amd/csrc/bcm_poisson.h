@@ -10,12 +10,20 @@
 namespace bcm {
 
 // Poisson operator (examples/SolverTest/bcut.f90:2848-2866): l = 1/h^2 per
-// axis, Ap = -(l0+l1)-(l2+l3)-(l4+l5) and, on an outer Dirichlet face,
-// Ap -= Anb; Anb = 0 (examples/SolverTest/bca.f90:12-42).
+// axis, Ap = -(l0+l1)-(l2+l3)-(l4+l5); then, face by face in X_M..Z_P
+// order (examples/SolverTest/BlockScalar3D.h:207-214), the outer-face fold:
+// Dirichlet Ap -= Anb (bca.f90:12-202), Neumann Ap += Anb (bca.f90:204-394);
+// Anb = 0.
 template <typename T>
 struct Coef7 {
   T p, w, e, s, n, b, t;
 };
+
+template <typename T>
+__device__ __forceinline__ void fold_face(T& p, T& a, bool neumann) {
+  p = neumann ? p + a : p - a;
+  a = T(0);
+}
 
 template <typename T>
 __device__ __forceinline__ Coef7<T> poisson_coef(const BlockMeta& me, int i, int j, int k, int n) {
@@ -23,16 +31,40 @@ __device__ __forceinline__ Coef7<T> poisson_coef(const BlockMeta& me, int i, int
   Coef7<T> a;
   a.w = lx; a.e = lx; a.s = ly; a.n = ly; a.b = lz; a.t = lz;
   a.p = ((-(a.w + a.e)) - (a.s + a.n)) - (a.b + a.t);
-  const int dm = me.dir_mask;
-  if (dm) {
-    if ((dm & 1) && i == 0) { a.p = a.p - a.w; a.w = T(0); }
-    if ((dm & 2) && i == n - 1) { a.p = a.p - a.e; a.e = T(0); }
-    if ((dm & 4) && j == 0) { a.p = a.p - a.s; a.s = T(0); }
-    if ((dm & 8) && j == n - 1) { a.p = a.p - a.n; a.n = T(0); }
-    if ((dm & 16) && k == 0) { a.p = a.p - a.b; a.b = T(0); }
-    if ((dm & 32) && k == n - 1) { a.p = a.p - a.t; a.t = T(0); }
+  const int bm = me.dir_mask | me.neu_mask;
+  if (bm) {
+    const int nm = me.neu_mask;
+    if ((bm & 1) && i == 0) fold_face(a.p, a.w, nm & 1);
+    if ((bm & 2) && i == n - 1) fold_face(a.p, a.e, nm & 2);
+    if ((bm & 4) && j == 0) fold_face(a.p, a.s, nm & 4);
+    if ((bm & 8) && j == n - 1) fold_face(a.p, a.n, nm & 8);
+    if ((bm & 16) && k == 0) fold_face(a.p, a.b, nm & 16);
+    if ((bm & 32) && k == n - 1) fold_face(a.p, a.t, nm & 32);
   }
   return a;
+}
+
+// The folded right-hand side of one cell: rhs, then per outer face in
+// X_M..Z_P order Dirichlet b -= Anb*v*2 (bca.f90:33) and Neumann
+// b += Anb*xc on a minus face, b -= Anb*xc on a plus face (bca.f90:225,257),
+// with xc = v * cellSize.x (examples/SolverTest/BlockScalar3D.h:98-100).
+__device__ __forceinline__ double fold_rhs(const BlockMeta& me, int i, int j, int k, int n, double bv,
+                                           const double* __restrict__ bval) {
+  const int bm = me.dir_mask | me.neu_mask;
+  if (!bm) return bv;
+  const double l[6] = {me.cx, me.cx, me.cy, me.cy, me.cz, me.cz};
+  const int at[6] = {i == 0, i == n - 1, j == 0, j == n - 1, k == 0, k == n - 1};
+  for (int f = 0; f < 6; ++f) {
+    if (!((bm >> f & 1) && at[f])) continue;
+    const double v = bval[me.bc_id[f]];
+    if (me.neu_mask >> f & 1) {
+      const double xc = v * me.hx;
+      bv = (f & 1) ? bv - l[f] * xc : bv + l[f] * xc;
+    } else {
+      bv = bv - l[f] * v * 2.0;
+    }
+  }
+  return bv;
 }
 
 // jacobi_smoother (blas.f90:485-493).

@@ -436,7 +436,7 @@ __global__ __launch_bounds__(kWG) void k_convert(void* __restrict__ dst, const v
 }
 
 // Poisson assembly: stored A (optional) and folded b
-// (bcut.f90:2848-2865 without cut cells; bca.f90 bc_A*_d: b -= Anb xc 2).
+// (bcut.f90:2848-2865 without cut cells; the bca.f90 Dirichlet and Neumann folds).
 __global__ __launch_bounds__(kWG) void k_assemble(double* Ap, double* Aw, double* Ae, double* As, double* An,
                                                   double* Ab, double* At, double* __restrict__ bout,
                                                   const double* __restrict__ rhs, const BlockMeta* __restrict__ meta,
@@ -448,21 +448,10 @@ __global__ __launch_bounds__(kWG) void k_assemble(double* Ap, double* Aw, double
     int bk, i, j, k;
     const long long o = cell_offset(g, c, bk, i, j, k);
     const BlockMeta me = meta[bk];
-    const double l[6] = {me.cx, me.cx, me.cy, me.cy, me.cz, me.cz};
-    double a[6] = {l[0], l[1], l[2], l[3], l[4], l[5]};
-    double ap = ((-(a[0] + a[1])) - (a[2] + a[3])) - (a[4] + a[5]);
-    double bv = rhs[o];
-    const int at[6] = {i == 0, i == g.n - 1, j == 0, j == g.n - 1, k == 0, k == g.n - 1};
-    for (int f = 0; f < 6; ++f) {
-      if ((me.dir_mask >> f & 1) && at[f]) {
-        ap = ap - a[f];
-        bv = bv - a[f] * bval[me.bc_id[f]] * 2.0;
-        a[f] = 0.0;
-      }
-    }
-    bout[o] = bv;
+    bout[o] = fold_rhs(me, i, j, k, g.n, rhs[o], bval);
     if (Ap) {
-      Ap[o] = ap; Aw[o] = a[0]; Ae[o] = a[1]; As[o] = a[2]; An[o] = a[3]; Ab[o] = a[4]; At[o] = a[5];
+      const Coef7<double> a = poisson_coef<double>(me, i, j, k, g.n);
+      Ap[o] = a.p; Aw[o] = a.w; Ae[o] = a.e; As[o] = a.s; An[o] = a.n; Ab[o] = a.b; At[o] = a.t;
     }
   }
 }

@@ -347,6 +347,9 @@ struct bcm_tree {
         if (p.bc_setter == BCM_BCSET_ALL_DIRICHLET) {
           bi.bc_type[f] = BCM_BC_DIRICHLET;
           bi.bc_id[f] = 0;
+        } else if (p.bc_setter == BCM_BCSET_PER_FACE) {
+          bi.bc_type[f] = p.face_bc_type[f];
+          bi.bc_id[f] = f;
         } else {
           bi.bc_type[f] = BCM_BC_PERIODIC;
           const int axis = p.bc_setter - BCM_BCSET_SOR1_X;
@@ -373,7 +376,10 @@ extern "C" int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out) {
   if (p.divider < BCM_DIV_FLAT || p.divider > BCM_DIV_SHELL) return BCM_ERR_ARG;
   if (p.divider == BCM_DIV_SHELL && (p.n_shells < 1 || p.n_shells > BCM_MAX_SHELLS)) return BCM_ERR_ARG;
   if (p.ordering != BCM_ORDER_Z && p.ordering != BCM_ORDER_HILBERT) return BCM_ERR_ARG;
-  if (p.bc_setter < BCM_BCSET_ALL_DIRICHLET || p.bc_setter > BCM_BCSET_SOR1_Z) return BCM_ERR_ARG;
+  if (p.bc_setter < BCM_BCSET_ALL_DIRICHLET || p.bc_setter > BCM_BCSET_PER_FACE) return BCM_ERR_ARG;
+  if (p.bc_setter == BCM_BCSET_PER_FACE)
+    for (int f = 0; f < 6; ++f)
+      if (p.face_bc_type[f] != BCM_BC_DIRICHLET && p.face_bc_type[f] != BCM_BC_NEUMANN) return BCM_ERR_ARG;
   if (p.shell_mode < 0 || p.shell_mode > 1 || !(p.shell_margin >= 0.0)) return BCM_ERR_ARG;
   if (!(p.root_length > 0.0)) return BCM_ERR_ARG;
   bcm_tree* t = new (std::nothrow) bcm_tree();

@@ -223,8 +223,13 @@ def bench_bicgstab(args):
         return out, time.perf_counter() - t0
 
     x = mesh.field(np.float64)
-    mixed = lambda xx: mesh.solve(xx, rhs, vals, precision="mixed", tol=tol, max_iter=20000)[0]
-    fp64 = lambda xx: mesh.solve(xx, rhs, vals, precision="fp64", tol=tol, max_iter=20000)[0]
+    ug = not args.no_graph
+    mixed = lambda xx: mesh.solve(xx, rhs, vals, precision="mixed", tol=tol, max_iter=20000, use_graph=ug)[0]
+    fp64 = lambda xx: mesh.solve(xx, rhs, vals, precision="fp64", tol=tol, max_iter=20000, use_graph=ug)[0]
+    if args.precision != "auto":
+        # One engine precision only (e.g. for a kernel trace of that path).
+        return bicgstab_one(args, mesh, rhs, vals, timed, mixed if args.precision == "mixed" else fp64,
+                            level, blocks, ncell, n)
     for _ in range(args.warmup):
         timed(mixed, x)
     times, st = [], None
@@ -266,9 +271,64 @@ def bench_bicgstab(args):
         "speedup_vs_fp64_host_driven": th / (elapsed / len(times)),
         "speedup_vs_fp64_engine": t64 / (elapsed / len(times)),
     }
+    # Algorithmic HBM bytes of one fused FP64 iteration, averaged over the
+    # whole solve (reductions, graph launches and checks included).
+    bpc = engine_bytes_per_cell(8, 2, n)
+    achieved = bpc * ncell * st64["iters"] / t64 / 1e9
+    res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                       "frac": achieved / 8000.0, "traffic": None,
+                       "kernel": f"fused FP64 BiCGSTAB iteration, all kernels ({bpc:.0f} B per cell per iteration)"}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_bicgstab(blocks, n, rhs_host, ith, args.cpu_steps)
     return res
+
+
+def bicgstab_one(args, mesh, rhs, vals, timed, solve, level, blocks, ncell, n):
+    x = mesh.field(np.float64)
+    for _ in range(args.warmup):
+        timed(solve, x)
+    times, st = [], None
+    for _ in range(args.steps):
+        st, dt = timed(solve, x)
+        times.append(dt)
+        print(f"[bench] {args.precision} solve: {st} {dt:.3f} s", file=sys.stderr, flush=True)
+    elapsed = sum(times)
+    es = 8 if args.precision == "fp64" else 4
+    res = {
+        "metric": f"solves/s (BiCGSTAB to FP64 relative residual 1e-10, {args.precision} engine)",
+        "value": len(times) / elapsed, "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / len(times) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"adaptive shell octree level {level} (|x-c| = 0.25, margin 0.25), "
+                               f"{len(blocks)} x 16^3 blocks, Jacobi(2 sweeps, 0.8)-preconditioned BiCGSTAB",
+                   "blocks": len(blocks), "interior_cells": ncell, "tol": 1e-10},
+        "iters": st["iters"], "refreshes": st["refreshes"], "relres": true_relres(mesh, x, rhs, vals),
+        "hip_graph": not args.no_graph,
+    }
+    if args.precision == "fp64":
+        bpc = engine_bytes_per_cell(es, 2, n)
+        achieved = bpc * ncell * st["iters"] / (elapsed / len(times)) / 1e9
+        res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                           "frac": achieved / 8000.0, "traffic": None,
+                           "kernel": f"fused FP64 BiCGSTAB iteration, all kernels ({bpc:.0f} B per cell per iteration)"}
+    return res
+
+
+def engine_bytes_per_cell(es, pre, n):
+    """Algorithmic bytes per interior cell of one iteration of the fused
+    BiCGSTAB engine with vectors of `es` bytes (FP64: 8) and `pre` >= 1 Jacobi
+    sweeps: every array read once and written once; a stencil input adds its
+    4-column export (4/n of a vector)."""
+    xf = 4.0 / n
+    d = 8.0
+    upd_p = 3 * es + es + es * (1 + xf)  # r, q, p -> p, first sweep + export
+    upd_s = 2 * es + es + es * (1 + xf)  # r, q -> s, first sweep + export
+    sweep = es * (1 + xf) + es + es * (1 + xf)  # input + export, rhs -> output + export
+    spmv = d * (1 + xf) + es + es  # p_ + export, r0 (or s) -> q (or t)
+    xr = 3 * d + 4 * es + d + es  # dx, p_, s_, r, s, t, r0 -> dx, r
+    return upd_p + upd_s + 2 * (pre - 1) * sweep + 2 * spmv + xr
 
 
 def cpu_bicgstab(blocks, n, rhs, gpu_iters, iters):
@@ -522,10 +582,13 @@ def main():
                     help="serving: fp64 for every request, or FP32 (tol 1e-6) / mixed (1e-10)")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--precision", default="auto", choices=["auto", "mixed", "fp64"],
-                    help="batched: precision of the full batch (auto: the faster on the subset)")
+                    help="batched: precision of the full batch (auto: the faster on the subset); "
+                         "bicgstab: auto times mixed, FP64 and the host-driven path, else that engine only")
     ap.add_argument("--level", type=int, default=7, help="bicgstab: shell refinement level")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="bicgstab: launch the iteration kernels one by one instead of replaying a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "

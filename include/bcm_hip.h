@@ -78,8 +78,18 @@ enum { BCM_ORDER_Z = 0, BCM_ORDER_HILBERT = 1 };
 /* Boundary setters: all outer faces Dirichlet id 0
  * (examples/sor2/BoundaryConditionSetter.h), or the sor1 'x'/'y'/'z'
  * setter (examples/sor1/BoundaryConditionSetter.h): outer faces periodic
- * except the two faces normal to the axis, Dirichlet ids 0 and 1. */
-enum { BCM_BCSET_ALL_DIRICHLET = 0, BCM_BCSET_SOR1_X = 1, BCM_BCSET_SOR1_Y = 2, BCM_BCSET_SOR1_Z = 3 };
+ * except the two faces normal to the axis, Dirichlet ids 0 and 1.
+ * BCM_BCSET_PER_FACE: the outer faces on domain face f get type
+ * face_bc_type[f] (BCM_BC_DIRICHLET or BCM_BC_NEUMANN) and boundary-value
+ * id f, as SolverTest's LocalScalar3D takes boundaryType[6] and
+ * boundaryValue[6] per domain face (examples/SolverTest/BlockScalar3D.h:67-105). */
+enum {
+  BCM_BCSET_ALL_DIRICHLET = 0,
+  BCM_BCSET_SOR1_X = 1,
+  BCM_BCSET_SOR1_Y = 2,
+  BCM_BCSET_SOR1_Z = 3,
+  BCM_BCSET_PER_FACE = 4
+};
 
 #define BCM_MAX_SHELLS 8
 typedef struct bcm_tree_params {
@@ -102,6 +112,7 @@ typedef struct bcm_tree_params {
    * the shell touches. */
   int32_t shell_mode;
   double shell_margin;
+  int32_t face_bc_type[6];    /* BCM_BCSET_PER_FACE: BCM_BC_* per domain face X_M..Z_P */
 } bcm_tree_params;
 
 typedef struct bcm_tree* bcm_tree_t;
@@ -180,6 +191,15 @@ int bcm_ghost_fill(bcm_field_t field, int mask);
  * setDirichletBoundaryInBlock (examples/sor1/Solver.cpp:202-294). */
 int bcm_dirichlet_mirror(bcm_field_t field, const double* values, int nvalues);
 
+/* Outer-face conditions of SolverTest's BlockScalar3D::ImposeBoundaryCondition
+ * (examples/SolverTest/BlockScalar3D.h:199-206, bcx.f90:18-376) on every
+ * outer DIRICHLET or NEUMANN face, v = values[bc_id], u the adjacent
+ * interior cell: Dirichlet g1 = 2v - u, g2 = v; Neumann g1 = u -/+ xc,
+ * g2 = u -/+ 2 xc on a minus/plus face, xc = v * cellSize.x
+ * (BlockScalar3D.h:98-100).  g2 (the second ghost layer) only when vc >= 2.
+ * FP64 arithmetic, rounded once for FP32 fields. */
+int bcm_impose_bc(bcm_field_t field, const double* values, int nvalues);
+
 /* ------------------------------------------------------------------ */
 /* Red-black SOR (examples/sor1/Solver.cpp:132-153,                     */
 /* examples/sor2/Solver.cpp:136-160).  Both colours, ghosts frozen.      */
@@ -242,8 +262,11 @@ int bcm_copy(bcm_field_t y, bcm_field_t x);
 
 /* Poisson operator A = lap_h (reference sign, bcut.f90:2848-2865 with no
  * cut cells: Aw..At = 1/h^2, Ap = -(l0+l1)-(l2+l3)-(l4+l5)) with the
- * Dirichlet fold of bca.f90:12-202 on outer faces (Ap -= Anb;
- * b -= 2 Anb v; Anb = 0).  Writes A[7] and b = rhs folded. */
+ * outer-face folds in X_M..Z_P order (BlockScalar3D.h:207-214):
+ * Dirichlet bca.f90:12-202 (Ap -= Anb; b -= Anb v 2; Anb = 0), Neumann
+ * bca.f90:204-394 (Ap += Anb; b += Anb xc on a minus face, b -= Anb xc on
+ * a plus face, xc = v * cellSize.x; Anb = 0).  Writes A[7] (nullable) and
+ * b = rhs folded. */
 int bcm_poisson_assemble(const bcm_field_t A[7], bcm_field_t b, bcm_field_t rhs,
                          const double* bvalues, int nbvalues);
 

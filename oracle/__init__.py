@@ -39,6 +39,8 @@ def load():
         "oracle_ghost_fill_f": [_P, _I, _I, _I, _P, _I],
         "oracle_dirichlet_d": [_P, _I, _I, _I, _P, _P],
         "oracle_dirichlet_f": [_P, _I, _I, _I, _P, _P],
+        "oracle_impose_bc_d": [_P, _I, _I, _I, _P, _P],
+        "oracle_impose_bc_f": [_P, _I, _I, _I, _P, _P],
         "oracle_sor_sweep_d": [_P, _I, _I, _I, _P, _P, _D],
         "oracle_sor_iterate_d": [_P, _I, _I, _I, _P, _P, _D, _P, _I],
         "oracle_jacobi7_d": [_I, _I, _I, _P, _P, _P, _P, _D],
@@ -86,6 +88,13 @@ def ghost_fill(blocks, n, vc, data, mask=7):
     """In-place ghost fill of `data` (float64 or float32)."""
     lib = load()
     getattr(lib, "oracle_ghost_fill_" + _suf(data))(blocks.ctypes.data, len(blocks), n, vc, data.ctypes.data, mask)
+
+
+def impose_bc(blocks, n, vc, data, values):
+    lib = load()
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    getattr(lib, "oracle_impose_bc_" + _suf(data))(blocks.ctypes.data, len(blocks), n, vc, data.ctypes.data,
+                                                   v.ctypes.data)
 
 
 def dirichlet(blocks, n, vc, data, values):

@@ -188,6 +188,52 @@ static int so1(int sf) { return sf >> 1; }
 DEFINE_GHOST(double, d)
 DEFINE_GHOST(float, f)
 
+/* BlockScalar3D::ImposeBoundaryCondition (examples/SolverTest/BlockScalar3D.h:
+ * 199-206, value scaling :98-100) with the per-face routines of
+ * examples/SolverTest/bcx.f90: bc_x1_d/bc_x3_d (:18-76) and the Y/Z forms,
+ * bc_x1_n/bc_x3_n (:198-256) and the Y/Z forms.  For each outer face, with
+ * u the interior cell next to it and (p, q) the two in-face indices:
+ *   Dirichlet: ghost1 = 2*xc - u, ghost2 = xc, xc = v;
+ *   Neumann:   ghost1 = u + s*xc, ghost2 = u + s*(xc*2), xc = v*cellSize.x,
+ *              s = +1 on a plus face, -1 on a minus face.
+ * ghost2 exists only for vc >= 2 (the Fortran writes it unconditionally). */
+#define DEFINE_IMPOSE_BC(T, SUF)                                                                              \
+  void oracle_impose_bc_##SUF(const bcm_block_info* blk, int nb, int n, int vc, T* data, const double* vals) {  \
+    geo_t g = mkgeo(n, vc);                                                                                   \
+    _Pragma("omp parallel for schedule(static)")                                                              \
+    for (int b = 0; b < nb; ++b) {                                                                            \
+      T* f = data + (long)b * g.bvol;                                                                         \
+      double cell_x = blk[b].block_size[0] / n;                                                               \
+      for (int face = 0; face < 6; ++face) {                                                                  \
+        int type = blk[b].bc_type[face];                                                                      \
+        if (type != BCM_BC_DIRICHLET && type != BCM_BC_NEUMANN) continue;                                     \
+        double xc = vals[blk[b].bc_id[face]];                                                                 \
+        if (type == BCM_BC_NEUMANN) xc = xc * cell_x;                                                         \
+        int axis = face / 2, plus = face & 1;                                                                 \
+        for (int q = 0; q < n; ++q)                                                                           \
+          for (int p = 0; p < n; ++p) {                                                                       \
+            int c[3], c1[3], c2[3];                                                                           \
+            int t0 = axis == 0 ? 1 : 0, t1 = axis == 2 ? 1 : 2;                                               \
+            c[t0] = p; c[t1] = q;                                                                             \
+            c[axis] = plus ? n - 1 : 0;                                                                       \
+            memcpy(c1, c, sizeof c); memcpy(c2, c, sizeof c);                                                 \
+            c1[axis] = plus ? n : -1;                                                                         \
+            c2[axis] = plus ? n + 1 : -2;                                                                     \
+            double u = f[IDX(&g, c[0], c[1], c[2])];                                                          \
+            double g1, g2;                                                                                    \
+            if (type == BCM_BC_DIRICHLET) { g1 = 2.0 * xc - u; g2 = xc; }                                     \
+            else if (plus) { g1 = u + xc; g2 = u + xc * 2.0; }                                                \
+            else { g1 = u - xc; g2 = u - xc * 2.0; }                                                          \
+            f[IDX(&g, c1[0], c1[1], c1[2])] = (T)g1;                                                          \
+            if (vc >= 2) f[IDX(&g, c2[0], c2[1], c2[2])] = (T)g2;                                             \
+          }                                                                                                   \
+      }                                                                                                       \
+    }                                                                                                         \
+  }
+
+DEFINE_IMPOSE_BC(double, d)
+DEFINE_IMPOSE_BC(float, f)
+
 /* ------------------------------------------------------------------ */
 /* Red-black SOR                                                        */
 /* ------------------------------------------------------------------ */
@@ -361,9 +407,12 @@ DEFINE_BLAS(double, d)
 DEFINE_BLAS(float, f)
 
 /* Poisson assembly without cut cells (examples/SolverTest/bcut.f90:2848-2865):
- * Aw..At = 1/h^2, Ap = -(l0+l1)-(l2+l3)-(l4+l5); then the Dirichlet fold of
- * bc_A*_d (examples/SolverTest/bca.f90:12-202), faces in X_M..Z_P order:
- * Ap -= Anb; b -= Anb*xc*2; Anb = 0.  A may be NULL (b only). */
+ * Aw..At = 1/h^2, Ap = -(l0+l1)-(l2+l3)-(l4+l5); then the outer-face folds
+ * in X_M..Z_P order (examples/SolverTest/BlockScalar3D.h:207-214):
+ * Dirichlet bc_A*_d (bca.f90:12-202): Ap -= Anb; b -= Anb*xc*2; Anb = 0;
+ * Neumann bc_A*_n (bca.f90:204-394): Ap += Anb; b += Anb*xc on a minus face,
+ * b -= Anb*xc on a plus face, xc = v*cellSize.x; Anb = 0.
+ * A may be NULL (b only). */
 void oracle_poisson_assemble_d(const bcm_block_info* blk, int nb, int n, int vc, double* const A[7], double* bout,
                                const double* rhs, const double* vals) {
   geo_t g = mkgeo(n, vc);
@@ -383,9 +432,16 @@ void oracle_poisson_assemble_d(const bcm_block_info* blk, int nb, int n, int vc,
           double bv = rhs[o];
           int at[6] = {i == 0, i == n - 1, j == 0, j == n - 1, k == 0, k == n - 1};
           for (int f = 0; f < 6; ++f) {
-            if (blk[b].bc_type[f] == BCM_BC_DIRICHLET && at[f]) {
+            if (!at[f]) continue;
+            if (blk[b].bc_type[f] == BCM_BC_DIRICHLET) {
               ap = ap - a[f];
               bv = bv - a[f] * vals[blk[b].bc_id[f]] * 2.0;
+              a[f] = 0.0;
+            } else if (blk[b].bc_type[f] == BCM_BC_NEUMANN) {
+              double xc = vals[blk[b].bc_id[f]] * dx;
+              ap = ap + a[f];
+              if (f & 1) bv = bv - a[f] * xc;
+              else bv = bv + a[f] * xc;
               a[f] = 0.0;
             }
           }

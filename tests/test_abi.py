@@ -38,7 +38,7 @@ def test_python_binding_covers_the_header():
 
 def test_struct_layouts():
     assert L.BLOCK_DTYPE.itemsize == 256
-    assert ctypes.sizeof(L.TreeParams) == 4 * 3 + 4 * 3 + 8 * 3 + 8 + 4 * 4 + 8 * 5 * 8 + 4 * 3 + 4 + 8
+    assert ctypes.sizeof(L.TreeParams) == 4 * 3 + 4 * 3 + 8 * 3 + 8 + 4 * 4 + 8 * 5 * 8 + 4 * 3 + 4 + 8 + 4 * 6
 
 
 def test_oracle_loads():
