@@ -13,10 +13,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbcm_hip.so")
 
 # Status codes (include/bcm_hip.h).
-OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BALANCE, ERR_NAN, ERR_NOCONV, ERR_BUSY = range(9)
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_BALANCE, ERR_NAN, ERR_NOCONV, ERR_BUSY, ERR_IO = range(10)
 STATUS_NAMES = {
     0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_STATE", 5: "ERR_BALANCE",
-    6: "ERR_NAN", 7: "ERR_NOCONV", 8: "ERR_BUSY",
+    6: "ERR_NAN", 7: "ERR_NOCONV", 8: "ERR_BUSY", 9: "ERR_IO",
 }
 
 F64, F32 = 0, 1
@@ -179,6 +179,10 @@ SIGNATURES = {
     "bcm_field_download": (_I, [_P, _P, ctypes.c_size_t]),
     "bcm_field_upload_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),
     "bcm_field_download_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),
+    "bcm_field_dump2": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
+    "bcm_field_load2": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
+    "bcm_field_dump": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
+    "bcm_field_load": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
     "bcm_field_device_ptr": (_P, [_P]),
     "bcm_field_bytes": (ctypes.c_size_t, [_P]),
     "bcm_ghost_fill": (_I, [_P, _I]),

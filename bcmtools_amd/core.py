@@ -106,6 +106,27 @@ class Field:
                 "bcm_field_download_blocks")
         return a
 
+    # -- restart files (examples/SolverTest/LocalScalar3D.cpp:84-357) ------
+    def dump2(self, step, label, rank=0, directory="./BIN"):
+        """LocalScalar3D::Dump2: one file per rank."""
+        L.check(self.mesh.lib.bcm_field_dump2(self.handle, directory.encode(), label.encode(), int(rank), int(step)),
+                "bcm_field_dump2")
+
+    def load2(self, step, label, rank=0, directory="./BIN"):
+        """LocalScalar3D::Load2 (same block edge, or half of it)."""
+        L.check(self.mesh.lib.bcm_field_load2(self.handle, directory.encode(), label.encode(), int(rank), int(step)),
+                "bcm_field_load2")
+
+    def dump(self, step, label, rank=0, directory="./BIN"):
+        """LocalScalar3D::Dump: one file per block."""
+        L.check(self.mesh.lib.bcm_field_dump(self.handle, directory.encode(), label.encode(), int(rank), int(step)),
+                "bcm_field_dump")
+
+    def load(self, step, label, rank=0, directory="./BIN"):
+        """LocalScalar3D::Load: one file per block."""
+        L.check(self.mesh.lib.bcm_field_load(self.handle, directory.encode(), label.encode(), int(rank), int(step)),
+                "bcm_field_load")
+
     def device_ptr(self):
         return self.mesh.lib.bcm_field_device_ptr(self.handle)
 

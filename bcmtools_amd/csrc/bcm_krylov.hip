@@ -201,49 +201,91 @@ struct Compact {
   }
 };
 
-// fr[f][p + N q]: X faces p = j, q = k; Y faces p = i, q = k; Z faces p = i, q = j.
+// Ghost value of face f at in-face position (p, q) of the block whose face
+// descriptors are df = desc + 8 f, computed from src with the exchange's
+// arithmetic; c receives the ghost cell (i, j, k).  X faces p = j, q = k;
+// Y faces p = i, q = k; Z faces p = i, q = j.  Faces without a neighbour
+// (outer boundaries) give 0: their folded coefficient is 0.
+template <typename T, int N>
+__device__ __forceinline__ T frame_value(const Compact<T, N>& src, const int* df, int f, int p, int q, int (&c)[3]) {
+  constexpr int h = N / 2;
+  const int ax = f >> 1;
+  const bool plus = f & 1;
+  const int kind = df[0];
+  c[ax] = plus ? N : -1;
+  c[ax == 0 ? 1 : 0] = p;
+  c[ax == 2 ? 1 : 2] = q;
+  const int t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
+  T val = T(0);
+  if (kind == FD_COPY) {
+    int sc[3] = {c[0], c[1], c[2]};
+    sc[ax] = plus ? 0 : N - 1;
+    val = ax == 0 ? src.x_at(df[1], sc[0], sc[1], sc[2]) : src.at(df[1], sc[0], sc[1], sc[2]);
+  } else if (kind == FD_F2C) {
+    const int s0 = c[t0] >= h, s1 = c[t1] >= h;
+    const int fb = df[1 + s0 + 2 * s1];
+    int I[3];
+    I[ax] = plus ? 0 : h - 1;
+    I[t0] = c[t0] - h * s0;
+    I[t1] = c[t1] - h * s1;
+    if (ax == 0)
+      val = f2c_sum<T>([&](int i, int j, int k) { return src.x_at(fb, i, j, k); }, I[0], I[1], I[2]);
+    else
+      val = f2c_sum<T>([&](int i, int j, int k) { return src.at(fb, i, j, k); }, I[0], I[1], I[2]);
+  } else if (kind == FD_C2F) {
+    const int sf = df[5], cb = df[1];
+    int pp[3];
+    pp[ax] = plus ? 0 : 2 * N - 1;
+    pp[t0] = c[t0] + N * (sf & 1);
+    pp[t1] = c[t1] + N * (sf >> 1);
+    if (ax == 0)
+      val = c2f_interp<T>([&](int i, int j, int k) { return src.x_at(cb, i, j, k); }, N, pp[0], pp[1], pp[2]);
+    else
+      val = c2f_interp<T>([&](int i, int j, int k) { return src.at(cb, i, j, k); }, N, pp[0], pp[1], pp[2]);
+  }
+  return val;
+}
+
+// fr[f][p + N q], the six faces of the ghost frame.
 template <typename T, int N>
 __device__ __forceinline__ void build_frame(const Compact<T, N>& src, const int* d, T (*fr)[N * N]) {
-  constexpr int h = N / 2;
   for (int e = threadIdx.x; e < 6 * N * N; e += KWG) {
-    const int f = e / (N * N), pq = e % (N * N), p = pq % N, q = pq / N;
-    const int ax = f >> 1;
-    const bool plus = f & 1;
-    const int* df = d + f * 8;
-    const int kind = df[0];
+    const int f = e / (N * N), pq = e % (N * N);
     int c[3];
-    c[ax] = plus ? N : -1;
-    c[ax == 0 ? 1 : 0] = p;
-    c[ax == 2 ? 1 : 2] = q;
-    const int t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
-    T val = T(0);
-    if (kind == FD_COPY) {
-      int sc[3] = {c[0], c[1], c[2]};
-      sc[ax] = plus ? 0 : N - 1;
-      val = ax == 0 ? src.x_at(df[1], sc[0], sc[1], sc[2]) : src.at(df[1], sc[0], sc[1], sc[2]);
-    } else if (kind == FD_F2C) {
-      const int s0 = c[t0] >= h, s1 = c[t1] >= h;
-      const int fb = df[1 + s0 + 2 * s1];
-      int I[3];
-      I[ax] = plus ? 0 : h - 1;
-      I[t0] = c[t0] - h * s0;
-      I[t1] = c[t1] - h * s1;
-      if (ax == 0)
-        val = f2c_sum<T>([&](int i, int j, int k) { return src.x_at(fb, i, j, k); }, I[0], I[1], I[2]);
-      else
-        val = f2c_sum<T>([&](int i, int j, int k) { return src.at(fb, i, j, k); }, I[0], I[1], I[2]);
-    } else if (kind == FD_C2F) {
-      const int sf = df[5], cb = df[1];
-      int pp[3];
-      pp[ax] = plus ? 0 : 2 * N - 1;
-      pp[t0] = c[t0] + N * (sf & 1);
-      pp[t1] = c[t1] + N * (sf >> 1);
-      if (ax == 0)
-        val = c2f_interp<T>([&](int i, int j, int k) { return src.x_at(cb, i, j, k); }, N, pp[0], pp[1], pp[2]);
-      else
-        val = c2f_interp<T>([&](int i, int j, int k) { return src.at(cb, i, j, k); }, N, pp[0], pp[1], pp[2]);
-    }
-    fr[f][pq] = val;
+    fr[f][pq] = frame_value<T, N>(src, d + f * 8, f, pq % N, pq / N, c);
+  }
+}
+
+// A block with its face ghosts in LDS, (N+2)^3 values in Index3DS order.
+template <int N>
+struct Gh {
+  static constexpr int NX = N + 2;
+  static constexpr int V = NX * NX * NX;
+  __device__ static __forceinline__ int at(int i, int j, int k) { return (i + 1) + NX * ((j + 1) + NX * (k + 1)); }
+};
+
+// Stage block b of src in LDS: the interior from the compact vector (loads
+// issued first, one coalesced pass), then the face ghosts from the
+// neighbours (descriptors read from global memory, so no barrier sits
+// between the two sets of loads).  The caller synchronises.
+template <typename T, int N>
+__device__ __forceinline__ void stage_block(const Compact<T, N>& src, const int* __restrict__ d, int b, T* lds) {
+  using B = Blk<N>;
+  T v[B::PER];
+  const long long cb = (long long)b * B::N3;
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) v[m] = src.v[cb + threadIdx.x + KWG * m];
+  for (int e = threadIdx.x; e < 6 * N * N; e += KWG) {
+    const int f = e / (N * N), pq = e % (N * N);
+    int c[3];
+    const T val = frame_value<T, N>(src, d + f * 8, f, pq % N, pq / N, c);
+    lds[Gh<N>::at(c[0], c[1], c[2])] = val;
+  }
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) {
+    int i, j, k;
+    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    lds[Gh<N>::at(i, j, k)] = v[m];
   }
 }
 
@@ -400,6 +442,101 @@ __global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, TO* _
     if (col >= 0) xs[(col * N + k) * N + j] = v;
   }
   export_flush<TO, N>(xs, gox, b);
+}
+
+// The same sweep for N <= 16 with the whole ghosted block staged in LDS:
+// the interior and frame loads are all in flight together, and the seven
+// stencil values of every cell come from LDS.  The outputs are then staged
+// in the same LDS space so the column export is written coalesced.
+template <typename T, int N, typename TI, typename TO>
+__global__ __launch_bounds__(KWG) void k_kry_jacobi_l(TO* __restrict__ gout, TO* __restrict__ gox,
+                                                      const TI* __restrict__ gin, const TI* __restrict__ gix,
+                                                      const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
+                                                      const int* __restrict__ blk_seg,
+                                                      const SegState* __restrict__ st, double omega,
+                                                      const int* __restrict__ blist, const int* __restrict__ desc) {
+  using B = Blk<N>;
+  using G = Gh<N>;
+  constexpr int ES = sizeof(TI) > sizeof(TO) ? sizeof(TI) : sizeof(TO);
+  __shared__ __align__(16) char raw[G::V * ES];
+  TI* lin = reinterpret_cast<TI*>(raw);
+  TO* lout = reinterpret_cast<TO*>(raw);
+  const int b = blist[blockIdx.x];
+  if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
+  const long long cb = (long long)b * B::N3;
+  T bv[B::PER];
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) bv[m] = rhs[cb + threadIdx.x + KWG * m];
+  stage_block<TI, N>(Compact<TI, N>{gin, gix}, desc + b * 48, b, lin);
+  __syncthreads();
+  const T om = T(omega);
+  const BlockMeta me = meta[b];
+  TO out[B::PER];
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) {
+    int i, j, k;
+    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    const int a0 = G::at(i, j, k);
+    const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+    out[m] = TO(jacobi_vals(a, T(lin[a0 - 1]), T(lin[a0 + 1]), T(lin[a0 - G::NX]), T(lin[a0 + G::NX]),
+                            T(lin[a0 - G::NX * G::NX]), T(lin[a0 + G::NX * G::NX]), T(lin[a0]), bv[m], om));
+    gout[cb + threadIdx.x + KWG * m] = out[m];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) {
+    int i, j, k;
+    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    if (export_col<N>(i) >= 0) lout[G::at(i, j, k)] = out[m];
+  }
+  __syncthreads();
+  TO* dst = gox + (long long)b * 4 * N * N;
+  for (int e = threadIdx.x; e < 4 * N * N; e += KWG) {
+    const int col = e / (N * N), k = (e / N) % N, j = e % N;
+    const int i = col < 2 ? col : col + N - 4;
+    dst[e] = lout[G::at(i, j, k)];
+  }
+}
+
+// k_kry_spmv for N <= 16 with the ghosted block staged in LDS.
+template <typename T, int N, int NDOT>
+__global__ __launch_bounds__(KWG) void k_kry_spmv_l(T* __restrict__ out, const double* __restrict__ g,
+                                                    const double* __restrict__ gx, const T* __restrict__ y,
+                                                    const BlockMeta* __restrict__ meta,
+                                                    const int* __restrict__ blk_seg, const SegState* __restrict__ st,
+                                                    double* __restrict__ parts, const int* __restrict__ blist,
+                                                    const int* __restrict__ desc) {
+  using B = Blk<N>;
+  using G = Gh<N>;
+  __shared__ __align__(16) double lin[G::V];
+  __shared__ double red[8];
+  const int b = blist[blockIdx.x];
+  if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
+  const long long cb = (long long)b * B::N3;
+  T yv[B::PER];
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) yv[m] = y[cb + threadIdx.x + KWG * m];
+  stage_block<double, N>(Compact<double, N>{g, gx}, desc + b * 48, b, lin);
+  __syncthreads();
+  const BlockMeta me = meta[b];
+  double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+  for (int m = 0; m < B::PER; ++m) {
+    int i, j, k;
+    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    const int a0 = G::at(i, j, k);
+    const Coef7<double> a = poisson_coef<double>(me, i, j, k, N);
+    const T v = T(ax_vals(a, lin[a0 - 1], lin[a0 + 1], lin[a0 - G::NX], lin[a0 + G::NX], lin[a0 - G::NX * G::NX],
+                          lin[a0 + G::NX * G::NX], lin[a0]));
+    out[cb + threadIdx.x + KWG * m] = v;
+    d0 += double(v) * double(yv[m]);
+    if (NDOT == 2) d1 += double(v) * double(v);
+  }
+  wg_sum2(d0, d1, red);
+  if (threadIdx.x == 0) {
+    parts[2 * b] = d0;
+    parts[2 * b + 1] = d1;
+  }
 }
 
 // out = A g (calc_ax) in FP64 from the FP64 preconditioned vector g, stored
@@ -907,12 +1044,21 @@ static int precondition(bcm_mesh* m, T* scratch, T* sx, double* result, double* 
   const int* bl = m->kry->list;
   const SegState* st = m->kry->st;
   for (int sweep = 2; sweep <= pre; ++sweep) {
-    if (in_result)
-      hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
-                         m->d_meta, bs, st, omega, bl, m->d_desc);
-    else
-      hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
-                         m->d_meta, bs, st, omega, bl, m->d_desc);
+    if (in_result) {
+      if constexpr (N <= 16)
+        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
+                           m->d_meta, bs, st, omega, bl, m->d_desc);
+      else
+        hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
+                           m->d_meta, bs, st, omega, bl, m->d_desc);
+    } else {
+      if constexpr (N <= 16)
+        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
+                           m->d_meta, bs, st, omega, bl, m->d_desc);
+      else
+        hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
+                           m->d_meta, bs, st, omega, bl, m->d_desc);
+    }
     BCM_HIP_TRY(hipGetLastError());
     in_result = !in_result;
   }
@@ -967,14 +1113,22 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
   SegState* st = K->st;
   BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, x1, gp, xp, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, x1, gp, xp, p, pre, omega)));
-  hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts, bl,
-                     m->d_desc);
+  if constexpr (N <= 16)
+    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
+                       bl, m->d_desc);
+  else
+    hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
+                       bl, m->d_desc);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_Q>(m));
   BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, x1, gs, xs, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, x1, gs, xs, s, pre, omega)));
-  hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts, bl,
-                     m->d_desc);
+  if constexpr (N <= 16)
+    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,
+                       bl, m->d_desc);
+  else
+    hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,
+                       bl, m->d_desc);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_T>(m));
   hipLaunchKernelGGL((k_kry_xr<T, N>), grid, blk, 0, m->stream, K->dx, gp, gs, r, s, t, r0, bs, st, K->parts, bl);

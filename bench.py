@@ -568,12 +568,99 @@ def bench_copy(args):
             "config": {"workload": "copy of one 3816 x 34^3 FP64 field", "bytes_per_copy": nbytes}}
 
 
+def bench_restart(args):
+    """Restart files of SolverTest (LocalScalar3D::Dump2 / Load2) of an FP64
+    field on the level-7 shell mesh, written from HBM and read back into it."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    level, n, vc = args.level, 16, 1
+    t = W.config3_tree(level)
+    blocks = t.blocks
+    data = W.rhs_field_fast(blocks, n, vc, seed=11)
+    mesh = Mesh(blocks, n, vc)
+    f = mesh.field(np.float64, data)
+    g = mesh.field(np.float64)
+    nbytes = data.nbytes
+    print(f"[bench] restart: {len(blocks)} blocks, field {nbytes / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    d = tempfile.mkdtemp(prefix="bcm_restart_", dir="/tmp")
+    try:
+        for _ in range(args.warmup):
+            f.dump2(0, "p", directory=d)
+            g.load2(0, "p", directory=d)
+        td, tl = [], []
+        for s in range(args.steps):
+            t0 = time.perf_counter()
+            f.dump2(s, "p", directory=d)
+            t1 = time.perf_counter()
+            g.load2(s, "p", directory=d)
+            mesh.sync()
+            t2 = time.perf_counter()
+            td.append(t1 - t0)
+            tl.append(t2 - t1)
+            os.remove(os.path.join(d, f"dump-p-00000-{s:010d}.bin"))
+            print(f"[bench] restart step {s}: dump2 {t1 - t0:.3f} s, load2 {t2 - t1:.3f} s", file=sys.stderr,
+                  flush=True)
+        exact = bool(np.array_equal(g.download(), data))
+        # PCIe reference rates: pinned host <-> device copies of the same size.
+        dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        host.copy_(dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        host.copy_(dev)
+        torch.cuda.synchronize()
+        d2h = nbytes / (time.perf_counter() - t0) / 1e9
+        t0 = time.perf_counter()
+        dev.copy_(host)
+        torch.cuda.synchronize()
+        h2d = nbytes / (time.perf_counter() - t0) / 1e9
+        del dev, host
+        res = {
+            "metric": "restart file throughput (LocalScalar3D Dump2 + Load2, GB/s of field data)",
+            "value": 2 * nbytes * len(td) / (sum(td) + sum(tl)) / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": (sum(td) + sum(tl)) / len(td) * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"FP64 field on the level-{level} shell mesh, {len(blocks)} x 18^3 stored cells, "
+                                   "one Dump2 file per step in /tmp (page cache, no fsync), read back by Load2",
+                       "blocks": len(blocks), "bytes": nbytes},
+            "dump2_GBps": nbytes * len(td) / sum(td) / 1e9,
+            "load2_GBps": nbytes * len(tl) / sum(tl) / 1e9,
+            "pcie_pinned_d2h_GBps": d2h, "pcie_pinned_h2d_GBps": h2d,
+            "round_trip_bit_exact": exact,
+        }
+        if not args.no_cpu:
+            from oracle import restart as R
+            path = os.path.join(d, "cpu.bin")
+            t0 = time.perf_counter()
+            R.dump2(data, n, vc, path)
+            tw = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            R.load2(np.empty_like(data), n, vc, path)
+            tr = time.perf_counter() - t0
+            os.remove(path)
+            res["cpu_baseline"] = {"value": 2 * nbytes / (tw + tr) / 1e9, "unit": "GB/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": "the same field written and read from host memory by the restatement "
+                                             f"(write {nbytes / tw / 1e9:.2f} GB/s, read {nbytes / tr / 1e9:.2f} GB/s)"}
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="sor", choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy"])
+    ap.add_argument("--workload", default="sor",
+                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart"])
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=16, help="serving: requests on the device at once")
@@ -595,7 +682,7 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
-           "serving": bench_serving, "copy": bench_copy}[args.workload](args)
+           "serving": bench_serving, "copy": bench_copy, "restart": bench_restart}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

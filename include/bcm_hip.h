@@ -41,7 +41,8 @@ enum {
   BCM_ERR_BALANCE = 5,  /* 2:1 constraint broken (Utils/src/BCMOctree.cpp:462-467) */
   BCM_ERR_NAN = 6,      /* NaN/Inf residual (examples/SolverTest/ILS3D.h:53-59) */
   BCM_ERR_NOCONV = 7,   /* iteration limit reached before the tolerance */
-  BCM_ERR_BUSY = 8      /* admission refused: HBM budget or slot table full */
+  BCM_ERR_BUSY = 8,     /* admission refused: HBM budget or slot table full */
+  BCM_ERR_IO = 9        /* a file could not be opened, read or written */
 };
 
 enum { BCM_F64 = 0, BCM_F32 = 1 };
@@ -164,6 +165,28 @@ int bcm_field_download(bcm_field_t field, void* host, size_t bytes);
  * segment of a batch, a block range of one mesh); bytes must match. */
 int bcm_field_upload_blocks(bcm_field_t field, int first_block, int nblocks, const void* host, size_t bytes);
 int bcm_field_download_blocks(bcm_field_t field, int first_block, int nblocks, void* host, size_t bytes);
+
+/* Restart files of SolverTest's LocalScalar3D<real>
+ * (examples/SolverTest/LocalScalar3D.cpp:84-357), written from and read
+ * into device memory through pinned staging buffers.  The file element
+ * is the field's type (the reference's `real`).
+ *   Dump2 / Load2 (:226-324): <dir>/dump-<label>-<rank %05d>-<step %010d>.bin,
+ *     int32 {nx, ny, nz, vc, 1, nblocks}, then every block's (n+2vc)^3 array.
+ *   Dump / Load (:84-222): one file per block,
+ *     <dir>/dump-<label>-<rank %05d>-<id %05d>-<step %010d>.bin,
+ *     int32 {nx, ny, nz, vc, 1}, then the block's array.
+ * The reference writes to ./BIN/ (dir = "./BIN"), creating it.  It imposes
+ * the field's boundary conditions before a dump and after a load
+ * (LocalScalar3D::ImposeBoundaryCondition); here the caller does that with
+ * bcm_ghost_fill and bcm_impose_bc.  A load accepts files of the field's
+ * block edge, or of half of it: each fine interior cell then takes its
+ * coarse cell (sf3d_copy_x2, examples/SolverTest/bsf3d.f90:18-46) and the
+ * ghosts keep their values.  Any other header returns BCM_ERR_ARG, where
+ * the reference calls Exit(0); a missing or short file BCM_ERR_IO. */
+int bcm_field_dump2(bcm_field_t field, const char* dir, const char* label, int rank, int step);
+int bcm_field_load2(bcm_field_t field, const char* dir, const char* label, int rank, int step);
+int bcm_field_dump(bcm_field_t field, const char* dir, const char* label, int rank, int step);
+int bcm_field_load(bcm_field_t field, const char* dir, const char* label, int rank, int step);
 void* bcm_field_device_ptr(bcm_field_t field);
 size_t bcm_field_bytes(bcm_field_t field);
 
