@@ -183,6 +183,7 @@ SIGNATURES = {
     "bcm_field_load2": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
     "bcm_field_dump": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
     "bcm_field_load": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
+    "bcm_field_write_vtk": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I, _I, _I, _DP, _D]),
     "bcm_field_device_ptr": (_P, [_P]),
     "bcm_field_bytes": (ctypes.c_size_t, [_P]),
     "bcm_ghost_fill": (_I, [_P, _I]),
@@ -205,6 +206,20 @@ SIGNATURES = {
     "bcm_axpbypz": (_I, [_P, _P, _P, _D, _D]),
     "bcm_fill": (_I, [_P, _D]),
     "bcm_copy": (_I, [_P, _P]),
+    "bcm_jacobi7_mask": (_I, [_P, _P, ctypes.POINTER(_P), _P, _P, _D]),
+    "bcm_calc_ax_mask": (_I, [_P, ctypes.POINTER(_P), _P, _P]),
+    "bcm_calc_r_mask": (_I, [_P, ctypes.POINTER(_P), _P, _P, _P]),
+    "bcm_calc_r2_mask": (_I, [ctypes.POINTER(_P), _P, _P, _P, _DP]),
+    "bcm_dot_mask": (_I, [_P, _P, _P, _DP]),
+    "bcm_axpy_mask": (_I, [_P, _P, _D, _P]),
+    "bcm_xpay_mask": (_I, [_P, _P, _D, _P]),
+    "bcm_axpyz_mask": (_I, [_P, _P, _P, _D, _P]),
+    "bcm_axpbypz_mask": (_I, [_P, _P, _P, _D, _D, _P]),
+    "bcm_copy_mask": (_I, [_P, _P, _P]),
+    "bcm_jacobi_solve_mask": (_I, [_P, ctypes.POINTER(_P), _P, _P, _P, ctypes.POINTER(KrylovParams), _DP, _I, _IP,
+                                   _DP]),
+    "bcm_bicgstab_mask": (_I, [_P, ctypes.POINTER(_P), _P, ctypes.POINTER(_P), _P, ctypes.POINTER(KrylovParams), _DP,
+                               _I, _IP, _DP]),
     "bcm_poisson_assemble": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _I]),
     "bcm_jacobi7_poisson": (_I, [_P, _P, _P, _D]),
     "bcm_calc_ax_poisson": (_I, [_P, _P]),

@@ -127,6 +127,14 @@ class Field:
         L.check(self.mesh.lib.bcm_field_load(self.handle, directory.encode(), label.encode(), int(rank), int(step)),
                 "bcm_field_load")
 
+    def write_vtk(self, step, name, difflevel, root_origin=(0.0, 0.0, 0.0), root_length=1.0, rank=0, start_id=0,
+                  directory="./VTK"):
+        """VtkWriter::writeScalar: one .vti per block and a .vthb index."""
+        o = np.ascontiguousarray(root_origin, dtype=np.float64)
+        L.check(self.mesh.lib.bcm_field_write_vtk(self.handle, directory.encode(), name.encode(), int(step),
+                                                  int(rank), int(start_id), int(difflevel), L.dptr(o),
+                                                  float(root_length)), "bcm_field_write_vtk")
+
     def device_ptr(self):
         return self.mesh.lib.bcm_field_device_ptr(self.handle)
 
@@ -279,7 +287,13 @@ class Mesh:
         return r.value
 
     # -- SolverTest kernels ---------------------------------------------------
-    def jacobi7(self, x1, x0, A, b, omega):
+    # mask: a field of the operands' type, 0/1; only cells where it is 1 are
+    # visited (the *_mask forms of blas.f90:753-1113).
+    def jacobi7(self, x1, x0, A, b, omega, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_jacobi7_mask(x1.handle, x0.handle, _handles(A), b.handle, mask.handle, float(omega)),
+                    "bcm_jacobi7_mask")
+            return
         L.check(self.lib.bcm_jacobi7(x1.handle, x0.handle, _handles(A), b.handle, float(omega)), "bcm_jacobi7")
 
     def jacobi7_poisson(self, x1, x0, b, omega):
@@ -291,33 +305,64 @@ class Mesh:
     def calc_r_poisson(self, r, x, b):
         L.check(self.lib.bcm_calc_r_poisson(r.handle, x.handle, b.handle), "bcm_calc_r_poisson")
 
-    def calc_ax(self, ax, A, x):
+    def calc_ax(self, ax, A, x, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_calc_ax_mask(ax.handle, _handles(A), x.handle, mask.handle), "bcm_calc_ax_mask")
+            return
         L.check(self.lib.bcm_calc_ax(ax.handle, _handles(A), x.handle), "bcm_calc_ax")
 
-    def calc_r(self, r, A, x, b):
+    def calc_r(self, r, A, x, b, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_calc_r_mask(r.handle, _handles(A), x.handle, b.handle, mask.handle),
+                    "bcm_calc_r_mask")
+            return
         L.check(self.lib.bcm_calc_r(r.handle, _handles(A), x.handle, b.handle), "bcm_calc_r")
 
-    def calc_r2(self, A, x, b):
+    def calc_r2(self, A, x, b, mask=None):
         v = ctypes.c_double()
+        if mask is not None:
+            L.check(self.lib.bcm_calc_r2_mask(_handles(A), x.handle, b.handle, mask.handle, ctypes.byref(v)),
+                    "bcm_calc_r2_mask")
+            return v.value
         L.check(self.lib.bcm_calc_r2(_handles(A), x.handle, b.handle, ctypes.byref(v)), "bcm_calc_r2")
         return v.value
 
-    def dot(self, x, y):
+    def dot(self, x, y, mask=None):
         v = ctypes.c_double()
+        if mask is not None:
+            L.check(self.lib.bcm_dot_mask(x.handle, y.handle, mask.handle, ctypes.byref(v)), "bcm_dot_mask")
+            return v.value
         L.check(self.lib.bcm_dot(x.handle, y.handle, ctypes.byref(v)), "bcm_dot")
         return v.value
 
-    def axpy(self, y, x, a):
+    def axpy(self, y, x, a, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_axpy_mask(y.handle, x.handle, float(a), mask.handle), "bcm_axpy_mask")
+            return
         L.check(self.lib.bcm_axpy(y.handle, x.handle, float(a)), "bcm_axpy")
 
-    def xpay(self, y, x, a):
+    def xpay(self, y, x, a, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_xpay_mask(y.handle, x.handle, float(a), mask.handle), "bcm_xpay_mask")
+            return
         L.check(self.lib.bcm_xpay(y.handle, x.handle, float(a)), "bcm_xpay")
 
-    def axpyz(self, z, x, y, a):
+    def axpyz(self, z, x, y, a, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_axpyz_mask(z.handle, x.handle, y.handle, float(a), mask.handle), "bcm_axpyz_mask")
+            return
         L.check(self.lib.bcm_axpyz(z.handle, x.handle, y.handle, float(a)), "bcm_axpyz")
 
-    def axpbypz(self, z, x, y, a, b):
+    def axpbypz(self, z, x, y, a, b, mask=None):
+        if mask is not None:
+            L.check(self.lib.bcm_axpbypz_mask(z.handle, x.handle, y.handle, float(a), float(b), mask.handle),
+                    "bcm_axpbypz_mask")
+            return
         L.check(self.lib.bcm_axpbypz(z.handle, x.handle, y.handle, float(a), float(b)), "bcm_axpbypz")
+
+    def copy_mask(self, y, x, mask):
+        """copy_mask (blas.f90:969-992): interior cells where mask is 1."""
+        L.check(self.lib.bcm_copy_mask(y.handle, x.handle, mask.handle), "bcm_copy_mask")
 
     def fill(self, x, a):
         L.check(self.lib.bcm_fill(x.handle, float(a)), "bcm_fill")
@@ -331,25 +376,37 @@ class Mesh:
                                               L.dptr(v), len(v)), "bcm_poisson_assemble")
 
     # -- solvers ----------------------------------------------------------------
-    def jacobi_solve(self, x, A, b, x0, omega=1.0, count_max=100, epsilon=1e-8, bvalues=()):
+    def jacobi_solve(self, x, A, b, x0, omega=1.0, count_max=100, epsilon=1e-8, bvalues=(), mask=None):
+        """ILS3D::Jacobi; with a mask ILS3D::Jacobi_Mask."""
         p = L.KrylovParams(float(omega), 0, int(count_max), float(epsilon))
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
         cnt = ctypes.c_int()
         res = ctypes.c_double()
-        rc = self.lib.bcm_jacobi_solve(x.handle, _handles(A), b.handle, x0.handle, ctypes.byref(p),
-                                       L.dptr(v) if len(v) else None, len(v), ctypes.byref(cnt), ctypes.byref(res))
+        vp = L.dptr(v) if len(v) else None
+        if mask is not None:
+            rc = self.lib.bcm_jacobi_solve_mask(x.handle, _handles(A), b.handle, x0.handle, mask.handle,
+                                                ctypes.byref(p), vp, len(v), ctypes.byref(cnt), ctypes.byref(res))
+        else:
+            rc = self.lib.bcm_jacobi_solve(x.handle, _handles(A), b.handle, x0.handle, ctypes.byref(p), vp, len(v),
+                                           ctypes.byref(cnt), ctypes.byref(res))
         if rc not in (L.OK, L.ERR_NOCONV):
             L.check(rc, "bcm_jacobi_solve")
         return rc, cnt.value, res.value
 
-    def bicgstab(self, x, A, b, work, omega=0.8, count_pre=4, count_max=100, epsilon=1e-8, bvalues=()):
-        """ILS3D::BiCGSTAB; `work` = 9 fields {r, r0, p, p_, q_, s, s_, t_, x0}."""
+    def bicgstab(self, x, A, b, work, omega=0.8, count_pre=4, count_max=100, epsilon=1e-8, bvalues=(), mask=None):
+        """ILS3D::BiCGSTAB (with a mask ILS3D::BiCGSTAB_Mask); `work` = 9 fields
+        {r, r0, p, p_, q_, s, s_, t_, x0}."""
         p = L.KrylovParams(float(omega), int(count_pre), int(count_max), float(epsilon))
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
         cnt = ctypes.c_int()
         res = ctypes.c_double()
-        rc = self.lib.bcm_bicgstab(x.handle, _handles(A), b.handle, _handles(work), ctypes.byref(p),
-                                   L.dptr(v) if len(v) else None, len(v), ctypes.byref(cnt), ctypes.byref(res))
+        vp = L.dptr(v) if len(v) else None
+        if mask is not None:
+            rc = self.lib.bcm_bicgstab_mask(x.handle, _handles(A), b.handle, _handles(work), mask.handle,
+                                            ctypes.byref(p), vp, len(v), ctypes.byref(cnt), ctypes.byref(res))
+        else:
+            rc = self.lib.bcm_bicgstab(x.handle, _handles(A), b.handle, _handles(work), ctypes.byref(p), vp, len(v),
+                                       ctypes.byref(cnt), ctypes.byref(res))
         if rc not in (L.OK, L.ERR_NOCONV):
             L.check(rc, "bcm_bicgstab")
         return rc, cnt.value, res.value

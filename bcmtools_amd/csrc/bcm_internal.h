@@ -158,8 +158,9 @@ template <typename T> int ghost_fill_raw(bcm_mesh* m, T* data, int mask);
 template <typename T> int jacobi_poisson_raw(bcm_mesh* m, T* x1, const T* x0, const T* b, double omega);
 template <typename T> int residual_poisson_raw(bcm_mesh* m, T* r, const T* x, const T* b, bool want_norm, int slot);
 template <typename T> int spmv_poisson_raw(bcm_mesh* m, T* ax, const T* x);
-template <typename T> int dot_raw(bcm_mesh* m, const T* x, const T* y, int slot);
-template <typename T> int blas_raw(bcm_mesh* m, int op, T* z, const T* x, const T* y, double a, double b);
+template <typename T> int dot_raw(bcm_mesh* m, const T* x, const T* y, int slot, const T* mask = nullptr);
+template <typename T> int blas_raw(bcm_mesh* m, int op, T* z, const T* x, const T* y, double a, double b,
+                                   const T* mask = nullptr);
 template <typename T> int fill_raw(bcm_mesh* m, T* x, double a);
 int assemble_poisson_b_raw(bcm_mesh* m, double* b, const double* rhs);
 int residual_d2f_raw(bcm_mesh* m, float* r32, const double* x, const double* b, int slot);

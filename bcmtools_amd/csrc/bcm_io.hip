@@ -19,6 +19,7 @@
 #include <cerrno>
 #include <cstdio>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -160,6 +161,63 @@ int load_blocks(bcm_field* f, int first, int nblk, int nfile, FILE* fp, Stage& s
   return rc;
 }
 
+// VtkWriter::printVTIC: the interior of each block as FP32, x fastest,
+// n^3 values per block in block order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_vtk_pack(const T* __restrict__ field, float* __restrict__ out, int first,
+                                                  int nblk, int n, int vc) {
+  const long long per = (long long)n * n * n;
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= per * nblk) return;
+  const int b = int(c / per), r = int(c % per);
+  const int i = r % n, j = (r / n) % n, k = r / (n * n);
+  const int nx = n + 2 * vc;
+  const long long o = (long long)(first + b) * nx * nx * nx + (i + vc) + (long long)nx * ((j + vc) + (long long)nx * (k + vc));
+  out[c] = float(field[o]);
+}
+
+// The XML of one block's .vti, up to the appended-data marker, as
+// VtkWriter::printVTIC writes it (examples/SolverTest/VtkWriter.h:118-168).
+std::string vti_head(int n, double ox, double oy, double oz, double dx, const char* label) {
+  std::ostringstream o;
+  o << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">" << std::endl;
+  o << "<ImageData WholeExtent=\"0 " << n << " 0 " << n << " 0 " << n << "\" ";
+  o << "Origin=\"";
+  o.setf(std::ios::scientific, std::ios::floatfield);
+  o.precision(16);
+  o << ox << " " << oy << " " << oz << "\" ";
+  o << "Spacing=\"" << dx << " " << dx << " " << dx << "\">" << std::endl;
+  o << "<Piece Extent=\"0 " << n << " 0 " << n << " 0 " << n << "\">" << std::endl;
+  o << "<PointData>" << std::endl << "</PointData>" << std::endl << "<CellData>" << std::endl;
+  o << "<DataArray type=\"Float32\" Name=\"" << label << "\" format=\"appended\" offset=\"" << 0 << "\"/>"
+    << std::endl;
+  o << "</CellData>" << std::endl << "<Coordinates>" << std::endl << "</Coordinates>" << std::endl;
+  o << "</Piece>" << std::endl << "</ImageData>" << std::endl << "<AppendedData encoding=\"raw\">" << std::endl;
+  o << "_";
+  return o.str();
+}
+
+std::string step_dir(const char* dir, int step) {
+  char buf[16];
+  std::snprintf(buf, sizeof buf, "%010d", step);
+  return std::string(dir) + "/" + buf;
+}
+
+int write_vti(const std::string& path, const std::string& head, const float* data, int n) {
+  File out;
+  out.fp = std::fopen(path.c_str(), "wb");
+  if (!out.fp) return BCM_ERR_IO;
+  const int32_t nbytes = int32_t(sizeof(float)) * n * n * n;
+  static const char tail[] = "\n</AppendedData>\n</VTKFile>\n";
+  if (std::fwrite(head.data(), 1, head.size(), out.fp) != head.size()) return BCM_ERR_IO;
+  if (std::fwrite(&nbytes, sizeof nbytes, 1, out.fp) != 1) return BCM_ERR_IO;
+  if (std::fwrite(data, 1, size_t(nbytes), out.fp) != size_t(nbytes)) return BCM_ERR_IO;
+  if (std::fwrite(tail, 1, sizeof tail - 1, out.fp) != sizeof tail - 1) return BCM_ERR_IO;
+  const int rc = std::fclose(out.fp);
+  out.fp = nullptr;
+  return rc == 0 ? BCM_OK : BCM_ERR_IO;
+}
+
 int check_args(bcm_field* f, const char* dir, const char* label) {
   if (!f || !dir || !label) return BCM_ERR_ARG;
   return BCM_OK;
@@ -248,4 +306,101 @@ extern "C" int bcm_field_load(bcm_field_t f, const char* dir, const char* label,
     BCM_TRY(load_blocks(f, id, 1, h[0], in.fp, st));
   }
   return BCM_OK;
+}
+
+extern "C" int bcm_field_write_vtk(bcm_field_t f, const char* dir, const char* name, int step, int rank,
+                                   int start_id, int difflevel, const double root_origin[3], double root_length) {
+  BCM_TRY(check_args(f, dir, name));
+  if (!root_origin || !(root_length > 0.0) || difflevel < 0 || start_id < 0) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  const int n = m->n;
+  if (::mkdir(dir, 0755) != 0 && errno != EEXIST) return BCM_ERR_IO;
+  const std::string sdir = step_dir(dir, step);
+  if (::mkdir(sdir.c_str(), 0755) != 0 && errno != EEXIST) return BCM_ERR_IO;
+  Stage st(m->stream);
+  BCM_TRY(st.init());
+  const size_t per = size_t(n) * n * n;
+  const int group = int(std::max<size_t>(1, kStage / (per * sizeof(float))));
+  void* tmp = nullptr;
+  BCM_TRY(dev_alloc(&tmp, size_t(std::min(group, m->nblocks)) * per * sizeof(float)));
+  int rc = BCM_OK;
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  char fname[256];
+  // Group g is packed and copied down while group g-1 is written out.
+  const int ngroups = (m->nblocks + group - 1) / group;
+  for (int g = 0; g <= ngroups && rc == BCM_OK; ++g) {
+    if (g < ngroups) {
+      const int b0 = g * group, nb = std::min(group, m->nblocks - b0);
+      // The staging buffer of group g-2 was written out before this point.
+      const long long cells = (long long)nb * (long long)per;
+      const dim3 grid(unsigned((cells + 255) / 256));
+      if (f->dtype == BCM_F64)
+        hipLaunchKernelGGL(k_vtk_pack<double>, grid, dim3(256), 0, m->stream, static_cast<const double*>(f->data),
+                           static_cast<float*>(tmp), b0, nb, n, m->vc);
+      else
+        hipLaunchKernelGGL(k_vtk_pack<float>, grid, dim3(256), 0, m->stream, static_cast<const float*>(f->data),
+                           static_cast<float*>(tmp), b0, nb, n, m->vc);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(st.buf[g & 1], tmp, size_t(nb) * per * sizeof(float), hipMemcpyDeviceToHost, m->stream) !=
+              hipSuccess ||
+          hipEventRecord(st.ev[g & 1], m->stream) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+    }
+    if (g > 0) {
+      const int p = g - 1, b0 = p * group, nb = std::min(group, m->nblocks - b0);
+      if (hipEventSynchronize(st.ev[p & 1]) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+      const float* data = reinterpret_cast<const float*>(st.buf[p & 1]);
+      for (int b = 0; b < nb && rc == BCM_OK; ++b) {
+        const bcm_block_info& bi = m->blocks[size_t(b0 + b)];
+        const double dx = bi.block_size[0] / n;
+        std::snprintf(fname, sizeof fname, "/data-%s-%05d-%05d-%010d.vti", name, rank, b0 + b, step);
+        rc = write_vti(sdir + fname, vti_head(n, bi.origin[0], bi.origin[1], bi.origin[2], dx, name),
+                       data + size_t(b) * per, n);
+      }
+    }
+    // The next pack reuses tmp: its copy into the staging buffer must be done.
+    if (rc == BCM_OK && g < ngroups && hipEventSynchronize(st.ev[g & 1]) != hipSuccess) rc = BCM_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(m->stream);
+  dev_free(tmp);
+  BCM_TRY(rc);
+  // The hierarchical-box index (VtkWriter::writeScalar, VtkWriter.h:771-889):
+  // one DataSet per block with its level and its box in cells of that level.
+  if (rank != 0) return BCM_OK;
+  File out;
+  std::ostringstream o;
+  o << "<VTKFile type=\"vtkHierarchicalBoxDataSet\" version=\"0.1\">" << std::endl;
+  o << "<vtkHierarchicalBoxDataSet>" << std::endl;
+  for (int l = 0; l < difflevel + 1; ++l) o << "<RefinementRatio level=\"" << l << "\" refinement=\"2\"/>" << std::endl;
+  o << std::endl;
+  for (int id = 0; id < m->nblocks; ++id) {
+    const bcm_block_info& bi = m->blocks[size_t(id)];
+    const int level = bi.level;
+    const double lx = double(n) * double(1 << level);
+    double box[3][2];
+    for (int a = 0; a < 3; ++a) {
+      const double o2 = (bi.origin[a] - root_origin[a]) / root_length;  // origin in root lengths
+      box[a][0] = o2 * lx;
+      box[a][1] = box[a][0] + n - 1;
+    }
+    std::snprintf(fname, sizeof fname, "./%010d/data-%s-%05d-%05d-%010d.vti", step, name, rank, id, step);
+    o << "<DataSet group=\"" << level << "\" dataset=\"" << id + start_id << "\" amr_box=\"" << box[0][0] << " "
+      << box[0][1] << " " << box[1][0] << " " << box[1][1] << " " << box[2][0] << " " << box[2][1] << "\" file=\""
+      << fname << "\"/>" << std::endl;
+  }
+  o << "</vtkHierarchicalBoxDataSet>" << std::endl << "</VTKFile>" << std::endl;
+  const std::string text = o.str();
+  char iname[256];
+  std::snprintf(iname, sizeof iname, "/data-%s-%010d.vthb", name, step);
+  out.fp = std::fopen((std::string(dir) + iname).c_str(), "wb");
+  if (!out.fp) return BCM_ERR_IO;
+  if (std::fwrite(text.data(), 1, text.size(), out.fp) != text.size()) return BCM_ERR_IO;
+  const int frc = std::fclose(out.fp);
+  out.fp = nullptr;
+  return frc == 0 ? BCM_OK : BCM_ERR_IO;
 }

@@ -108,6 +108,29 @@ __device__ __forceinline__ void wg_sum2(double& a, double& b, double* red) {
   }
 }
 
+// The same for a work-group of NW waves (wave totals summed in wave order).
+template <int NW>
+__device__ __forceinline__ void wg_sum2n(double& a, double& b, double* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[NW + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = red[0];
+    b = red[NW];
+    for (int i = 1; i < NW; ++i) {
+      a += red[i];
+      b += red[NW + i];
+    }
+  }
+}
+
 // Gate of a block-wise kernel: 0 every block, 1 blocks of active
 // segments, 2 blocks of active segments with a reliable update pending.
 template <int GATE>
@@ -268,23 +291,24 @@ struct Gh {
 // issued first, one coalesced pass), then the face ghosts from the
 // neighbours (descriptors read from global memory, so no barrier sits
 // between the two sets of loads).  The caller synchronises.
-template <typename T, int N>
+template <typename T, int N, int W>
 __device__ __forceinline__ void stage_block(const Compact<T, N>& src, const int* __restrict__ d, int b, T* lds) {
   using B = Blk<N>;
-  T v[B::PER];
+  constexpr int PER = B::N3 / W;
+  T v[PER];
   const long long cb = (long long)b * B::N3;
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) v[m] = src.v[cb + threadIdx.x + KWG * m];
-  for (int e = threadIdx.x; e < 6 * N * N; e += KWG) {
+  for (int m = 0; m < PER; ++m) v[m] = src.v[cb + threadIdx.x + W * m];
+  for (int e = threadIdx.x; e < 6 * N * N; e += W) {
     const int f = e / (N * N), pq = e % (N * N);
     int c[3];
     const T val = frame_value<T, N>(src, d + f * 8, f, pq % N, pq / N, c);
     lds[Gh<N>::at(c[0], c[1], c[2])] = val;
   }
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) {
+  for (int m = 0; m < PER; ++m) {
     int i, j, k;
-    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    B::ijk(threadIdx.x + W * m, i, j, k);
     lds[Gh<N>::at(i, j, k)] = v[m];
   }
 }
@@ -444,19 +468,23 @@ __global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, TO* _
   export_flush<TO, N>(xs, gox, b);
 }
 
-// The same sweep for N <= 16 with the whole ghosted block staged in LDS:
-// the interior and frame loads are all in flight together, and the seven
-// stencil values of every cell come from LDS.  The outputs are then staged
-// in the same LDS space so the column export is written coalesced.
-template <typename T, int N, typename TI, typename TO>
-__global__ __launch_bounds__(KWG) void k_kry_jacobi_l(TO* __restrict__ gout, TO* __restrict__ gox,
-                                                      const TI* __restrict__ gin, const TI* __restrict__ gix,
-                                                      const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
-                                                      const int* __restrict__ blk_seg,
-                                                      const SegState* __restrict__ st, double omega,
-                                                      const int* __restrict__ blist, const int* __restrict__ desc) {
+// The same sweep for N <= 16 with the whole ghosted block staged in LDS,
+// W threads per block: the interior and frame loads are all in flight
+// together, and the seven stencil values of every cell come from LDS.  The
+// outputs are then staged in the same LDS space so the column export is
+// written coalesced.
+constexpr int KWG_L = 512;
+
+template <typename T, int N, typename TI, typename TO, int W>
+__global__ __launch_bounds__(W) void k_kry_jacobi_l(TO* __restrict__ gout, TO* __restrict__ gox,
+                                                    const TI* __restrict__ gin, const TI* __restrict__ gix,
+                                                    const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
+                                                    const int* __restrict__ blk_seg,
+                                                    const SegState* __restrict__ st, double omega,
+                                                    const int* __restrict__ blist, const int* __restrict__ desc) {
   using B = Blk<N>;
   using G = Gh<N>;
+  constexpr int PER = B::N3 / W;
   constexpr int ES = sizeof(TI) > sizeof(TO) ? sizeof(TI) : sizeof(TO);
   __shared__ __align__(16) char raw[G::V * ES];
   TI* lin = reinterpret_cast<TI*>(raw);
@@ -464,34 +492,34 @@ __global__ __launch_bounds__(KWG) void k_kry_jacobi_l(TO* __restrict__ gout, TO*
   const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
   const long long cb = (long long)b * B::N3;
-  T bv[B::PER];
+  T bv[PER];
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) bv[m] = rhs[cb + threadIdx.x + KWG * m];
-  stage_block<TI, N>(Compact<TI, N>{gin, gix}, desc + b * 48, b, lin);
+  for (int m = 0; m < PER; ++m) bv[m] = rhs[cb + threadIdx.x + W * m];
+  stage_block<TI, N, W>(Compact<TI, N>{gin, gix}, desc + b * 48, b, lin);
   __syncthreads();
   const T om = T(omega);
   const BlockMeta me = meta[b];
-  TO out[B::PER];
+  TO out[PER];
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) {
+  for (int m = 0; m < PER; ++m) {
     int i, j, k;
-    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    B::ijk(threadIdx.x + W * m, i, j, k);
     const int a0 = G::at(i, j, k);
     const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
     out[m] = TO(jacobi_vals(a, T(lin[a0 - 1]), T(lin[a0 + 1]), T(lin[a0 - G::NX]), T(lin[a0 + G::NX]),
                             T(lin[a0 - G::NX * G::NX]), T(lin[a0 + G::NX * G::NX]), T(lin[a0]), bv[m], om));
-    gout[cb + threadIdx.x + KWG * m] = out[m];
+    gout[cb + threadIdx.x + W * m] = out[m];
   }
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) {
+  for (int m = 0; m < PER; ++m) {
     int i, j, k;
-    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    B::ijk(threadIdx.x + W * m, i, j, k);
     if (export_col<N>(i) >= 0) lout[G::at(i, j, k)] = out[m];
   }
   __syncthreads();
   TO* dst = gox + (long long)b * 4 * N * N;
-  for (int e = threadIdx.x; e < 4 * N * N; e += KWG) {
+  for (int e = threadIdx.x; e < 4 * N * N; e += W) {
     const int col = e / (N * N), k = (e / N) % N, j = e % N;
     const int i = col < 2 ? col : col + N - 4;
     dst[e] = lout[G::at(i, j, k)];
@@ -499,40 +527,41 @@ __global__ __launch_bounds__(KWG) void k_kry_jacobi_l(TO* __restrict__ gout, TO*
 }
 
 // k_kry_spmv for N <= 16 with the ghosted block staged in LDS.
-template <typename T, int N, int NDOT>
-__global__ __launch_bounds__(KWG) void k_kry_spmv_l(T* __restrict__ out, const double* __restrict__ g,
-                                                    const double* __restrict__ gx, const T* __restrict__ y,
-                                                    const BlockMeta* __restrict__ meta,
-                                                    const int* __restrict__ blk_seg, const SegState* __restrict__ st,
-                                                    double* __restrict__ parts, const int* __restrict__ blist,
-                                                    const int* __restrict__ desc) {
+template <typename T, int N, int NDOT, int W>
+__global__ __launch_bounds__(W) void k_kry_spmv_l(T* __restrict__ out, const double* __restrict__ g,
+                                                  const double* __restrict__ gx, const T* __restrict__ y,
+                                                  const BlockMeta* __restrict__ meta,
+                                                  const int* __restrict__ blk_seg, const SegState* __restrict__ st,
+                                                  double* __restrict__ parts, const int* __restrict__ blist,
+                                                  const int* __restrict__ desc) {
   using B = Blk<N>;
   using G = Gh<N>;
+  constexpr int PER = B::N3 / W;
   __shared__ __align__(16) double lin[G::V];
-  __shared__ double red[8];
+  __shared__ double red[2 * (W / 64)];
   const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
   const long long cb = (long long)b * B::N3;
-  T yv[B::PER];
+  T yv[PER];
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) yv[m] = y[cb + threadIdx.x + KWG * m];
-  stage_block<double, N>(Compact<double, N>{g, gx}, desc + b * 48, b, lin);
+  for (int m = 0; m < PER; ++m) yv[m] = y[cb + threadIdx.x + W * m];
+  stage_block<double, N, W>(Compact<double, N>{g, gx}, desc + b * 48, b, lin);
   __syncthreads();
   const BlockMeta me = meta[b];
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
-  for (int m = 0; m < B::PER; ++m) {
+  for (int m = 0; m < PER; ++m) {
     int i, j, k;
-    B::ijk(threadIdx.x + KWG * m, i, j, k);
+    B::ijk(threadIdx.x + W * m, i, j, k);
     const int a0 = G::at(i, j, k);
     const Coef7<double> a = poisson_coef<double>(me, i, j, k, N);
     const T v = T(ax_vals(a, lin[a0 - 1], lin[a0 + 1], lin[a0 - G::NX], lin[a0 + G::NX], lin[a0 - G::NX * G::NX],
                           lin[a0 + G::NX * G::NX], lin[a0]));
-    out[cb + threadIdx.x + KWG * m] = v;
+    out[cb + threadIdx.x + W * m] = v;
     d0 += double(v) * double(yv[m]);
     if (NDOT == 2) d1 += double(v) * double(v);
   }
-  wg_sum2(d0, d1, red);
+  wg_sum2n<W / 64>(d0, d1, red);
   if (threadIdx.x == 0) {
     parts[2 * b] = d0;
     parts[2 * b + 1] = d1;
@@ -1046,14 +1075,14 @@ static int precondition(bcm_mesh* m, T* scratch, T* sx, double* result, double* 
   for (int sweep = 2; sweep <= pre; ++sweep) {
     if (in_result) {
       if constexpr (N <= 16)
-        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
+        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, double, T, KWG_L>), grid, dim3(KWG_L), 0, m->stream, scratch, sx, result, rx, rhs,
                            m->d_meta, bs, st, omega, bl, m->d_desc);
       else
         hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
                            m->d_meta, bs, st, omega, bl, m->d_desc);
     } else {
       if constexpr (N <= 16)
-        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
+        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, T, double, KWG_L>), grid, dim3(KWG_L), 0, m->stream, result, rx, scratch, sx, rhs,
                            m->d_meta, bs, st, omega, bl, m->d_desc);
       else
         hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
@@ -1114,7 +1143,7 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
   BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, x1, gp, xp, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, x1, gp, xp, p, pre, omega)));
   if constexpr (N <= 16)
-    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
+    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 1, KWG_L>), grid, dim3(KWG_L), 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
                        bl, m->d_desc);
   else
     hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
@@ -1124,7 +1153,7 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
   BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, x1, gs, xs, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, x1, gs, xs, s, pre, omega)));
   if constexpr (N <= 16)
-    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,
+    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 2, KWG_L>), grid, dim3(KWG_L), 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,
                        bl, m->d_desc);
   else
     hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,

@@ -60,6 +60,7 @@ struct StoredOp {
   bcm_mesh* m;
   const bcm_field_t* A;
   int dtype;
+  bcm_field_t mask = nullptr;  // ILS3D's *_Mask forms: Jacobi sweeps and A x on mask == 1 cells
   // The C ABI works on field handles; wrap raw pointers in temporaries.
   bcm_field tmp(void* p) const {
     bcm_field f;
@@ -72,7 +73,7 @@ struct StoredOp {
   }
   template <typename T> int ax(T* out, const T* x) const {
     bcm_field o = tmp(out), xi = tmp(const_cast<T*>(x));
-    return bcm_calc_ax(&o, A, &xi);
+    return mask ? bcm_calc_ax_mask(&o, A, &xi, mask) : bcm_calc_ax(&o, A, &xi);
   }
   template <typename T> int r(T* out, const T* x, const T* b) const {
     bcm_field o = tmp(out), xi = tmp(const_cast<T*>(x)), bi = tmp(const_cast<T*>(b));
@@ -80,8 +81,9 @@ struct StoredOp {
   }
   template <typename T> int jacobi(T* x1, const T* x0, const T* b, double w) const {
     bcm_field o = tmp(x1), xi = tmp(const_cast<T*>(x0)), bi = tmp(const_cast<T*>(b));
-    return bcm_jacobi7(&o, &xi, A, &bi, w);
+    return mask ? bcm_jacobi7_mask(&o, &xi, A, &bi, mask, w) : bcm_jacobi7(&o, &xi, A, &bi, w);
   }
+  template <typename T> const T* mask_data() const { return mask ? static_cast<const T*>(mask->data) : nullptr; }
 };
 
 template <typename T>
@@ -108,8 +110,8 @@ struct Bc {
 };
 
 template <typename T>
-int dot_host(bcm_mesh* m, const T* x, const T* y, double* out) {
-  BCM_TRY(dot_raw<T>(m, x, y, 0));
+int dot_host(bcm_mesh* m, const T* x, const T* y, double* out, const T* mask = nullptr) {
+  BCM_TRY(dot_raw<T>(m, x, y, 0, mask));
   return fetch_result(m, 1, out);
 }
 
@@ -126,13 +128,16 @@ int jacobi_pre(const Op& op, const Bc<T>& bc, T*& x, T*& x0, const T* b, double 
 }
 
 // ILS3D::BiCGSTAB (ILS3D.h:663-844).  w = {r, r0, p, p_, q_, s, s_, t_, x0}.
-// Pointers in w may be permuted by the preconditioner's swaps.
+// Pointers in w may be permuted by the preconditioner's swaps.  With a mask
+// (the operator's), ILS3D::BiCGSTAB_Mask (ILS3D.h:901-1097): the dot
+// products, the preconditioner sweeps and A p_, A s_ visit only mask == 1
+// cells; the initial residual and the vector updates are unmasked.
 template <typename T, typename Op>
 int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[9], double omega, int count_pre,
-             int count_max, double eps, int* count_out, double* residual) {
+             int count_max, double eps, int* count_out, double* residual, const T* mask = nullptr) {
   T *r = w[0], *r0 = w[1], *p = w[2], *p_ = w[3], *q_ = w[4], *s = w[5], *s_ = w[6], *t_ = w[7], *x0 = w[8];
   double bb = 0.0;
-  BCM_TRY(dot_host<T>(m, b, b, &bb));
+  BCM_TRY(dot_host<T>(m, b, b, &bb, mask));
   if (std::fabs(bb) < FLT_MIN) {
     *residual = 0.0;
     *count_out = 0;
@@ -145,7 +150,7 @@ int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[
   int status = BCM_ERR_NOCONV;
   for (count = 1; count <= count_max; ++count) {
     double rr1 = 0.0;
-    BCM_TRY(dot_host<T>(m, r, r0, &rr1));
+    BCM_TRY(dot_host<T>(m, r, r0, &rr1, mask));
     if (std::fabs(rr1) < FLT_MIN) {
       *residual = rr1;
       count = 0;
@@ -164,7 +169,7 @@ int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[
     BCM_TRY(jacobi_pre<T>(op, bc, p_, x0, p, omega, count_pre));
     BCM_TRY(op.template ax<T>(q_, p_));
     double q_r0 = 0.0;
-    BCM_TRY(dot_host<T>(m, q_, r0, &q_r0));
+    BCM_TRY(dot_host<T>(m, q_, r0, &q_r0, mask));
     alpha = rr1 / q_r0;
     BCM_TRY(blas_raw<T>(m, OP_AXPYZ, s, q_, r, -alpha, 0.0));
     BCM_TRY(bc.impose(s));
@@ -172,14 +177,14 @@ int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[
     BCM_TRY(jacobi_pre<T>(op, bc, s_, x0, s, omega, count_pre));
     BCM_TRY(op.template ax<T>(t_, s_));
     double ts[2];
-    BCM_TRY(dot_raw<T>(m, t_, s, 0));
-    BCM_TRY(dot_raw<T>(m, t_, t_, 1));
+    BCM_TRY(dot_raw<T>(m, t_, s, 0, mask));
+    BCM_TRY(dot_raw<T>(m, t_, t_, 1, mask));
     BCM_TRY(fetch_result(m, 2, ts));
     gamma = ts[0] / ts[1];
     BCM_TRY(blas_raw<T>(m, OP_AXPBYPZ, x, p_, s_, alpha, gamma));
     BCM_TRY(blas_raw<T>(m, OP_AXPYZ, r, t_, s, -gamma, 0.0));
     double rr = 0.0;
-    BCM_TRY(dot_host<T>(m, r, r, &rr));
+    BCM_TRY(dot_host<T>(m, r, r, &rr, mask));
     bool done = false;
     BCM_TRY(is_converged(rr, bb, eps, residual, &done));
     if (done) {
@@ -203,23 +208,26 @@ void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc) { return ws_get(m,
 
 using namespace bcm;
 
-extern "C" int bcm_jacobi_solve(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_field_t x0,
-                                const bcm_krylov_params* p, const double* bvalues, int nbvalues, int* count,
-                                double* residual) {
+// ILS3D::Jacobi (ILS3D.h:478-529); with a mask ILS3D::Jacobi_Mask
+// (ILS3D.h:846-899), where only the sweep is masked.
+static int jacobi_solve_impl(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_field_t x0,
+                             const bcm_krylov_params* p, const double* bvalues, int nbvalues, int* count,
+                             double* residual, bcm_field_t mask) {
   if (!x || !b || !x0 || !A || !p || !count || !residual) return BCM_ERR_ARG;
   if (x->mesh != b->mesh || x->mesh != x0->mesh || x->dtype != b->dtype || x->dtype != x0->dtype) return BCM_ERR_ARG;
   for (int i = 0; i < 7; ++i)
     if (!A[i] || A[i]->mesh != x->mesh || A[i]->dtype != x->dtype) return BCM_ERR_ARG;
+  if (mask && (mask->mesh != x->mesh || mask->dtype != x->dtype)) return BCM_ERR_ARG;
   bcm_mesh* m = x->mesh;
   const bool mirror = nbvalues > 0;
   if (mirror) BCM_TRY(set_bvalues(m, bvalues, nbvalues));
-  // ILS3D::Jacobi (ILS3D.h:478-529).
   double bb = 0.0;
   BCM_TRY(bcm_dot(b, b, &bb));
   int status = BCM_ERR_NOCONV;
   int c = 1;
   for (c = 1; c <= p->count_max; ++c) {
-    BCM_TRY(bcm_jacobi7(x0, x, A, b, p->omega));
+    if (mask) BCM_TRY(bcm_jacobi7_mask(x0, x, A, b, mask, p->omega));
+    else BCM_TRY(bcm_jacobi7(x0, x, A, b, p->omega));
     std::swap(x->data, x0->data);
     BCM_TRY(bcm_ghost_fill(x, BCM_GF_ALL));
     if (mirror) BCM_TRY(dirichlet_device(m, x));
@@ -239,9 +247,22 @@ extern "C" int bcm_jacobi_solve(bcm_field_t x, const bcm_field_t A[7], bcm_field
   return status;
 }
 
-extern "C" int bcm_bicgstab(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
-                            const bcm_krylov_params* p, const double* bvalues, int nbvalues, int* count,
-                            double* residual) {
+extern "C" int bcm_jacobi_solve(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_field_t x0,
+                                const bcm_krylov_params* p, const double* bvalues, int nbvalues, int* count,
+                                double* residual) {
+  return jacobi_solve_impl(x, A, b, x0, p, bvalues, nbvalues, count, residual, nullptr);
+}
+
+extern "C" int bcm_jacobi_solve_mask(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_field_t x0,
+                                     bcm_field_t mask, const bcm_krylov_params* p, const double* bvalues,
+                                     int nbvalues, int* count, double* residual) {
+  if (!mask) return BCM_ERR_ARG;
+  return jacobi_solve_impl(x, A, b, x0, p, bvalues, nbvalues, count, residual, mask);
+}
+
+static int bicgstab_impl(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
+                         const bcm_krylov_params* p, const double* bvalues, int nbvalues, int* count,
+                         double* residual, bcm_field_t mask) {
   if (!x || !b || !A || !work || !p || !count || !residual) return BCM_ERR_ARG;
   bcm_mesh* m = x->mesh;
   for (int i = 0; i < 7; ++i)
@@ -249,26 +270,40 @@ extern "C" int bcm_bicgstab(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b
   for (int i = 0; i < 9; ++i)
     if (!work[i] || work[i]->mesh != m || work[i]->dtype != x->dtype) return BCM_ERR_ARG;
   if (b->mesh != m || b->dtype != x->dtype) return BCM_ERR_ARG;
+  if (mask && (mask->mesh != m || mask->dtype != x->dtype)) return BCM_ERR_ARG;
   const bool mirror = nbvalues > 0;
   if (mirror) BCM_TRY(set_bvalues(m, bvalues, nbvalues));
-  StoredOp op{m, A, x->dtype};
+  StoredOp op{m, A, x->dtype, mask};
   int rc;
   if (x->dtype == BCM_F64) {
     double* w[9];
     for (int i = 0; i < 9; ++i) w[i] = static_cast<double*>(work[i]->data);
     rc = bicgstab<double>(m, op, Bc<double>{m, mirror}, static_cast<double*>(x->data),
                           static_cast<const double*>(b->data), w, p->omega, p->count_pre, p->count_max, p->epsilon,
-                          count, residual);
+                          count, residual, op.mask_data<double>());
     for (int i = 0; i < 9; ++i) work[i]->data = w[i];
   } else {
     float* w[9];
     for (int i = 0; i < 9; ++i) w[i] = static_cast<float*>(work[i]->data);
     rc = bicgstab<float>(m, op, Bc<float>{m, mirror}, static_cast<float*>(x->data),
                          static_cast<const float*>(b->data), w, p->omega, p->count_pre, p->count_max, p->epsilon,
-                         count, residual);
+                         count, residual, op.mask_data<float>());
     for (int i = 0; i < 9; ++i) work[i]->data = w[i];
   }
   return rc;
+}
+
+extern "C" int bcm_bicgstab(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
+                            const bcm_krylov_params* p, const double* bvalues, int nbvalues, int* count,
+                            double* residual) {
+  return bicgstab_impl(x, A, b, work, p, bvalues, nbvalues, count, residual, nullptr);
+}
+
+extern "C" int bcm_bicgstab_mask(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
+                                 bcm_field_t mask, const bcm_krylov_params* p, const double* bvalues, int nbvalues,
+                                 int* count, double* residual) {
+  if (!mask) return BCM_ERR_ARG;
+  return bicgstab_impl(x, A, b, work, p, bvalues, nbvalues, count, residual, mask);
 }
 
 // Workspace slots on the mesh.

@@ -304,7 +304,7 @@ __device__ __forceinline__ Coef7<T> stored_coef(const StoredA<T>& A, long long o
 template <typename T, bool STORED>
 __global__ __launch_bounds__(kWG) void k_jacobi(T* __restrict__ x1, const T* __restrict__ x0, StoredA<T> A,
                                                 const BlockMeta* __restrict__ meta, const T* __restrict__ b,
-                                                double omega, CellGeo g) {
+                                                double omega, CellGeo g, const T* __restrict__ mask) {
   const int sy = g.nx0, sz = g.nx0 * g.nx0;
   const T om = T(omega);
 #pragma unroll
@@ -313,6 +313,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi(T* __restrict__ x1, const T* __r
     if (c >= g.ncell) return;
     int bk, i, j, k;
     const long long o = cell_offset(g, c, bk, i, j, k);
+    if (mask && mask[o] != T(1)) continue;
     const Coef7<T> a = STORED ? stored_coef(A, o) : poisson_coef<T>(meta[bk], i, j, k, g.n);
     x1[o] = jacobi_cell(a, x0, o, sy, sz, b[o], om);
   }
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi(T* __restrict__ x1, const T* __r
 template <typename T, bool STORED, int MODE>
 __global__ __launch_bounds__(kWG) void k_apply(T* __restrict__ out, const T* __restrict__ x, StoredA<T> A,
                                                const BlockMeta* __restrict__ meta, const T* __restrict__ b,
-                                               CellGeo g, double* __restrict__ parts) {
+                                               CellGeo g, double* __restrict__ parts, const T* __restrict__ mask) {
   __shared__ double red[4];
   const int sy = g.nx0, sz = g.nx0 * g.nx0;
   double acc = 0.0;
@@ -332,6 +333,7 @@ __global__ __launch_bounds__(kWG) void k_apply(T* __restrict__ out, const T* __r
     if (c >= g.ncell) break;
     int bk, i, j, k;
     const long long o = cell_offset(g, c, bk, i, j, k);
+    if (mask && mask[o] != T(1)) continue;
     const Coef7<T> a = STORED ? stored_coef(A, o) : poisson_coef<T>(meta[bk], i, j, k, g.n);
     if (MODE == 0) {
       out[o] = ax_cell(a, x, o, sy, sz);
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(kWG) void k_residual_d2f(float* __restrict__ r32, c
 // global sums in FP64 by a fixed tree.
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_dot(const T* __restrict__ x, const T* __restrict__ y, CellGeo g,
-                                             double* __restrict__ parts) {
+                                             double* __restrict__ parts, const T* __restrict__ mask) {
   __shared__ double red[4];
   T acc = T(0);
 #pragma unroll
@@ -384,6 +386,7 @@ __global__ __launch_bounds__(kWG) void k_dot(const T* __restrict__ x, const T* _
     if (c >= g.ncell) break;
     int bk, i, j, k;
     const long long o = cell_offset(g, c, bk, i, j, k);
+    if (mask && mask[o] != T(1)) continue;
     acc = acc + x[o] * y[o];
   }
   const double s = wg_sum(double(acc), red);
@@ -393,7 +396,7 @@ __global__ __launch_bounds__(kWG) void k_dot(const T* __restrict__ x, const T* _
 // blas.f90:242-377 (interior cells only).
 template <typename T, int OP>
 __global__ __launch_bounds__(kWG) void k_blas(T* __restrict__ z, const T* __restrict__ x, const T* __restrict__ y,
-                                              double a, double bcoef, CellGeo g) {
+                                              double a, double bcoef, CellGeo g, const T* __restrict__ mask) {
   const T ta = T(a), tb = T(bcoef);
 #pragma unroll
   for (int q = 0; q < kCPT; ++q) {
@@ -401,6 +404,7 @@ __global__ __launch_bounds__(kWG) void k_blas(T* __restrict__ z, const T* __rest
     if (c >= g.ncell) return;
     int bk, i, j, k;
     const long long o = cell_offset(g, c, bk, i, j, k);
+    if (mask && mask[o] != T(1)) continue;
     if (OP == OP_AXPY) z[o] = z[o] + ta * x[o];                       // y = y + a x
     else if (OP == OP_XPAY) z[o] = x[o] + ta * z[o];                  // y = x + a y
     else if (OP == OP_AXPYZ) z[o] = ta * x[o] + y[o];                 // z = a x + y
@@ -464,7 +468,7 @@ int jacobi_poisson_raw(bcm_mesh* m, T* x1, const T* x0, const T* b, double omega
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_JACOBI, &ev));
   hipLaunchKernelGGL((k_jacobi<T, false>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream, x1, x0, StoredA<T>{},
-                     m->d_meta, b, omega, cell_geo(m));
+                     m->d_meta, b, omega, cell_geo(m), (const T*)nullptr);
   BCM_HIP_TRY(hipGetLastError());
   return timer_end(m, &ev);
 }
@@ -476,7 +480,7 @@ int spmv_poisson_raw(bcm_mesh* m, T* ax, const T* x) {
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_SPMV, &ev));
   hipLaunchKernelGGL((k_apply<T, false, 0>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream, ax, x, StoredA<T>{},
-                     m->d_meta, (const T*)nullptr, cell_geo(m), m->d_partials);
+                     m->d_meta, (const T*)nullptr, cell_geo(m), m->d_partials, (const T*)nullptr);
   BCM_HIP_TRY(hipGetLastError());
   return timer_end(m, &ev);
 }
@@ -490,10 +494,10 @@ int residual_poisson_raw(bcm_mesh* m, T* r, const T* x, const T* b, bool want_no
   const int grid = cell_grid(m);
   if (want_norm)
     hipLaunchKernelGGL((k_apply<T, false, 2>), dim3(grid), dim3(kWG), 0, m->stream, r, x, StoredA<T>{}, m->d_meta, b,
-                       cell_geo(m), m->d_partials);
+                       cell_geo(m), m->d_partials, (const T*)nullptr);
   else
     hipLaunchKernelGGL((k_apply<T, false, 1>), dim3(grid), dim3(kWG), 0, m->stream, r, x, StoredA<T>{}, m->d_meta, b,
-                       cell_geo(m), m->d_partials);
+                       cell_geo(m), m->d_partials, (const T*)nullptr);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(timer_end(m, &ev));
   if (want_norm) BCM_TRY(reduce_partials(m, grid, slot));
@@ -514,36 +518,36 @@ int residual_d2f_raw(bcm_mesh* m, float* r32, const double* x, const double* b, 
 }
 
 template <typename T>
-int dot_raw(bcm_mesh* m, const T* x, const T* y, int slot) {
+int dot_raw(bcm_mesh* m, const T* x, const T* y, int slot, const T* mask) {
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_DOT, &ev));
   const int grid = cell_grid(m);
-  hipLaunchKernelGGL(k_dot<T>, dim3(grid), dim3(kWG), 0, m->stream, x, y, cell_geo(m), m->d_partials);
+  hipLaunchKernelGGL(k_dot<T>, dim3(grid), dim3(kWG), 0, m->stream, x, y, cell_geo(m), m->d_partials, mask);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(timer_end(m, &ev));
   return reduce_partials(m, grid, slot);
 }
-template int dot_raw<double>(bcm_mesh*, const double*, const double*, int);
-template int dot_raw<float>(bcm_mesh*, const float*, const float*, int);
+template int dot_raw<double>(bcm_mesh*, const double*, const double*, int, const double*);
+template int dot_raw<float>(bcm_mesh*, const float*, const float*, int, const float*);
 
 template <typename T>
-int blas_raw(bcm_mesh* m, int op, T* z, const T* x, const T* y, double a, double b) {
+int blas_raw(bcm_mesh* m, int op, T* z, const T* x, const T* y, double a, double b, const T* mask) {
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_BLAS1, &ev));
   const dim3 grid(cell_grid(m)), blk(kWG);
   const CellGeo g = cell_geo(m);
   switch (op) {
-    case OP_AXPY: hipLaunchKernelGGL((k_blas<T, OP_AXPY>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
-    case OP_XPAY: hipLaunchKernelGGL((k_blas<T, OP_XPAY>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
-    case OP_AXPYZ: hipLaunchKernelGGL((k_blas<T, OP_AXPYZ>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
-    case OP_AXPBYPZ: hipLaunchKernelGGL((k_blas<T, OP_AXPBYPZ>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
-    default: hipLaunchKernelGGL((k_blas<T, OP_COPY_INTERIOR>), grid, blk, 0, m->stream, z, x, y, a, b, g); break;
+    case OP_AXPY: hipLaunchKernelGGL((k_blas<T, OP_AXPY>), grid, blk, 0, m->stream, z, x, y, a, b, g, mask); break;
+    case OP_XPAY: hipLaunchKernelGGL((k_blas<T, OP_XPAY>), grid, blk, 0, m->stream, z, x, y, a, b, g, mask); break;
+    case OP_AXPYZ: hipLaunchKernelGGL((k_blas<T, OP_AXPYZ>), grid, blk, 0, m->stream, z, x, y, a, b, g, mask); break;
+    case OP_AXPBYPZ: hipLaunchKernelGGL((k_blas<T, OP_AXPBYPZ>), grid, blk, 0, m->stream, z, x, y, a, b, g, mask); break;
+    default: hipLaunchKernelGGL((k_blas<T, OP_COPY_INTERIOR>), grid, blk, 0, m->stream, z, x, y, a, b, g, mask); break;
   }
   BCM_HIP_TRY(hipGetLastError());
   return timer_end(m, &ev);
 }
-template int blas_raw<double>(bcm_mesh*, int, double*, const double*, const double*, double, double);
-template int blas_raw<float>(bcm_mesh*, int, float*, const float*, const float*, double, double);
+template int blas_raw<double>(bcm_mesh*, int, double*, const double*, const double*, double, double, const double*);
+template int blas_raw<float>(bcm_mesh*, int, float*, const float*, const float*, double, double, const float*);
 
 template <typename T>
 int fill_raw(bcm_mesh* m, T* x, double a) {
@@ -771,71 +775,125 @@ extern "C" int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relre
   return BCM_OK;
 }
 
-extern "C" int bcm_jacobi7(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, double omega) {
-  if (!same_mesh(x1, x0) || !same_mesh(x1, b) || !check_A(x1, A) || x1 == x0) return BCM_ERR_ARG;
+// A mask field (nullable) has the operands' mesh and type; a cell is
+// visited when its value is 1, as blas.f90's `mask(ijk) == 1`.
+static bool mask_ok(bcm_field_t x, bcm_field_t mask) { return !mask || same_mesh(x, mask); }
+
+template <typename T>
+static const T* mask_ptr(bcm_field_t mask) {
+  return mask ? static_cast<const T*>(mask->data) : nullptr;
+}
+
+static int jacobi7_impl(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, double omega,
+                        bcm_field_t mask) {
+  if (!same_mesh(x1, x0) || !same_mesh(x1, b) || !check_A(x1, A) || x1 == x0 || !mask_ok(x1, mask))
+    return BCM_ERR_ARG;
   bcm_mesh* m = x1->mesh;
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_JACOBI, &ev));
   if (x1->dtype == BCM_F64)
     hipLaunchKernelGGL((k_jacobi<double, true>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream,
                        static_cast<double*>(x1->data), static_cast<const double*>(x0->data), stored<double>(A),
-                       m->d_meta, static_cast<const double*>(b->data), omega, cell_geo(m));
+                       m->d_meta, static_cast<const double*>(b->data), omega, cell_geo(m), mask_ptr<double>(mask));
   else
     hipLaunchKernelGGL((k_jacobi<float, true>), dim3(cell_grid(m)), dim3(kWG), 0, m->stream,
                        static_cast<float*>(x1->data), static_cast<const float*>(x0->data), stored<float>(A),
-                       m->d_meta, static_cast<const float*>(b->data), omega, cell_geo(m));
+                       m->d_meta, static_cast<const float*>(b->data), omega, cell_geo(m), mask_ptr<float>(mask));
   BCM_HIP_TRY(hipGetLastError());
   return timer_end(m, &ev);
 }
 
+extern "C" int bcm_jacobi7(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, double omega) {
+  return jacobi7_impl(x1, x0, A, b, omega, nullptr);
+}
+
+extern "C" int bcm_jacobi7_mask(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b,
+                                bcm_field_t mask, double omega) {
+  if (!mask) return BCM_ERR_ARG;
+  return jacobi7_impl(x1, x0, A, b, omega, mask);
+}
+
 template <typename T, int MODE>
-static int apply_stored(bcm_field_t out, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b) {
+static int apply_stored(bcm_field_t out, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, bcm_field_t mask) {
   bcm_mesh* m = x->mesh;
   EventPair ev;
   BCM_TRY(timer_begin(m, MODE == 0 ? BCM_K_SPMV : BCM_K_RESID, &ev));
   const int grid = cell_grid(m);
   hipLaunchKernelGGL((k_apply<T, true, MODE>), dim3(grid), dim3(kWG), 0, m->stream,
                      out ? static_cast<T*>(out->data) : nullptr, static_cast<const T*>(x->data), stored<T>(A),
-                     m->d_meta, b ? static_cast<const T*>(b->data) : nullptr, cell_geo(m), m->d_partials);
+                     m->d_meta, b ? static_cast<const T*>(b->data) : nullptr, cell_geo(m), m->d_partials,
+                     mask_ptr<T>(mask));
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(timer_end(m, &ev));
   if (MODE == 2) BCM_TRY(reduce_partials(m, grid, 0));
   return BCM_OK;
 }
 
-extern "C" int bcm_calc_ax(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x) {
-  if (!same_mesh(ax, x) || !check_A(x, A) || ax == x) return BCM_ERR_ARG;
-  return x->dtype == BCM_F64 ? apply_stored<double, 0>(ax, A, x, nullptr) : apply_stored<float, 0>(ax, A, x, nullptr);
+static int calc_ax_impl(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x, bcm_field_t mask) {
+  if (!same_mesh(ax, x) || !check_A(x, A) || ax == x || !mask_ok(x, mask)) return BCM_ERR_ARG;
+  return x->dtype == BCM_F64 ? apply_stored<double, 0>(ax, A, x, nullptr, mask)
+                             : apply_stored<float, 0>(ax, A, x, nullptr, mask);
 }
 
-extern "C" int bcm_calc_r(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b) {
-  if (!same_mesh(r, x) || !same_mesh(x, b) || !check_A(x, A) || r == x) return BCM_ERR_ARG;
-  return x->dtype == BCM_F64 ? apply_stored<double, 1>(r, A, x, b) : apply_stored<float, 1>(r, A, x, b);
+static int calc_r_impl(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, bcm_field_t mask) {
+  if (!same_mesh(r, x) || !same_mesh(x, b) || !check_A(x, A) || r == x || !mask_ok(x, mask)) return BCM_ERR_ARG;
+  return x->dtype == BCM_F64 ? apply_stored<double, 1>(r, A, x, b, mask) : apply_stored<float, 1>(r, A, x, b, mask);
 }
 
-extern "C" int bcm_calc_r2(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, double* rr) {
-  if (!same_mesh(x, b) || !check_A(x, A) || !rr) return BCM_ERR_ARG;
-  const int rc = x->dtype == BCM_F64 ? apply_stored<double, 2>(nullptr, A, x, b) : apply_stored<float, 2>(nullptr, A, x, b);
+static int calc_r2_impl(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, double* rr, bcm_field_t mask) {
+  if (!same_mesh(x, b) || !check_A(x, A) || !rr || !mask_ok(x, mask)) return BCM_ERR_ARG;
+  const int rc = x->dtype == BCM_F64 ? apply_stored<double, 2>(nullptr, A, x, b, mask)
+                                     : apply_stored<float, 2>(nullptr, A, x, b, mask);
   if (rc != BCM_OK) return rc;
   return fetch_result(x->mesh, 1, rr);
 }
 
-extern "C" int bcm_dot(bcm_field_t x, bcm_field_t y, double* xy) {
-  if (!same_mesh(x, y) || !xy) return BCM_ERR_ARG;
+static int dot_impl(bcm_field_t x, bcm_field_t y, double* xy, bcm_field_t mask) {
+  if (!same_mesh(x, y) || !xy || !mask_ok(x, mask)) return BCM_ERR_ARG;
   bcm_mesh* m = x->mesh;
-  if (x->dtype == BCM_F64) BCM_TRY(dot_raw<double>(m, static_cast<const double*>(x->data), static_cast<const double*>(y->data), 0));
-  else BCM_TRY(dot_raw<float>(m, static_cast<const float*>(x->data), static_cast<const float*>(y->data), 0));
+  if (x->dtype == BCM_F64)
+    BCM_TRY(dot_raw<double>(m, static_cast<const double*>(x->data), static_cast<const double*>(y->data), 0,
+                            mask_ptr<double>(mask)));
+  else
+    BCM_TRY(dot_raw<float>(m, static_cast<const float*>(x->data), static_cast<const float*>(y->data), 0,
+                           mask_ptr<float>(mask)));
   return fetch_result(m, 1, xy);
 }
 
-static int blas_c(int op, bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b) {
-  if (!same_mesh(z, x) || (y && !same_mesh(z, y))) return BCM_ERR_ARG;
+extern "C" int bcm_calc_ax(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x) {
+  return calc_ax_impl(ax, A, x, nullptr);
+}
+extern "C" int bcm_calc_r(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b) {
+  return calc_r_impl(r, A, x, b, nullptr);
+}
+extern "C" int bcm_calc_r2(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, double* rr) {
+  return calc_r2_impl(A, x, b, rr, nullptr);
+}
+extern "C" int bcm_dot(bcm_field_t x, bcm_field_t y, double* xy) { return dot_impl(x, y, xy, nullptr); }
+
+extern "C" int bcm_calc_ax_mask(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x, bcm_field_t mask) {
+  return mask ? calc_ax_impl(ax, A, x, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_calc_r_mask(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b,
+                               bcm_field_t mask) {
+  return mask ? calc_r_impl(r, A, x, b, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_calc_r2_mask(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, bcm_field_t mask, double* rr) {
+  return mask ? calc_r2_impl(A, x, b, rr, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_dot_mask(bcm_field_t x, bcm_field_t y, bcm_field_t mask, double* xy) {
+  return mask ? dot_impl(x, y, xy, mask) : BCM_ERR_ARG;
+}
+
+static int blas_c(int op, bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b,
+                  bcm_field_t mask = nullptr) {
+  if (!same_mesh(z, x) || (y && !same_mesh(z, y)) || !mask_ok(z, mask)) return BCM_ERR_ARG;
   bcm_mesh* m = z->mesh;
   if (z->dtype == BCM_F64)
     return blas_raw<double>(m, op, static_cast<double*>(z->data), static_cast<const double*>(x->data),
-                            y ? static_cast<const double*>(y->data) : nullptr, a, b);
+                            y ? static_cast<const double*>(y->data) : nullptr, a, b, mask_ptr<double>(mask));
   return blas_raw<float>(m, op, static_cast<float*>(z->data), static_cast<const float*>(x->data),
-                         y ? static_cast<const float*>(y->data) : nullptr, a, b);
+                         y ? static_cast<const float*>(y->data) : nullptr, a, b, mask_ptr<float>(mask));
 }
 
 extern "C" int bcm_axpy(bcm_field_t y, bcm_field_t x, double a) { return blas_c(OP_AXPY, y, x, nullptr, a, 0.0); }
@@ -847,6 +905,21 @@ extern "C" int bcm_axpyz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a) 
 extern "C" int bcm_axpbypz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b) {
   if (!y) return BCM_ERR_ARG;
   return blas_c(OP_AXPBYPZ, z, x, y, a, b);
+}
+extern "C" int bcm_axpy_mask(bcm_field_t y, bcm_field_t x, double a, bcm_field_t mask) {
+  return mask ? blas_c(OP_AXPY, y, x, nullptr, a, 0.0, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_xpay_mask(bcm_field_t y, bcm_field_t x, double a, bcm_field_t mask) {
+  return mask ? blas_c(OP_XPAY, y, x, nullptr, a, 0.0, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_axpyz_mask(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, bcm_field_t mask) {
+  return (mask && y) ? blas_c(OP_AXPYZ, z, x, y, a, 0.0, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_axpbypz_mask(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b, bcm_field_t mask) {
+  return (mask && y) ? blas_c(OP_AXPBYPZ, z, x, y, a, b, mask) : BCM_ERR_ARG;
+}
+extern "C" int bcm_copy_mask(bcm_field_t y, bcm_field_t x, bcm_field_t mask) {
+  return mask ? blas_c(OP_COPY_INTERIOR, y, x, nullptr, 0.0, 0.0, mask) : BCM_ERR_ARG;
 }
 
 extern "C" int bcm_fill(bcm_field_t x, double a) {

@@ -187,6 +187,20 @@ int bcm_field_dump2(bcm_field_t field, const char* dir, const char* label, int r
 int bcm_field_load2(bcm_field_t field, const char* dir, const char* label, int rank, int step);
 int bcm_field_dump(bcm_field_t field, const char* dir, const char* label, int rank, int step);
 int bcm_field_load(bcm_field_t field, const char* dir, const char* label, int rank, int step);
+
+/* VTK output of SolverTest's VtkWriter::writeScalar
+ * (examples/SolverTest/VtkWriter.h:733-889) with printVTIC (:75-200): one
+ * ImageData file per block, <dir>/<step %010d>/data-<name>-<rank %05d>-
+ * <id %05d>-<step %010d>.vti, holding the block's interior as appended raw
+ * Float32 cell data (origin and spacing printed in scientific notation with
+ * 16 digits), and, on rank 0, the index <dir>/data-<name>-<step %010d>.vthb
+ * (vtkHierarchicalBoxDataSet: difflevel + 1 refinement ratios of 2, one
+ * DataSet per block with its level, dataset = start_id + id and its
+ * amr_box in cells of its level, from the block origin in root lengths:
+ * (origin - root_origin) / root_length).  The reference writes to ./VTK
+ * (dir = "./VTK").  The interior is packed to FP32 on the device. */
+int bcm_field_write_vtk(bcm_field_t field, const char* dir, const char* name, int step, int rank, int start_id,
+                        int difflevel, const double root_origin[3], double root_length);
 void* bcm_field_device_ptr(bcm_field_t field);
 size_t bcm_field_bytes(bcm_field_t field);
 
@@ -283,6 +297,28 @@ int bcm_axpbypz(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b)
 int bcm_fill(bcm_field_t x, double a);
 int bcm_copy(bcm_field_t y, bcm_field_t x);
 
+/* Masked forms (blas.f90:753-1113: jacobi_smoother_mask, calc_ax_mask,
+ * calc_r_mask, calc_r2_mask, dot_mask, axpy_mask, xpay_mask, axpyz_mask,
+ * axpbypz_mask, copy_mask): only interior cells whose mask value is 1 are
+ * written or summed; the others keep their values.  The reference's mask
+ * is a LocalScalar3D<int> set to 1 inside and 0 in the ghosts by
+ * setup_mask (blas.f90:700-751); here the mask is a field of the
+ * operands' type holding 0/1.  The Fortran walks the linear range from the
+ * first to the last interior cell, ghosts between rows included; those
+ * ghosts carry mask 0 from setup_mask, so visiting interior cells only is
+ * the same. */
+int bcm_jacobi7_mask(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, bcm_field_t mask,
+                     double omega);
+int bcm_calc_ax_mask(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x, bcm_field_t mask);
+int bcm_calc_r_mask(bcm_field_t r, const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, bcm_field_t mask);
+int bcm_calc_r2_mask(const bcm_field_t A[7], bcm_field_t x, bcm_field_t b, bcm_field_t mask, double* rr);
+int bcm_dot_mask(bcm_field_t x, bcm_field_t y, bcm_field_t mask, double* xy);
+int bcm_axpy_mask(bcm_field_t y, bcm_field_t x, double a, bcm_field_t mask);
+int bcm_xpay_mask(bcm_field_t y, bcm_field_t x, double a, bcm_field_t mask);
+int bcm_axpyz_mask(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, bcm_field_t mask);
+int bcm_axpbypz_mask(bcm_field_t z, bcm_field_t x, bcm_field_t y, double a, double b, bcm_field_t mask);
+int bcm_copy_mask(bcm_field_t y, bcm_field_t x, bcm_field_t mask);
+
 /* Poisson operator A = lap_h (reference sign, bcut.f90:2848-2865 with no
  * cut cells: Aw..At = 1/h^2, Ap = -(l0+l1)-(l2+l3)-(l4+l5)) with the
  * outer-face folds in X_M..Z_P order (BlockScalar3D.h:207-214):
@@ -323,6 +359,16 @@ int bcm_jacobi_solve(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_f
 int bcm_bicgstab(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
                  const bcm_krylov_params* p, const double* bvalues, int nbvalues,
                  int* count, double* residual);
+/* ILS3D::Jacobi_Mask (ILS3D.h:846-899): the sweeps are masked, the
+ * residual norm is not.  ILS3D::BiCGSTAB_Mask (ILS3D.h:901-1097): the dot
+ * products, the preconditioner sweeps and A p_, A s_ are masked; the
+ * initial residual and the vector updates are not. */
+int bcm_jacobi_solve_mask(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, bcm_field_t x0, bcm_field_t mask,
+                          const bcm_krylov_params* p, const double* bvalues, int nbvalues,
+                          int* count, double* residual);
+int bcm_bicgstab_mask(bcm_field_t x, const bcm_field_t A[7], bcm_field_t b, const bcm_field_t work[9],
+                      bcm_field_t mask, const bcm_krylov_params* p, const double* bvalues, int nbvalues,
+                      int* count, double* residual);
 
 /* Mixed precision: FP64 outer iterative refinement around an FP32
  * Jacobi-preconditioned BiCGSTAB.  Only the FP64 residual r = b - A u,

@@ -54,15 +54,25 @@ def load():
         "oracle_poisson_assemble_d": [_P, _I, _I, _I, _P, _P, _P, _P],
         "oracle_bicgstab_d": [_P, _I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _D, _P, _P],
         "oracle_jacobi_solve_d": [_P, _I, _I, _I, _P, _P, _P, _P, _D, _I, _D, _P, _P],
+        "oracle_bicgstab_mask_d": [_P, _I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _D, _P, _P, _P],
+        "oracle_jacobi_solve_mask_d": [_P, _I, _I, _I, _P, _P, _P, _P, _D, _I, _D, _P, _P, _P],
     }
+    for suf in ("d", "f"):
+        sig["oracle_jacobi7_mask_" + suf] = sig["oracle_jacobi7_" + suf] + [_P]
+        sig["oracle_calc_ax_mask_" + suf] = sig["oracle_calc_ax_" + suf] + [_P]
+        sig["oracle_calc_r_mask_" + suf] = sig["oracle_calc_r_" + suf] + [_P]
+        sig["oracle_blas_mask_" + suf] = sig["oracle_blas_" + suf] + [_P]
     for name, args in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = None
     for name in ("oracle_calc_r2_d", "oracle_calc_r2_f", "oracle_dot_d", "oracle_dot_f"):
-        fn = getattr(lib, name)
-        fn.argtypes = [_I, _I, _I, _P, _P, _P] if "r2" in name else [_I, _I, _I, _P, _P]
-        fn.restype = _D
+        for nm, extra in ((name, []), (name[:-2] + "_mask" + name[-2:], [_P])):
+            fn = getattr(lib, nm)
+            fn.argtypes = ([_I, _I, _I, _P, _P, _P] if "r2" in nm else [_I, _I, _I, _P, _P]) + extra
+            fn.restype = _D
+    lib.oracle_bicgstab_mask_d.restype = _I
+    lib.oracle_jacobi_solve_mask_d.restype = _I
     lib.oracle_bicgstab_d.restype = _I
     lib.oracle_jacobi_solve_d.restype = _I
     _lib = lib
@@ -114,33 +124,43 @@ def sor_iterate(blocks, n, vc, f, s, omega, values, iters):
                                 v.ctypes.data, int(iters))
 
 
-def jacobi7(nb, n, vc, x1, x0, A, b, omega):
-    getattr(load(), "oracle_jacobi7_" + _suf(x1))(nb, n, vc, x1.ctypes.data, x0.ctypes.data, _arr7(A),
-                                                  b.ctypes.data, float(omega))
+def _m(name, arr, mask):
+    """Symbol of the plain or the masked (mask != None) variant."""
+    return getattr(load(), name + ("_mask_" if mask is not None else "_") + _suf(arr))
 
 
-def calc_ax(nb, n, vc, ax, A, x):
-    getattr(load(), "oracle_calc_ax_" + _suf(ax))(nb, n, vc, ax.ctypes.data, _arr7(A), x.ctypes.data)
+def _mask_args(mask):
+    return (mask.ctypes.data,) if mask is not None else ()
 
 
-def calc_r(nb, n, vc, r, A, x, b):
-    getattr(load(), "oracle_calc_r_" + _suf(r))(nb, n, vc, r.ctypes.data, _arr7(A), x.ctypes.data, b.ctypes.data)
+def jacobi7(nb, n, vc, x1, x0, A, b, omega, mask=None):
+    _m("oracle_jacobi7", x1, mask)(nb, n, vc, x1.ctypes.data, x0.ctypes.data, _arr7(A), b.ctypes.data, float(omega),
+                                   *_mask_args(mask))
 
 
-def calc_r2(nb, n, vc, A, x, b):
-    return getattr(load(), "oracle_calc_r2_" + _suf(x))(nb, n, vc, _arr7(A), x.ctypes.data, b.ctypes.data)
+def calc_ax(nb, n, vc, ax, A, x, mask=None):
+    _m("oracle_calc_ax", ax, mask)(nb, n, vc, ax.ctypes.data, _arr7(A), x.ctypes.data, *_mask_args(mask))
 
 
-def dot(nb, n, vc, x, y):
-    return getattr(load(), "oracle_dot_" + _suf(x))(nb, n, vc, x.ctypes.data, y.ctypes.data)
+def calc_r(nb, n, vc, r, A, x, b, mask=None):
+    _m("oracle_calc_r", r, mask)(nb, n, vc, r.ctypes.data, _arr7(A), x.ctypes.data, b.ctypes.data,
+                                 *_mask_args(mask))
+
+
+def calc_r2(nb, n, vc, A, x, b, mask=None):
+    return _m("oracle_calc_r2", x, mask)(nb, n, vc, _arr7(A), x.ctypes.data, b.ctypes.data, *_mask_args(mask))
+
+
+def dot(nb, n, vc, x, y, mask=None):
+    return _m("oracle_dot", x, mask)(nb, n, vc, x.ctypes.data, y.ctypes.data, *_mask_args(mask))
 
 
 BLAS_OPS = {"axpy": 0, "xpay": 1, "axpyz": 2, "axpbypz": 3, "copy": 4}
 
 
-def blas(nb, n, vc, op, z, x, y=None, a=0.0, b=0.0):
-    getattr(load(), "oracle_blas_" + _suf(z))(nb, n, vc, BLAS_OPS[op], z.ctypes.data, x.ctypes.data, _ptr(y),
-                                              float(a), float(b))
+def blas(nb, n, vc, op, z, x, y=None, a=0.0, b=0.0, mask=None):
+    _m("oracle_blas", z, mask)(nb, n, vc, BLAS_OPS[op], z.ctypes.data, x.ctypes.data, _ptr(y), float(a), float(b),
+                               *_mask_args(mask))
 
 
 def poisson_assemble(blocks, n, vc, A, b, rhs, values):
@@ -149,26 +169,26 @@ def poisson_assemble(blocks, n, vc, A, b, rhs, values):
                                      b.ctypes.data, rhs.ctypes.data, v.ctypes.data)
 
 
-def bicgstab(blocks, n, vc, x, A, b, work, omega, count_pre, count_max, eps):
+def bicgstab(blocks, n, vc, x, A, b, work, omega, count_pre, count_max, eps, mask=None):
     w = (ctypes.c_void_p * 9)()
     for i, a in enumerate(work):
         w[i] = a.ctypes.data
     cnt = ctypes.c_int()
     res = ctypes.c_double()
-    st = load().oracle_bicgstab_d(blocks.ctypes.data, len(blocks), n, vc, x.ctypes.data, _arr7(A), b.ctypes.data, w,
-                                  float(omega), int(count_pre), int(count_max), float(eps), ctypes.byref(cnt),
-                                  ctypes.byref(res))
+    fn = load().oracle_bicgstab_mask_d if mask is not None else load().oracle_bicgstab_d
+    st = fn(blocks.ctypes.data, len(blocks), n, vc, x.ctypes.data, _arr7(A), b.ctypes.data, w, float(omega),
+            int(count_pre), int(count_max), float(eps), ctypes.byref(cnt), ctypes.byref(res), *_mask_args(mask))
     return st, cnt.value, res.value
 
 
-def jacobi_solve(blocks, n, vc, x, A, b, x0, omega, count_max, eps):
+def jacobi_solve(blocks, n, vc, x, A, b, x0, omega, count_max, eps, mask=None):
     """Returns (status, count, residual, result array) -- the result may live in x or x0 after swaps."""
     px = ctypes.c_void_p(x.ctypes.data)
     p0 = ctypes.c_void_p(x0.ctypes.data)
     cnt = ctypes.c_int()
     res = ctypes.c_double()
-    st = load().oracle_jacobi_solve_d(blocks.ctypes.data, len(blocks), n, vc, ctypes.byref(px), _arr7(A),
-                                      b.ctypes.data, ctypes.byref(p0), float(omega), int(count_max), float(eps),
-                                      ctypes.byref(cnt), ctypes.byref(res))
+    fn = load().oracle_jacobi_solve_mask_d if mask is not None else load().oracle_jacobi_solve_d
+    st = fn(blocks.ctypes.data, len(blocks), n, vc, ctypes.byref(px), _arr7(A), b.ctypes.data, ctypes.byref(p0),
+            float(omega), int(count_max), float(eps), ctypes.byref(cnt), ctypes.byref(res), *_mask_args(mask))
     out = x if px.value == x.ctypes.data else x0
     return st, cnt.value, res.value, out

@@ -298,8 +298,10 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
 /* A[7] = {Ap, Aw, Ae, As, An, Ab, At}; OpenMP over blocks as ILS3D.    */
 /* ------------------------------------------------------------------ */
 #define DEFINE_BLAS(T, SUF)                                                                                   \
-  /* jacobi_smoother (blas.f90:452-502). */                                                                  \
-  void oracle_jacobi7_##SUF(int nb, int n, int vc, T* x1, const T* x0, T* const A[7], const T* bv, double w) { \
+  /* jacobi_smoother (blas.f90:452-502); with a mask, jacobi_smoother_mask                                    \
+   * (blas.f90:915-967): only cells whose mask is 1 are written. */                                         \
+  void oracle_jacobi7_mask_##SUF(int nb, int n, int vc, T* x1, const T* x0, T* const A[7], const T* bv,        \
+                                 double w, const T* mask) {                                                   \
     geo_t g = mkgeo(n, vc);                                                                                   \
     const T omega = (T)w;                                                                                     \
     _Pragma("omp parallel for schedule(static)")                                                              \
@@ -307,6 +309,7 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       long o0 = (long)b * g.bvol;                                                                             \
       for (int k = 0; k < n; ++k) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) {                  \
         long o = o0 + IDX(&g, i, j, k);                                                                       \
+        if (mask && mask[o] != (T)1) continue;                                                                \
         T r = bv[o] - A[1][o] * x0[o - 1] - A[2][o] * x0[o + 1] - A[3][o] * x0[o - g.nx0] -                   \
               A[4][o] * x0[o + g.nx0] - A[5][o] * x0[o - (long)g.nx0 * g.nx0] -                               \
               A[6][o] * x0[o + (long)g.nx0 * g.nx0];                                                          \
@@ -315,8 +318,8 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       }                                                                                                       \
     }                                                                                                         \
   }                                                                                                           \
-  /* calc_ax (blas.f90:556-600). */                                                                          \
-  void oracle_calc_ax_##SUF(int nb, int n, int vc, T* ax, T* const A[7], const T* x) {                        \
+  /* calc_ax (blas.f90:556-600); calc_ax_mask (:994-1030). */                                                \
+  void oracle_calc_ax_mask_##SUF(int nb, int n, int vc, T* ax, T* const A[7], const T* x, const T* mask) {    \
     geo_t g = mkgeo(n, vc);                                                                                   \
     const long sy = g.nx0, sz = (long)g.nx0 * g.nx0;                                                          \
     _Pragma("omp parallel for schedule(static)")                                                              \
@@ -324,13 +327,15 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       long o0 = (long)b * g.bvol;                                                                             \
       for (int k = 0; k < n; ++k) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) {                  \
         long o = o0 + IDX(&g, i, j, k);                                                                       \
+        if (mask && mask[o] != (T)1) continue;                                                                \
         ax[o] = +A[1][o] * x[o - 1] + A[2][o] * x[o + 1] + A[3][o] * x[o - sy] + A[4][o] * x[o + sy] +        \
                 A[5][o] * x[o - sz] + A[6][o] * x[o + sz] + A[0][o] * x[o];                                   \
       }                                                                                                       \
     }                                                                                                         \
   }                                                                                                           \
-  /* calc_r (blas.f90:602-648). */                                                                           \
-  void oracle_calc_r_##SUF(int nb, int n, int vc, T* r, T* const A[7], const T* x, const T* bv) {             \
+  /* calc_r (blas.f90:602-648); calc_r_mask (:1032-1070). */                                                 \
+  void oracle_calc_r_mask_##SUF(int nb, int n, int vc, T* r, T* const A[7], const T* x, const T* bv,           \
+                                const T* mask) {                                                              \
     geo_t g = mkgeo(n, vc);                                                                                   \
     const long sy = g.nx0, sz = (long)g.nx0 * g.nx0;                                                          \
     _Pragma("omp parallel for schedule(static)")                                                              \
@@ -338,6 +343,7 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       long o0 = (long)b * g.bvol;                                                                             \
       for (int k = 0; k < n; ++k) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) {                  \
         long o = o0 + IDX(&g, i, j, k);                                                                       \
+        if (mask && mask[o] != (T)1) continue;                                                                \
         r[o] = bv[o] - A[1][o] * x[o - 1] - A[2][o] * x[o + 1] - A[3][o] * x[o - sy] - A[4][o] * x[o + sy] -  \
                A[5][o] * x[o - sz] - A[6][o] * x[o + sz] - A[0][o] * x[o];                                    \
       }                                                                                                       \
@@ -345,7 +351,8 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
   }                                                                                                           \
   /* calc_r2 + CalcR2 (blas.f90:650-698, ILS3D.h:168-217): per-block sum in T,                               \
    * blocks summed in block order (the reference's OpenMP order varies). */                                  \
-  double oracle_calc_r2_##SUF(int nb, int n, int vc, T* const A[7], const T* x, const T* bv) {                 \
+  double oracle_calc_r2_mask_##SUF(int nb, int n, int vc, T* const A[7], const T* x, const T* bv,              \
+                                   const T* mask) {                                                           \
     geo_t g = mkgeo(n, vc);                                                                                   \
     const long sy = g.nx0, sz = (long)g.nx0 * g.nx0;                                                          \
     double* part = (double*)malloc(sizeof(double) * (size_t)nb);                                              \
@@ -355,6 +362,7 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       T rr = 0;                                                                                               \
       for (int k = 0; k < n; ++k) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) {                  \
         long o = o0 + IDX(&g, i, j, k);                                                                       \
+        if (mask && mask[o] != (T)1) continue;                                                                \
         T r = bv[o] - A[1][o] * x[o - 1] - A[2][o] * x[o + 1] - A[3][o] * x[o - sy] - A[4][o] * x[o + sy] -   \
               A[5][o] * x[o - sz] - A[6][o] * x[o + sz] - A[0][o] * x[o];                                     \
         rr = rr + r * r;                                                                                      \
@@ -367,7 +375,7 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
     return s;                                                                                                 \
   }                                                                                                           \
   /* dot (blas.f90:379-412) + ILS3D::DOT (ILS3D.h:272-311). */                                               \
-  double oracle_dot_##SUF(int nb, int n, int vc, const T* x, const T* y) {                                    \
+  double oracle_dot_mask_##SUF(int nb, int n, int vc, const T* x, const T* y, const T* mask) {                 \
     geo_t g = mkgeo(n, vc);                                                                                   \
     double* part = (double*)malloc(sizeof(double) * (size_t)nb);                                              \
     _Pragma("omp parallel for schedule(static)")                                                              \
@@ -376,6 +384,7 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       T xy = 0;                                                                                               \
       for (int k = 0; k < n; ++k) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) {                  \
         long o = o0 + IDX(&g, i, j, k);                                                                       \
+        if (mask && mask[o] != (T)1) continue;                                                                \
         xy = xy + x[o] * y[o];                                                                                \
       }                                                                                                       \
       part[b] = xy;                                                                                           \
@@ -386,7 +395,8 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
     return s;                                                                                                 \
   }                                                                                                           \
   /* axpy / xpay / axpyz / axpbypz (blas.f90:242-377), interior cells. */                                    \
-  void oracle_blas_##SUF(int nb, int n, int vc, int op, T* z, const T* x, const T* y, double a, double bc) {   \
+  void oracle_blas_mask_##SUF(int nb, int n, int vc, int op, T* z, const T* x, const T* y, double a,           \
+                              double bc, const T* mask) {                                                     \
     geo_t g = mkgeo(n, vc);                                                                                   \
     const T ta = (T)a, tb = (T)bc;                                                                            \
     _Pragma("omp parallel for schedule(static)")                                                              \
@@ -394,6 +404,7 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
       long o0 = (long)b * g.bvol;                                                                             \
       for (int k = 0; k < n; ++k) for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) {                  \
         long o = o0 + IDX(&g, i, j, k);                                                                       \
+        if (mask && mask[o] != (T)1) continue;                                                                \
         if (op == 0) z[o] = z[o] + ta * x[o];                                                                 \
         else if (op == 1) z[o] = x[o] + ta * z[o];                                                            \
         else if (op == 2) z[o] = ta * x[o] + y[o];                                                            \
@@ -401,6 +412,24 @@ void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, doub
         else z[o] = x[o];                                                                                     \
       }                                                                                                       \
     }                                                                                                         \
+  }                                                                                                           \
+  void oracle_jacobi7_##SUF(int nb, int n, int vc, T* x1, const T* x0, T* const A[7], const T* bv, double w) { \
+    oracle_jacobi7_mask_##SUF(nb, n, vc, x1, x0, A, bv, w, NULL);                                             \
+  }                                                                                                           \
+  void oracle_calc_ax_##SUF(int nb, int n, int vc, T* ax, T* const A[7], const T* x) {                        \
+    oracle_calc_ax_mask_##SUF(nb, n, vc, ax, A, x, NULL);                                                     \
+  }                                                                                                           \
+  void oracle_calc_r_##SUF(int nb, int n, int vc, T* r, T* const A[7], const T* x, const T* bv) {             \
+    oracle_calc_r_mask_##SUF(nb, n, vc, r, A, x, bv, NULL);                                                   \
+  }                                                                                                           \
+  double oracle_calc_r2_##SUF(int nb, int n, int vc, T* const A[7], const T* x, const T* bv) {                 \
+    return oracle_calc_r2_mask_##SUF(nb, n, vc, A, x, bv, NULL);                                              \
+  }                                                                                                           \
+  double oracle_dot_##SUF(int nb, int n, int vc, const T* x, const T* y) {                                    \
+    return oracle_dot_mask_##SUF(nb, n, vc, x, y, NULL);                                                      \
+  }                                                                                                           \
+  void oracle_blas_##SUF(int nb, int n, int vc, int op, T* z, const T* x, const T* y, double a, double bc) {   \
+    oracle_blas_mask_##SUF(nb, n, vc, op, z, x, y, a, bc, NULL);                                              \
   }
 
 DEFINE_BLAS(double, d)
@@ -465,20 +494,23 @@ static void swapp(double** a, double** b) {
 }
 
 static void jacobi_pre(const bcm_block_info* blk, int nb, int n, int vc, double** x, double** x0,
-                       double* const A[7], const double* bv, double omega, int count) {
+                       double* const A[7], const double* bv, double omega, int count, const double* mask) {
   for (int c = 0; c < count; ++c) {
-    oracle_jacobi7_d(nb, n, vc, *x0, *x, A, bv, omega);
+    oracle_jacobi7_mask_d(nb, n, vc, *x0, *x, A, bv, omega, mask);
     swapp(x, x0);
     oracle_ghost_fill_d(blk, nb, n, vc, *x, BCM_GF_ALL);
   }
 }
 
-int oracle_bicgstab_d(const bcm_block_info* blk, int nb, int n, int vc, double* x, double* const A[7],
-                      const double* bv, double* w[9], double omega, int count_pre, int count_max, double eps,
-                      int* count_out, double* residual) {
+/* With a mask, ILS3D::BiCGSTAB_Mask (ILS3D.h:901-1097): the dot products, the
+ * preconditioner sweeps and the products A p_, A s_ visit only the cells
+ * whose mask is 1; the initial residual and the vector updates do not. */
+static int bicgstab_impl(const bcm_block_info* blk, int nb, int n, int vc, double* x, double* const A[7],
+                         const double* bv, double* w[9], double omega, int count_pre, int count_max, double eps,
+                         int* count_out, double* residual, const double* mask) {
   double *r = w[0], *r0 = w[1], *p = w[2], *p_ = w[3], *q_ = w[4], *s = w[5], *s_ = w[6], *t_ = w[7], *x0 = w[8];
   size_t total = (size_t)nb * (size_t)mkgeo(n, vc).bvol;
-  double bb = oracle_dot_d(nb, n, vc, bv, bv);
+  double bb = oracle_dot_mask_d(nb, n, vc, bv, bv, mask);
   if (fabs(bb) < FLT_MIN) {
     *residual = 0.0;
     *count_out = 0;
@@ -489,7 +521,7 @@ int oracle_bicgstab_d(const bcm_block_info* blk, int nb, int n, int vc, double* 
   double rr0 = 1.0, alpha = 0.0, gamma = 1.0;
   int count, status = 1;
   for (count = 1; count <= count_max; ++count) {
-    double rr1 = oracle_dot_d(nb, n, vc, r, r0);
+    double rr1 = oracle_dot_mask_d(nb, n, vc, r, r0, mask);
     if (fabs(rr1) < FLT_MIN) {
       *residual = rr1;
       count = 0;
@@ -505,21 +537,21 @@ int oracle_bicgstab_d(const bcm_block_info* blk, int nb, int n, int vc, double* 
     }
     oracle_ghost_fill_d(blk, nb, n, vc, p, BCM_GF_ALL);
     memset(p_, 0, total * sizeof(double));
-    jacobi_pre(blk, nb, n, vc, &p_, &x0, A, p, omega, count_pre);
-    oracle_calc_ax_d(nb, n, vc, q_, A, p_);
-    double q_r0 = oracle_dot_d(nb, n, vc, q_, r0);
+    jacobi_pre(blk, nb, n, vc, &p_, &x0, A, p, omega, count_pre, mask);
+    oracle_calc_ax_mask_d(nb, n, vc, q_, A, p_, mask);
+    double q_r0 = oracle_dot_mask_d(nb, n, vc, q_, r0, mask);
     alpha = rr1 / q_r0;
     oracle_blas_d(nb, n, vc, 2, s, q_, r, -alpha, 0.0);
     oracle_ghost_fill_d(blk, nb, n, vc, s, BCM_GF_ALL);
     memset(s_, 0, total * sizeof(double));
-    jacobi_pre(blk, nb, n, vc, &s_, &x0, A, s, omega, count_pre);
-    oracle_calc_ax_d(nb, n, vc, t_, A, s_);
-    double t_s = oracle_dot_d(nb, n, vc, t_, s);
-    double t_t = oracle_dot_d(nb, n, vc, t_, t_);
+    jacobi_pre(blk, nb, n, vc, &s_, &x0, A, s, omega, count_pre, mask);
+    oracle_calc_ax_mask_d(nb, n, vc, t_, A, s_, mask);
+    double t_s = oracle_dot_mask_d(nb, n, vc, t_, s, mask);
+    double t_t = oracle_dot_mask_d(nb, n, vc, t_, t_, mask);
     gamma = t_s / t_t;
     oracle_blas_d(nb, n, vc, 3, x, p_, s_, alpha, gamma);
     oracle_blas_d(nb, n, vc, 2, r, t_, s, -gamma, 0.0);
-    double rr = oracle_dot_d(nb, n, vc, r, r);
+    double rr = oracle_dot_mask_d(nb, n, vc, r, r, mask);
     double res = (fabs(bb) > FLT_MIN) ? fabs(rr) / fabs(bb) : fabs(rr);
     *residual = res;
     if (isnan(res) || isinf(res)) return 2;
@@ -536,14 +568,27 @@ int oracle_bicgstab_d(const bcm_block_info* blk, int nb, int n, int vc, double* 
   return status;
 }
 
-/* ILS3D::Jacobi (examples/SolverTest/ILS3D.h:478-529), FP64, stored A. */
-int oracle_jacobi_solve_d(const bcm_block_info* blk, int nb, int n, int vc, double** x, double* const A[7],
-                          const double* bv, double** x0, double omega, int count_max, double eps, int* count_out,
-                          double* residual) {
+int oracle_bicgstab_d(const bcm_block_info* blk, int nb, int n, int vc, double* x, double* const A[7],
+                      const double* bv, double* w[9], double omega, int count_pre, int count_max, double eps,
+                      int* count_out, double* residual) {
+  return bicgstab_impl(blk, nb, n, vc, x, A, bv, w, omega, count_pre, count_max, eps, count_out, residual, NULL);
+}
+
+int oracle_bicgstab_mask_d(const bcm_block_info* blk, int nb, int n, int vc, double* x, double* const A[7],
+                           const double* bv, double* w[9], double omega, int count_pre, int count_max, double eps,
+                           int* count_out, double* residual, const double* mask) {
+  return bicgstab_impl(blk, nb, n, vc, x, A, bv, w, omega, count_pre, count_max, eps, count_out, residual, mask);
+}
+
+/* ILS3D::Jacobi (examples/SolverTest/ILS3D.h:478-529), FP64, stored A; with
+ * a mask ILS3D::Jacobi_Mask (:846-899), where only the sweep is masked. */
+static int jacobi_solve_impl(const bcm_block_info* blk, int nb, int n, int vc, double** x, double* const A[7],
+                             const double* bv, double** x0, double omega, int count_max, double eps,
+                             int* count_out, double* residual, const double* mask) {
   double bb = oracle_dot_d(nb, n, vc, bv, bv);
   int c, status = 1;
   for (c = 1; c <= count_max; ++c) {
-    oracle_jacobi7_d(nb, n, vc, *x0, *x, A, bv, omega);
+    oracle_jacobi7_mask_d(nb, n, vc, *x0, *x, A, bv, omega, mask);
     swapp(x, x0);
     oracle_ghost_fill_d(blk, nb, n, vc, *x, BCM_GF_ALL);
     double rr = oracle_calc_r2_d(nb, n, vc, A, *x, bv);
@@ -558,4 +603,16 @@ int oracle_jacobi_solve_d(const bcm_block_info* blk, int nb, int n, int vc, doub
   *count_out = c > count_max ? count_max : c;
   oracle_ghost_fill_d(blk, nb, n, vc, *x, BCM_GF_ALL);
   return status;
+}
+
+int oracle_jacobi_solve_d(const bcm_block_info* blk, int nb, int n, int vc, double** x, double* const A[7],
+                          const double* bv, double** x0, double omega, int count_max, double eps, int* count_out,
+                          double* residual) {
+  return jacobi_solve_impl(blk, nb, n, vc, x, A, bv, x0, omega, count_max, eps, count_out, residual, NULL);
+}
+
+int oracle_jacobi_solve_mask_d(const bcm_block_info* blk, int nb, int n, int vc, double** x, double* const A[7],
+                               const double* bv, double** x0, double omega, int count_max, double eps,
+                               int* count_out, double* residual, const double* mask) {
+  return jacobi_solve_impl(blk, nb, n, vc, x, A, bv, x0, omega, count_max, eps, count_out, residual, mask);
 }
