@@ -1219,7 +1219,12 @@ static int begin_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve
     z.refresh_max = rmax;
     z.restart_every = rs_every;
     z.stall = stall;
-    z.stag_max = P.stagnation > 0 ? P.stagnation : (P.stagnation == 0 ? 8 : 0);
+    // FP64: 8 checks (each follows a recurrence that met the tolerance).
+    // MIXED/FP32 check after every inner solve, and an inner solve can set x
+    // back before later ones recover (measured on the level-7 shell mesh),
+    // so they get 30.
+    const int stag_default = P.precision == BCM_PREC_FP64 ? 8 : 30;
+    z.stag_max = P.stagnation > 0 ? P.stagnation : (P.stagnation == 0 ? stag_default : 0);
     z.rr_best = DBL_MAX;
     z.rho_prev = 1.0;
     z.gamma = 1.0;

@@ -327,7 +327,7 @@ def engine_bytes_per_cell(es, pre, n):
     upd_s = 2 * es + es + es * (1 + xf)  # r, q -> s, first sweep + export
     sweep = es * (1 + xf) + es + es * (1 + xf)  # input + export, rhs -> output + export
     spmv = d * (1 + xf) + es + es  # p_ + export, r0 (or s) -> q (or t)
-    xr = 3 * d + 4 * es + d + es  # dx, p_, s_, r, s, t, r0 -> dx, r
+    xr = 3 * d + 3 * es + d + es  # dx, p_, s_, s, t, r0 -> dx, r
     return upd_p + upd_s + 2 * (pre - 1) * sweep + 2 * spmv + xr
 
 
@@ -464,21 +464,13 @@ def bench_batched(args):
     }
 
 
-def bench_serving(args):
-    """Poisson-arrival stream of solves (block counts log-uniform in [64, 50k],
-    tol 1e-6 -> FP32 path, 1e-10 -> FP32 + FP64 refinement), HBM budget 200 GB."""
+def serve_stream(args, lam, trees, tols, gaps):
+    """One pass of the request stream at arrival rate lam; returns the
+    per-request outcomes and the stream's statistics."""
     from bcmtools_amd import Server
-    from bcmtools_amd import workloads as W
 
-    R, lam, n = args.requests, args.lam, 16
-    rng = np.random.default_rng(args.seed)
-    targets = np.exp(rng.uniform(np.log(64), np.log(50000), size=R)).astype(int)
-    tols = rng.choice([1e-6, 1e-10], size=R)
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=R))
-    trees = [W.serving_tree(7000 + i, int(targets[i])) for i in range(R)]
-    counts = [len(t.blocks) for t in trees]
-    print(f"[bench] serving: {R} requests at {lam}/s, {sum(counts)} blocks ({min(counts)}..{max(counts)})",
-          file=sys.stderr, flush=True)
+    R, n = len(trees), 16
+    arrivals = np.cumsum(gaps / lam)
     sv = Server(200e9, max_active=args.slots, n=n, max_iter=20000, check_every=8)
     done, k = {}, 0
     t0 = sv.clock()
@@ -499,15 +491,15 @@ def bench_serving(args):
         if sv.clock() - last_print > 15.0:
             last_print = sv.clock()
             st = sv.stats()
-            print(f"[bench] serving: {len(done)}/{R} done, {st['active']} active, {st['queued']} queued, "
+            print(f"[bench] serving at {lam}/s: {len(done)}/{R} done, {st['active']} active, {st['queued']} queued, "
                   f"{st['bytes_in_use'] / 1e9:.1f} GB held", file=sys.stderr, flush=True)
+    sv.close()
     t_end = max(r["t_done"] for r in done.values()) - t0
     lat = np.array([done[i]["t_done"] - t0 - arrivals[i] for i in range(R)])
     wait_q = np.array([done[i]["t_admit"] - t0 - arrivals[i] for i in range(R)])
     ok = [done[i]["status"] == 1 and done[i]["relres"] <= tols[i] for i in range(R)]
     failed = [{"tag": i, "status": done[i]["status"], "relres": done[i]["relres"], "iters": done[i]["iters"],
-               "nblocks": done[i]["nblocks"]} for i in range(R) if not ok[i]]
-    sv.close()
+               "nblocks": done[i]["nblocks"], "tol": float(tols[i])} for i in range(R) if not ok[i]]
     by_tol = {}
     for tv in (1e-6, 1e-10):
         sel = [i for i in range(R) if tols[i] == tv]
@@ -515,30 +507,60 @@ def bench_serving(args):
             by_tol[f"{tv:g}"] = {"requests": len(sel), "p50_s": float(np.percentile(lat[sel], 50)),
                                  "p99_s": float(np.percentile(lat[sel], 99)),
                                  "mean_iters": float(np.mean([done[i]["iters"] for i in sel]))}
-    return {
+    span = t_end - arrivals[0]
+    print(f"[bench] serving at {lam}/s: {R / span:.3f} solves/s, p50 {np.percentile(lat, 50):.2f} s, "
+          f"p99 {np.percentile(lat, 99):.2f} s", file=sys.stderr, flush=True)
+    return {"lambda_per_s": lam, "solves_per_s": R / span, "makespan_s": float(t_end),
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "mean": float(lat.mean()), "max": float(lat.max())},
+            "queue_wait_s": {"p50": float(np.percentile(wait_q, 50)), "p99": float(np.percentile(wait_q, 99))},
+            "by_tolerance": by_tol, "all_met_tolerance": bool(all(ok)), "not_met": failed}
+
+
+def bench_serving(args):
+    """Poisson-arrival stream of solves (block counts log-uniform in [64, 50k],
+    tol 1e-6 or 1e-10), HBM budget 200 GB, at one or several arrival rates
+    (the same requests, arrival gaps rescaled)."""
+    from bcmtools_amd import workloads as W
+
+    R = args.requests
+    lams = [float(v) for v in args.lams.split(",")] if args.lams else [args.lam]
+    rng = np.random.default_rng(args.seed)
+    targets = np.exp(rng.uniform(np.log(64), np.log(50000), size=R)).astype(int)
+    tols = rng.choice([1e-6, 1e-10], size=R)
+    gaps = rng.exponential(1.0, size=R)
+    tb = time.perf_counter()
+    trees = [W.serving_tree(7000 + i, int(targets[i])) for i in range(R)]
+    t_tree = time.perf_counter() - tb
+    counts = [len(t.blocks) for t in trees]
+    print(f"[bench] serving: {R} requests, {sum(counts)} blocks ({min(counts)}..{max(counts)}), meshes built in "
+          f"{t_tree:.2f} s on the host", file=sys.stderr, flush=True)
+    runs = [serve_stream(args, lam, trees, tols, gaps) for lam in lams]
+    head = runs[0]
+    res = {
         "metric": "served Poisson solves/s with p50/p99 latency (continuous batching, HBM budget 200 GB)",
-        "value": R / (t_end - arrivals[0]),
+        "value": head["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1, "steps": R, "warmup": 0,
-        "ms_per_step": (t_end - arrivals[0]) / R * 1e3,
+        "ms_per_step": 1e3 / head["solves_per_s"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.policy == "fp64" else "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{R} requests, Poisson arrivals at {lam}/s, meshes of 1-3 random sphere surfaces "
-                               "with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 or 1e-10, "
+        "config": {"workload": f"{R} requests, Poisson arrivals at {lams[0]:g}/s, meshes of 1-3 random sphere "
+                               "surfaces with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 or "
+                               "1e-10, "
                                + ("FP32 path for 1e-6 and FP32 + FP64 refinement for 1e-10"
                                   if args.policy == "spec" else "fused FP64 BiCGSTAB for every request")
                                + f", up to {args.slots} requests on the device, HBM budget 200 GB",
                    "precision_policy": args.policy,
-                   "requests": R, "lambda_per_s": lam, "blocks_total": int(sum(counts)),
-                   "blocks_per_request": [int(min(counts)), int(np.median(counts)), int(max(counts))]},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "mean": float(lat.mean()), "max": float(lat.max())},
-        "queue_wait_s": {"p50": float(np.percentile(wait_q, 50)), "p99": float(np.percentile(wait_q, 99))},
-        "by_tolerance": by_tol,
-        "all_met_tolerance": bool(all(ok)),
-        "not_met": failed,
-        "makespan_s": float(t_end),
+                   "requests": R, "lambda_per_s": lams[0], "blocks_total": int(sum(counts)),
+                   "blocks_per_request": [int(min(counts)), int(np.median(counts)), int(max(counts))],
+                   "host_mesh_build_s_per_request": t_tree / R},
     }
+    res.update({k: v for k, v in head.items() if k not in ("lambda_per_s", "solves_per_s")})
+    if len(runs) > 1:
+        res["sweep"] = [{k: r[k] for k in ("lambda_per_s", "solves_per_s", "makespan_s", "latency_s",
+                                          "all_met_tolerance")} for r in runs]
+    return res
 
 
 def bench_copy(args):
@@ -663,6 +685,7 @@ def main():
                     choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart"])
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
+    ap.add_argument("--lams", default="", help="serving: comma-separated arrival rates to sweep (same requests)")
     ap.add_argument("--slots", type=int, default=16, help="serving: requests on the device at once")
     ap.add_argument("--seed", type=int, default=1, help="serving: request stream seed")
     ap.add_argument("--policy", default="fp64", choices=["fp64", "spec"],

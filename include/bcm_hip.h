@@ -459,7 +459,8 @@ typedef struct bcm_solve_params {
   int32_t stagnation;         /* end a segment as BCM_SOLVE_NOCONV when this many consecutive
                                  true-residual checks did not halve the best ||b - A x||:
                                  the tolerance is below what the arithmetic attains
-                                 (0: 8, < 0: off, run to max_iter) */
+                                 (0: 8 for FP64, 30 for MIXED/FP32; < 0: off, run to
+                                 max_iter) */
 } bcm_solve_params;
 typedef struct bcm_solve_stats {
   int32_t status;             /* BCM_SOLVE_CONVERGED / NOCONV / FAILED */
