@@ -473,10 +473,25 @@ __global__ __launch_bounds__(KWG) void k_kry_jacobi(TO* __restrict__ gout, TO* _
 // together, and the seven stencil values of every cell come from LDS.  The
 // outputs are then staged in the same LDS space so the column export is
 // written coalesced.
-constexpr int KWG_L = 512;
+// Threads per block of the LDS-staged kernels (at most one per cell),
+// the faster of 256/512/1024 measured on the level-6 shell mesh: the
+// sweep at 512 (176 us vs 180 us at 1024), the SpMV at 1024 (143 us vs
+// 159 us at 512).
+template <int N, int W>
+constexpr int kwg_cap() {
+  return N * N * N < W ? N * N * N : W;
+}
+template <int N>
+constexpr int kwg_jac() {
+  return kwg_cap<N, 512>();
+}
+template <int N>
+constexpr int kwg_spmv() {
+  return kwg_cap<N, 1024>();
+}
 
 template <typename T, int N, typename TI, typename TO, int W>
-__global__ __launch_bounds__(W) void k_kry_jacobi_l(TO* __restrict__ gout, TO* __restrict__ gox,
+__global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) void k_kry_jacobi_l(TO* __restrict__ gout, TO* __restrict__ gox,
                                                     const TI* __restrict__ gin, const TI* __restrict__ gix,
                                                     const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
                                                     const int* __restrict__ blk_seg,
@@ -528,7 +543,7 @@ __global__ __launch_bounds__(W) void k_kry_jacobi_l(TO* __restrict__ gout, TO* _
 
 // k_kry_spmv for N <= 16 with the ghosted block staged in LDS.
 template <typename T, int N, int NDOT, int W>
-__global__ __launch_bounds__(W) void k_kry_spmv_l(T* __restrict__ out, const double* __restrict__ g,
+__global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) void k_kry_spmv_l(T* __restrict__ out, const double* __restrict__ g,
                                                   const double* __restrict__ gx, const T* __restrict__ y,
                                                   const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blk_seg, const SegState* __restrict__ st,
@@ -542,9 +557,6 @@ __global__ __launch_bounds__(W) void k_kry_spmv_l(T* __restrict__ out, const dou
   const int b = blist[blockIdx.x];
   if (st[blk_seg[b]].status != BCM_SOLVE_ACTIVE) return;
   const long long cb = (long long)b * B::N3;
-  T yv[PER];
-#pragma unroll
-  for (int m = 0; m < PER; ++m) yv[m] = y[cb + threadIdx.x + W * m];
   stage_block<double, N, W>(Compact<double, N>{g, gx}, desc + b * 48, b, lin);
   __syncthreads();
   const BlockMeta me = meta[b];
@@ -558,7 +570,7 @@ __global__ __launch_bounds__(W) void k_kry_spmv_l(T* __restrict__ out, const dou
     const T v = T(ax_vals(a, lin[a0 - 1], lin[a0 + 1], lin[a0 - G::NX], lin[a0 + G::NX], lin[a0 - G::NX * G::NX],
                           lin[a0 + G::NX * G::NX], lin[a0]));
     out[cb + threadIdx.x + W * m] = v;
-    d0 += double(v) * double(yv[m]);
+    d0 += double(v) * double(y[cb + threadIdx.x + W * m]);
     if (NDOT == 2) d1 += double(v) * double(v);
   }
   wg_sum2n<W / 64>(d0, d1, red);
@@ -1075,14 +1087,14 @@ static int precondition(bcm_mesh* m, T* scratch, T* sx, double* result, double* 
   for (int sweep = 2; sweep <= pre; ++sweep) {
     if (in_result) {
       if constexpr (N <= 16)
-        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, double, T, KWG_L>), grid, dim3(KWG_L), 0, m->stream, scratch, sx, result, rx, rhs,
+        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, double, T, kwg_jac<N>()>), grid, dim3(kwg_jac<N>()), 0, m->stream, scratch, sx, result, rx, rhs,
                            m->d_meta, bs, st, omega, bl, m->d_desc);
       else
         hipLaunchKernelGGL((k_kry_jacobi<T, N, double, T>), grid, blk, 0, m->stream, scratch, sx, result, rx, rhs,
                            m->d_meta, bs, st, omega, bl, m->d_desc);
     } else {
       if constexpr (N <= 16)
-        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, T, double, KWG_L>), grid, dim3(KWG_L), 0, m->stream, result, rx, scratch, sx, rhs,
+        hipLaunchKernelGGL((k_kry_jacobi_l<T, N, T, double, kwg_jac<N>()>), grid, dim3(kwg_jac<N>()), 0, m->stream, result, rx, scratch, sx, rhs,
                            m->d_meta, bs, st, omega, bl, m->d_desc);
       else
         hipLaunchKernelGGL((k_kry_jacobi<T, N, T, double>), grid, blk, 0, m->stream, result, rx, scratch, sx, rhs,
@@ -1143,7 +1155,7 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
   BCM_TRY((update_and_first_sweep<T, N, true>(m, p, r, q, r0, g1, x1, gp, xp, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, x1, gp, xp, p, pre, omega)));
   if constexpr (N <= 16)
-    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 1, KWG_L>), grid, dim3(KWG_L), 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
+    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 1, kwg_spmv<N>()>), grid, dim3(kwg_spmv<N>()), 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
                        bl, m->d_desc);
   else
     hipLaunchKernelGGL((k_kry_spmv<T, N, 1>), grid, blk, 0, m->stream, q, gp, xp, r0, m->d_meta, bs, st, K->parts,
@@ -1153,7 +1165,7 @@ static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
   BCM_TRY((update_and_first_sweep<T, N, false>(m, s, r, q, nullptr, g1, x1, gs, xs, pre, omega)));
   BCM_TRY((precondition<T, N>(m, g1, x1, gs, xs, s, pre, omega)));
   if constexpr (N <= 16)
-    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 2, KWG_L>), grid, dim3(KWG_L), 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,
+    hipLaunchKernelGGL((k_kry_spmv_l<T, N, 2, kwg_spmv<N>()>), grid, dim3(kwg_spmv<N>()), 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,
                        bl, m->d_desc);
   else
     hipLaunchKernelGGL((k_kry_spmv<T, N, 2>), grid, blk, 0, m->stream, t, gs, xs, s, m->d_meta, bs, st, K->parts,

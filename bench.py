@@ -177,13 +177,32 @@ def bench_jacobi(args):
     jms, jn = mesh.kernel_time(L.K_JACOBI)
     javg = jms / max(jn, 1) * 1e-3
     achieved = 24.0 * ncell / javg / 1e9
-    return {"metric": "stencil GCUPS (FP64 Jacobi7 + ghost fill, config 1)", "value": ncell * args.steps / el / 1e9,
-            "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "uniform level-2 octree, 64 x 16^3, matrix-free Jacobi7 + ghost fill"},
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+    res = {"metric": "stencil GCUPS (FP64 Jacobi7 + ghost fill, config 1)", "value": ncell * args.steps / el / 1e9,
+           "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "uniform level-2 octree, 64 x 16^3, matrix-free Jacobi7 + ghost fill"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+    if not args.no_cpu:
+        import oracle
+
+        A = [np.zeros_like(f) for _ in range(7)]
+        b = np.zeros_like(f)
+        oracle.poisson_assemble(blocks, n, vc, A, b, -f, [0.0])
+        x, x0 = np.zeros_like(f), np.zeros_like(f)
+        sweeps = 100
+        t0 = time.perf_counter()
+        for _ in range(sweeps):
+            oracle.jacobi7(len(blocks), n, vc, x0, x, A, b, 1.0)
+            x, x0 = x0, x
+            oracle.ghost_fill(blocks, n, vc, x)
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        res["cpu_baseline"] = {"value": ncell * sweeps / dt / 1e9, "unit": "GCUPS", "cores": cores, "kind": "port",
+                               "sample": f"same mesh, {sweeps} sweeps of jacobi_smoother (stored A) + ghost fill "
+                                         f"({dt:.2f} s) on {cores} threads of {cpu_model()}"}
+    return res
 
 
 def true_relres(mesh, x, rhs_field, bvalues):
