@@ -56,7 +56,7 @@ def cpu_model():
 
 
 def traffic_from_profiles(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.json), if recorded."""
+    """HBM bytes measured with rocprofv3 --pmc (profiles/traffic.json), if recorded."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fp:
@@ -243,8 +243,9 @@ def bench_bicgstab(args):
 
     x = mesh.field(np.float64)
     ug = not args.no_graph
-    mixed = lambda xx: mesh.solve(xx, rhs, vals, precision="mixed", tol=tol, max_iter=20000, use_graph=ug)[0]
-    fp64 = lambda xx: mesh.solve(xx, rhs, vals, precision="fp64", tol=tol, max_iter=20000, use_graph=ug)[0]
+    mi = args.max_iter
+    mixed = lambda xx: mesh.solve(xx, rhs, vals, precision="mixed", tol=tol, max_iter=mi, use_graph=ug)[0]
+    fp64 = lambda xx: mesh.solve(xx, rhs, vals, precision="fp64", tol=tol, max_iter=mi, use_graph=ug)[0]
     if args.precision != "auto":
         # One engine precision only (e.g. for a kernel trace of that path).
         return bicgstab_one(args, mesh, rhs, vals, timed, mixed if args.precision == "mixed" else fp64,
@@ -294,9 +295,11 @@ def bench_bicgstab(args):
     # whole solve (reductions, graph launches and checks included).
     bpc = engine_bytes_per_cell(8, 2, n)
     achieved = bpc * ncell * st64["iters"] / t64 / 1e9
+    tpc = traffic_from_profiles("bicgstab_fp64_n16_bytes_per_cell_iteration") if n == 16 else None
     res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                       "frac": achieved / 8000.0, "traffic": None,
-                       "kernel": f"fused FP64 BiCGSTAB iteration, all kernels ({bpc:.0f} B per cell per iteration)"}
+                       "frac": achieved / 8000.0, "traffic": tpc * ncell if tpc else None,
+                       "kernel": f"fused FP64 BiCGSTAB iteration, all kernels ({bpc:.0f} B per cell per iteration; "
+                                 "traffic = measured HBM bytes per iteration)"}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_bicgstab(blocks, n, rhs_host, ith, args.cpu_steps)
     return res
@@ -329,9 +332,11 @@ def bicgstab_one(args, mesh, rhs, vals, timed, solve, level, blocks, ncell, n):
     if args.precision == "fp64":
         bpc = engine_bytes_per_cell(es, 2, n)
         achieved = bpc * ncell * st["iters"] / (elapsed / len(times)) / 1e9
+        tpc = traffic_from_profiles("bicgstab_fp64_n16_bytes_per_cell_iteration") if n == 16 else None
         res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                           "frac": achieved / 8000.0, "traffic": None,
-                           "kernel": f"fused FP64 BiCGSTAB iteration, all kernels ({bpc:.0f} B per cell per iteration)"}
+                           "frac": achieved / 8000.0, "traffic": tpc * ncell if tpc else None,
+                           "kernel": f"fused FP64 BiCGSTAB iteration, all kernels ({bpc:.0f} B per cell per "
+                                     "iteration; traffic = measured HBM bytes per iteration)"}
     return res
 
 
@@ -716,6 +721,8 @@ def main():
     ap.add_argument("--level", type=int, default=7, help="bicgstab: shell refinement level")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-iter", type=int, default=20000,
+                    help="bicgstab: iteration limit of the engine solves (a short limit for counter runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="bicgstab: launch the iteration kernels one by one instead of replaying a hipGraph")
     args = ap.parse_args()
