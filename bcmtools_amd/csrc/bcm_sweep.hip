@@ -423,6 +423,14 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   // writes without splitting a cache line more often (2 % faster).
   constexpr int NPAIR = N * (HALF + 1);
   constexpr int NT = N * N / 2;
+  // Export columns the next sweep reads: the outer column (0 or N-1) on a
+  // side with a neighbour or a Dirichlet condition (copy, mirror), the inner
+  // one (1 or N-2) only across a level change (F2C and C2F read two
+  // columns).  Thread t < 4N owns column t / N of the export.
+  const int xcol = t / N;
+  const int xkind = bdesc[(xcol < 2 ? BCM_X_M : BCM_X_P) * 8];
+  const bool xneed = t < 4 * N && xkind != FD_NONE &&
+                     (xcol == 0 || xcol == 3 || xkind == FD_F2C || xkind == FD_C2F);
   auto store_plane = [&](int p) __attribute__((always_inline)) {
     const int S = p % 3;
     T* dst = Bb + (long long)(p + 1) * PL;
@@ -435,10 +443,10 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
         store_pair(dst + off, lds[S][off], lds[S][off + 1]);
       }
     }
-    if (t < 4 * N) {
-      const int col = t / N, j = t % N;
-      const int i = col < 2 ? col : col + N - 4;
-      xout[((long long)(b * 4 + col) * N + p) * N + j] = lds[S][L(i, j)];
+    if (xneed) {
+      const int j = t % N;
+      const int i = xcol < 2 ? xcol : xcol + N - 4;
+      xout[((long long)(b * 4 + xcol) * N + p) * N + j] = lds[S][L(i, j)];
     }
   };
 
