@@ -216,6 +216,7 @@ SIGNATURES = {
     "bcm_axpyz_mask": (_I, [_P, _P, _P, _D, _P]),
     "bcm_axpbypz_mask": (_I, [_P, _P, _P, _D, _D, _P]),
     "bcm_copy_mask": (_I, [_P, _P, _P]),
+    "bcm_setup_mask": (_I, [_P]),
     "bcm_jacobi_solve_mask": (_I, [_P, ctypes.POINTER(_P), _P, _P, _P, ctypes.POINTER(KrylovParams), _DP, _I, _IP,
                                    _DP]),
     "bcm_bicgstab_mask": (_I, [_P, ctypes.POINTER(_P), _P, ctypes.POINTER(_P), _P, ctypes.POINTER(KrylovParams), _DP,

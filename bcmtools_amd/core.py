@@ -364,6 +364,10 @@ class Mesh:
         """copy_mask (blas.f90:969-992): interior cells where mask is 1."""
         L.check(self.lib.bcm_copy_mask(y.handle, x.handle, mask.handle), "bcm_copy_mask")
 
+    def setup_mask(self, mask):
+        """setup_mask (blas.f90:700-751): 1 on interior cells, 0 on ghosts."""
+        L.check(self.lib.bcm_setup_mask(mask.handle), "bcm_setup_mask")
+
     def fill(self, x, a):
         L.check(self.lib.bcm_fill(x.handle, float(a)), "bcm_fill")
 

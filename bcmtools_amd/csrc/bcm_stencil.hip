@@ -419,6 +419,18 @@ __global__ __launch_bounds__(256) void k_fill_all(T* __restrict__ x, T v, long l
   if (i < count) x[i] = v;
 }
 
+// setup_mask: 1 on interior cells, 0 on ghosts, every block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_setup_mask(T* __restrict__ x, int n, int vc, long long count) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= count) return;
+  const int nx0 = n + 2 * vc;
+  const int r = int(c % ((long long)nx0 * nx0 * nx0));
+  const int i = r % nx0 - vc, j = (r / nx0) % nx0 - vc, k = r / (nx0 * nx0) - vc;
+  const bool inside = i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n;
+  x[c] = inside ? T(1) : T(0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_all(T* __restrict__ y, const T* __restrict__ x, long long count) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -926,6 +938,23 @@ extern "C" int bcm_fill(bcm_field_t x, double a) {
   if (!x) return BCM_ERR_ARG;
   if (x->dtype == BCM_F64) return fill_raw<double>(x->mesh, static_cast<double*>(x->data), a);
   return fill_raw<float>(x->mesh, static_cast<float*>(x->data), a);
+}
+
+extern "C" int bcm_setup_mask(bcm_field_t mask) {
+  if (!mask) return BCM_ERR_ARG;
+  bcm_mesh* m = mask->mesh;
+  const long long count = (long long)m->nblocks * (long long)m->bvol;
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_BLAS1, &ev));
+  const unsigned grid = unsigned((count + 255) / 256);
+  if (mask->dtype == BCM_F64)
+    hipLaunchKernelGGL(k_setup_mask<double>, dim3(grid), dim3(256), 0, m->stream, static_cast<double*>(mask->data),
+                       m->n, m->vc, count);
+  else
+    hipLaunchKernelGGL(k_setup_mask<float>, dim3(grid), dim3(256), 0, m->stream, static_cast<float*>(mask->data),
+                       m->n, m->vc, count);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
 }
 
 extern "C" int bcm_copy(bcm_field_t y, bcm_field_t x) {

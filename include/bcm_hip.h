@@ -307,6 +307,9 @@ int bcm_copy(bcm_field_t y, bcm_field_t x);
  * first to the last interior cell, ghosts between rows included; those
  * ghosts carry mask 0 from setup_mask, so visiting interior cells only is
  * the same. */
+/* setup_mask (blas.f90:700-751, called at examples/SolverTest/Solver.cpp:
+ * 1022): every cell of `mask` to 1 inside the block, 0 in the ghosts. */
+int bcm_setup_mask(bcm_field_t mask);
 int bcm_jacobi7_mask(bcm_field_t x1, bcm_field_t x0, const bcm_field_t A[7], bcm_field_t b, bcm_field_t mask,
                      double omega);
 int bcm_calc_ax_mask(bcm_field_t ax, const bcm_field_t A[7], bcm_field_t x, bcm_field_t mask);

@@ -124,6 +124,15 @@ def sor_iterate(blocks, n, vc, f, s, omega, values, iters):
                                 v.ctypes.data, int(iters))
 
 
+def setup_mask(shape, vc, dtype=np.float64):
+    """setup_mask (examples/SolverTest/blas.f90:700-751): every cell 0, then
+    the interior cells 1.  `shape` is (nblocks, n+2vc, n+2vc, n+2vc)."""
+    m = np.zeros(shape, dtype=dtype)
+    n = shape[-1] - 2 * vc
+    m[:, vc:vc + n, vc:vc + n, vc:vc + n] = 1
+    return m
+
+
 def _m(name, arr, mask):
     """Symbol of the plain or the masked (mask != None) variant."""
     return getattr(load(), name + ("_mask_" if mask is not None else "_") + _suf(arr))

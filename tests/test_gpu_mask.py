@@ -117,3 +117,25 @@ def test_jacobi_solve_mask_vs_oracle():
                                        epsilon=1e-14, mask=mesh.field(np.float64, mask))
     assert gcnt == cnt == 60
     np.testing.assert_array_equal(W.interior(gx.download(), vc), W.interior(out, vc))
+
+
+@pytest.mark.parametrize("dtype,vc", [(np.float64, 1), (np.float32, 2)])
+def test_setup_mask_on_device(dtype, vc):
+    """bcm_setup_mask writes the oracle's setup_mask exactly, and the masked
+    kernels with it equal the plain kernels bit for bit."""
+    blocks, n, _, A, b, _, rng = problem(dtype, 6)
+    case(f"setup_mask {np.dtype(dtype).name} vc={vc} nb={len(blocks)}")
+    nx0 = n + 2 * vc
+    shape = (len(blocks), nx0, nx0, nx0)
+    mesh = Mesh(blocks, n, vc)
+    gm = mesh.field(dtype, rng.standard_normal(shape).astype(dtype))
+    mesh.setup_mask(gm)
+    np.testing.assert_array_equal(gm.download(), oracle.setup_mask(shape, vc, dtype))
+    x = rng.standard_normal(shape).astype(dtype)
+    out0 = rng.standard_normal(shape).astype(dtype)
+    gx = mesh.field(dtype, x)
+    o1, o2 = mesh.field(dtype, out0), mesh.field(dtype, out0)
+    mesh.axpy(o1, gx, 0.7)
+    mesh.axpy(o2, gx, 0.7, mask=gm)
+    np.testing.assert_array_equal(o1.download(), o2.download())
+    assert mesh.dot(gx, gx) == mesh.dot(gx, gx, mask=gm)

@@ -21,14 +21,6 @@ from bcmtools_amd import workloads as W
 from test_oracle import assemble, uniform_poisson_matrix
 
 
-def setup_mask(shape, vc):
-    """setup_mask (blas.f90:700-751): 1 on interior cells, 0 on ghosts."""
-    m = np.zeros(shape)
-    n = shape[-1] - 2 * vc
-    m[:, vc:vc + n, vc:vc + n, vc:vc + n] = 1.0
-    return m
-
-
 def system(seed=1, n=6, vc=1):
     t = W.Tree(root_n=(1, 1, 1), divider=L.DIV_FLAT, max_level=1)
     blocks = t.blocks
@@ -57,7 +49,7 @@ def test_setup_mask_equals_unmasked():
     blocks, n, vc, A, b, x, y = system()
     out0 = np.random.default_rng(5).standard_normal(x.shape)
     plain = kernels(blocks, n, vc, A, b, x, y, out0, None)
-    masked = kernels(blocks, n, vc, A, b, x, y, out0, setup_mask(x.shape, vc))
+    masked = kernels(blocks, n, vc, A, b, x, y, out0, oracle.setup_mask(x.shape, vc))
     for k in plain:
         np.testing.assert_array_equal(masked[k], plain[k], err_msg=k)
 
@@ -65,7 +57,7 @@ def test_setup_mask_equals_unmasked():
 def test_random_mask_writes_only_open_cells():
     blocks, n, vc, A, b, x, y = system(seed=2)
     rng = np.random.default_rng(3)
-    mask = setup_mask(x.shape, vc) * (rng.random(x.shape) < 0.6)
+    mask = oracle.setup_mask(x.shape, vc) * (rng.random(x.shape) < 0.6)
     out0 = rng.standard_normal(x.shape)
     plain = kernels(blocks, n, vc, A, b, x, y, out0, None)
     masked = kernels(blocks, n, vc, A, b, x, y, out0, mask)
@@ -82,7 +74,7 @@ def test_random_mask_writes_only_open_cells():
 def test_bicgstab_mask_with_setup_mask_equals_bicgstab():
     blocks, n, vc, A, b, _, _ = system(seed=4, n=8)
     runs = []
-    for mask in (None, setup_mask(b.shape, vc)):
+    for mask in (None, oracle.setup_mask(b.shape, vc)):
         x = np.zeros_like(b)
         work = [np.zeros_like(b) for _ in range(9)]
         st = oracle.bicgstab(blocks, n, vc, x, A, b, work, 0.8, 2, 200, 1e-10, mask=mask)
@@ -100,7 +92,7 @@ def test_bicgstab_mask_solves_the_open_cells():
     A, b = assemble(blocks, n, vc, -f, [0.0])
     m = 16
     h = 1.0 / m
-    mask = setup_mask(b.shape, vc)
+    mask = oracle.setup_mask(b.shape, vc)
     gidx = np.full(b.shape, -1, dtype=np.int64)
     for bk, bi in enumerate(blocks):
         o = np.rint(bi["origin"] / (n * h)).astype(int)
