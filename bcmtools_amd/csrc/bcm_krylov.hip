@@ -229,8 +229,10 @@ struct Compact {
 // arithmetic; c receives the ghost cell (i, j, k).  X faces p = j, q = k;
 // Y faces p = i, q = k; Z faces p = i, q = j.  Faces without a neighbour
 // (outer boundaries) give 0: their folded coefficient is 0.
-template <typename T, int N>
-__device__ __forceinline__ T frame_value(const Compact<T, N>& src, const int* df, int f, int p, int q, int (&c)[3]) {
+// src is any accessor with at(blk, i, j, k) and, for the X faces, x_at()
+// returning T (Compact, or the on-the-fly sweep values of SweepOne).
+template <typename T, int N, typename S>
+__device__ __forceinline__ T frame_value(const S& src, const int* df, int f, int p, int q, int (&c)[3]) {
   constexpr int h = N / 2;
   const int ax = f >> 1;
   const bool plus = f & 1;
@@ -541,14 +543,15 @@ __global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) vo
   }
 }
 
-// k_kry_spmv for N <= 16 with the ghosted block staged in LDS.
-template <typename T, int N, int NDOT, int W>
+// k_kry_spmv for N <= 16 with the ghosted block staged in LDS; with OX the
+// column export of `out` too (q, read by the fused update kernels).
+template <typename T, int N, int NDOT, int W, bool OX = false>
 __global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) void k_kry_spmv_l(T* __restrict__ out, const double* __restrict__ g,
                                                   const double* __restrict__ gx, const T* __restrict__ y,
                                                   const BlockMeta* __restrict__ meta,
                                                   const int* __restrict__ blk_seg, const SegState* __restrict__ st,
                                                   double* __restrict__ parts, const int* __restrict__ blist,
-                                                  const int* __restrict__ desc) {
+                                                  const int* __restrict__ desc, T* __restrict__ ox = nullptr) {
   using B = Blk<N>;
   using G = Gh<N>;
   constexpr int PER = B::N3 / W;
@@ -561,6 +564,7 @@ __global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) vo
   __syncthreads();
   const BlockMeta me = meta[b];
   double d0 = 0.0, d1 = 0.0;
+  T ov[OX ? PER : 1];
 #pragma unroll
   for (int m = 0; m < PER; ++m) {
     int i, j, k;
@@ -570,13 +574,175 @@ __global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) vo
     const T v = T(ax_vals(a, lin[a0 - 1], lin[a0 + 1], lin[a0 - G::NX], lin[a0 + G::NX], lin[a0 - G::NX * G::NX],
                           lin[a0 + G::NX * G::NX], lin[a0]));
     out[cb + threadIdx.x + W * m] = v;
+    if (OX) ov[m] = v;
     d0 += double(v) * double(y[cb + threadIdx.x + W * m]);
     if (NDOT == 2) d1 += double(v) * double(v);
+  }
+  if (OX) {
+    T* xs = reinterpret_cast<T*>(lin);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      int i, j, k;
+      B::ijk(threadIdx.x + W * m, i, j, k);
+      const int col = export_col<N>(i);
+      if (col >= 0) xs[(col * N + k) * N + j] = ov[m];
+    }
+    __syncthreads();
+    T* dst = ox + (long long)b * 4 * N * N;
+    for (int e = threadIdx.x; e < 4 * N * N; e += W) dst[e] = xs[e];
   }
   wg_sum2n<W / 64>(d0, d1, red);
   if (threadIdx.x == 0) {
     parts[2 * b] = d0;
     parts[2 * b + 1] = d1;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Two-sweep preconditioner fused into the vector update (N <= 16,
+// count_pre == 2).  The update kernel of p (or s) computes the new vector,
+// its first Jacobi sweep (pointwise: the start is zero) and, in the same
+// pass, the second sweep, which needs the first sweep's values on the
+// block's ghost frame.  Those are functions of the neighbours' r, p, q
+// (s: r, q), read from the stored vectors (Y/Z faces) and their column
+// exports (X faces), so each frame value is recomputed exactly as the
+// neighbour's own kernel computes it and then combined with the
+// exchange's arithmetic.  This removes the sweep scratch vector and the
+// separate sweep kernel.  p is read by the neighbours while it is
+// replaced, so it alternates between two buffers (the iteration count's
+// parity picks the one holding the current p); s is not read by them.
+// ---------------------------------------------------------------------
+// The update of the vector at one cell, as k_kry_p / k_kry_s write it:
+// p = r (first) or r + c1 (p + c2 q) with c1 = beta, c2 = -gamma;
+// s = c1 q + r with c1 = -alpha.
+template <typename T, bool IS_P>
+__device__ __forceinline__ T updated(bool first, T c1, T c2, T rv, T pv, T qv) {
+  if (IS_P) return first ? rv : rv + c1 * (pv + c2 * qv);
+  return c1 * qv + rv;
+}
+
+// First-sweep value of the updated vector at any cell of any block.
+template <typename T, int N, bool IS_P>
+struct SweepOne {
+  const T* __restrict__ r;
+  const T* __restrict__ rx;
+  const T* __restrict__ p;  // IS_P only
+  const T* __restrict__ px;
+  const T* __restrict__ q;
+  const T* __restrict__ qx;
+  const BlockMeta* __restrict__ meta;
+  T c1, c2, om;
+  bool first;
+  __device__ __forceinline__ T sweep(int blk, int i, int j, int k, T v) const {
+    const Coef7<T> a = poisson_coef<T>(meta[blk], i, j, k, N);
+    return first_sweep(v, a.p, om);
+  }
+  __device__ __forceinline__ T at(int blk, int i, int j, int k) const {
+    const long long o = (long long)blk * (N * N * N) + i + N * (j + N * k);
+    return sweep(blk, i, j, k, updated<T, IS_P>(first, c1, c2, r[o], IS_P ? p[o] : T(0), q[o]));
+  }
+  // i must be 0, 1, N-2 or N-1.
+  __device__ __forceinline__ T x_at(int blk, int i, int j, int k) const {
+    const int col = i < 2 ? i : i - (N - 4);
+    const long long e = ((long long)(blk * 4 + col) * N + k) * N + j;
+    return sweep(blk, i, j, k, updated<T, IS_P>(first, c1, c2, rx[e], IS_P ? px[e] : T(0), qx[e]));
+  }
+};
+
+// v0/v1: p's two buffers (IS_P; read v[par], write v[1 - par]) or s (v0);
+// vx0/vx1 their column exports.  g, gx: the preconditioned vector (FP64)
+// and its export, as k_kry_jacobi_l writes them.
+template <typename T, int N, int W, bool IS_P>
+__global__ __launch_bounds__(W) __attribute__((amdgpu_waves_per_eu(W / 128))) void k_kry_upre2(
+    T* __restrict__ v0, T* __restrict__ v1, T* __restrict__ vx0, T* __restrict__ vx1, const T* __restrict__ r,
+    const T* __restrict__ rx, const T* __restrict__ q, const T* __restrict__ qx, T* __restrict__ r0,
+    double* __restrict__ g, double* __restrict__ gx, const BlockMeta* __restrict__ meta,
+    const int* __restrict__ blk_seg, const SegState* __restrict__ st, double omega, const int* __restrict__ blist,
+    const int* __restrict__ desc) {
+  using B = Blk<N>;
+  using G = Gh<N>;
+  constexpr int PER = B::N3 / W;
+  constexpr int LB = G::V * int(sizeof(T));
+  constexpr int XB = 4 * N * N * int(sizeof(double) + sizeof(T));
+  __shared__ __align__(16) char raw[LB > XB ? LB : XB];
+  T* lin = reinterpret_cast<T*>(raw);
+  const int b = blist[blockIdx.x];
+  const SegState* sp = st + blk_seg[b];
+  if (sp->status != BCM_SOLVE_ACTIVE) return;
+  const int par = IS_P ? (sp->iters & 1) : 0;
+  const T* vin = par ? v1 : v0;
+  T* vout = IS_P ? (par ? v0 : v1) : v0;
+  const T* vxin = par ? vx1 : vx0;
+  T* vxout = par ? vx0 : vx1;
+  const bool first = IS_P && sp->first != 0;
+  const bool reset = IS_P && sp->reset_r0 != 0;
+  T c1, c2 = T(0);
+  if (IS_P) {
+    const double beta = first ? 0.0 : sp->rho / sp->rho_prev * sp->alpha / sp->gamma;
+    c1 = T(beta);
+    c2 = T(-sp->gamma);
+  } else {
+    c1 = T(-sp->alpha);
+  }
+  const T om = T(omega);
+  const SweepOne<T, N, IS_P> src{r, rx, vin, vxin, q, qx, meta, c1, c2, om, first};
+  const BlockMeta me = meta[b];
+  const long long cb = (long long)b * B::N3;
+  // The frame first, then the own cells: the update (kept for the second
+  // sweep's right-hand side) and its first sweep into LDS.
+  const int* d = desc + b * 48;
+  for (int e = threadIdx.x; e < 6 * N * N; e += W) {
+    const int f = e / (N * N), pq = e % (N * N);
+    int c[3];
+    const T val = frame_value<T, N>(src, d + f * 8, f, pq % N, pq / N, c);
+    lin[G::at(c[0], c[1], c[2])] = val;
+  }
+  T vv[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const long long o = cb + threadIdx.x + W * m;
+    const T rv = r[o];
+    vv[m] = updated<T, IS_P>(first, c1, c2, rv, IS_P ? vin[o] : T(0), q[o]);
+    vout[o] = vv[m];
+    if (reset) r0[o] = rv;
+    int i, j, k;
+    B::ijk(threadIdx.x + W * m, i, j, k);
+    const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+    lin[G::at(i, j, k)] = first_sweep(vv[m], a.p, om);
+  }
+  __syncthreads();
+  // Second sweep (jacobi_smoother with b = the updated vector).
+  double out[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    int i, j, k;
+    B::ijk(threadIdx.x + W * m, i, j, k);
+    const int a0 = G::at(i, j, k);
+    const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+    out[m] = double(jacobi_vals(a, lin[a0 - 1], lin[a0 + 1], lin[a0 - G::NX], lin[a0 + G::NX],
+                                lin[a0 - G::NX * G::NX], lin[a0 + G::NX * G::NX], lin[a0], vv[m], om));
+    g[cb + threadIdx.x + W * m] = out[m];
+  }
+  __syncthreads();
+  // Column exports: the preconditioned vector and, for p, the new p.
+  double* xg = reinterpret_cast<double*>(raw);
+  T* xv = reinterpret_cast<T*>(raw + 4 * N * N * sizeof(double));
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    int i, j, k;
+    B::ijk(threadIdx.x + W * m, i, j, k);
+    const int col = export_col<N>(i);
+    if (col >= 0) {
+      xg[(col * N + k) * N + j] = out[m];
+      if (IS_P) xv[(col * N + k) * N + j] = vv[m];
+    }
+  }
+  __syncthreads();
+  const long long xb = (long long)b * 4 * N * N;
+  for (int e = threadIdx.x; e < 4 * N * N; e += W) {
+    gx[xb + e] = xg[e];
+    if (IS_P) vxout[xb + e] = xv[e];
   }
 }
 
@@ -625,15 +791,17 @@ __global__ __launch_bounds__(KWG) void k_kry_spmv(T* __restrict__ out, const dou
 
 // dx += alpha p_ + gamma s_  [AXPBYPZ(x, p_, s_, alpha, gamma)] in FP64;
 // r = s - gamma t  [AXPYZ(r, t_, s, -gamma)] in T; partial (r, r), (r, r0).
-template <typename T, int N>
+// With RX, also r's column export (read by the fused update kernels).
+template <typename T, int N, bool RX = false>
 __global__ __launch_bounds__(KWG) void k_kry_xr(double* __restrict__ dx, const double* __restrict__ gp,
                                                 const double* __restrict__ gs, T* __restrict__ r,
                                                 const T* __restrict__ s, const T* __restrict__ t,
                                                 const T* __restrict__ r0, const int* __restrict__ blk_seg,
                                                 const SegState* __restrict__ st, double* __restrict__ parts,
-    const int* __restrict__ blist) {
+    const int* __restrict__ blist, T* __restrict__ rx = nullptr) {
   using B = Blk<N>;
   __shared__ double red[8];
+  __shared__ T xs[RX ? 4 * N * N : 1];
   const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE) return;
@@ -647,9 +815,16 @@ __global__ __launch_bounds__(KWG) void k_kry_xr(double* __restrict__ dx, const d
     dx[o] = (al * gp[o] + ga * gs[o]) + dx[o];
     const T rv = tmg * t[o] + s[o];
     r[o] = rv;
+    if (RX) {
+      int i, j, k;
+      B::ijk(c, i, j, k);
+      const int col = export_col<N>(i);
+      if (col >= 0) xs[(col * N + k) * N + j] = rv;
+    }
     d0 += double(rv) * double(rv);
     d1 += double(rv) * double(r0[o]);
   }
+  if (RX) export_flush<T, N>(xs, rx, b);
   wg_sum2(d0, d1, red);
   if (threadIdx.x == 0) {
     parts[2 * b] = d0;
@@ -681,14 +856,16 @@ __global__ __launch_bounds__(KWG) void k_ref_x(double* __restrict__ x, double* _
 
 // r = T(b - A x) in FP64 (calc_r).  Partials: FP64 (r, r) of the
 // unrounded residual and (r, r0) of the rounded one.
-template <typename T, int N>
+// With RX, also r's column export.
+template <typename T, int N, bool RX = false>
 __global__ __launch_bounds__(KWG) void k_ref_r(T* __restrict__ r, const T* __restrict__ r0,
                                                const double* __restrict__ x, const double* __restrict__ b64,
                                                const BlockMeta* __restrict__ meta, const int* __restrict__ blk_seg,
                                                const SegState* __restrict__ st, double* __restrict__ parts,
-    const int* __restrict__ blist) {
+    const int* __restrict__ blist, T* __restrict__ rx = nullptr) {
   using B = Blk<N>;
   __shared__ double red[8];
+  __shared__ T xs[RX ? 4 * N * N : 1];
   const int b = blist[blockIdx.x];
   const SegState* sp = st + blk_seg[b];
   if (sp->status != BCM_SOLVE_ACTIVE || !sp->refresh) return;
@@ -704,9 +881,14 @@ __global__ __launch_bounds__(KWG) void k_ref_r(T* __restrict__ r, const T* __res
     const double rv = r_cell(a, x, gb + B::g(i, j, k), B::NX, B::NX * B::NX, b64[o]);
     const T rt = T(rv);
     r[o] = rt;
+    if (RX) {
+      const int col = export_col<N>(i);
+      if (col >= 0) xs[(col * N + k) * N + j] = rt;
+    }
     acc += rv * rv;
     d1 += double(rt) * double(r0[o]);
   }
+  if (RX) export_flush<T, N>(xs, rx, b);
   wg_sum2(acc, d1, red);
   if (threadIdx.x == 0) {
     parts[2 * b] = acc;
@@ -908,6 +1090,12 @@ struct Krylov {
   double* dx = nullptr;     // compact FP64 correction since the last reliable update
   void* g1 = nullptr;       // compact sweep scratch, sweep precision
   void* x1 = nullptr;       // its column export
+  // Fused two-sweep path (N <= 16, count_pre == 2): p's second buffer and
+  // the column exports of r, q and both p buffers, sweep precision.
+  void* p2 = nullptr;
+  void* rxe = nullptr;
+  void* qxe = nullptr;
+  void* pxe[2] = {};
   double* g[2] = {};        // compact FP64 preconditioned vectors p_, s_ (SpMV inputs)
   double* xg[2] = {};       // their column exports
   double* b64 = nullptr;    // folded right-hand side, compact
@@ -932,6 +1120,10 @@ static void krylov_free_vectors(Krylov* k) {
   dev_free(k->g1);
   dev_free(k->x1);
   k->g1 = k->x1 = nullptr;
+  for (void** p : {&k->p2, &k->rxe, &k->qxe, &k->pxe[0], &k->pxe[1]}) {
+    dev_free(*p);
+    *p = nullptr;
+  }
   k->esize = 0;
 }
 
@@ -1000,12 +1192,15 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
     }
     BCM_TRY(dev_alloc(&k->g1, nc * esize));
     BCM_TRY(dev_alloc(&k->x1, nx * esize));
-    if (esize == 8) {
-      BCM_TRY(zero_async(m, static_cast<double*>(k->g1), nc));
-      BCM_TRY(zero_async(m, static_cast<double*>(k->x1), nx));
-    } else {
-      BCM_TRY(zero_async(m, static_cast<float*>(k->g1), nc));
-      BCM_TRY(zero_async(m, static_cast<float*>(k->x1), nx));
+    BCM_TRY(dev_alloc(&k->p2, nc * esize));
+    for (void** p : {&k->rxe, &k->qxe, &k->pxe[0], &k->pxe[1]}) BCM_TRY(dev_alloc(p, nx * esize));
+    for (void* p : {k->g1, k->p2}) {
+      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nc));
+      else BCM_TRY(zero_async(m, static_cast<float*>(p), nc));
+    }
+    for (void* p : {k->x1, k->rxe, k->qxe, k->pxe[0], k->pxe[1]}) {
+      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nx));
+      else BCM_TRY(zero_async(m, static_cast<float*>(p), nx));
     }
     k->esize = esize;
   }
@@ -1140,9 +1335,63 @@ static int update_and_first_sweep(bcm_mesh* m, T* v, const T* r, const T* q, T* 
   return BCM_OK;
 }
 
+// Whether the iteration uses the fused two-sweep update kernels.
+template <int N>
+static constexpr bool fused_pre2_ok() {
+  return N <= 16;
+}
+static bool fused_pre2(int n, int pre) {
+  return n <= 16 && pre == 2;
+}
+
+// One iteration of the fused path: p and s updates carry both
+// preconditioner sweeps; q and r carry their column exports.
+template <typename T, int N>
+static int iteration_fused(bcm_mesh* m, double* x, double omega) {
+  Krylov* K = m->kry;
+  T *r = static_cast<T*>(K->v[0]), *r0 = static_cast<T*>(K->v[1]), *p = static_cast<T*>(K->v[2]),
+    *q = static_cast<T*>(K->v[3]), *s = static_cast<T*>(K->v[4]), *t = static_cast<T*>(K->v[5]);
+  T *p2 = static_cast<T*>(K->p2), *rx = static_cast<T*>(K->rxe), *qx = static_cast<T*>(K->qxe);
+  T *px0 = static_cast<T*>(K->pxe[0]), *px1 = static_cast<T*>(K->pxe[1]);
+  double *gp = K->g[0], *gs = K->g[1], *xp = K->xg[0], *xs = K->xg[1];
+  const dim3 grid(K->n_list), blk(KWG);
+  const int* bs = m->d_blk_seg;
+  const int* bl = K->list;
+  SegState* st = K->st;
+  constexpr int WJ = kwg_jac<N>(), WS = kwg_spmv<N>();
+  hipLaunchKernelGGL((k_kry_upre2<T, N, WJ, true>), grid, dim3(WJ), 0, m->stream, p, p2, px0, px1, r, rx, q, qx, r0,
+                     gp, xp, m->d_meta, bs, st, omega, bl, m->d_desc);
+  BCM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((k_kry_spmv_l<T, N, 1, WS, true>), grid, dim3(WS), 0, m->stream, q, gp, xp, r0, m->d_meta, bs,
+                     st, K->parts, bl, m->d_desc, qx);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_Q>(m));
+  hipLaunchKernelGGL((k_kry_upre2<T, N, WJ, false>), grid, dim3(WJ), 0, m->stream, s, static_cast<T*>(nullptr),
+                     static_cast<T*>(nullptr), static_cast<T*>(nullptr), r, rx, q, qx, static_cast<T*>(nullptr), gs,
+                     xs, m->d_meta, bs, st, omega, bl, m->d_desc);
+  BCM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((k_kry_spmv_l<T, N, 2, WS>), grid, dim3(WS), 0, m->stream, t, gs, xs, s, m->d_meta, bs, st,
+                     K->parts, bl, m->d_desc, static_cast<T*>(nullptr));
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_T>(m));
+  hipLaunchKernelGGL((k_kry_xr<T, N, true>), grid, blk, 0, m->stream, K->dx, gp, gs, r, s, t, r0, bs, st, K->parts,
+                     bl, rx);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY(seg_reduce<PH_R>(m));
+  hipLaunchKernelGGL((k_ref_x<N>), grid, blk, 0, m->stream, x, K->dx, bs, st, bl);
+  BCM_HIP_TRY(hipGetLastError());
+  BCM_TRY((gather<double, N>(m, x, 2)));
+  hipLaunchKernelGGL((k_ref_r<T, N, true>), grid, blk, 0, m->stream, r, r0, x, K->b64, m->d_meta, bs, st, K->parts,
+                     bl, rx);
+  BCM_HIP_TRY(hipGetLastError());
+  return seg_reduce<PH_F>(m);
+}
+
 // One BiCGSTAB iteration plus the (gated) reliable update.
 template <typename T, int N>
 static int iteration(bcm_mesh* m, double* x, int pre, double omega) {
+  if constexpr (fused_pre2_ok<N>())
+    if (pre == 2) return iteration_fused<T, N>(m, x, omega);
   Krylov* K = m->kry;
   T *r = static_cast<T*>(K->v[0]), *r0 = static_cast<T*>(K->v[1]), *p = static_cast<T*>(K->v[2]),
     *q = static_cast<T*>(K->v[3]), *s = static_cast<T*>(K->v[4]), *t = static_cast<T*>(K->v[5]);
@@ -1253,8 +1502,13 @@ static int begin_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve
   BCM_TRY(set_list(m, true));
   // Initial residual r = b - A x; the first pass sets r0 = r.
   BCM_TRY((gather<double, N>(m, x, 2)));
-  hipLaunchKernelGGL((k_ref_r<T, N>), dim3(K->n_list), blk, 0, m->stream, static_cast<T*>(K->v[0]),
-                     static_cast<const T*>(K->v[1]), x, K->b64, m->d_meta, m->d_blk_seg, K->st, K->parts, K->list);
+  if (fused_pre2(N, P.count_pre))
+    hipLaunchKernelGGL((k_ref_r<T, N, fused_pre2_ok<N>()>), dim3(K->n_list), blk, 0, m->stream,
+                       static_cast<T*>(K->v[0]), static_cast<const T*>(K->v[1]), x, K->b64, m->d_meta, m->d_blk_seg,
+                       K->st, K->parts, K->list, static_cast<T*>(K->rxe));
+  else
+    hipLaunchKernelGGL((k_ref_r<T, N>), dim3(K->n_list), blk, 0, m->stream, static_cast<T*>(K->v[0]),
+                       static_cast<const T*>(K->v[1]), x, K->b64, m->d_meta, m->d_blk_seg, K->st, K->parts, K->list);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(seg_reduce<PH_F>(m));
   Run& R = K->run;

@@ -344,9 +344,18 @@ def engine_bytes_per_cell(es, pre, n):
     """Algorithmic bytes per interior cell of one iteration of the fused
     BiCGSTAB engine with vectors of `es` bytes (FP64: 8) and `pre` >= 1 Jacobi
     sweeps: every array read once and written once; a stencil input adds its
-    4-column export (4/n of a vector)."""
+    4-column export (4/n of a vector).  With two sweeps and n <= 16 both
+    sweeps run inside the p and s updates, which read the X-face frame from
+    the exports of r, q and p."""
     xf = 4.0 / n
     d = 8.0
+    if pre == 2 and n <= 16:
+        upd_p = 3 * es * (1 + xf) + es * (1 + xf) + d * (1 + xf)  # r, q, p + exports -> p + export, p_ + export
+        upd_s = 2 * es * (1 + xf) + es + d * (1 + xf)  # r, q + exports -> s, s_ + export
+        spmv_q = d * (1 + xf) + es + es * (1 + xf)  # p_ + export, r0 -> q + export
+        spmv_t = d * (1 + xf) + es + es  # s_ + export, s -> t
+        xr = 3 * d + 3 * es + d + es * (1 + xf)  # dx, p_, s_, s, t, r0 -> dx, r + export
+        return upd_p + upd_s + spmv_q + spmv_t + xr
     upd_p = 3 * es + es + es * (1 + xf)  # r, q, p -> p, first sweep + export
     upd_s = 2 * es + es + es * (1 + xf)  # r, q -> s, first sweep + export
     sweep = es * (1 + xf) + es + es * (1 + xf)  # input + export, rhs -> output + export
