@@ -184,7 +184,7 @@ int krylov_begin(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_para
 int krylov_advance(bcm_mesh* m, bool wait, bool* active);
 int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats);
 // Device bytes the fused solver allocates for a mesh of nblocks n^3 blocks.
-size_t krylov_bytes(int nblocks, int n, int precision);
+size_t krylov_bytes(int nblocks, int n, int precision, int count_pre);
 // Field-sized workspace buffer kept on the mesh (bcm_solver.hip).
 void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc);
 enum WorkspaceSlot { WS_SOR_PONG_F64 = 30, WS_SOR_PONG_F32 = 31 };

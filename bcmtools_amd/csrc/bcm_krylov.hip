@@ -1086,12 +1086,14 @@ struct Krylov {
   Run run;
   hipEvent_t ev = nullptr;  // the state mirror is valid once this completes
   int esize = 0;            // bytes per value of the sweep-precision vectors (4 or 8)
+  bool fused = false;       // vectors allocated for the fused two-sweep path
   void* v[6] = {};          // compact, sweep precision: r, r0, p, q, s, t
   double* dx = nullptr;     // compact FP64 correction since the last reliable update
-  void* g1 = nullptr;       // compact sweep scratch, sweep precision
-  void* x1 = nullptr;       // its column export
+  // Unfused path: compact sweep scratch and its column export.
+  void* g1 = nullptr;
+  void* x1 = nullptr;
   // Fused two-sweep path (N <= 16, count_pre == 2): p's second buffer and
-  // the column exports of r, q and both p buffers, sweep precision.
+  // the column exports of r, q and both p buffers.  All sweep precision.
   void* p2 = nullptr;
   void* rxe = nullptr;
   void* qxe = nullptr;
@@ -1125,6 +1127,7 @@ static void krylov_free_vectors(Krylov* k) {
     *p = nullptr;
   }
   k->esize = 0;
+  k->fused = false;
 }
 
 static void krylov_drop_graph(Krylov* k) {
@@ -1161,7 +1164,7 @@ static int zero_async(bcm_mesh* m, T* p, size_t count) {
   return BCM_OK;
 }
 
-static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
+static int krylov_get(bcm_mesh* m, int esize, bool fused, Krylov** out) {
   const size_t nc = m->ncell, nx = size_t(m->nblocks) * 4 * m->n * m->n;
   if (!m->kry) {
     Krylov* k = new (std::nothrow) Krylov();
@@ -1181,7 +1184,7 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
     }
   }
   Krylov* k = m->kry;
-  if (k->esize != esize) {
+  if (k->esize != esize || k->fused != fused) {
     BCM_HIP_TRY(hipStreamSynchronize(m->stream));
     krylov_drop_graph(k);
     krylov_free_vectors(k);
@@ -1190,19 +1193,26 @@ static int krylov_get(bcm_mesh* m, int esize, Krylov** out) {
       if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nc));
       else BCM_TRY(zero_async(m, static_cast<float*>(p), nc));
     }
-    BCM_TRY(dev_alloc(&k->g1, nc * esize));
-    BCM_TRY(dev_alloc(&k->x1, nx * esize));
-    BCM_TRY(dev_alloc(&k->p2, nc * esize));
-    for (void** p : {&k->rxe, &k->qxe, &k->pxe[0], &k->pxe[1]}) BCM_TRY(dev_alloc(p, nx * esize));
-    for (void* p : {k->g1, k->p2}) {
-      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nc));
-      else BCM_TRY(zero_async(m, static_cast<float*>(p), nc));
+    std::vector<void**> full, cols;
+    if (fused) {
+      full = {&k->p2};
+      cols = {&k->rxe, &k->qxe, &k->pxe[0], &k->pxe[1]};
+    } else {
+      full = {&k->g1};
+      cols = {&k->x1};
     }
-    for (void* p : {k->x1, k->rxe, k->qxe, k->pxe[0], k->pxe[1]}) {
-      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(p), nx));
-      else BCM_TRY(zero_async(m, static_cast<float*>(p), nx));
+    for (void** p : full) {
+      BCM_TRY(dev_alloc(p, nc * esize));
+      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(*p), nc));
+      else BCM_TRY(zero_async(m, static_cast<float*>(*p), nc));
+    }
+    for (void** p : cols) {
+      BCM_TRY(dev_alloc(p, nx * esize));
+      if (esize == 8) BCM_TRY(zero_async(m, static_cast<double*>(*p), nx));
+      else BCM_TRY(zero_async(m, static_cast<float*>(*p), nx));
     }
     k->esize = esize;
+    k->fused = fused;
   }
   *out = k;
   return BCM_OK;
@@ -1455,7 +1465,7 @@ template <typename T, int N>
 static int begin_impl(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& P, const double* seg_tol) {
   static_assert(Blk<N>::N3 % KWG == 0, "block volume must be a multiple of the work-group");
   Krylov* K = nullptr;
-  BCM_TRY(krylov_get(m, int(sizeof(T)), &K));
+  BCM_TRY(krylov_get(m, int(sizeof(T)), fused_pre2(N, P.count_pre), &K));
   const dim3 grid(m->nblocks), blk(KWG);
   // Defaults measured on the shell meshes of levels 4-7 (16^3 blocks): an
   // FP64 update each time the FP32 recurrence has gained three digits or
@@ -1648,12 +1658,15 @@ int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg
   return BCM_OK;
 }
 
-size_t krylov_bytes(int nblocks, int n, int precision) {
+size_t krylov_bytes(int nblocks, int n, int precision, int count_pre) {
   const size_t nc = size_t(nblocks) * n * n * n, nx = size_t(nblocks) * 4 * n * n;
   const size_t es = precision == BCM_PREC_FP64 ? 8 : 4;
   // b, dx, p_, s_ (FP64 compact) + the exports of p_, s_; r, r0, p, q, s, t
-  // and the sweep scratch (compact) + its export; partials, list.
-  return 4 * nc * 8 + 2 * nx * 8 + 7 * nc * es + nx * es + size_t(nblocks) * 20;
+  // (compact); then either p's second buffer and the exports of r, q and
+  // both p buffers (fused two-sweep path) or the sweep scratch and its
+  // export; partials, list.
+  const size_t path = fused_pre2(n, count_pre) ? nc * es + 4 * nx * es : nc * es + nx * es;
+  return 4 * nc * 8 + 2 * nx * 8 + 6 * nc * es + path + size_t(nblocks) * 20;
 }
 
 }  // namespace bcm

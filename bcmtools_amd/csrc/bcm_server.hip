@@ -103,7 +103,7 @@ namespace {
 size_t request_bytes(const bcm_server* sv, int nblocks, int precision) {
   const int n = sv->p.n;
   const size_t ng = size_t(nblocks) * (n + 2) * (n + 2) * (n + 2);
-  return size_t(nblocks) * 640 + 2 * ng * 8 + krylov_bytes(nblocks, n, precision);
+  return size_t(nblocks) * 640 + 2 * ng * 8 + krylov_bytes(nblocks, n, precision, sv->p.count_pre);
 }
 
 void release(bcm_server* sv, Request* r) {

@@ -164,3 +164,29 @@ def test_engine_stops_at_the_attainable_accuracy(precision):
     rr = oracle_relres(blocks, 16, gx.download(), rhs)
     assert rr <= 1e-10, (rr, st)
     assert abs(rr - st["relres"]) <= 0.1 * rr, (rr, st)
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_engine_preconditioner_paths_on_one_mesh(n):
+    """count_pre = 2 runs both sweeps inside the p and s updates (frames
+    recomputed from r, p, q and their column exports); other counts keep the
+    separate sweep kernels, with differently allocated solver state.  Solving
+    on one mesh with 2, then 3, then 1, then 2 sweeps gives each time the
+    bits of a fresh mesh solved the same way, and every solution meets its
+    tolerance by the oracle's residual."""
+    blocks, rhs = _problem(3, n, 15)
+    case(f"engine preconditioner paths n={n} nb={len(blocks)}")
+    mesh = Mesh(blocks, n, 1)
+    grhs = mesh.field(np.float64, rhs)
+    for pre in (2, 3, 1, 2):
+        for precision in ("fp64", "mixed"):
+            gx = mesh.field(np.float64)
+            st = mesh.solve(gx, grhs, [0.0], precision=precision, tol=1e-10, count_pre=pre, check_every=5)[0]
+            fresh = Mesh(blocks, n, 1)
+            fx = fresh.field(np.float64)
+            fst = fresh.solve(fx, fresh.field(np.float64, rhs), [0.0], precision=precision, tol=1e-10,
+                              count_pre=pre, check_every=5)[0]
+            assert st == fst, (pre, precision, st, fst)
+            np.testing.assert_array_equal(gx.download(), fx.download())
+            assert st["status"] == "converged", (pre, precision, st)
+            assert oracle_relres(blocks, n, gx.download(), rhs) <= 1e-10 * (1 + 1e-6)
