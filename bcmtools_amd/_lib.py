@@ -28,6 +28,7 @@ BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z, BCSET_PER_FACE = 
 GF_FLAT, GF_F2C, GF_C2F, GF_ALL = 1, 2, 4, 7
 GF_X, GF_Y, GF_Z = 8, 16, 32
 GF_BLOCKWISE = 64
+VC_AT_ONCE, VC_SEPARATE_XYZ, VC_SEPARATE_XYZ_SEPARATE_LEVEL_DIFF = 0, 1, 2
 SOR_AUTO, SOR_FUSED_TASKS, SOR_FUSED_GATHER, SOR_HALO, SOR_COLOUR_TASKS, SOR_NUM_VARIANTS = range(6)
 PREC_FP64, PREC_MIXED, PREC_FP32 = range(3)
 SOLVE_ACTIVE, SOLVE_CONVERGED, SOLVE_NOCONV, SOLVE_FAILED = range(4)
@@ -217,6 +218,8 @@ SIGNATURES = {
     "bcm_axpbypz_mask": (_I, [_P, _P, _P, _D, _D, _P]),
     "bcm_copy_mask": (_I, [_P, _P, _P]),
     "bcm_setup_mask": (_I, [_P]),
+    "bcm_update_virtual_cells": (_I, [_P, _I]),
+    "bcm_impose_boundary_condition": (_I, [_P, _I, _DP, _I]),
     "bcm_jacobi_solve_mask": (_I, [_P, ctypes.POINTER(_P), _P, _P, _P, ctypes.POINTER(KrylovParams), _DP, _I, _IP,
                                    _DP]),
     "bcm_bicgstab_mask": (_I, [_P, ctypes.POINTER(_P), _P, ctypes.POINTER(_P), _P, ctypes.POINTER(KrylovParams), _DP,

@@ -245,6 +245,18 @@ class Mesh:
         """updateVC_{X,Y,Z}[_{Flat,F2C,C2F}] (include/BlockManager.h:327-343)."""
         self.update_vc(f, kinds | (L.GF_X << axis))
 
+    def update_virtual_cells(self, f, method=L.VC_AT_ONCE):
+        """LocalScalar3D::UpdateVirtualCells (examples/SolverTest/LocalScalar3D.h:198-223)
+        with the field's VCUpdateMethod."""
+        L.check(self.lib.bcm_update_virtual_cells(f.handle, int(method)), "bcm_update_virtual_cells")
+
+    def impose_boundary_condition(self, f, values, method=L.VC_AT_ONCE):
+        """LocalScalar3D::ImposeBoundaryCondition (LocalScalar3D.h:225-257): the ghost
+        exchange, then the outer-face conditions."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        L.check(self.lib.bcm_impose_boundary_condition(f.handle, int(method), L.dptr(v), len(v)),
+                "bcm_impose_boundary_condition")
+
     def dirichlet_mirror(self, f, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
         L.check(self.lib.bcm_dirichlet_mirror(f.handle, L.dptr(v), len(v)), "bcm_dirichlet_mirror")

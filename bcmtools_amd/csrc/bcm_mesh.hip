@@ -745,6 +745,28 @@ extern "C" int bcm_impose_bc(bcm_field_t f, const double* values, int nvalues) {
   return impose_bc_raw<float>(m, static_cast<float*>(f->data));
 }
 
+extern "C" int bcm_update_virtual_cells(bcm_field_t f, int method) {
+  if (!f) return BCM_ERR_ARG;
+  switch (method) {
+    case BCM_VC_AT_ONCE:
+      return bcm_ghost_fill(f, BCM_GF_ALL);
+    case BCM_VC_SEPARATE_XYZ:
+      for (int ax : {BCM_GF_X, BCM_GF_Y, BCM_GF_Z}) BCM_TRY(bcm_ghost_fill(f, BCM_GF_ALL | ax));
+      return BCM_OK;
+    case BCM_VC_SEPARATE_XYZ_SEPARATE_LEVEL_DIFF:
+      for (int kind : {BCM_GF_F2C, BCM_GF_FLAT, BCM_GF_C2F})
+        for (int ax : {BCM_GF_X, BCM_GF_Y, BCM_GF_Z}) BCM_TRY(bcm_ghost_fill(f, kind | ax));
+      return BCM_OK;
+    default:
+      return BCM_ERR_ARG;
+  }
+}
+
+extern "C" int bcm_impose_boundary_condition(bcm_field_t f, int method, const double* values, int nvalues) {
+  BCM_TRY(bcm_update_virtual_cells(f, method));
+  return bcm_impose_bc(f, values, nvalues);
+}
+
 extern "C" int bcm_dirichlet_mirror(bcm_field_t f, const double* values, int nvalues) {
   if (!f) return BCM_ERR_ARG;
   bcm_mesh* m = f->mesh;

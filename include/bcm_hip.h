@@ -223,6 +223,19 @@ size_t bcm_field_bytes(bcm_field_t field);
  * (include/Scalar3DUpdater.h:321-347, 519-708). Face ghosts only. */
 int bcm_ghost_fill(bcm_field_t field, int mask);
 
+/* VCUpdateMethod (include/BlockManager.h:39-46). */
+#define BCM_VC_AT_ONCE 0
+#define BCM_VC_SEPARATE_XYZ 1
+#define BCM_VC_SEPARATE_XYZ_SEPARATE_LEVEL_DIFF 2
+/* LocalScalar3D::UpdateVirtualCells (examples/SolverTest/LocalScalar3D.h:
+ * 198-223): the ghost exchange in the method's order -- all at once; X, Y,
+ * then Z; or F2C on X, Y, Z, then same-level, then C2F.  Every exchange
+ * reads interiors only, so the three give the same bits. */
+int bcm_update_virtual_cells(bcm_field_t field, int method);
+/* LocalScalar3D::ImposeBoundaryCondition (LocalScalar3D.h:225-257):
+ * bcm_update_virtual_cells, then bcm_impose_bc with the face values. */
+int bcm_impose_boundary_condition(bcm_field_t field, int method, const double* values, int nvalues);
+
 /* Dirichlet mirror g = 2v - u0 on every outer face whose bc_type is
  * DIRICHLET, v = values[bc_id]: Solver::setBoundaryCondition /
  * setDirichletBoundaryInBlock (examples/sor1/Solver.cpp:202-294). */

@@ -134,3 +134,27 @@ def test_host_bicgstab_neumann_vs_oracle():
     assert rc == L.OK
     xi, ri = W.interior(gx.download(), vc), W.interior(x, vc)
     assert np.abs(xi - ri).max() <= 1e-8 * np.abs(ri).max()
+
+
+@pytest.mark.parametrize("method", [L.VC_AT_ONCE, L.VC_SEPARATE_XYZ, L.VC_SEPARATE_XYZ_SEPARATE_LEVEL_DIFF])
+@pytest.mark.parametrize("dtype,n,vc", [(np.float64, 8, 1), (np.float32, 8, 2), (np.float64, 16, 1)])
+def test_update_virtual_cells_and_impose_boundary_condition(method, dtype, n, vc):
+    """LocalScalar3D::UpdateVirtualCells / ImposeBoundaryCondition with each
+    VCUpdateMethod: the exchange equals the oracle's ghost fill, and with the
+    outer-face conditions after it the oracle's ghost fill + impose_bc, bit
+    for bit."""
+    blocks = bc_tree(TYPES, 3, True).blocks
+    case(f"update_virtual_cells method={method} n={n} vc={vc} {np.dtype(dtype).name} nb={len(blocks)}")
+    rng = np.random.default_rng(7 + method)
+    nx0 = n + 2 * vc
+    data = rng.standard_normal((len(blocks), nx0, nx0, nx0)).astype(dtype)
+    mesh = Mesh(blocks, n, vc)
+    f = mesh.field(dtype, data)
+    mesh.update_virtual_cells(f, method)
+    ref = data.copy()
+    oracle.ghost_fill(blocks, n, vc, ref)
+    np.testing.assert_array_equal(f.download(), ref)
+    g = mesh.field(dtype, data)
+    mesh.impose_boundary_condition(g, VALS, method)
+    oracle.impose_bc(blocks, n, vc, ref, VALS)
+    np.testing.assert_array_equal(g.download(), ref)
