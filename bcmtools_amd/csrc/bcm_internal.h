@@ -170,7 +170,8 @@ int dirichlet_device(bcm_mesh* m, bcm_field* f);
 // Halo-gathering red-black sweep A -> B (bcm_sweep.hip) and the X-face
 // column export it reads.
 template <typename T> int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout);
-template <typename T> int xexport_raw(bcm_mesh* m, const T* A, T* xout);
+template <typename T> int to_compact_raw(bcm_mesh* m, const T* f, T* c, T* xout);
+template <typename T> int from_compact_raw(bcm_mesh* m, const T* c, T* f);
 // Block-wise ghost fill (all faces, all kinds; n in {8, 16, 32}, vc = 1).
 template <typename T> int ghost_gather_raw(bcm_mesh* m, T* data);
 // Frees the fused solver's state (bcm_krylov.hip).
@@ -187,7 +188,8 @@ int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats);
 size_t krylov_bytes(int nblocks, int n, int precision, int count_pre);
 // Field-sized workspace buffer kept on the mesh (bcm_solver.hip).
 void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc);
-enum WorkspaceSlot { WS_SOR_PONG_F64 = 30, WS_SOR_PONG_F32 = 31 };
+// Compact iterates A, B and source of the halo-gathering SOR schedule.
+enum WorkspaceSlot { WS_SOR_CA_F64 = 30, WS_SOR_CB_F64, WS_SOR_CS_F64, WS_SOR_CA_F32, WS_SOR_CB_F32, WS_SOR_CS_F32 };
 
 enum BlasOp { OP_AXPY = 0, OP_XPAY = 1, OP_AXPYZ = 2, OP_AXPBYPZ = 3, OP_COPY_INTERIOR = 4 };
 enum ConvOp { CONV_D2F = 0, CONV_ADD_F2D = 1 };

@@ -628,9 +628,11 @@ extern "C" int bcm_rbsor_sweep(bcm_field_t f, bcm_field_t source, double omega) 
                           source ? static_cast<const float*>(source->data) : nullptr, omega);
 }
 
-// `iters` further outer iterations by the halo-gathering sweep, ping-ponging
-// between f's buffer and a mesh workspace buffer; f ends with the final
-// iterate (its device pointer may change) and complete ghosts.
+// `iters` further outer iterations by the halo-gathering sweep on compact
+// copies of the iterate (ping-pong between two workspace buffers) and of
+// the source term; f receives the final interior, then complete ghosts.
+// f's own ghost cells are not read meanwhile: every face of a mesh this
+// schedule runs on has a neighbour or a Dirichlet condition.
 template <typename T>
 static int sor_iterate_halo(bcm_field_t f, bcm_field_t source, double omega, int iters) {
   bcm_mesh* m = f->mesh;
@@ -638,29 +640,30 @@ static int sor_iterate_halo(bcm_field_t f, bcm_field_t source, double omega, int
   const size_t xbytes = size_t(m->nblocks) * 4 * m->n * m->n * sizeof(double);
   for (int i = 0; i < 2; ++i)
     if (!m->d_xexp[i]) BCM_TRY(dev_alloc(&m->d_xexp[i], xbytes));
-  const int slot = sizeof(T) == 8 ? WS_SOR_PONG_F64 : WS_SOR_PONG_F32;
-  T* pong = static_cast<T*>(workspace(m, slot, f->bytes, &rc));
+  const size_t cbytes = size_t(m->ncell) * sizeof(T);
+  const bool f64 = sizeof(T) == 8;
+  T* A = static_cast<T*>(workspace(m, f64 ? WS_SOR_CA_F64 : WS_SOR_CA_F32, cbytes, &rc));
   if (rc != BCM_OK) return rc;
-  T* A = static_cast<T*>(f->data);
-  T* B = pong;
+  T* B = static_cast<T*>(workspace(m, f64 ? WS_SOR_CB_F64 : WS_SOR_CB_F32, cbytes, &rc));
+  if (rc != BCM_OK) return rc;
+  T* s = nullptr;
+  if (source) {
+    s = static_cast<T*>(workspace(m, f64 ? WS_SOR_CS_F64 : WS_SOR_CS_F32, cbytes, &rc));
+    if (rc != BCM_OK) return rc;
+    BCM_TRY(to_compact_raw<T>(m, static_cast<const T*>(source->data), s, nullptr));
+  }
   T* xe[2] = {static_cast<T*>(m->d_xexp[0]), static_cast<T*>(m->d_xexp[1])};
-  const T* s = source ? static_cast<const T*>(source->data) : nullptr;
-  // Both buffers start identical, so cells no kernel writes (edge and
-  // corner ghosts) keep their values whichever buffer ends up in f.
-  BCM_HIP_TRY(hipMemcpyAsync(B, A, f->bytes, hipMemcpyDeviceToDevice, m->stream));
-  BCM_TRY(xexport_raw<T>(m, A, xe[0]));
+  BCM_TRY(to_compact_raw<T>(m, static_cast<const T*>(f->data), A, xe[0]));
   int cur = 0;
   for (int it = 0; it < iters; ++it) {
     BCM_TRY(halo_sweep_raw<T>(m, A, B, s, omega, xe[cur], xe[cur ^ 1]));
     std::swap(A, B);
     cur ^= 1;
   }
-  if (A != f->data) {
-    m->ws[slot] = f->data;
-    f->data = A;
-  }
+  T* d = static_cast<T*>(f->data);
+  BCM_TRY(from_compact_raw<T>(m, A, d));
   // Complete the ghosts of the final iterate: exchange, then the mirror.
-  BCM_TRY(ghost_fill_raw<T>(m, A, BCM_GF_ALL));
+  BCM_TRY(ghost_fill_raw<T>(m, d, BCM_GF_ALL));
   return dirichlet_device(m, f);
 }
 
@@ -761,7 +764,6 @@ extern "C" int bcm_sor_autotune(bcm_field_t f, bcm_field_t source, double omega,
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(m->stream);
-  // The halo variant may have swapped the scratch buffer with the workspace.
   dev_free(scratch.data);
   if (rc == BCM_OK) m->sor_variant = *best;
   return rc;

@@ -8,8 +8,10 @@
 // This kernel runs sweep k+1 reading the iterate of sweep k from buffer A
 // and writing buffer B, and builds each block's ghost frame on the fly from
 // A with the same arithmetic as the exchange, so no separate ghost pass and
-// no ghost writes are needed between sweeps.  The X-face sources (columns
-// 0, 1, n-2, n-1 of every block) come from a compact export written by the
+// no ghost writes are needed between sweeps.  A, B and the source term are
+// compact (n^3 values per block, x fastest, no ghost cells), so every row
+// is read and written whole and aligned.  The X-face sources (columns 0,
+// 1, n-2, n-1 of every block) come from a compact export written by the
 // previous sweep, which keeps every read coalesced.
 #include <type_traits>
 
@@ -22,14 +24,14 @@ namespace {
 
 template <typename T, int N>
 struct HaloSrc {
-  const T* __restrict__ A;   // previous iterate
+  const T* __restrict__ A;   // previous iterate, compact
   const T* __restrict__ xe;  // its X-face export [blk][4][k][j]
   const int* desc;   // face descriptors of this work-group's block (LDS copy)
   const double* __restrict__ bval;
   const double* bface;  // Dirichlet value per face of this block (LDS copy)
-  static constexpr int NX = N + 2;
+  static constexpr int N3 = N * N * N;
   __device__ __forceinline__ T a_at(int blk, int i, int j, int k) const {
-    return A[(long long)blk * NX * NX * NX + (i + 1) + NX * ((j + 1) + NX * (k + 1))];
+    return A[(long long)blk * N3 + i + N * (j + N * k)];
   }
   // i must be 0, 1, N-2 or N-1.
   __device__ __forceinline__ T x_at(int blk, int i, int j, int k) const {
@@ -40,7 +42,7 @@ struct HaloSrc {
     return a == 0 ? x_at(blk, i, j, k) : a_at(blk, i, j, k);
   }
   __device__ __forceinline__ const T* a_ptr(int blk, int i, int j, int k) const {
-    return A + (long long)blk * NX * NX * NX + (i + 1) + NX * ((j + 1) + NX * (k + 1));
+    return A + (long long)blk * N3 + i + N * (j + N * k);
   }
   __device__ __forceinline__ const T* x_ptr(int blk, int i, int j, int k) const {
     const int col = i < 2 ? i : i - (N - 4);
@@ -96,7 +98,10 @@ struct HaloSrc {
         c[a] = plus ? N - 1 : 0;
         base = ptr(a, b, c[0], c[1], c[2]);
       } else {
-        base = a_ptr(b, ci, cj, ck);  // no neighbour, no condition: the stored ghost
+        // No neighbour and no condition: the halo schedule is only used on
+        // meshes without such faces (the stored ghost is not in the compact
+        // iterate); an in-block cell keeps the address valid.
+        base = a_ptr(b, min(max(ci, 0), N - 1), min(max(cj, 0), N - 1), min(max(ck, 0), N - 1));
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) o[q] = 0;
@@ -186,7 +191,7 @@ struct HaloSrc {
       inp = cin;
       blk0 = blk1 = b;
     } else {
-      nrm = a == 0 ? ci : cj;  // the stored ghost itself
+      nrm = a == 0 ? min(max(ci, 0), N - 1) : min(max(cj, 0), N - 1);  // not reached (see gather)
       inp = cin;
       blk0 = blk1 = b;
     }
@@ -194,8 +199,8 @@ struct HaloSrc {
     const bool viax = a == 0 && kind != FD_NONE;  // X faces read the column export
     fl.base[0] = viax ? x_ptr(blk0, x, y, 0) : a_ptr(blk0, x, y, 0);
     fl.base[1] = viax ? x_ptr(blk1, x, y, 0) : a_ptr(blk1, x, y, 0);
-    fl.zs = viax ? N : NX * NX;
-    const int xs = viax ? N * N : 1, ys = viax ? 1 : NX;
+    fl.zs = viax ? N : N * N;
+    const int xs = viax ? N * N : 1, ys = viax ? 1 : N;
     const bool many = kind == FD_F2C || kind == FD_C2F;
 #pragma unroll
     for (int e = 0; e < VPT; ++e) {
@@ -280,17 +285,18 @@ __device__ __forceinline__ T sor_cell(T fc, T fw, T fe, T fs, T fn, T fb, T ft, 
 // (2pi..2pi+1, pj) of every z-plane (one red, one black).  The thread's own
 // cells of planes k-2..k+3 sit in registers (vertical neighbours never go
 // through LDS); LDS holds three ghosted planes for the in-plane neighbours
-// and two staging planes for the X/Y ghost frame; two barriers per z-step:
+// and two staging planes for the X/Y ghost frame; one barrier per z-step:
 //   step k: commit plane k (own old values); park the frame sources of plane
 //   k+1; issue the loads of plane k+3 (own cells and frame sources) and of
 //   the source term of plane k+2; barrier; combine the frame of plane k+1;
 //   red cells of plane k (in-plane from LDS, vertical from registers; red
 //   results also to LDS); black cells of plane k-1 (in-plane red values from
-//   LDS, black results too); barrier; write plane k-1 from LDS to B in
-//   aligned whole rows (and its X-face columns to the export).
+//   LDS, black results too), then plane k-1's final pair straight from
+//   registers to B (one aligned 16-byte store) and, where a neighbour reads
+//   it, to the X-face export.
 // Every LDS cell is written by one thread; slot k%3 is rewritten at step
-// k+3, after the barrier of step k+1 and k+2; staging slot p&1 is rewritten
-// two steps after it was read.
+// k+3, after the barriers of steps k+1 and k+2; staging slot p&1 is
+// rewritten two steps after it was read.
 template <typename T, int N, bool SRC>
 __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_rbsor_halo(
     const T* __restrict__ A, T* __restrict__ B, const T* __restrict__ s, const double* __restrict__ coef,
@@ -299,6 +305,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int NX = N + 2;
   constexpr int PL = NX * NX;
   constexpr int HALF = N / 2;
+  constexpr int NN = N * N;
   __shared__ __align__(16) T lds[3][PL];
   __shared__ T stage[2][8][4 * N];
   __shared__ int bdesc[48];
@@ -311,7 +318,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   }
   __syncthreads();
   const HaloSrc<T, N> src{A, xin, bdesc, bval, bface};
-  const long long base = (long long)b * PL * NX;
+  const long long base = (long long)b * NN * N;
   const T* __restrict__ Ab = A + base;
   T* __restrict__ Bb = B + base;
   const T* __restrict__ sb = SRC ? s + base : nullptr;
@@ -319,6 +326,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   const T om = T(omega), om1 = T(1.0 - omega);
   const int t = threadIdx.x;
   const int i0 = 2 * (t % HALF), j0 = t / HALF;
+  const int own = j0 * N + i0;  // offset of the thread's pair in a compact plane (even: 16-byte aligned)
   const bool framer = t < 4 * N;
   const int edge = t / N, u = t % N;
   const int fci = edge == 0 ? -1 : (edge == 1 ? N : u);
@@ -329,9 +337,9 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   // Own cells of plane p: interior rows of A, or the gathered z ghost plane.
   auto load_own = [&](int p, T* o) __attribute__((always_inline)) {
     if (p >= 0 && p < N) {
-      const T* row = Ab + (long long)(p + 1) * PL;
+      const T* row = Ab + (long long)p * NN + own;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) o[q] = row[LQ(q)];
+      for (int q = 0; q < 2; ++q) o[q] = row[q];
     } else {
 #pragma unroll 1
       for (int q = 0; q < 2; ++q) {
@@ -351,9 +359,9 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   };
   auto load_s = [&](int p, T* sv) __attribute__((always_inline)) {
     if (!SRC) return;
-    const T* row = sb + (long long)(p + 1) * PL;
+    const T* row = sb + (long long)p * NN + own;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) sv[q] = row[LQ(q)];
+    for (int q = 0; q < 2; ++q) sv[q] = row[q];
   };
   // X/Y ghost frame: 4N cells per plane, eight source values each, spread
   // over all threads: thread t loads values VPT*gq .. VPT*gq+VPT-1 of frame
@@ -411,43 +419,21 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
       lds[S][c] = o[q];
     }
   };
-  // Plane p to B from LDS slot p % 3, complete once the barrier after
-  // black(p) has passed: every row in aligned cell pairs (-1,0), (1,2), ...,
-  // (N-1,N) -- the X frame included -- so each wave writes whole cache
-  // lines with 16-byte stores; the X-face columns to the export, one
-  // coalesced row per column.  The X frame is the ghost exchange of A,
-  // which only unconditioned faces read back, and there it is the stored
-  // ghost itself.  The Y frame rows are not written: no sweep reads them
-  // back except on unconditioned faces, whose stored ghosts are the same in
-  // A and B, and rows -1 and N are whole rows, so leaving them out removes
-  // writes without splitting a cache line more often (2 % faster).
-  constexpr int NPAIR = N * (HALF + 1);
-  constexpr int NT = N * N / 2;
   // Export columns the next sweep reads: the outer column (0 or N-1) on a
   // side with a neighbour or a Dirichlet condition (copy, mirror), the inner
   // one (1 or N-2) only across a level change (F2C and C2F read two
-  // columns).  Thread t < 4N owns column t / N of the export.
-  const int xcol = t / N;
-  const int xkind = bdesc[(xcol < 2 ? BCM_X_M : BCM_X_P) * 8];
-  const bool xneed = t < 4 * N && xkind != FD_NONE &&
-                     (xcol == 0 || xcol == 3 || xkind == FD_F2C || xkind == FD_C2F);
-  auto store_plane = [&](int p) __attribute__((always_inline)) {
-    const int S = p % 3;
-    T* dst = Bb + (long long)(p + 1) * PL;
-#pragma unroll
-    for (int e0 = 0; e0 < NPAIR; e0 += NT) {
-      const int e = e0 + t;
-      if (e < NPAIR) {
-        const int j = e / (HALF + 1), m = e % (HALF + 1);
-        const int off = (j + 1) * NX + 2 * m;
-        store_pair(dst + off, lds[S][off], lds[S][off + 1]);
-      }
-    }
-    if (xneed) {
-      const int j = t % N;
-      const int i = xcol < 2 ? xcol : xcol + N - 4;
-      xout[((long long)(b * 4 + xcol) * N + p) * N + j] = lds[S][L(i, j)];
-    }
+  // columns).  The threads whose pair is cells 0, 1 or N-2, N-1 write them.
+  const bool xlo = i0 == 0, xhi = i0 == N - 2;
+  const int xkind = bdesc[(xlo ? BCM_X_M : BCM_X_P) * 8];
+  const bool lvl = xkind == FD_F2C || xkind == FD_C2F;
+  const bool xo0 = (xlo && (xkind != FD_NONE)) || (xhi && lvl);   // cell i0 (column 0 or N-2)
+  const bool xo1 = (xlo && lvl) || (xhi && (xkind != FD_NONE));   // cell i0+1 (column 1 or N-1)
+  const int xcol = xlo ? 0 : 2;
+  // Plane p's final pair, once black(p) has run: B and the export.
+  auto store_own = [&](int p, const T* o) __attribute__((always_inline)) {
+    store_pair(Bb + (long long)p * NN + own, o[0], o[1]);
+    if (xo0) xout[((long long)(b * 4 + xcol) * N + p) * N + j0] = o[0];
+    if (xo1) xout[((long long)(b * 4 + xcol + 1) * N + p) * N + j0] = o[1];
   };
 
   // Register window: own cells of planes k-2 .. k+3 (loads of plane k+3
@@ -476,9 +462,9 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
     commit(k, u0);
     if (k + 1 < N) park_frame(k + 1, fr1);
     if (k + 3 < N) {
-      const T* row = Ab + (long long)(k + 4) * PL;
+      const T* row = Ab + (long long)(k + 3) * NN + own;
 #pragma unroll
-      for (int q = 0; q < 2; ++q) up3[q] = row[LQ(q)];
+      for (int q = 0; q < 2; ++q) up3[q] = row[q];
     } else if (k + 3 == N) {
       up3[0] = zp[0];
       up3[1] = zp[1];
@@ -490,8 +476,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
     red(std::integral_constant<int, PAR>(), k, u0, um1, up1, sc);
     if (k >= 1) {
       black(std::integral_constant<int, 1 - PAR>(), k - 1, um1, um2, u0, sp);
-      __syncthreads();
-      store_plane(k - 1);
+      store_own(k - 1, um1);
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -520,20 +505,39 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   // um1 = plane N-1, um2 = plane N-2, u0 = plane N (ghost); sp = s(N-1).
   __syncthreads();
   black(std::integral_constant<int, (N - 1) & 1>(), N - 1, um1, um2, u0, sp);
-  __syncthreads();
-  store_plane(N - 1);
+  store_own(N - 1, um1);
 }
 
-// X-face columns 0, 1, N-2, N-1 of every block of `A` into `xout`.
+// Ghosted field -> compact iterate (interior cells, x fastest) and, when
+// xout is given, its X-face export; and back.
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_xexport(const T* __restrict__ A, T* __restrict__ xout) {
-  constexpr int NX = N + 2;
+__global__ __launch_bounds__(256) void k_to_compact(const T* __restrict__ f, T* __restrict__ c,
+                                                    T* __restrict__ xout) {
+  constexpr int NX = N + 2, NN = N * N;
   const int b = blockIdx.x;
-  const T* Ab = A + (long long)b * NX * NX * NX;
-  for (int e = threadIdx.x; e < 4 * N * N; e += blockDim.x) {
-    const int j = e % N, k = (e / N) % N, col = e / (N * N);
+  const T* fb = f + (long long)b * NX * NX * NX;
+  T* cb = c + (long long)b * NN * N;
+  for (int e = threadIdx.x; e < NN * N; e += blockDim.x) {
+    const int i = e % N, j = (e / N) % N, k = e / NN;
+    cb[e] = fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
+  }
+  if (!xout) return;
+  for (int e = threadIdx.x; e < 4 * NN; e += blockDim.x) {
+    const int j = e % N, k = (e / N) % N, col = e / NN;
     const int i = col < 2 ? col : col + (N - 4);
-    xout[((long long)(b * 4 + col) * N + k) * N + j] = Ab[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
+    xout[(long long)b * 4 * NN + e] = fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_from_compact(const T* __restrict__ c, T* __restrict__ f) {
+  constexpr int NX = N + 2, NN = N * N;
+  const int b = blockIdx.x;
+  T* fb = f + (long long)b * NX * NX * NX;
+  const T* cb = c + (long long)b * NN * N;
+  for (int e = threadIdx.x; e < NN * N; e += blockDim.x) {
+    const int i = e % N, j = (e / N) % N, k = e / NN;
+    fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))] = cb[e];
   }
 }
 
@@ -558,14 +562,25 @@ template int halo_sweep_raw<double>(bcm_mesh*, const double*, double*, const dou
 template int halo_sweep_raw<float>(bcm_mesh*, const float*, float*, const float*, double, const float*, float*);
 
 template <typename T>
-int xexport_raw(bcm_mesh* m, const T* A, T* xout) {
-  if (m->n == 32) hipLaunchKernelGGL((k_xexport<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, A, xout);
-  else if (m->n == 16) hipLaunchKernelGGL((k_xexport<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, A, xout);
+int to_compact_raw(bcm_mesh* m, const T* f, T* c, T* xout) {
+  if (m->n == 32) hipLaunchKernelGGL((k_to_compact<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c, xout);
+  else if (m->n == 16) hipLaunchKernelGGL((k_to_compact<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c, xout);
   else return BCM_ERR_ARG;
   BCM_HIP_TRY(hipGetLastError());
   return BCM_OK;
 }
-template int xexport_raw<double>(bcm_mesh*, const double*, double*);
-template int xexport_raw<float>(bcm_mesh*, const float*, float*);
+template int to_compact_raw<double>(bcm_mesh*, const double*, double*, double*);
+template int to_compact_raw<float>(bcm_mesh*, const float*, float*, float*);
+
+template <typename T>
+int from_compact_raw(bcm_mesh* m, const T* c, T* f) {
+  if (m->n == 32) hipLaunchKernelGGL((k_from_compact<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, c, f);
+  else if (m->n == 16) hipLaunchKernelGGL((k_from_compact<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, c, f);
+  else return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+template int from_compact_raw<double>(bcm_mesh*, const double*, double*);
+template int from_compact_raw<float>(bcm_mesh*, const float*, float*);
 
 }  // namespace bcm
