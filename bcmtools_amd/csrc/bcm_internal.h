@@ -169,9 +169,9 @@ int convert_raw(bcm_mesh* m, int op, void* dst, const void* src);
 int dirichlet_device(bcm_mesh* m, bcm_field* f);
 // Halo-gathering red-black sweep A -> B (bcm_sweep.hip) and the X-face
 // column export it reads.
-template <typename T> int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout);
-template <typename T> int to_compact_raw(bcm_mesh* m, const T* f, T* c, T* xout);
-template <typename T> int from_compact_raw(bcm_mesh* m, const T* c, T* f);
+template <typename T>
+int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout);
+template <typename T> int to_compact_raw(bcm_mesh* m, const T* f, T* c);
 // Block-wise ghost fill (all faces, all kinds; n in {8, 16, 32}, vc = 1).
 template <typename T> int ghost_gather_raw(bcm_mesh* m, T* data);
 // Frees the fused solver's state (bcm_krylov.hip).

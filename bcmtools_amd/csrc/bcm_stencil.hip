@@ -115,7 +115,8 @@ __device__ __forceinline__ T sor_update(T fc, T fw, T fe, T fs, T fn, T fb, T ft
 // written back.  Plane k+2 is prefetched into registers during step k.
 template <typename T, int N, bool SRC>
 __global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, const T* __restrict__ s,
-                                                         const double* __restrict__ coef, double omega) {
+                                                         const double* __restrict__ coef, double omega,
+                                                         T* __restrict__ cout, T* __restrict__ xout) {
   constexpr int NX = N + 2;
   constexpr int PL = NX * NX;
   constexpr int NT = N * N / 4;
@@ -161,8 +162,23 @@ __global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, con
       lds[P][c] = v;
     }
   };
+  // The swept plane back to f, or (cout given) to the compact iterate of
+  // the halo schedule together with its X-face export (columns 0, 1, N-2,
+  // N-1); f then keeps its old values.
   auto store = [&](int p) {
     const int P = p & 3;
+    if (cout) {
+      T* dst = cout + (long long)b * N * N * N + (long long)p * N * N;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + (q & 1), j = j0 + (q >> 1);
+        const T v = lds[P][L(i, j)];
+        dst[i + N * j] = v;
+        const int col = i < 2 ? i : (i >= N - 2 ? i - (N - 4) : -1);
+        if (col >= 0) xout[((long long)(b * 4 + col) * N + p) * N + j] = v;
+      }
+      return;
+    }
     T* dst = fb + (long long)(p + 1) * PL;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -226,8 +242,11 @@ __global__ __launch_bounds__(256) void k_rbsor_colour(T* __restrict__ f, const T
                             SRC ? s[o] : T(0), c0, cx, cy, cz, T(omega), T(1.0 - omega));
 }
 
+// cout (vc = 1, n = 16 or 32 only): write the swept iterate compact, with
+// its X-face export, instead of back into f.
 template <typename T>
-static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_passes = false) {
+static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_passes = false, T* cout = nullptr,
+                     T* xout = nullptr) {
   const int form = s ? 2 : 1;
   BCM_TRY(sor_coef(m, omega, form));
   EventPair ev;
@@ -236,11 +255,11 @@ static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_pa
   if (colour_passes) {
     // fall through to the generic kernel below
   } else if (m->vc == 1 && m->n == 32) {
-    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 32, true>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega);
-    else hipLaunchKernelGGL((k_rbsor_fused<T, 32, false>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega);
+    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 32, true>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
+    else hipLaunchKernelGGL((k_rbsor_fused<T, 32, false>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
   } else if (m->vc == 1 && m->n == 16) {
-    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 16, true>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega);
-    else hipLaunchKernelGGL((k_rbsor_fused<T, 16, false>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega);
+    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 16, true>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
+    else hipLaunchKernelGGL((k_rbsor_fused<T, 16, false>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
   }
   if (colour_passes || !(m->vc == 1 && (m->n == 32 || m->n == 16))) {
     const CellGeo g = cell_geo(m);
@@ -628,10 +647,13 @@ extern "C" int bcm_rbsor_sweep(bcm_field_t f, bcm_field_t source, double omega) 
                           source ? static_cast<const float*>(source->data) : nullptr, omega);
 }
 
-// `iters` further outer iterations by the halo-gathering sweep on compact
-// copies of the iterate (ping-pong between two workspace buffers) and of
-// the source term; f receives the final interior, then complete ghosts.
-// f's own ghost cells are not read meanwhile: every face of a mesh this
+// `iters` >= 2 outer iterations with the halo-gathering schedule.  Sweep 1
+// uses the ghosts stored in f, as the reference's first iteration does,
+// and writes its result compact with the X-face export; sweeps 2..iters
+// build each block's ghost frame from the previous compact iterate
+// (ping-pong between two workspace buffers, the source term compact too),
+// the last one writing f's interior; then f's ghosts are completed.  f's
+// own ghost cells are not read after sweep 1: every face of a mesh this
 // schedule runs on has a neighbour or a Dirichlet condition.
 template <typename T>
 static int sor_iterate_halo(bcm_field_t f, bcm_field_t source, double omega, int iters) {
@@ -646,22 +668,22 @@ static int sor_iterate_halo(bcm_field_t f, bcm_field_t source, double omega, int
   if (rc != BCM_OK) return rc;
   T* B = static_cast<T*>(workspace(m, f64 ? WS_SOR_CB_F64 : WS_SOR_CB_F32, cbytes, &rc));
   if (rc != BCM_OK) return rc;
+  T* d = static_cast<T*>(f->data);
+  const T* sg = source ? static_cast<const T*>(source->data) : nullptr;
   T* s = nullptr;
   if (source) {
     s = static_cast<T*>(workspace(m, f64 ? WS_SOR_CS_F64 : WS_SOR_CS_F32, cbytes, &rc));
     if (rc != BCM_OK) return rc;
-    BCM_TRY(to_compact_raw<T>(m, static_cast<const T*>(source->data), s, nullptr));
+    BCM_TRY(to_compact_raw<T>(m, sg, s));
   }
   T* xe[2] = {static_cast<T*>(m->d_xexp[0]), static_cast<T*>(m->d_xexp[1])};
-  BCM_TRY(to_compact_raw<T>(m, static_cast<const T*>(f->data), A, xe[0]));
+  BCM_TRY(rbsor_raw<T>(m, d, sg, omega, false, A, xe[0]));
   int cur = 0;
-  for (int it = 0; it < iters; ++it) {
-    BCM_TRY(halo_sweep_raw<T>(m, A, B, s, omega, xe[cur], xe[cur ^ 1]));
+  for (int it = 1; it < iters; ++it) {
+    BCM_TRY(halo_sweep_raw<T>(m, A, B, s, omega, xe[cur], xe[cur ^ 1], it == iters - 1 ? d : nullptr));
     std::swap(A, B);
     cur ^= 1;
   }
-  T* d = static_cast<T*>(f->data);
-  BCM_TRY(from_compact_raw<T>(m, A, d));
   // Complete the ghosts of the final iterate: exchange, then the mirror.
   BCM_TRY(ghost_fill_raw<T>(m, d, BCM_GF_ALL));
   return dirichlet_device(m, f);
@@ -685,10 +707,8 @@ static int sor_iterate_t(bcm_field_t f, bcm_field_t source, double omega, const 
   T* d = static_cast<T*>(f->data);
   const T* s = source ? static_cast<const T*>(source->data) : nullptr;
   if (variant == BCM_SOR_AUTO) variant = variant_supported(m, BCM_SOR_HALO) ? BCM_SOR_HALO : BCM_SOR_FUSED_TASKS;
-  const bool halo = variant == BCM_SOR_HALO && iters >= 2;
-  // The first iteration uses the ghosts stored in f, as the reference does.
-  const int classic = halo ? 1 : iters;
-  for (int it = 0; it < classic; ++it) {
+  if (variant == BCM_SOR_HALO && iters >= 2) return sor_iterate_halo<T>(f, source, omega, iters);
+  for (int it = 0; it < iters; ++it) {
     d = static_cast<T*>(f->data);
     BCM_TRY(rbsor_raw<T>(m, d, s, omega, variant == BCM_SOR_COLOUR_TASKS));
     BCM_TRY(dirichlet_device(m, f));
@@ -696,8 +716,7 @@ static int sor_iterate_t(bcm_field_t f, bcm_field_t source, double omega, const 
     else BCM_TRY(ghost_fill_raw<T>(m, d, BCM_GF_ALL));
     BCM_TRY(dirichlet_device(m, f));
   }
-  if (!halo) return BCM_OK;
-  return sor_iterate_halo<T>(f, source, omega, iters - 1);
+  return BCM_OK;
 }
 
 extern "C" int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,

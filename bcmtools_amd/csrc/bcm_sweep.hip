@@ -301,7 +301,7 @@ template <typename T, int N, bool SRC>
 __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_rbsor_halo(
     const T* __restrict__ A, T* __restrict__ B, const T* __restrict__ s, const double* __restrict__ coef,
     double omega, const int* __restrict__ desc, const T* __restrict__ xin, T* __restrict__ xout,
-    const double* __restrict__ bval) {
+    const double* __restrict__ bval, T* __restrict__ gout) {
   constexpr int NX = N + 2;
   constexpr int PL = NX * NX;
   constexpr int HALF = N / 2;
@@ -429,8 +429,15 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   const bool xo0 = (xlo && (xkind != FD_NONE)) || (xhi && lvl);   // cell i0 (column 0 or N-2)
   const bool xo1 = (xlo && lvl) || (xhi && (xkind != FD_NONE));   // cell i0+1 (column 1 or N-1)
   const int xcol = xlo ? 0 : 2;
-  // Plane p's final pair, once black(p) has run: B and the export.
+  // Plane p's final pair, once black(p) has run: B and the export, or
+  // (the last sweep of a call, gout given) the ghosted field's interior.
   auto store_own = [&](int p, const T* o) __attribute__((always_inline)) {
+    if (gout) {
+      T* g = gout + (long long)b * PL * NX + (long long)(p + 1) * PL + LQ(0);
+      g[0] = o[0];
+      g[1] = o[1];
+      return;
+    }
     store_pair(Bb + (long long)p * NN + own, o[0], o[1]);
     if (xo0) xout[((long long)(b * 4 + xcol) * N + p) * N + j0] = o[0];
     if (xo1) xout[((long long)(b * 4 + xcol + 1) * N + p) * N + j0] = o[1];
@@ -510,77 +517,57 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
 
 // Ghosted field -> compact iterate (interior cells, x fastest) and, when
 // xout is given, its X-face export; and back.
+// Ghosted field -> compact interior (x fastest), one work-group per block;
+// each thread moves UNR values per pass with all loads issued first.
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_to_compact(const T* __restrict__ f, T* __restrict__ c,
-                                                    T* __restrict__ xout) {
-  constexpr int NX = N + 2, NN = N * N;
+__global__ __launch_bounds__(256) void k_to_compact(const T* __restrict__ f, T* __restrict__ c) {
+  constexpr int NX = N + 2, NN = N * N, UNR = 8;
+  static_assert((NN * N) % (256 * UNR) == 0, "block volume");
   const int b = blockIdx.x;
   const T* fb = f + (long long)b * NX * NX * NX;
   T* cb = c + (long long)b * NN * N;
-  for (int e = threadIdx.x; e < NN * N; e += blockDim.x) {
-    const int i = e % N, j = (e / N) % N, k = e / NN;
-    cb[e] = fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
-  }
-  if (!xout) return;
-  for (int e = threadIdx.x; e < 4 * NN; e += blockDim.x) {
-    const int j = e % N, k = (e / N) % N, col = e / NN;
-    const int i = col < 2 ? col : col + (N - 4);
-    xout[(long long)b * 4 * NN + e] = fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
-  }
-}
-
-template <typename T, int N>
-__global__ __launch_bounds__(256) void k_from_compact(const T* __restrict__ c, T* __restrict__ f) {
-  constexpr int NX = N + 2, NN = N * N;
-  const int b = blockIdx.x;
-  T* fb = f + (long long)b * NX * NX * NX;
-  const T* cb = c + (long long)b * NN * N;
-  for (int e = threadIdx.x; e < NN * N; e += blockDim.x) {
-    const int i = e % N, j = (e / N) % N, k = e / NN;
-    fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))] = cb[e];
+  for (int e0 = threadIdx.x; e0 < NN * N; e0 += 256 * UNR) {
+    T v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const int e = e0 + 256 * u, i = e % N, j = (e / N) % N, k = e / NN;
+      v[u] = fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) cb[e0 + 256 * u] = v[u];
   }
 }
 
 template <typename T>
-int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout) {
+int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout) {
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_SOR, &ev));
   const int nb = m->nblocks;
   if (m->n == 32) {
-    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 32, true>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
-    else hipLaunchKernelGGL((k_rbsor_halo<T, 32, false>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
+    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 32, true>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+    else hipLaunchKernelGGL((k_rbsor_halo<T, 32, false>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
   } else if (m->n == 16) {
-    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 16, true>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
-    else hipLaunchKernelGGL((k_rbsor_halo<T, 16, false>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues);
+    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 16, true>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+    else hipLaunchKernelGGL((k_rbsor_halo<T, 16, false>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
   } else {
     return BCM_ERR_ARG;
   }
   BCM_HIP_TRY(hipGetLastError());
   return timer_end(m, &ev);
 }
-template int halo_sweep_raw<double>(bcm_mesh*, const double*, double*, const double*, double, const double*, double*);
-template int halo_sweep_raw<float>(bcm_mesh*, const float*, float*, const float*, double, const float*, float*);
+template int halo_sweep_raw<double>(bcm_mesh*, const double*, double*, const double*, double, const double*, double*,
+                                    double*);
+template int halo_sweep_raw<float>(bcm_mesh*, const float*, float*, const float*, double, const float*, float*, float*);
 
 template <typename T>
-int to_compact_raw(bcm_mesh* m, const T* f, T* c, T* xout) {
-  if (m->n == 32) hipLaunchKernelGGL((k_to_compact<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c, xout);
-  else if (m->n == 16) hipLaunchKernelGGL((k_to_compact<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c, xout);
+int to_compact_raw(bcm_mesh* m, const T* f, T* c) {
+  if (m->n == 32) hipLaunchKernelGGL((k_to_compact<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c);
+  else if (m->n == 16) hipLaunchKernelGGL((k_to_compact<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c);
   else return BCM_ERR_ARG;
   BCM_HIP_TRY(hipGetLastError());
   return BCM_OK;
 }
-template int to_compact_raw<double>(bcm_mesh*, const double*, double*, double*);
-template int to_compact_raw<float>(bcm_mesh*, const float*, float*, float*);
-
-template <typename T>
-int from_compact_raw(bcm_mesh* m, const T* c, T* f) {
-  if (m->n == 32) hipLaunchKernelGGL((k_from_compact<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, c, f);
-  else if (m->n == 16) hipLaunchKernelGGL((k_from_compact<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, c, f);
-  else return BCM_ERR_ARG;
-  BCM_HIP_TRY(hipGetLastError());
-  return BCM_OK;
-}
-template int from_compact_raw<double>(bcm_mesh*, const double*, double*);
-template int from_compact_raw<float>(bcm_mesh*, const float*, float*);
+template int to_compact_raw<double>(bcm_mesh*, const double*, double*);
+template int to_compact_raw<float>(bcm_mesh*, const float*, float*);
 
 }  // namespace bcm
