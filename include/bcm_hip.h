@@ -269,9 +269,12 @@ int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega,
  *                 thread, z-planes through LDS) + per-face-task ghost fill
  *   FUSED_GATHER  the same sweep + block-wise ghost gather
  *   HALO          ghost frame built inside the sweep from the previous
- *                 iterate, ping-pong buffers, no separate fill (2x1 cell
- *                 patches per thread; vc = 1, n in {16, 32}, every face
- *                 connected or Dirichlet)
+ *                 iterate, no separate fill; the iterates and the source
+ *                 are held compact (n^3 per block, mesh workspace) between
+ *                 the call's first sweep, which reads f's stored ghosts,
+ *                 and its last, which writes f (2x1 cell patches per
+ *                 thread; vc = 1, n in {16, 32}, every face connected or
+ *                 Dirichlet)
  *   COLOUR_TASKS  two one-colour passes (one thread per cell) + task fill */
 enum { BCM_SOR_AUTO = 0, BCM_SOR_FUSED_TASKS, BCM_SOR_FUSED_GATHER, BCM_SOR_HALO, BCM_SOR_COLOUR_TASKS,
        BCM_SOR_NUM_VARIANTS };
