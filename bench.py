@@ -595,7 +595,69 @@ def bench_serving(args):
     if len(runs) > 1:
         res["sweep"] = [{k: r[k] for k in ("lambda_per_s", "solves_per_s", "makespan_s", "latency_s",
                                           "all_met_tolerance")} for r in runs]
+    if args.per_request_every > 0:
+        res["one_at_a_time"] = serve_baseline(trees, tols, args.per_request_every, head["solves_per_s"],
+                                              np.cumsum(gaps / lams[0]))
     return res
+
+
+def serve_baseline(trees, tols, every, served_rate, arrivals):
+    """The requests of the stream solved alone, one after the other: every
+    request by the fused FP64 engine, every `every`-th also by the plain
+    host-driven FP64 path (bcm_solve_fp64, ILS3D's BiCGSTAB step by step).
+    The plain path's time for the whole stream is estimated as the fused
+    time times the plain/fused time ratio on the sampled requests; the
+    latencies a first-come-first-served queue of one-at-a-time fused solves
+    would give follow from the fused times and the arrival times.
+    Right-hand sides of the served form (sin x sin y sin z + 0.01 N(0,1),
+    negated) are drawn on the host."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    n = 16
+    t_fused = t_fused_s = t_plain_s = 0.0
+    ok, times = [], []
+    for i, tr in enumerate(trees):
+        blocks = tr.blocks
+        mesh = Mesh(blocks, n, 1)
+        rhs = mesh.field(np.float64, -W.rhs_field_fast(blocks, n, 1, seed=9000 + i))
+        x = mesh.field(np.float64)
+        mesh.sync()
+        t0 = time.perf_counter()
+        st = mesh.solve(x, rhs, [0.0], precision="fp64", tol=float(tols[i]), max_iter=20000)[0]
+        mesh.sync()
+        dt = time.perf_counter() - t0
+        t_fused += dt
+        times.append(dt)
+        line = f"[bench] alone {i}: {len(blocks)} blocks, fused {st['iters']} its {dt:.2f} s"
+        if i % every == 0:
+            t_fused_s += dt
+            x2 = mesh.field(np.float64)
+            mesh.sync()
+            t0 = time.perf_counter()
+            _, it, rr = mesh.solve_fp64(x2, rhs, [0.0], tol=float(tols[i]), max_iter=5000)
+            mesh.sync()
+            dp = time.perf_counter() - t0
+            t_plain_s += dp
+            ok.append(rr <= tols[i])
+            line += f", plain {it} its {dp:.2f} s"
+            del x2
+        print(line, file=sys.stderr, flush=True)
+        del mesh, rhs, x
+    R = len(trees)
+    fused_rate = R / t_fused
+    plain_est = t_fused * t_plain_s / t_fused_s
+    lat, free = [], 0.0
+    for i in range(R):
+        free = max(free, float(arrivals[i])) + times[i]
+        lat.append(free - float(arrivals[i]))
+    return {"requests": R, "fused_fp64_one_at_a_time_s": t_fused, "fused_fp64_one_at_a_time_solves_per_s": fused_rate,
+            "fifo_one_at_a_time_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "plain_sampled_every": every, "plain_sampled_all_met_tolerance": bool(all(ok)),
+            "plain_over_fused_time_on_sample": t_plain_s / t_fused_s,
+            "plain_fp64_host_driven_est_s": plain_est, "plain_fp64_host_driven_est_solves_per_s": R / plain_est,
+            "served_vs_fused_one_at_a_time": served_rate / fused_rate,
+            "served_vs_plain_est": served_rate / (R / plain_est)}
 
 
 def bench_copy(args):
@@ -725,6 +787,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1, help="serving: request stream seed")
     ap.add_argument("--policy", default="fp64", choices=["fp64", "spec"],
                     help="serving: fp64 for every request, or FP32 (tol 1e-6) / mixed (1e-10)")
+    ap.add_argument("--per-request-every", type=int, default=4,
+                    help="serving: also solve every request alone on the fused FP64 path and every k-th on the "
+                         "plain host-driven path (0: off)")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="batched: precision of the full batch (auto: the faster on the subset); "
