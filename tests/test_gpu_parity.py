@@ -395,3 +395,33 @@ def test_halo_schedule_equals_classic_schedule(dtype, iters):
             mesh.sor_iterate(f, sf, 1.25, vals, iters)
             out.append(f.download())
         np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("variant", [L.SOR_HALO, L.SOR_FUSED_TASKS])
+def test_sor_iterate_on_a_batch_equals_each_mesh_alone(variant):
+    """Several independent meshes in one device mesh (segments): one
+    sor_iterate over the batch gives every segment the bits of that mesh
+    iterated alone, with the halo schedule and the classic one."""
+    n = 16
+    tables = [W.random_tree(s, max_level=lv).blocks for s, lv in ((31, 3), (32, 2), (33, 3))]
+    case(f"sor batch variant={variant} segments={[len(t) for t in tables]}")
+    rng = np.random.default_rng(variant)
+    f0 = [rng.standard_normal((len(t), n + 2, n + 2, n + 2)) for t in tables]
+    s0 = [W.rhs_field(t, n, 1, seed=40 + i) for i, t in enumerate(tables)]
+    mesh = Mesh.batch(tables, n)
+    mesh.set_sor_variant(variant)
+    f = mesh.field(np.float64, np.concatenate(f0))
+    sf = mesh.field(np.float64, np.concatenate(s0))
+    mesh.sor_iterate(f, sf, 1.3, [0.5], 5)
+    out = f.download()
+    for i, t in enumerate(tables):
+        first, cnt = mesh.segment(i)
+        alone = Mesh(t, n, 1)
+        alone.set_sor_variant(variant)
+        g = alone.field(np.float64, f0[i])
+        alone.sor_iterate(g, alone.field(np.float64, s0[i]), 1.3, [0.5], 5)
+        np.testing.assert_array_equal(out[first:first + cnt], g.download())
+        ref = f0[i].copy()
+        oracle.dirichlet(t, n, 1, ref, [0.5])
+        oracle.sor_iterate(t, n, 1, ref, s0[i], 1.3, [0.5], 5)
+        np.testing.assert_array_equal(g.download(), ref)
