@@ -297,7 +297,7 @@ __device__ __forceinline__ T sor_cell(T fc, T fw, T fe, T fs, T fn, T fb, T ft, 
 // Every LDS cell is written by one thread; slot k%3 is rewritten at step
 // k+3, after the barriers of steps k+1 and k+2; staging slot p&1 is
 // rewritten two steps after it was read.
-template <typename T, int N, bool SRC>
+template <typename T, int N, bool SRC, bool GOUT>
 __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) void k_rbsor_halo(
     const T* __restrict__ A, T* __restrict__ B, const T* __restrict__ s, const double* __restrict__ coef,
     double omega, const int* __restrict__ desc, const T* __restrict__ xin, T* __restrict__ xout,
@@ -430,9 +430,13 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   const bool xo1 = (xlo && lvl) || (xhi && (xkind != FD_NONE));   // cell i0+1 (column 1 or N-1)
   const int xcol = xlo ? 0 : 2;
   // Plane p's final pair, once black(p) has run: B and the export, or
-  // (the last sweep of a call, gout given) the ghosted field's interior.
+  // (GOUT: the last sweep of a call) the ghosted field's interior.  GOUT is
+  // a template parameter so that no store sits under a run-time branch:
+  // the compiler then counts the outstanding memory operations exactly
+  // (a store under a branch counts as possibly not issued, which makes the
+  // waits for earlier loads drain more than they need).
   auto store_own = [&](int p, const T* o) __attribute__((always_inline)) {
-    if (gout) {
+    if constexpr (GOUT) {
       T* g = gout + (long long)b * PL * NX + (long long)(p + 1) * PL + LQ(0);
       g[0] = o[0];
       g[1] = o[1];
@@ -538,17 +542,28 @@ __global__ __launch_bounds__(256) void k_to_compact(const T* __restrict__ f, T* 
   }
 }
 
+template <typename T, int N>
+static void launch_halo(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout) {
+  const dim3 grid(m->nblocks), blk(N * N / 2);
+  if (s && gout)
+    hipLaunchKernelGGL((k_rbsor_halo<T, N, true, true>), grid, blk, 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+  else if (s)
+    hipLaunchKernelGGL((k_rbsor_halo<T, N, true, false>), grid, blk, 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+  else if (gout)
+    hipLaunchKernelGGL((k_rbsor_halo<T, N, false, true>), grid, blk, 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+  else
+    hipLaunchKernelGGL((k_rbsor_halo<T, N, false, false>), grid, blk, 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+}
+
 template <typename T>
 int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout) {
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_SOR, &ev));
   const int nb = m->nblocks;
   if (m->n == 32) {
-    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 32, true>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
-    else hipLaunchKernelGGL((k_rbsor_halo<T, 32, false>), dim3(nb), dim3(512), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+    launch_halo<T, 32>(m, A, B, s, omega, xin, xout, gout);
   } else if (m->n == 16) {
-    if (s) hipLaunchKernelGGL((k_rbsor_halo<T, 16, true>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
-    else hipLaunchKernelGGL((k_rbsor_halo<T, 16, false>), dim3(nb), dim3(128), 0, m->stream, A, B, s, m->d_sor_coef, omega, m->d_desc, xin, xout, m->d_bvalues, gout);
+    launch_halo<T, 16>(m, A, B, s, omega, xin, xout, gout);
   } else {
     return BCM_ERR_ARG;
   }
