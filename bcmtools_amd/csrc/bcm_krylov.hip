@@ -957,10 +957,28 @@ __global__ __launch_bounds__(KWG) void k_seg(SegState* __restrict__ st, const do
   SegState* sp = st + sg;
   if (sp->status != BCM_SOLVE_ACTIVE) return;
   if (PH == PH_F && !sp->refresh) return;
+  // Each thread sums its blocks in block order; the loads of UNR blocks are
+  // issued together and then added one after the other, so the sum is the
+  // same as a plain loop's while the loads overlap.
+  constexpr int UNR = 8;
+  const double2* pp = reinterpret_cast<const double2*>(parts);
+  const int last = seg_first[sg + 1];
   double a = 0.0, c = 0.0;
-  for (int b = seg_first[sg] + threadIdx.x; b < seg_first[sg + 1]; b += KWG) {
-    a += parts[2 * b];
-    c += parts[2 * b + 1];
+  int b = seg_first[sg] + threadIdx.x;
+  for (; b + (UNR - 1) * KWG < last; b += UNR * KWG) {
+    double2 v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) v[u] = pp[b + u * KWG];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      a += v[u].x;
+      c += v[u].y;
+    }
+  }
+  for (; b < last; b += KWG) {
+    const double2 v = pp[b];
+    a += v.x;
+    c += v.y;
   }
   wg_sum2(a, c, red);
   if (threadIdx.x != 0) return;
