@@ -316,8 +316,20 @@ int dirichlet_device(bcm_mesh* m, bcm_field* f) {
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ parts, int n,
                                                          double* __restrict__ out) {
   __shared__ double red[4];
+  // Each thread sums its partials in order; the loads of UNR of them are
+  // issued together and then added one after the other (the same sum as a
+  // plain loop, with the loads overlapped).
+  constexpr int UNR = 8;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += parts[i];
+  int i = threadIdx.x;
+  for (; i + (UNR - 1) * 256 < n; i += UNR * 256) {
+    double v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) v[u] = parts[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) s += v[u];
+  }
+  for (; i < n; i += 256) s += parts[i];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
