@@ -158,3 +158,14 @@ def test_update_virtual_cells_and_impose_boundary_condition(method, dtype, n, vc
     mesh.impose_boundary_condition(g, VALS, method)
     oracle.impose_bc(blocks, n, vc, ref, VALS)
     np.testing.assert_array_equal(g.download(), ref)
+
+
+def test_update_virtual_cells_rejects_an_unknown_method():
+    blocks = bc_tree(TYPES, 2, False).blocks
+    case(f"update_virtual_cells bad method nb={len(blocks)}")
+    mesh = Mesh(blocks, 8, 1)
+    f = mesh.field(np.float64)
+    with pytest.raises(L.BCMError):
+        mesh.update_virtual_cells(f, 3)
+    with pytest.raises(L.BCMError):
+        mesh.impose_boundary_condition(f, VALS, -1)
