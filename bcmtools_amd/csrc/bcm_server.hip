@@ -233,7 +233,7 @@ extern "C" int bcm_server_create(const bcm_server_params* p, bcm_server_t* out) 
 }
 
 extern "C" int bcm_server_destroy(bcm_server_t sv) {
-  if (!sv) return BCM_ERR_ARG;
+  if (!sv) return BCM_OK;
   for (Request* r : sv->active) release(sv, r);
   for (Request* r : sv->queue) delete r;
   delete sv;

@@ -125,6 +125,7 @@ int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out);
 int bcm_tree_num_blocks(bcm_tree_t tree);
 /* Copies the block table (one record per leaf in block-ID order). */
 int bcm_tree_get_blocks(bcm_tree_t tree, bcm_block_info* out, int capacity);
+/* The destroy calls accept NULL (no-op, BCM_OK), like free(). */
 int bcm_tree_destroy(bcm_tree_t tree);
 
 /* ------------------------------------------------------------------ */
@@ -279,6 +280,7 @@ int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega,
 enum { BCM_SOR_AUTO = 0, BCM_SOR_FUSED_TASKS, BCM_SOR_FUSED_GATHER, BCM_SOR_HALO, BCM_SOR_COLOUR_TASKS,
        BCM_SOR_NUM_VARIANTS };
 int bcm_sor_set_variant(bcm_mesh_t mesh, int variant);
+/* Returns the mesh's variant, or -BCM_ERR_ARG for a NULL mesh. */
 int bcm_sor_get_variant(bcm_mesh_t mesh);
 /* Autotuner: times every variant the mesh supports for `iters` outer
  * iterations on a scratch copy of f (f is not modified), keeps the
@@ -542,7 +544,9 @@ int bcm_server_submit(bcm_server_t server, const bcm_block_info* blocks, int nbl
 /* Drive the admitted requests for up to max_seconds (returns early when
  * nothing is left); finished requests move to the result list. */
 int bcm_server_progress(bcm_server_t server, double max_seconds, int* n_active, int* n_queued);
-/* Move up to capacity finished results into out; returns how many. */
+/* Move up to capacity finished results into out; returns how many, or
+ * -BCM_ERR_ARG for a NULL server, a negative capacity, or a NULL out with
+ * capacity > 0. */
 int bcm_server_poll(bcm_server_t server, bcm_server_result* out, int capacity);
 int bcm_server_stats(bcm_server_t server, double* bytes_in_use, int* n_active, int* n_queued);
 

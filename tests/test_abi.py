@@ -57,3 +57,30 @@ def test_field_api_rejects_bad_arguments_without_gpu():
     assert lib.bcm_mesh_create(blocks.ctypes.data, 1, 4, 1, ctypes.byref(ctypes.c_void_p())) == L.ERR_ARG
     assert lib.bcm_ghost_fill(None, 7) == L.ERR_ARG
     assert lib.bcm_dot(None, None, None) == L.ERR_ARG
+
+
+# Entry points that report a count return -BCM_ERR_ARG for a NULL handle;
+# the destroy calls accept NULL like free(); every other int entry point
+# returns BCM_ERR_ARG.
+_COUNTS = {"bcm_tree_num_blocks", "bcm_sor_get_variant", "bcm_server_poll"}
+_DESTROYS = {"bcm_tree_destroy", "bcm_mesh_destroy", "bcm_field_destroy", "bcm_server_destroy"}
+
+
+def test_every_entry_point_rejects_null_handles_without_gpu():
+    lib = L.load()
+    ints = (ctypes.c_int, ctypes.c_long, ctypes.c_longlong, ctypes.c_size_t, ctypes.c_uint, ctypes.c_int64,
+            ctypes.c_uint64)
+    checked = 0
+    for name, (res, args) in sorted(L.SIGNATURES.items()):
+        if res is not ctypes.c_int:
+            continue
+        vals = [0 if a in ints else 0.0 if a in (ctypes.c_double, ctypes.c_float) else None for a in args]
+        got = getattr(lib, name)(*vals)
+        if name in _DESTROYS:
+            assert got == L.OK, name
+        elif name in _COUNTS:
+            assert got == -L.ERR_ARG, name
+        else:
+            assert got == L.ERR_ARG, (name, got)
+        checked += 1
+    assert checked >= 70
