@@ -6,7 +6,7 @@ its block tables and fields as .npz.  Only runs where the reference tree
 exists (this container); the fixtures are committed and the tests read
 them everywhere.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [fixture names: all when none]
 """
 import os
 import subprocess
@@ -41,6 +41,10 @@ GHOSTS = {
     "ghost_sphere_atonce_f": (0, "f", "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
     "ghost_simple_vc2_d": (0, "d", "simple 1 2 Z 1 1 1 dir 8 2"),
     "ghost_periodic_x_d": (0, "d", "simple 1 3 Z 1 1 1 x 4 1"),
+    # two-cell ghost frames across level differences (F2C sums and C2F
+    # interpolation reach two cells deep)
+    "ghost_sphere_vc2_d": (0, "d", "sphere 1 3 Z 1 1 1 dir 4 2 0.5 0.5 0.5 0.3"),
+    "ghost_sphere_vc2_leveldiff_f": (2, "f", "sphere 1 3 Hilbert 1 1 1 dir 4 2 0.5 0.5 0.5 0.3"),
 }
 
 SOR1 = {
@@ -72,12 +76,17 @@ def run(args, cwd):
     return out.stdout
 
 
-def main():
+def main(only=()):
+    def wanted(name):
+        return not only or name in only
+
     if not os.path.isdir(REF):
         sys.exit("reference tree not present; fixtures are only regenerated where it is")
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref")], check=True)
     with tempfile.TemporaryDirectory() as tmp:
         for name, margs in MESHES.items():
+            if not wanted(name):
+                continue
             out = os.path.join(tmp, name + ".bin")
             run(["mesh", out] + margs.split(), tmp)
             blocks, n, vc, _ = read_dump(out, 0, np.float64)
@@ -85,6 +94,8 @@ def main():
                                 args=margs)
             print(name, len(blocks))
         for name, (method, dt, margs) in GHOSTS.items():
+            if not wanted(name):
+                continue
             out = os.path.join(tmp, name + ".bin")
             run(["ghost", out, str(method), dt] + margs.split(), tmp)
             dtype = np.float64 if dt == "d" else np.float32
@@ -93,6 +104,8 @@ def main():
                                 method=method, fin=fin, fout=fout, args=margs)
             print(name, len(blocks), fin.shape)
         for name, (conf, rn, ordering) in SOR1.items():
+            if not wanted(name):
+                continue
             cfile = os.path.join(tmp, name + ".conf")
             if conf is None:
                 with open(os.path.join(REF, "examples", "sor1", "test.conf")) as fp:
@@ -116,4 +129,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

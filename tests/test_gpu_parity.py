@@ -40,7 +40,8 @@ def gpu_update(mesh, fld, method):
 
 
 @pytest.mark.parametrize("name", ["ghost_sphere_atonce_d", "ghost_sphere_xyz_d", "ghost_sphere_leveldiff_d",
-                                  "ghost_sphere_atonce_f", "ghost_simple_vc2_d", "ghost_periodic_x_d"])
+                                  "ghost_sphere_atonce_f", "ghost_simple_vc2_d", "ghost_periodic_x_d",
+                                  "ghost_sphere_vc2_d", "ghost_sphere_vc2_leveldiff_f"])
 def test_ghost_fill_vs_reference_fixture(name):
     g = load_golden(name)
     case(f"ghost fill {name} nb={len(g['blocks'])} n={g['n']} vc={g['vc']}")

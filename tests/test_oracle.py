@@ -18,7 +18,7 @@ from bcmtools_amd import workloads as W
 from conftest import load_golden
 
 GHOST_FIXTURES = ["ghost_sphere_atonce_d", "ghost_sphere_xyz_d", "ghost_sphere_leveldiff_d", "ghost_sphere_atonce_f",
-                  "ghost_simple_vc2_d", "ghost_periodic_x_d"]
+                  "ghost_simple_vc2_d", "ghost_periodic_x_d", "ghost_sphere_vc2_d", "ghost_sphere_vc2_leveldiff_f"]
 
 
 def oracle_update(blocks, n, vc, data, method):
