@@ -121,8 +121,9 @@ def bench_sor(args):
                    "interior_cells": ncell, "omega": omega},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("rbsor_halo_n32"),
-                     "kernel": "k_rbsor_halo<double,32,true> (HIP-event average over the call's sweeps; "
-                               "the first of each call is k_rbsor_fused on the stored ghosts)",
+                     "kernel": "k_rbsor_halo<double,32,true,false> (HIP-event average over the call's sweeps; "
+                               "the first of each call is k_rbsor_fused on the stored ghosts, the last "
+                               "k_rbsor_halo<double,32,true,true>, which writes the ghosted field)",
                      "avg_ms": sor_avg * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes},
         "kernels_ms_per_step": {"rbsor": sor_ms / max(args.steps, 1), "ghost_fill": gh_ms / max(args.steps, 1),
