@@ -437,9 +437,18 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   // waits for earlier loads drain more than they need).
   auto store_own = [&](int p, const T* o) __attribute__((always_inline)) {
     if constexpr (GOUT) {
+      // The row's two X ghost cells are written as well (with the row's end
+      // values), so the rows j = 0..N-1 of a plane form one contiguous span
+      // of whole cache lines instead of rows with 8-byte holes at both ends
+      // (partial lines cost a read-modify-write).  Those cells are not part
+      // of the result: this sweep only runs on meshes whose every face is
+      // connected or Dirichlet, and the call ends with the ghost fill and
+      // the Dirichlet mirror, which overwrite them.
       T* g = gout + (long long)b * PL * NX + (long long)(p + 1) * PL + LQ(0);
+      if (xlo) g[-1] = o[0];
       g[0] = o[0];
       g[1] = o[1];
+      if (xhi) g[2] = o[1];
       return;
     }
     store_pair(Bb + (long long)p * NN + own, o[0], o[1]);
