@@ -171,7 +171,6 @@ int dirichlet_device(bcm_mesh* m, bcm_field* f);
 // column export it reads.
 template <typename T>
 int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout);
-template <typename T> int to_compact_raw(bcm_mesh* m, const T* f, T* c);
 // Block-wise ghost fill (all faces, all kinds; n in {8, 16, 32}, vc = 1).
 template <typename T> int ghost_gather_raw(bcm_mesh* m, T* data);
 // Frees the fused solver's state (bcm_krylov.hip).

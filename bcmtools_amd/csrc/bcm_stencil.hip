@@ -106,6 +106,13 @@ __device__ __forceinline__ T sor_update(T fc, T fw, T fe, T fs, T fn, T fb, T ft
   }
 }
 
+static __device__ __forceinline__ void store2(double* p, double a, double b) {
+  *reinterpret_cast<double2*>(p) = make_double2(a, b);
+}
+static __device__ __forceinline__ void store2(float* p, float a, float b) {
+  *reinterpret_cast<float2*>(p) = make_float2(a, b);
+}
+
 // Fused two-colour sweep of one block per work-group, vc == 1.
 // Thread t owns the 2x2 patch (2pi..2pi+1, 2pj..2pj+1) of every z-plane,
 // so each thread updates two red and two black cells per plane.  A ring
@@ -116,7 +123,8 @@ __device__ __forceinline__ T sor_update(T fc, T fw, T fe, T fs, T fn, T fb, T ft
 template <typename T, int N, bool SRC>
 __global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, const T* __restrict__ s,
                                                          const double* __restrict__ coef, double omega,
-                                                         T* __restrict__ cout, T* __restrict__ xout) {
+                                                         T* __restrict__ cout, T* __restrict__ xout,
+                                                         T* __restrict__ scout) {
   constexpr int NX = N + 2;
   constexpr int PL = NX * NX;
   constexpr int NT = N * N / 4;
@@ -164,18 +172,26 @@ __global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, con
   };
   // The swept plane back to f, or (cout given) to the compact iterate of
   // the halo schedule together with its X-face export (columns 0, 1, N-2,
-  // N-1); f then keeps its old values.
-  auto store = [&](int p) {
+  // N-1); f then keeps its old values.  scout (with cout, SRC only)
+  // receives the plane's source values compact, for the later sweeps.
+  auto store = [&](int p, const T* sv) {
     const int P = p & 3;
     if (cout) {
-      T* dst = cout + (long long)b * N * N * N + (long long)p * N * N;
+      // Each row pair (i0, i0+1) starts at an even compact index: one
+      // aligned two-value store per row and array.
+      const long long cp = (long long)b * N * N * N + (long long)p * N * N;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = i0 + (q & 1), j = j0 + (q >> 1);
-        const T v = lds[P][L(i, j)];
-        dst[i + N * j] = v;
-        const int col = i < 2 ? i : (i >= N - 2 ? i - (N - 4) : -1);
-        if (col >= 0) xout[((long long)(b * 4 + col) * N + p) * N + j] = v;
+      for (int r = 0; r < 2; ++r) {
+        const int j = j0 + r;
+        const T v0 = lds[P][L(i0, j)], v1 = lds[P][L(i0 + 1, j)];
+        store2(cout + cp + i0 + N * j, v0, v1);
+        if (SRC && scout) store2(scout + cp + i0 + N * j, sv[2 * r], sv[2 * r + 1]);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const int i = i0 + a;
+          const int col = i < 2 ? i : (i >= N - 2 ? i - (N - 4) : -1);
+          if (col >= 0) xout[((long long)(b * 4 + col) * N + p) * N + j] = a ? v1 : v0;
+        }
       }
       return;
     }
@@ -208,14 +224,14 @@ __global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, con
     __syncthreads();
     if (k >= 1) {
       update(k - 1, 1, s_prev);
-      store(k - 1);
+      store(k - 1, s_prev);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) s_prev[q] = s_cur[q];
   }
   update(N - 1, 1, s_prev);
-  store(N - 1);
+  store(N - 1, s_prev);
 }
 
 // Generic one-colour pass for any n, vc: one thread per cell of the colour.
@@ -243,10 +259,11 @@ __global__ __launch_bounds__(256) void k_rbsor_colour(T* __restrict__ f, const T
 }
 
 // cout (vc = 1, n = 16 or 32 only): write the swept iterate compact, with
-// its X-face export, instead of back into f.
+// its X-face export, instead of back into f; scout: the source term
+// compact as well.
 template <typename T>
 static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_passes = false, T* cout = nullptr,
-                     T* xout = nullptr) {
+                     T* xout = nullptr, T* scout = nullptr) {
   const int form = s ? 2 : 1;
   BCM_TRY(sor_coef(m, omega, form));
   EventPair ev;
@@ -255,11 +272,11 @@ static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_pa
   if (colour_passes) {
     // fall through to the generic kernel below
   } else if (m->vc == 1 && m->n == 32) {
-    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 32, true>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
-    else hipLaunchKernelGGL((k_rbsor_fused<T, 32, false>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
+    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 32, true>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout, scout);
+    else hipLaunchKernelGGL((k_rbsor_fused<T, 32, false>), dim3(nb), dim3(256), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout, scout);
   } else if (m->vc == 1 && m->n == 16) {
-    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 16, true>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
-    else hipLaunchKernelGGL((k_rbsor_fused<T, 16, false>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout);
+    if (s) hipLaunchKernelGGL((k_rbsor_fused<T, 16, true>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout, scout);
+    else hipLaunchKernelGGL((k_rbsor_fused<T, 16, false>), dim3(nb), dim3(64), 0, m->stream, f, s, m->d_sor_coef, omega, cout, xout, scout);
   }
   if (colour_passes || !(m->vc == 1 && (m->n == 32 || m->n == 16))) {
     const CellGeo g = cell_geo(m);
@@ -674,10 +691,11 @@ static int sor_iterate_halo(bcm_field_t f, bcm_field_t source, double omega, int
   if (source) {
     s = static_cast<T*>(workspace(m, f64 ? WS_SOR_CS_F64 : WS_SOR_CS_F32, cbytes, &rc));
     if (rc != BCM_OK) return rc;
-    BCM_TRY(to_compact_raw<T>(m, sg, s));
   }
   T* xe[2] = {static_cast<T*>(m->d_xexp[0]), static_cast<T*>(m->d_xexp[1])};
-  BCM_TRY(rbsor_raw<T>(m, d, sg, omega, false, A, xe[0]));
+  // The first sweep reads f's stored ghosts and writes the compact iterate,
+  // its X export and the compact source term.
+  BCM_TRY(rbsor_raw<T>(m, d, sg, omega, false, A, xe[0], s));
   int cur = 0;
   for (int it = 1; it < iters; ++it) {
     BCM_TRY(halo_sweep_raw<T>(m, A, B, s, omega, xe[cur], xe[cur ^ 1], it == iters - 1 ? d : nullptr));

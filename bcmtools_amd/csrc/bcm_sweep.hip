@@ -528,29 +528,6 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   store_own(N - 1, um1);
 }
 
-// Ghosted field -> compact iterate (interior cells, x fastest) and, when
-// xout is given, its X-face export; and back.
-// Ghosted field -> compact interior (x fastest), one work-group per block;
-// each thread moves UNR values per pass with all loads issued first.
-template <typename T, int N>
-__global__ __launch_bounds__(256) void k_to_compact(const T* __restrict__ f, T* __restrict__ c) {
-  constexpr int NX = N + 2, NN = N * N, UNR = 8;
-  static_assert((NN * N) % (256 * UNR) == 0, "block volume");
-  const int b = blockIdx.x;
-  const T* fb = f + (long long)b * NX * NX * NX;
-  T* cb = c + (long long)b * NN * N;
-  for (int e0 = threadIdx.x; e0 < NN * N; e0 += 256 * UNR) {
-    T v[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      const int e = e0 + 256 * u, i = e % N, j = (e / N) % N, k = e / NN;
-      v[u] = fb[(i + 1) + NX * ((j + 1) + NX * (k + 1))];
-    }
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) cb[e0 + 256 * u] = v[u];
-  }
-}
-
 template <typename T, int N>
 static void launch_halo(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout) {
   const dim3 grid(m->nblocks), blk(N * N / 2);
@@ -583,15 +560,5 @@ template int halo_sweep_raw<double>(bcm_mesh*, const double*, double*, const dou
                                     double*);
 template int halo_sweep_raw<float>(bcm_mesh*, const float*, float*, const float*, double, const float*, float*, float*);
 
-template <typename T>
-int to_compact_raw(bcm_mesh* m, const T* f, T* c) {
-  if (m->n == 32) hipLaunchKernelGGL((k_to_compact<T, 32>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c);
-  else if (m->n == 16) hipLaunchKernelGGL((k_to_compact<T, 16>), dim3(m->nblocks), dim3(256), 0, m->stream, f, c);
-  else return BCM_ERR_ARG;
-  BCM_HIP_TRY(hipGetLastError());
-  return BCM_OK;
-}
-template int to_compact_raw<double>(bcm_mesh*, const double*, double*);
-template int to_compact_raw<float>(bcm_mesh*, const float*, float*);
 
 }  // namespace bcm
