@@ -201,6 +201,17 @@ __global__ __launch_bounds__(N* N / 4) void k_rbsor_fused(T* __restrict__ f, con
       const int i = i0 + (q & 1), j = j0 + (q >> 1);
       dst[L(i, j)] = lds[P][L(i, j)];
     }
+    // The rows' X ghost cells go back too, with the values read into LDS
+    // (the sweep never changes a ghost, and only this work-group writes the
+    // block), so the rows j = 0..N-1 of the plane are stored as one
+    // contiguous span instead of rows with holes at both ends: whole cache
+    // lines, no partial-line read-modify-write.
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int j = j0 + r;
+      if (i0 == 0) dst[L(-1, j)] = lds[P][L(-1, j)];
+      if (i0 == N - 2) dst[L(N, j)] = lds[P][L(N, j)];
+    }
   };
   auto load_s = [&](int p, T* sv) {
     if (!SRC) return;
