@@ -120,6 +120,9 @@ struct bcm_mesh {
   double progress_s = 0.0;
   // Scheduling of bcm_sor_iterate (BCM_SOR_*), set by hand or by the autotuner.
   int sor_variant = 0;
+  // Live fields of the mesh: destroying the mesh frees their device memory
+  // and detaches them, so a later bcm_field_destroy only frees the handle.
+  std::vector<struct bcm_field*> fields;
 };
 
 namespace bcm {
@@ -148,6 +151,29 @@ int set_bvalues(bcm_mesh* m, const double* values, int n);
 int reduce_partials(bcm_mesh* m, int nparts, int slot);
 int reduce_partials_at(bcm_mesh* m, const double* parts, int nparts, int slot);
 int fetch_result(bcm_mesh* m, int nslots, double* out);
+
+// Deterministic sum of parts[0..n) by one 256-thread work-group: each
+// thread sums its partials in order (the loads of eight issued together,
+// then added one after the other), then a wave64 butterfly and a
+// fixed-order sum of the four wave totals.  Thread 0 returns the total.
+// red: 4 doubles of LDS.
+__device__ __forceinline__ double sum_partials_wg256(const double* __restrict__ parts, int n, double* red) {
+  constexpr int UNR = 8;
+  double s = 0.0;
+  int i = threadIdx.x;
+  for (; i + (UNR - 1) * 256 < n; i += UNR * 256) {
+    double v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) v[u] = parts[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) s += v[u];
+  }
+  for (; i < n; i += 256) s += parts[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return ((red[0] + red[1]) + red[2]) + red[3];
+}
 
 // Device allocation helpers.
 int dev_alloc(void** p, size_t bytes);

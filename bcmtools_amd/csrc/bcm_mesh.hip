@@ -316,24 +316,8 @@ int dirichlet_device(bcm_mesh* m, bcm_field* f) {
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ parts, int n,
                                                          double* __restrict__ out) {
   __shared__ double red[4];
-  // Each thread sums its partials in order; the loads of UNR of them are
-  // issued together and then added one after the other (the same sum as a
-  // plain loop, with the loads overlapped).
-  constexpr int UNR = 8;
-  double s = 0.0;
-  int i = threadIdx.x;
-  for (; i + (UNR - 1) * 256 < n; i += UNR * 256) {
-    double v[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) v[u] = parts[i + u * 256];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) s += v[u];
-  }
-  for (; i < n; i += 256) s += parts[i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) out[0] = ((red[0] + red[1]) + red[2]) + red[3];
+  const double s = sum_partials_wg256(parts, n, red);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 int reduce_partials(bcm_mesh* m, int nparts, int slot) { return reduce_partials_at(m, m->d_partials, nparts, slot); }
@@ -384,6 +368,11 @@ static void mesh_release(bcm_mesh* m) {
   dev_free(m->d_xexp[0]);
   dev_free(m->d_xexp[1]);
   for (void* p : m->ws) dev_free(p);
+  for (bcm_field* f : m->fields) {
+    dev_free(f->data);
+    f->data = nullptr;
+    f->mesh = nullptr;
+  }
   if (m->h_result) (void)hipHostFree(m->h_result);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -680,13 +669,17 @@ extern "C" int bcm_field_create(bcm_mesh_t m, int dtype, bcm_field_t* out) {
     delete f;
     return BCM_ERR_HIP;
   }
+  m->fields.push_back(f);
   *out = f;
   return BCM_OK;
 }
 
 extern "C" int bcm_field_destroy(bcm_field_t f) {
   if (!f) return BCM_OK;
-  if (f->mesh) (void)hipStreamSynchronize(f->mesh->stream);
+  if (bcm_mesh* m = f->mesh) {
+    (void)hipStreamSynchronize(m->stream);
+    m->fields.erase(std::remove(m->fields.begin(), m->fields.end(), f), m->fields.end());
+  }
   dev_free(f->data);
   delete f;
   return BCM_OK;

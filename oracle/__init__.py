@@ -124,6 +124,29 @@ def sor_iterate(blocks, n, vc, f, s, omega, values, iters):
                                 v.ctypes.data, int(iters))
 
 
+def sor_residual(blocks, n, vc, f, s):
+    """Relative residual of the sor2 system lap_h(f) = s over the interior
+    cells: ||s - lap_h f||_2 / ||s||_2, lap_h f = cx((fW + fE) - 2 fc) +
+    cy((fS + fN) - 2 fc) + cz((fB + fT) - 2 fc) with c = 1/h^2 per axis of
+    each block (the operator whose sweep examples/sor2/Solver.cpp:136-160
+    relaxes; the reference itself reports no residual).  Plain numpy."""
+    nb = len(blocks)
+    v = vc
+    c = f[:, v:v + n, v:v + n, v:v + n]
+    h = blocks["block_size"] / n
+    lam = 1.0 / (h * h)
+    cx, cy, cz = (lam[:, a][:, None, None, None] for a in range(3))
+    lap = ((cx * ((f[:, v:v + n, v:v + n, v - 1:v + n - 1] + f[:, v:v + n, v:v + n, v + 1:v + n + 1]) - 2.0 * c)
+            + cy * ((f[:, v:v + n, v - 1:v + n - 1, v:v + n] + f[:, v:v + n, v + 1:v + n + 1, v:v + n]) - 2.0 * c))
+           + cz * ((f[:, v - 1:v + n - 1, v:v + n, v:v + n] + f[:, v + 1:v + n + 1, v:v + n, v:v + n]) - 2.0 * c))
+    si = s[:, v:v + n, v:v + n, v:v + n]
+    r = si - lap
+    ss = float(np.sum(si * si))
+    rr = float(np.sum(r * r))
+    assert nb == f.shape[0]
+    return float(np.sqrt(rr / ss)) if ss > 0.0 else float(np.sqrt(rr))
+
+
 def setup_mask(shape, vc, dtype=np.float64):
     """setup_mask (examples/SolverTest/blas.f90:700-751): every cell 0, then
     the interior cells 1.  `shape` is (nblocks, n+2vc, n+2vc, n+2vc)."""

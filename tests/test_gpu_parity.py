@@ -340,6 +340,51 @@ def test_halo_gathering_sor_iterations_vs_oracle(n, maker, form):
     np.testing.assert_array_equal(f.download(), ref)
 
 
+def test_sor_residual_vs_oracle():
+    """bcm_sor_residual after a few outer iterations against the oracle's
+    ||s - lap_h f|| / ||s|| (FP64 reductions in another order: rel 1e-12)."""
+    blocks, n = W.shell_tree(3).blocks, 16
+    case(f"sor residual nb={len(blocks)}")
+    s = -W.rhs_field(blocks, n, 1, seed=11)
+    mesh = Mesh(blocks, n, 1)
+    f = mesh.field(np.float64)
+    sf = mesh.field(np.float64, s)
+    mesh.dirichlet_mirror(f, [0.0])
+    for it in (0, 3, 40):
+        if it:
+            mesh.sor_iterate(f, sf, 1.4, [0.0], it)
+        got = mesh.sor_residual(f, sf)
+        want = oracle.sor_residual(blocks, n, 1, f.download(), s)
+        assert abs(got - want) <= 1e-12 * want, (it, got, want)
+
+
+def test_sor_to_tolerance_vs_oracle():
+    """The sor2 outer iteration run to relres 1e-8 at omega 1.4 (omega 1.6
+    diverges with ghosts frozen between exchanges) on the level-3 shell mesh
+    with 8^3 blocks: the GPU iterate after the same number of sweeps equals
+    the oracle's bit for bit, and the oracle's residual of it is <= 1e-8."""
+    blocks, n, om = W.shell_tree(3).blocks, 8, 1.4
+    case(f"sor to tolerance nb={len(blocks)} n={n}")
+    s = -W.rhs_field(blocks, n, 1, seed=12)
+    mesh = Mesh(blocks, n, 1)
+    f = mesh.field(np.float64)
+    sf = mesh.field(np.float64, s)
+    mesh.dirichlet_mirror(f, [0.0])
+    sweeps, chunk = 0, 250
+    while True:
+        mesh.sor_iterate(f, sf, om, [0.0], chunk)
+        sweeps += chunk
+        rr = mesh.sor_residual(f, sf)
+        if rr <= 1e-8 or sweeps >= 20000:
+            break
+    assert rr <= 1e-8, (sweeps, rr)
+    ref = np.zeros_like(s)
+    oracle.dirichlet(blocks, n, 1, ref, [0.0])
+    oracle.sor_iterate(blocks, n, 1, ref, s, om, [0.0], sweeps)
+    np.testing.assert_array_equal(f.download(), ref)
+    assert oracle.sor_residual(blocks, n, 1, ref, s) <= 1e-8
+
+
 def test_full_size_config2_sor_vs_oracle():
     """Config 2 at full size: 4 outer iterations (1 classic + 3 halo-gathering) bit-exact vs the oracle."""
     t = W.config2_tree()
@@ -398,13 +443,15 @@ def test_every_sor_schedule_matches_the_oracle_and_autotune(n, maker):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("iters", [1, 2, 3])
-def test_halo_schedule_equals_classic_schedule(dtype, iters):
+@pytest.mark.parametrize("n,level", [(16, 3), (32, 2)])
+def test_halo_schedule_equals_classic_schedule(dtype, iters, n, level):
     """The halo-gathering schedule (compact iterates; its last sweep writes
     the field) gives the classic schedule's bits for 1, 2 and 3 iterations,
-    FP64 and FP32, sor1 and sor2 forms."""
-    blocks = W.shell_tree(3).blocks
-    n, vals = 16, [0.25]
-    case(f"halo vs classic {np.dtype(dtype).name} iters={iters} nb={len(blocks)}")
+    FP64 and FP32, sor1 and sor2 forms, blocks of 16^3 and 32^3 (every
+    instance of the first, middle and last sweep kernels)."""
+    blocks = W.shell_tree(level).blocks
+    vals = [0.25]
+    case(f"halo vs classic {np.dtype(dtype).name} n={n} iters={iters} nb={len(blocks)}")
     rng = np.random.default_rng(iters)
     f0 = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2)).astype(dtype)
     s = W.rhs_field(blocks, n, 1, seed=5).astype(dtype)

@@ -197,6 +197,51 @@ def test_frozen_ghost_sor_diverges_at_omega_1_6():
     assert out[1.6] > 1e10
 
 
+def _cell_centres(blocks, n, vc):
+    """Cell-centre coordinates of every stored cell (ghosts included), each (nb, nx0, nx0, nx0)."""
+    idx = np.arange(-vc, n + vc) + 0.5
+    h = blocks["block_size"] / n
+    o = blocks["origin"]
+    x = o[:, 0, None] + idx[None, :] * h[:, 0, None]
+    y = o[:, 1, None] + idx[None, :] * h[:, 1, None]
+    z = o[:, 2, None] + idx[None, :] * h[:, 2, None]
+    return (x[:, None, None, :] + 0 * y[:, None, :, None], y[:, None, :, None] + 0 * x[:, None, None, :],
+            z[:, :, None, None] + 0 * x[:, None, None, :])
+
+
+def test_sor_residual_pinned_by_exact_quadratics():
+    """The second difference is exact on quadratics: f = x^2 + 2 y^2 + 3 z^2
+    (ghosts included) has lap_h f = 12 in every block of an adaptive mesh,
+    whatever its cell size.  A dropped term, a wrong sign, a wrong 1/h^2 or
+    a shifted neighbour index moves the residual of s = 12 off zero, and
+    f = 0 gives relres 1 exactly."""
+    blocks, n, vc = W.shell_tree(3).blocks, 8, 1
+    x, y, z = _cell_centres(blocks, n, vc)
+    f = (x * x + 2.0 * y * y) + 3.0 * z * z
+    s = np.full_like(f, 12.0)
+    assert oracle.sor_residual(blocks, n, vc, f, s) < 1e-9
+    assert oracle.sor_residual(blocks, n, vc, np.zeros_like(f), s) == 1.0
+    # s = 13 leaves the residual 1 in every cell: relres = 1/13.
+    assert abs(oracle.sor_residual(blocks, n, vc, f, s + 1.0) - 1.0 / 13.0) < 1e-9
+    # Brute force over every interior cell of a small random case.
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal(f.shape)
+    src = rng.standard_normal(f.shape)
+    rr = ss = 0.0
+    for b in range(len(blocks)):
+        lam = 1.0 / (blocks["block_size"][b] / n) ** 2
+        for k in range(vc, vc + n):
+            for j in range(vc, vc + n):
+                for i in range(vc, vc + n):
+                    c = g[b, k, j, i]
+                    lap = (lam[0] * (g[b, k, j, i - 1] - 2 * c + g[b, k, j, i + 1])
+                           + lam[1] * (g[b, k, j - 1, i] - 2 * c + g[b, k, j + 1, i])
+                           + lam[2] * (g[b, k - 1, j, i] - 2 * c + g[b, k + 1, j, i]))
+                    rr += (src[b, k, j, i] - lap) ** 2
+                    ss += src[b, k, j, i] ** 2
+    assert abs(oracle.sor_residual(blocks, n, vc, g, src) - np.sqrt(rr / ss)) < 1e-12 * np.sqrt(rr / ss)
+
+
 # ---------------------------------------------------------------------------
 # SolverTest kernels vs a numpy restatement with the same operand order.
 # ---------------------------------------------------------------------------
