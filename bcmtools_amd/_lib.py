@@ -193,6 +193,7 @@ SIGNATURES = {
     "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
+    "bcm_sor_residual_noalloc": (_I, [_P, _P, _DP]),
     "bcm_sor_set_variant": (_I, [_P, _I]),
     "bcm_sor_get_variant": (_I, [_P]),
     "bcm_sor_autotune": (_I, [_P, _P, _D, _DP, _I, _I, _IP, _DP]),

@@ -10,6 +10,7 @@
 // disabled at compile time, so FP64 results are bit-identical to the
 // CPU path (no fused multiply-add, IEEE division).
 #include <cmath>
+#include <mutex>
 #include <utility>
 
 #include "bcm_internal.h"
@@ -303,8 +304,13 @@ static int rbsor_raw(bcm_mesh* m, T* f, const T* s, double omega, bool colour_pa
 }
 
 // Relative residual of lap_h f = s: r = s - (cx(W+E-2C) + cy(..) + cz(..)).
+// The sor2 coefficients 1/h^2 per axis come from the per-block table
+// {c0, cx, cy, cz} (FROM_META false) or from the mesh's BlockMeta, which
+// holds the same values computed the same way (FROM_META true).
+template <bool FROM_META>
 __global__ __launch_bounds__(kWG) void k_sor_residual(const double* __restrict__ f, const double* __restrict__ s,
-                                                      const double* __restrict__ coef_sor2, CellGeo g,
+                                                      const double* __restrict__ coef_sor2,
+                                                      const BlockMeta* __restrict__ meta, CellGeo g,
                                                       double* __restrict__ parts) {
   __shared__ double red[4];
   const int sy = g.nx0, sz = g.nx0 * g.nx0;
@@ -315,7 +321,9 @@ __global__ __launch_bounds__(kWG) void k_sor_residual(const double* __restrict__
     if (c >= g.ncell) break;
     int b, i, j, k;
     const long long o = cell_offset(g, c, b, i, j, k);
-    const double cx = coef_sor2[4 * b + 1], cy = coef_sor2[4 * b + 2], cz = coef_sor2[4 * b + 3];
+    const double cx = FROM_META ? meta[b].cx : coef_sor2[4 * b + 1];
+    const double cy = FROM_META ? meta[b].cy : coef_sor2[4 * b + 2];
+    const double cz = FROM_META ? meta[b].cz : coef_sor2[4 * b + 3];
     const double fc = f[o];
     const double lap = (cx * ((f[o - 1] + f[o + 1]) - 2.0 * fc) + cy * ((f[o - sy] + f[o + sy]) - 2.0 * fc)) +
                        cz * ((f[o - sz] + f[o + sz]) - 2.0 * fc);
@@ -825,14 +833,94 @@ extern "C" int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relre
   if (size_t(2 * grid) > m->n_partials) return BCM_ERR_STATE;
   EventPair ev;
   BCM_TRY(timer_begin(m, BCM_K_RESID, &ev));
-  hipLaunchKernelGGL(k_sor_residual, dim3(grid), dim3(kWG), 0, m->stream, static_cast<const double*>(f->data),
-                     static_cast<const double*>(source->data), m->d_sor_coef, cell_geo(m), m->d_partials);
+  hipLaunchKernelGGL(k_sor_residual<false>, dim3(grid), dim3(kWG), 0, m->stream,
+                     static_cast<const double*>(f->data), static_cast<const double*>(source->data), m->d_sor_coef,
+                     (const BlockMeta*)nullptr, cell_geo(m), m->d_partials);
   BCM_HIP_TRY(hipGetLastError());
   BCM_TRY(timer_end(m, &ev));
   BCM_TRY(reduce_partials(m, grid, 0));
   BCM_TRY(reduce_partials_at(m, m->d_partials + grid, grid, 1));
   double v[2];
   BCM_TRY(fetch_result(m, 2, v));
+  *relres = v[1] > 0.0 ? std::sqrt(v[0] / v[1]) : std::sqrt(v[0]);
+  return BCM_OK;
+}
+
+// ---------------------------------------------------------------------
+// bcm_sor_residual without allocation.  The partial sums and the two
+// totals live in `scratch`, a file-scope device array every GPU holds once
+// for the life of the process: kNoAllocParts partials per sum (one per
+// work-group of kCellsPerWG cells, so meshes up to
+// kNoAllocParts * kCellsPerWG = 268,435,456 interior cells) plus the two
+// totals.  Every partial a call reads is written by the same call, so
+// nothing needs clearing; calls that overlap on one GPU share the array
+// and race.
+// ---------------------------------------------------------------------
+constexpr int kNoAllocParts = 1 << 18;
+constexpr int kNoAllocScratch = 2 * kNoAllocParts + 64;  // 4 MiB + 512 B
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+// Both totals in one launch: work-group w sums parts[w*n .. w*n+n) into out[w].
+__global__ __launch_bounds__(256) void k_reduce_pair(const double* __restrict__ parts, int n,
+                                                     double* __restrict__ out) {
+  __shared__ double red[4];
+  const double s = sum_partials_wg256(parts + (long long)blockIdx.x * n, n, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+static int scratch_on(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return BCM_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    BCM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) BCM_HIP_TRY(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) BCM_HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess || !p) {
+      std::fprintf(stderr, "bcm: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return BCM_ERR_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return BCM_OK;
+}
+
+extern "C" int bcm_sor_residual_noalloc(bcm_field_t f, bcm_field_t source, double* relres) {
+  if (!f || !source || !relres || !same_mesh(f, source) || f->dtype != BCM_F64) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  const long long grid = ((long long)m->ncell + kCellsPerWG - 1) / kCellsPerWG;
+  if (grid > kNoAllocParts) {
+    std::fprintf(stderr,
+                 "bcm: bcm_sor_residual_noalloc: %lld interior cells exceed its fixed scratch (at most %lld)\n",
+                 (long long)m->ncell, (long long)kNoAllocParts * kCellsPerWG);
+    return BCM_ERR_ARG;
+  }
+  double* ptr = nullptr;
+  BCM_TRY(scratch_on(m->device, &ptr));
+  double* parts = ptr;
+  double* totals = ptr + 2 * kNoAllocParts;
+  // No per-kernel timing here: the timer's events come from a pool that
+  // grows on demand.
+  hipLaunchKernelGGL(k_sor_residual<true>, dim3(int(grid)), dim3(kWG), 0, m->stream,
+                     static_cast<const double*>(f->data), static_cast<const double*>(source->data),
+                     (const double*)nullptr, m->d_meta, cell_geo(m), parts);
+  BCM_HIP_TRY(hipGetLastError());
+  // k_sor_residual writes the r^2 partials at parts[0..grid) and the s^2
+  // partials at parts[grid..2 grid).
+  hipLaunchKernelGGL(k_reduce_pair, dim3(2), dim3(256), 0, m->stream, (const double*)parts, int(grid), totals);
+  BCM_HIP_TRY(hipGetLastError());
+  double v[2] = {0.0, 0.0};
+  BCM_HIP_TRY(hipMemcpyAsync(v, totals, sizeof(v), hipMemcpyDeviceToHost, m->stream));
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
   *relres = v[1] > 0.0 ? std::sqrt(v[0] / v[1]) : std::sqrt(v[0]);
   return BCM_OK;
 }

@@ -291,6 +291,16 @@ int bcm_sor_autotune(bcm_field_t f, bcm_field_t source, double omega, const doub
 /* Relative residual of the sor2 system lap_h(f) = s:
  * ||s - lap_h f||_2 / ||s||_2 over interior cells (FP64 reduction). */
 int bcm_sor_residual(bcm_field_t f, bcm_field_t source, double* relres);
+/* No counterpart in the reference.  bcm_sor_residual for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same kernels and
+ * the same result bits, with the partial sums in a fixed device array
+ * that every GPU holds once for the life of the process (4 MiB), instead
+ * of the mesh's coefficient table, which bcm_sor_residual allocates on
+ * first use.  Calls no hipMalloc/hipFree.  Meshes of at most 268,435,456
+ * interior cells; larger ones return BCM_ERR_ARG.  Calls that overlap on
+ * one GPU (from several host threads) share the array and race. */
+int bcm_sor_residual_noalloc(bcm_field_t f, bcm_field_t source, double* relres);
 
 /* ------------------------------------------------------------------ */
 /* SolverTest per-block kernels (examples/SolverTest/blas.h:23-126),   */
