@@ -93,6 +93,7 @@ class IRParams(ctypes.Structure):
         ("count_pre", ctypes.c_int32),
         ("omega", ctypes.c_double),
         ("fp32_only", ctypes.c_int32),
+        ("fp32_strict", ctypes.c_int32),
     ]
 
 

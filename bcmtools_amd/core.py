@@ -428,7 +428,7 @@ class Mesh:
         return rc, cnt.value, res.value
 
     def solve_fp64(self, x, rhs, bvalues, tol=1e-10, max_iter=2000, count_pre=2, omega=0.8):
-        p = L.IRParams(float(tol), float(tol), int(max_iter), 1, int(count_pre), float(omega), 0)
+        p = L.IRParams(float(tol), float(tol), int(max_iter), 1, int(count_pre), float(omega), 0, 0)
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
         it = ctypes.c_int()
         rr = ctypes.c_double()
@@ -461,9 +461,12 @@ class Mesh:
                  "relres": e.relres} for e in st]
 
     def solve_mixed(self, x, rhs, bvalues, tol=1e-10, inner_tol=1e-4, inner_max=500, outer_max=20, count_pre=2,
-                    omega=0.8, fp32_only=False):
+                    omega=0.8, fp32_only=False, fp32_strict=False):
+        """FP32 inner BiCGSTAB inside FP64 iterative refinement (bcm_solve_mixed).
+        fp32_strict: stop when an FP32 correction does not lower the FP64
+        residual, instead of solving the remaining corrections in FP64."""
         p = L.IRParams(float(tol), float(inner_tol), int(inner_max), int(outer_max), int(count_pre), float(omega),
-                       int(bool(fp32_only)))
+                       int(bool(fp32_only)), int(bool(fp32_strict)))
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
         outer = ctypes.c_int()
         inner = ctypes.c_int()
@@ -507,6 +510,10 @@ class Server:
         return self.lib.bcm_server_clock(self.handle)
 
     def submit(self, blocks, tol, precision="mixed", rhs=None, seed=0, tag=0, want_x=False):
+        """Queue a request; `tag` must not name a request still pending (the
+        server reads rhs and writes x of the request it names)."""
+        if int(tag) in self._keep:
+            raise L.BCMError(L.ERR_ARG, f"bcm_server_submit (tag {int(tag)} already pending)")
         blocks = np.ascontiguousarray(blocks, dtype=L.BLOCK_DTYPE)
         nx = self.n + 2
         r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).reshape(len(blocks), nx, nx, nx)

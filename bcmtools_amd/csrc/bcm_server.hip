@@ -250,6 +250,11 @@ extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, 
     for (int f = 0; f < 6; ++f)
       for (int q = 0; q < 4; ++q)
         if (blocks[b].nb_id[f][q] < -1 || blocks[b].nb_id[f][q] >= nblocks) return BCM_ERR_ARG;
+  // A tag names one pending request: the caller's buffers are found by it.
+  for (const Request* q : sv->queue)
+    if (q->tag == tag) return BCM_ERR_ARG;
+  for (const Request* q : sv->active)
+    if (q->tag == tag) return BCM_ERR_ARG;
   Request* r = new (std::nothrow) Request();
   if (!r) return BCM_ERR_NOMEM;
   r->tag = tag;

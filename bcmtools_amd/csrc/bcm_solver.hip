@@ -6,6 +6,7 @@
 // The host drives the iteration; every vector operation is a kernel on the
 // mesh stream and each scalar (dot product, norm) is reduced on the device
 // by a fixed tree, then read back, as ILS3D::DOT reads its MPI allreduce.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <utility>
@@ -132,9 +133,26 @@ int jacobi_pre(const Op& op, const Bc<T>& bc, T*& x, T*& x0, const T* b, double 
 // (the operator's), ILS3D::BiCGSTAB_Mask (ILS3D.h:901-1097): the dot
 // products, the preconditioner sweeps and A p_, A s_ visit only mask == 1
 // cells; the initial residual and the vector updates are unmasked.
+//
+// xbest (the FP32 inner solve of iterative refinement, not in ILS3D): the
+// iterate with the lowest recurrence residual so far is kept in xbest, and
+// when the iteration ends without meeting eps, x returns that iterate.  The
+// iteration also ends once the recurrence residual has set no new minimum
+// for kInnerStall iterations after the first kInnerMin (it has reached the
+// accuracy its arithmetic attains, about cond(A) eps32 relative, and would
+// only drift; the first iterations are exempt because BiCGSTAB's residual
+// often rises well above its start before it falls), before an
+// update whose alpha or gamma is not finite (a breakdown: q.r0 or t.t of
+// zero), and at a non-finite residual.  Each of these returns
+// BCM_ERR_NOCONV with the best iterate in x.
+constexpr int kInnerStall = 100;
+constexpr int kInnerMin = 150;
+
 template <typename T, typename Op>
 int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[9], double omega, int count_pre,
-             int count_max, double eps, int* count_out, double* residual, const T* mask = nullptr) {
+             int count_max, double eps, int* count_out, double* residual, const T* mask = nullptr,
+             T* xbest = nullptr) {
+  const bool guard = xbest != nullptr;
   T *r = w[0], *r0 = w[1], *p = w[2], *p_ = w[3], *q_ = w[4], *s = w[5], *s_ = w[6], *t_ = w[7], *x0 = w[8];
   double bb = 0.0;
   BCM_TRY(dot_host<T>(m, b, b, &bb, mask));
@@ -146,6 +164,10 @@ int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[
   BCM_TRY(op.template r<T>(r, x, b));
   BCM_TRY(blas_raw<T>(m, OP_COPY_INTERIOR, r0, r, nullptr, 0.0, 0.0));
   double rr0 = 1.0, alpha = 0.0, gamma = 1.0;
+  double best_rr = HUGE_VAL;
+  int since_best = 0;
+  bool best_is_x = true;  // x = 0 at the start is the only iterate so far
+  if (guard) BCM_TRY(blas_raw<T>(m, OP_COPY_INTERIOR, xbest, x, nullptr, 0.0, 0.0));
   int count = 1;
   int status = BCM_ERR_NOCONV;
   for (count = 1; count <= count_max; ++count) {
@@ -171,6 +193,10 @@ int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[
     double q_r0 = 0.0;
     BCM_TRY(dot_host<T>(m, q_, r0, &q_r0, mask));
     alpha = rr1 / q_r0;
+    if (guard && !std::isfinite(alpha)) {
+      --count;
+      break;
+    }
     BCM_TRY(blas_raw<T>(m, OP_AXPYZ, s, q_, r, -alpha, 0.0));
     BCM_TRY(bc.impose(s));
     BCM_TRY(fill_raw<T>(m, s_, 0.0));
@@ -181,18 +207,40 @@ int bicgstab(bcm_mesh* m, const Op& op, const Bc<T>& bc, T* x, const T* b, T* w[
     BCM_TRY(dot_raw<T>(m, t_, t_, 1, mask));
     BCM_TRY(fetch_result(m, 2, ts));
     gamma = ts[0] / ts[1];
+    if (guard && !std::isfinite(gamma)) {
+      --count;
+      break;
+    }
     BCM_TRY(blas_raw<T>(m, OP_AXPBYPZ, x, p_, s_, alpha, gamma));
     BCM_TRY(blas_raw<T>(m, OP_AXPYZ, r, t_, s, -gamma, 0.0));
     double rr = 0.0;
     BCM_TRY(dot_host<T>(m, r, r, &rr, mask));
     bool done = false;
-    BCM_TRY(is_converged(rr, bb, eps, residual, &done));
+    const int crc = is_converged(rr, bb, eps, residual, &done);
+    if (crc != BCM_OK) {
+      if (!guard) return crc;
+      best_is_x = false;
+      break;
+    }
     if (done) {
       status = BCM_OK;
+      best_is_x = true;
       break;
+    }
+    if (guard) {
+      if (rr < best_rr) {
+        best_rr = rr;
+        since_best = 0;
+        best_is_x = true;
+        BCM_TRY(blas_raw<T>(m, OP_COPY_INTERIOR, xbest, x, nullptr, 0.0, 0.0));
+      } else {
+        best_is_x = false;
+        if (++since_best >= kInnerStall && count >= kInnerMin) break;
+      }
     }
     rr0 = rr1;
   }
+  if (guard && !best_is_x) BCM_TRY(blas_raw<T>(m, OP_COPY_INTERIOR, x, xbest, nullptr, 0.0, 0.0));
   if (count > count_max) count = count_max;
   BCM_TRY(bc.impose(x));
   *count_out = count;
@@ -307,7 +355,19 @@ extern "C" int bcm_bicgstab_mask(bcm_field_t x, const bcm_field_t A[7], bcm_fiel
 }
 
 // Workspace slots on the mesh.
-enum { WS_B64 = 0, WS_F64_FIRST = 1, WS_F32_FIRST = 12 };
+// Slots: b; 9 FP64 BiCGSTAB vectors; 12 FP32 vectors of the mixed solver;
+// the FP64 trial iterate and the FP64 correction's residual, correction and
+// best iterate.  The SOR workspaces use slots 30 and up.
+enum {
+  WS_B64 = 0,
+  WS_F64_FIRST = 1,
+  WS_F32_FIRST = 12,
+  WS_X64_TRIAL = WS_F32_FIRST + 12,
+  WS_R64,
+  WS_D64,
+  WS_BEST64
+};
+static_assert(WS_BEST64 < WS_SOR_CA_F64, "workspace slots overlap");
 
 extern "C" int bcm_solve_fp64(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
                               const bcm_ir_params* p, int* iters, double* relres) {
@@ -361,8 +421,8 @@ extern "C" int bcm_solve_mixed(bcm_field_t x, bcm_field_t rhs, const double* bva
   int rc = BCM_OK;
   double* b = static_cast<double*>(ws_get(m, WS_B64, n * 8, &rc));
   if (rc != BCM_OK) return rc;
-  float* w[11];  // r, r0, p, p_, q_, s, s_, t_, x0, rhs32, d32
-  for (int i = 0; i < 11; ++i) {
+  float* w[12];  // r, r0, p, p_, q_, s, s_, t_, x0, rhs32, d32, best d32
+  for (int i = 0; i < 12; ++i) {
     w[i] = static_cast<float*>(ws_get(m, WS_F32_FIRST + i, n * 4, &rc));
     if (rc != BCM_OK) return rc;
   }
@@ -381,36 +441,85 @@ extern "C" int bcm_solve_mixed(bcm_field_t x, bcm_field_t rhs, const double* bva
   const Bc<float> bc32{m, false};
   float* r32 = w[9];
   float* d32 = w[10];
+  // The refined iterate alternates between x and an FP64 workspace: a
+  // correction is applied into the other buffer and kept only when it
+  // lowers the FP64 residual.  The FP32 inner solve returns its best iterate
+  // by its recurrence residual, which can drift away from the true one by
+  // about cond(A) eps32 times the largest residual on its way.  A correction
+  // that does not lower the FP64 residual is discarded; the remaining steps
+  // then solve their corrections in FP64 (unless p->fp32_strict, which
+  // stops with BCM_ERR_NOCONV and the best x so far).
+  double* xt = static_cast<double*>(ws_get(m, WS_X64_TRIAL, n * 8, &rc));
+  if (rc != BCM_OK) return rc;
+  double* cur = xd;
+  double* alt = xt;
+  bool corr64 = false;
   int status = BCM_ERR_NOCONV;
-  for (int it = 0; it <= p->outer_max; ++it) {
-    // FP64 residual r = b - A x (x ghosts refreshed), rounded to FP32.
-    BCM_TRY(ghost_fill_raw<double>(m, xd, BCM_GF_ALL));
-    BCM_TRY(residual_d2f_raw(m, r32, xd, b, 0));
+  // FP64 residual r = b - A x (x ghosts refreshed), rounded to FP32.
+  auto residual = [&](double* v, double* rel) -> int {
+    BCM_TRY(ghost_fill_raw<double>(m, v, BCM_GF_ALL));
+    BCM_TRY(residual_d2f_raw(m, r32, v, b, 0));
     double rr = 0.0;
     BCM_TRY(fetch_result(m, 1, &rr));
-    if (std::isnan(rr) || std::isinf(rr)) return BCM_ERR_NAN;
-    *relres = std::sqrt(rr / bb);
-    if (*relres <= p->tol) {
-      status = BCM_OK;
-      break;
-    }
-    if (it == p->outer_max || (p->fp32_only && it == 1)) break;
-    // Correction A d = r in FP32, Jacobi-preconditioned BiCGSTAB.
-    BCM_TRY(fill_raw<float>(m, d32, 0.0));
+    *rel = std::sqrt(rr / bb);
+    return BCM_OK;
+  };
+  BCM_TRY(residual(cur, relres));
+  if (!std::isfinite(*relres)) return BCM_ERR_NAN;
+  for (int it = 0; it < p->outer_max && *relres > p->tol; ++it) {
+    if (p->fp32_only && it == 1) break;
     const double itol = p->fp32_only ? 0.5 * p->tol : p->inner_tol;
     int cnt = 0;
     double res = 0.0;
-    float* ww[9] = {w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
-    rc = bicgstab<float>(m, op32, bc32, d32, r32, ww, p->omega, p->count_pre, p->inner_max, itol, &cnt, &res);
-    for (int i = 0; i < 9; ++i) w[i] = ww[i];
-    for (int i = 0; i < 9; ++i) m->ws[WS_F32_FIRST + i] = w[i];
-    if (rc == BCM_ERR_NAN) return rc;
-    if (rc != BCM_OK && rc != BCM_ERR_NOCONV) return rc;
+    BCM_HIP_TRY(hipMemcpyAsync(alt, cur, n * 8, hipMemcpyDeviceToDevice, m->stream));
+    if (!corr64) {
+      // Correction A d = r in FP32, Jacobi-preconditioned BiCGSTAB.
+      BCM_TRY(fill_raw<float>(m, d32, 0.0));
+      float* ww[9] = {w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8]};
+      rc = bicgstab<float>(m, op32, bc32, d32, r32, ww, p->omega, p->count_pre, p->inner_max, itol, &cnt, &res,
+                           nullptr, w[11]);
+      for (int i = 0; i < 9; ++i) w[i] = ww[i];
+      for (int i = 0; i < 9; ++i) m->ws[WS_F32_FIRST + i] = w[i];
+      if (rc != BCM_OK && rc != BCM_ERR_NOCONV) return rc;
+      BCM_TRY(convert_raw(m, CONV_ADD_F2D, alt, d32));
+    } else {
+      // The same correction in FP64: r = b - A x, then A d = r.
+      double* w64[9];
+      for (int i = 0; i < 9; ++i) {
+        w64[i] = static_cast<double*>(ws_get(m, WS_F64_FIRST + i, n * 8, &rc));
+        if (rc != BCM_OK) return rc;
+      }
+      double* r64 = static_cast<double*>(ws_get(m, WS_R64, n * 8, &rc));
+      if (rc != BCM_OK) return rc;
+      double* d64 = static_cast<double*>(ws_get(m, WS_D64, n * 8, &rc));
+      if (rc != BCM_OK) return rc;
+      double* best64 = static_cast<double*>(ws_get(m, WS_BEST64, n * 8, &rc));
+      if (rc != BCM_OK) return rc;
+      BCM_TRY(residual_poisson_raw<double>(m, r64, cur, b, false, 0));
+      BCM_TRY(fill_raw<double>(m, d64, 0.0));
+      rc = bicgstab<double>(m, PoissonOp{m}, Bc<double>{m, false}, d64, r64, w64, p->omega, p->count_pre,
+                            p->inner_max, p->inner_tol, &cnt, &res, nullptr, best64);
+      for (int i = 0; i < 9; ++i) m->ws[WS_F64_FIRST + i] = w64[i];
+      if (rc != BCM_OK && rc != BCM_ERR_NOCONV) return rc;
+      BCM_TRY(blas_raw<double>(m, OP_AXPY, alt, d64, nullptr, 1.0, 0.0));
+    }
     *inner_total += cnt > 0 ? cnt : 0;
-    // FP64 update x += d.
-    BCM_TRY(convert_raw(m, CONV_ADD_F2D, xd, d32));
     *outer = it + 1;
+    // Trial x' = x + d in FP64, kept if its FP64 residual is lower.
+    double rel_alt = 0.0;
+    BCM_TRY(residual(alt, &rel_alt));
+    if (std::isfinite(rel_alt) && rel_alt < *relres) {
+      std::swap(cur, alt);
+      *relres = rel_alt;
+    } else {
+      // Rejected: r32 must hold the residual of the kept iterate again.
+      BCM_TRY(residual(cur, relres));
+      if (corr64 || p->fp32_strict || p->fp32_only) break;
+      corr64 = true;
+    }
   }
+  if (*relres <= p->tol) status = BCM_OK;
+  if (cur != xd) BCM_HIP_TRY(hipMemcpyAsync(xd, cur, n * 8, hipMemcpyDeviceToDevice, m->stream));
   BCM_TRY(ghost_fill_raw<double>(m, xd, BCM_GF_ALL));
   return status;
 }

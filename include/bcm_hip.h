@@ -415,6 +415,9 @@ typedef struct bcm_ir_params {
   int32_t count_pre;          /* Jacobi preconditioner sweeps */
   double omega;               /* Jacobi weight */
   int32_t fp32_only;          /* 1: no refinement, plain FP32 solve to tol */
+  int32_t fp32_strict;        /* 1: stop (BCM_ERR_NOCONV, best x so far) when an FP32
+                                 correction does not lower the FP64 residual;
+                                 0: solve the remaining corrections in FP64 */
 } bcm_ir_params;
 int bcm_solve_mixed(bcm_field_t x, bcm_field_t rhs, const double* bvalues, int nbvalues,
                     const bcm_ir_params* p, int* outer, int* inner_total, double* relres);
@@ -543,7 +546,10 @@ int bcm_server_create(const bcm_server_params* params, bcm_server_t* out);
 int bcm_server_destroy(bcm_server_t server);
 /* Seconds since the server was created. */
 double bcm_server_clock(bcm_server_t server);
-/* Queue a request.  blocks: its block table (copied); rhs: host FP64
+/* Queue a request.  tag: the caller's name for it, unique among the
+ * requests queued or running (a tag already pending returns BCM_ERR_ARG;
+ * it may be reused once its result has been produced).  blocks: its
+ * block table (copied); rhs: host FP64
  * ghosted field (nblocks*(n+2)^3 values, read when the request is
  * admitted, so it must stay valid until then) or NULL for the synthetic
  * right-hand side sin(pi x) sin(pi y) sin(pi z) + 0.01 N(0,1) generated on

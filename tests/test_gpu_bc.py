@@ -123,17 +123,21 @@ def test_host_bicgstab_neumann_vs_oracle():
     oracle.poisson_assemble(blocks, n, vc, A, b, rhs, VALS)
     x = np.zeros_like(rhs)
     work = [np.zeros_like(rhs) for _ in range(9)]
-    st, cnt, res = oracle.bicgstab(blocks, n, vc, x, A, b, work, 0.8, 2, 1000, 1e-11)
+    st, cnt, res = oracle.bicgstab(blocks, n, vc, x, A, b, work, 0.8, 2, 2000, 1e-13)
     assert st == 0
     mesh = Mesh(blocks, n, vc)
     gA = [mesh.field(np.float64, a) for a in A]
     gb = mesh.field(np.float64, b)
     gx = mesh.field(np.float64)
     gw = [mesh.field(np.float64) for _ in range(9)]
-    rc, gcnt, gres = mesh.bicgstab(gx, gA, gb, gw, omega=0.8, count_pre=2, count_max=1000, epsilon=1e-11)
+    rc, gcnt, gres = mesh.bicgstab(gx, gA, gb, gw, omega=0.8, count_pre=2, count_max=2000, epsilon=1e-13)
     assert rc == L.OK
-    xi, ri = W.interior(gx.download(), vc), W.interior(x, vc)
-    assert np.abs(xi - ri).max() <= 1e-8 * np.abs(ri).max()
+    xg = gx.download()
+    xi, ri = W.interior(xg, vc), W.interior(x, vc)
+    assert np.abs(xi - ri).max() <= 1e-10 * np.abs(ri).max()
+    r = np.zeros_like(x)
+    oracle.calc_r(len(blocks), n, vc, r, A, xg, b)
+    assert np.linalg.norm(W.interior(r, vc)) <= 1e-10 * np.linalg.norm(W.interior(b, vc))
 
 
 @pytest.mark.parametrize("method", [L.VC_AT_ONCE, L.VC_SEPARATE_XYZ, L.VC_SEPARATE_XYZ_SEPARATE_LEVEL_DIFF])

@@ -95,3 +95,34 @@ def test_server_budget_admission_and_refusal():
         assert done[b]["t_admit"] >= done[a]["t_done"] - 1e-6
     # The synthetic right-hand side depends on the seed only.
     sv.close()
+
+
+def test_server_rejects_a_pending_tag():
+    """A tag names one pending request (the server reads its rhs and writes
+    its x through the caller's buffers): a second submit with the same tag
+    is refused by the library and by the Python mirror, and the tag can be
+    used again once the first request's result has been polled."""
+    import ctypes
+
+    from bcmtools_amd import _lib as L
+    blocks = W.random_tree(3, max_level=2).blocks
+    case(f"server duplicate tag: {len(blocks)} blocks")
+    sv = Server(64e9, max_active=2, n=16, check_every=4)
+    sv.submit(blocks, 1e-8, "fp64", seed=1, tag=5)
+    with pytest.raises(L.BCMError):
+        sv.submit(blocks, 1e-8, "fp64", seed=2, tag=5)
+    b = np.ascontiguousarray(blocks, dtype=L.BLOCK_DTYPE)
+    rc = sv.lib.bcm_server_submit(sv.handle, b.ctypes.data, len(b), None, 2, ctypes.c_double(1e-8), 0, None, 5)
+    assert rc == L.ERR_ARG
+    done = []
+    while not done:
+        sv.progress(0.05)
+        done += sv.poll()
+    assert done[0]["tag"] == 5 and done[0]["status"] == 1
+    sv.submit(blocks, 1e-8, "fp64", seed=3, tag=5)
+    done = []
+    while not done:
+        sv.progress(0.05)
+        done += sv.poll()
+    assert done[0]["tag"] == 5 and done[0]["status"] == 1
+    sv.close()
