@@ -260,12 +260,22 @@ struct HaloSrc {
   }
 };
 
-// One aligned two-cell store (16 B for FP64, 8 B for FP32).
+// One aligned two-cell store (16 B for FP64, 8 B for FP32), marked
+// non-temporal: the next sweep reads the iterate back long after this
+// block's data has left the caches, so the store streams past them.
 __device__ __forceinline__ void store_pair(double* p, double a, double b) {
-  *reinterpret_cast<double2*>(p) = make_double2(a, b);
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  v2 v;
+  v.x = a;
+  v.y = b;
+  __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
 }
 __device__ __forceinline__ void store_pair(float* p, float a, float b) {
-  *reinterpret_cast<float2*>(p) = make_float2(a, b);
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  v2 v;
+  v.x = a;
+  v.y = b;
+  __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
 }
 
 template <typename T, bool SRC>
@@ -357,11 +367,14 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
       }
     }
   };
+  // The source term is read once per sweep and by no other work-group:
+  // non-temporal loads.  The iterate's rows stay cacheable (the neighbours
+  // read this block's faces for their ghost frames).
   auto load_s = [&](int p, T* sv) __attribute__((always_inline)) {
     if (!SRC) return;
     const T* row = sb + (long long)p * NN + own;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) sv[q] = row[q];
+    for (int q = 0; q < 2; ++q) sv[q] = __builtin_nontemporal_load(row + q);
   };
   // X/Y ghost frame: 4N cells per plane, eight source values each, spread
   // over all threads: thread t loads values VPT*gq .. VPT*gq+VPT-1 of frame
