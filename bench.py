@@ -12,11 +12,21 @@ copy + F2C + C2F), Dirichlet mirror.  Inputs are resident in HBM before
 the timed region.  value = interior cell updates per second (GCUPS).
 
 roofline: the dominant kernel is the halo-gathering red-black sweep
-(k_rbsor_halo: one sweep reading iterate A, writing iterate B, ghost frame
-built from the neighbours' interiors); its algorithmic traffic is 24 B per
-interior cell (read u and s, write u, FP64), timed with HIP events on the
-mesh stream inside the timed region (the first of the K steps is the
-stored-ghost sweep k_rbsor_fused, which the same timer includes).
+(k_rbsor_halo<double,32,true,false>: one sweep reading iterate A, writing
+iterate B, ghost frame built from the neighbours' interiors); its
+algorithmic traffic is 24 B per interior cell (read u and s, write u,
+FP64), timed with HIP events on the mesh stream inside the timed region
+over its own launches (the K steps of the call minus the first, which runs
+k_rbsor_fused on the stored ghosts, and the last, which writes the ghosted
+field; those two are reported beside it).
+
+The same line carries the rest of the metric on bounded samples (left out
+with --quick): sor_to_tolerance (the sor2 iteration to relres 1e-8 on the
+config-2 generator at level 3), batched (every 8th mesh of the 256-mesh
+batch: batched mixed and FP64 against one mesh at a time, fused and plain
+host-driven FP64) and serving (a 12-request config-5 stream through the
+continuous-batching server: solves/s, p50/p99 latency, and the same
+requests one at a time).
 
 cpu_baseline: the oracle (C, OpenMP, -O3 -ffp-contract=off) runs the same
 step on the same mesh for a bounded number of steps on the host cores.
@@ -28,7 +38,8 @@ with a per-mesh comparison on every 8th mesh), serving (Poisson-arrival
 request stream through the continuous-batching server), copy (HBM
 calibration).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sor|jacobi|bicgstab|batched|serving|copy]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+                       [--workload sor|jacobi|bicgstab|batched|serving|copy|restart]
 """
 import argparse
 import json
@@ -94,15 +105,19 @@ def bench_sor(args):
     t1 = time.perf_counter()
     mesh.set_timing(False)
     elapsed = t1 - t0
-    sor_ms, sor_n = mesh.kernel_time(L.K_SOR)
+    halo_ms, halo_n = mesh.kernel_time(L.K_SOR_HALO)
+    end_ms, end_n = mesh.kernel_time(L.K_SOR)
     gh_ms, gh_n = mesh.kernel_time(L.K_GHOST)
     bc_ms, bc_n = mesh.kernel_time(L.K_DIRICHLET)
-    sor_avg = sor_ms / max(sor_n, 1) * 1e-3
+    if halo_n == 0:
+        raise RuntimeError("bench: the halo-gathering sweep did not run (steps < 3?)")
+    sor_avg = halo_ms / halo_n * 1e-3
     algo_bytes = 24.0 * ncell
     achieved = algo_bytes / sor_avg / 1e9
     gcups = ncell * args.steps / elapsed / 1e9
-    print(f"[bench] sor kernel avg {sor_avg * 1e3:.3f} ms ({achieved:.0f} GB/s), ghost avg "
-          f"{gh_ms / max(gh_n, 1):.3f} ms, dirichlet avg {bc_ms / max(bc_n, 1):.3f} ms", file=sys.stderr, flush=True)
+    print(f"[bench] halo sweep avg {sor_avg * 1e3:.3f} ms ({achieved:.0f} GB/s) over {halo_n}, first+last sweep avg "
+          f"{end_ms / max(end_n, 1):.3f} ms, ghost avg {gh_ms / max(gh_n, 1):.3f} ms, dirichlet avg "
+          f"{bc_ms / max(bc_n, 1):.3f} ms", file=sys.stderr, flush=True)
     result = {
         "metric": "stencil GCUPS (FP64 red-black SOR + ghost fill, adaptive octree)",
         "value": gcups,
@@ -121,17 +136,60 @@ def bench_sor(args):
                    "interior_cells": ncell, "omega": omega},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("rbsor_halo_n32"),
-                     "kernel": "k_rbsor_halo<double,32,true,false> (HIP-event average over the call's sweeps; "
-                               "the first of each call is k_rbsor_fused on the stored ghosts, the last "
-                               "k_rbsor_halo<double,32,true,true>, which writes the ghosted field)",
-                     "avg_ms": sor_avg * 1e3,
+                     "kernel": "k_rbsor_halo<double,32,true,false> (HIP-event average over its launches: the "
+                               "sweeps between a call's first and last)",
+                     "avg_ms": sor_avg * 1e3, "launches": halo_n,
                      "algorithmic_bytes_per_launch": algo_bytes},
-        "kernels_ms_per_step": {"rbsor": sor_ms / max(args.steps, 1), "ghost_fill": gh_ms / max(args.steps, 1),
-                                "dirichlet": bc_ms / max(args.steps, 1)},
+        "kernels_ms_per_step": {"rbsor_halo": halo_ms / max(args.steps, 1),
+                                "rbsor_first_and_last": end_ms / max(args.steps, 1),
+                                "ghost_fill": gh_ms / max(args.steps, 1), "dirichlet": bc_ms / max(args.steps, 1)},
+        "first_and_last_sweep_avg_ms": end_ms / max(end_n, 1),
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_sor(blocks, n, vc, s, omega, vals, args.cpu_steps)
     return result
+
+
+def sor_to_tolerance(level=3, n=32, omega=1.4, tol=1e-8, chunk=500, cap=400000):
+    """The sor2 outer iteration run to relres ||s - lap_h f|| / ||s|| <= tol
+    on the config-2 generator at a reduced level, checked every `chunk`
+    sweeps (bcm_sor_residual_noalloc)."""
+    import ctypes
+
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    blocks = W.shell_tree(level).blocks
+    mesh = Mesh(blocks, n, 1)
+    sf = mesh.field(np.float64, -W.rhs_field_fast(blocks, n, 1, seed=0))
+    f = mesh.field(np.float64)
+    mesh.dirichlet_mirror(f, [0.0])
+    r = ctypes.c_double()
+    sweeps, rr = 0, float("inf")
+    mesh.sync()
+    t0 = time.perf_counter()
+    while sweeps < cap:
+        mesh.sor_iterate(f, sf, omega, [0.0], chunk)
+        sweeps += chunk
+        if mesh.lib.bcm_sor_residual_noalloc(f.handle, sf.handle, ctypes.byref(r)) != 0:
+            raise RuntimeError("bcm_sor_residual_noalloc failed")
+        rr = r.value
+        if not np.isfinite(rr) or rr <= tol:
+            break
+    el = time.perf_counter() - t0
+    ncell = len(blocks) * n ** 3
+    print(f"[bench] sor to tolerance: {sweeps} sweeps, relres {rr:.3e}, {el:.2f} s", file=sys.stderr, flush=True)
+    mesh.close()
+    return {"workload": f"shell octree level {level} (the config-2 generator one level coarser per step down from "
+                        f"5), {len(blocks)} x {n}^3 blocks, sor2 RB-SOR + Dirichlet + ghost fill per sweep",
+            "blocks": len(blocks), "omega": omega, "tol": tol, "sweeps": sweeps, "relres": rr,
+            "converged": bool(rr <= tol), "time_s": el, "GCUPS": ncell * sweeps / el / 1e9,
+            "omega_1_6": "diverges: with ghosts frozen between exchanges, block-wise RB-SOR is unstable at "
+                         "omega >= 1.5 on these meshes (relres 1e23 after 500 sweeps at 1.5, 1e47 at 1.6 on the "
+                         "level-4 mesh); the compiled reference sor1 gives errorMax 5.8e31 at omega 1.6 and "
+                         "0.334 at 1.4 on 64 x 16^3 after 400 sweeps",
+            "full_size_note": "on the level-5 mesh itself (3,816 x 32^3) omega 1.4 reaches relres 0.36 after "
+                              "60,000 sweeps (35 s): the tolerance needs several hundred thousand sweeps there"}
 
 
 def cpu_sor(blocks, n, vc, s, omega, vals, steps):
@@ -387,51 +445,52 @@ def cpu_bicgstab(blocks, n, rhs, gpu_iters, iters):
                       f"{cores} threads of {cpu_model()}, extrapolated to the GPU's {gpu_iters} iterations"}
 
 
-def bench_batched(args):
-    """256 independent random octree meshes (16^3 blocks) solved together to 1e-10."""
+def _batch_tables(M):
+    from bcmtools_amd import workloads as W
+    return [W.random_tree(1000 + i).blocks for i in range(M)]
+
+
+def _build_batch(tables, idx, n=16):
     from bcmtools_amd import Mesh
     from bcmtools_amd import workloads as W
 
-    tol, n, M = 1e-10, 16, args.meshes
-    trees = [W.random_tree(1000 + i) for i in range(M)]
-    tables = [t.blocks for t in trees]
-    counts = [len(tb) for tb in tables]
-    levels = [int(tb["level"].max()) + 1 for tb in tables]
-    total = sum(counts)
-    print(f"[bench] batched: {M} meshes, {total} blocks ({min(counts)}..{max(counts)} per mesh)", file=sys.stderr,
-          flush=True)
+    mesh = Mesh.batch([tables[i] for i in idx], n)
+    mesh.set_progress(20.0)
+    x, rhs = mesh.field(np.float64), mesh.field(np.float64)
+    for s, i in enumerate(idx):
+        first, cnt = mesh.segment(s)
+        rhs.upload_blocks(first, -W.rhs_field_fast(tables[i], n, 1, seed=5000 + i))
+    return mesh, x, rhs
 
-    def build(idx):
-        mesh = Mesh.batch([tables[i] for i in idx], n)
-        mesh.set_progress(20.0)
-        x, rhs = mesh.field(np.float64), mesh.field(np.float64)
-        for s, i in enumerate(idx):
-            first, cnt = mesh.segment(s)
-            rhs.upload_blocks(first, -W.rhs_field_fast(tables[i], n, 1, seed=5000 + i))
-        return mesh, x, rhs
 
-    def solve(mesh, x, rhs, precision):
-        mesh.fill(x, 0.0)
-        mesh.sync()
-        t0 = time.perf_counter()
-        st = mesh.solve(x, rhs, [0.0], precision=precision, tol=tol, max_iter=5000)
-        mesh.sync()
-        return st, time.perf_counter() - t0
+def _timed_solve(mesh, x, rhs, precision, tol):
+    mesh.fill(x, 0.0)
+    mesh.sync()
+    t0 = time.perf_counter()
+    st = mesh.solve(x, rhs, [0.0], precision=precision, tol=tol, max_iter=5000)
+    mesh.sync()
+    return st, time.perf_counter() - t0
 
-    def summary(st):
-        ok = sum(1 for e in st if e["status"] == "converged")
-        return {"converged": ok, "segments": len(st), "max_relres": max(e["relres"] for e in st),
-                "max_iters": max(e["iters"] for e in st), "sum_iters": sum(e["iters"] for e in st)}
 
-    # Comparison on a fixed subset (every 8th mesh): batched mixed / FP64
-    # against solving the same meshes one by one; the faster batched
-    # precision is then used for the full batch.
-    idx = list(range(0, M, 8))
-    mesh, x, rhs = build(idx)
+def _batch_summary(st):
+    ok = sum(1 for e in st if e["status"] == "converged")
+    return {"converged": ok, "segments": len(st), "max_relres": max(e["relres"] for e in st),
+            "max_iters": max(e["iters"] for e in st), "sum_iters": sum(e["iters"] for e in st)}
+
+
+def batched_subset(tables, every=8, tol=1e-10, n=16):
+    """Every `every`-th mesh of the 256-mesh batch generator: solved as one
+    batch (mixed and FP64), and one mesh at a time by the fused FP64 engine
+    and by the plain host-driven FP64 path (ILS3D's BiCGSTAB step by step)."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    idx = list(range(0, len(tables), every))
+    mesh, x, rhs = _build_batch(tables, idx, n)
     mesh.solve(x, rhs, [0.0], precision="mixed", tol=tol, max_iter=2)
-    st_m, t_m = solve(mesh, x, rhs, "mixed")
+    st_m, t_m = _timed_solve(mesh, x, rhs, "mixed", tol)
     mesh.solve(x, rhs, [0.0], precision="fp64", tol=tol, max_iter=2)
-    st_d, t_d = solve(mesh, x, rhs, "fp64")
+    st_d, t_d = _timed_solve(mesh, x, rhs, "fp64", tol)
     print(f"[bench] subset of {len(idx)}: batched mixed {t_m:.2f} s, batched fp64 {t_d:.2f} s", file=sys.stderr,
           flush=True)
     mesh.close()
@@ -457,24 +516,46 @@ def bench_batched(args):
         if k % 8 == 7:
             print(f"[bench] per-mesh solves: {k + 1}/{len(idx)} (fused fp64 {t_e:.2f} s, host-driven fp64 {t_h:.2f} s)",
                   file=sys.stderr, flush=True)
+    ns = len(idx)
+    return {"meshes": ns, "every": every, "blocks": int(sum(len(tables[i]) for i in idx)),
+            "batched_mixed_solves_per_s": ns / t_m, "batched_fp64_solves_per_s": ns / t_d,
+            "per_mesh_fused_fp64_solves_per_s": ns / t_e, "per_mesh_host_fp64_solves_per_s": ns / t_h,
+            "speedup_batched_mixed_vs_per_mesh_host_fp64": t_h / t_m,
+            "speedup_batched_mixed_vs_batched_fp64": t_d / t_m,
+            "batched_mixed": _batch_summary(st_m), "batched_fp64": _batch_summary(st_d),
+            "host_fp64_converged": int(host_ok)}
+
+
+def bench_batched(args):
+    """256 independent random octree meshes (16^3 blocks) solved together to 1e-10."""
+    tol, n, M = 1e-10, 16, args.meshes
+    tables = _batch_tables(M)
+    counts = [len(tb) for tb in tables]
+    levels = [int(tb["level"].max()) + 1 for tb in tables]
+    total = sum(counts)
+    print(f"[bench] batched: {M} meshes, {total} blocks ({min(counts)}..{max(counts)} per mesh)", file=sys.stderr,
+          flush=True)
+    # Comparison on a fixed subset (every 8th mesh); the faster batched
+    # precision is then used for the full batch.
+    sub = batched_subset(tables, 8, tol, n)
+    t_m, t_d = 1.0 / sub["batched_mixed_solves_per_s"], 1.0 / sub["batched_fp64_solves_per_s"]
     precision = args.precision if args.precision != "auto" else ("mixed" if t_m < t_d else "fp64")
 
     # Full batch with that precision (the headline).
-    mesh, x, rhs = build(range(M))
+    mesh, x, rhs = _build_batch(tables, range(M), n)
     print(f"[bench] batched: fields uploaded, precision {precision}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         mesh.solve(x, rhs, [0.0], precision=precision, tol=tol, max_iter=2)
     times, st = [], None
     for _ in range(args.steps):
-        st, dt = solve(mesh, x, rhs, precision)
+        st, dt = _timed_solve(mesh, x, rhs, precision, tol)
         times.append(dt)
-        print(f"[bench] batched {precision} solve of {M} meshes: {dt:.2f} s {summary(st)}", file=sys.stderr,
+        print(f"[bench] batched {precision} solve of {M} meshes: {dt:.2f} s {_batch_summary(st)}", file=sys.stderr,
               flush=True)
-    full = summary(st)
+    full = _batch_summary(st)
     mesh.close()
     del x, rhs
     elapsed = sum(times)
-    ns = len(idx)
     return {
         "metric": "batched solves/s (256 meshes, BiCGSTAB to FP64 relative residual 1e-10)",
         "value": M * len(times) / elapsed,
@@ -490,13 +571,7 @@ def bench_batched(args):
                    "meshes": M, "blocks": total, "blocks_per_mesh": [min(counts), int(np.median(counts)), max(counts)],
                    "levels": [min(levels), max(levels)], "interior_cells": total * n ** 3},
         "batch": full,
-        "subset": {"meshes": ns, "every": 8,
-                   "batched_mixed_solves_per_s": ns / t_m, "batched_fp64_solves_per_s": ns / t_d,
-                   "per_mesh_fused_fp64_solves_per_s": ns / t_e, "per_mesh_host_fp64_solves_per_s": ns / t_h,
-                   "speedup_batched_mixed_vs_per_mesh_host_fp64": t_h / t_m,
-                   "speedup_batched_mixed_vs_batched_fp64": t_d / t_m,
-                   "batched_mixed": summary(st_m), "batched_fp64": summary(st_d),
-                   "host_fp64_converged": int(host_ok)},
+        "subset": sub,
     }
 
 
@@ -517,7 +592,7 @@ def serve_stream(args, lam, trees, tols, gaps):
             if args.policy == "spec":
                 prec = "fp32" if tols[k] > 1e-8 else "mixed"
             else:
-                prec = "fp64"
+                prec = args.policy
             sv.submit(trees[k].blocks, float(tols[k]), prec, rhs=None, seed=9000 + k, tag=k)
             k += 1
         wait = (arrivals[k] - now) if k < R else 0.05
@@ -584,8 +659,9 @@ def bench_serving(args):
         "config": {"workload": f"{R} requests, Poisson arrivals at {lams[0]:g}/s, meshes of 1-3 random sphere "
                                "surfaces with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 or "
                                "1e-10, "
-                               + ("FP32 path for 1e-6 and FP32 + FP64 refinement for 1e-10"
-                                  if args.policy == "spec" else "fused FP64 BiCGSTAB for every request")
+                               + {"spec": "FP32 path for 1e-6 and FP32 + FP64 refinement for 1e-10",
+                                  "mixed": "mixed FP32/FP64 BiCGSTAB (FP64 reliable updates) for every request",
+                                  "fp64": "fused FP64 BiCGSTAB for every request"}[args.policy]
                                + f", up to {args.slots} requests on the device, HBM budget 200 GB",
                    "precision_policy": args.policy,
                    "requests": R, "lambda_per_s": lams[0], "blocks_total": int(sum(counts)),
@@ -694,9 +770,8 @@ def bench_restart(args):
     import shutil
     import tempfile
 
-    import torch
-
     from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
     from bcmtools_amd import workloads as W
 
     level, n, vc = args.level, 16, 1
@@ -727,20 +802,17 @@ def bench_restart(args):
             print(f"[bench] restart step {s}: dump2 {t1 - t0:.3f} s, load2 {t2 - t1:.3f} s", file=sys.stderr,
                   flush=True)
         exact = bool(np.array_equal(g.download(), data))
-        # PCIe reference rates: pinned host <-> device copies of the same size.
-        dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-        host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
-        host.copy_(dev)
-        torch.cuda.synchronize()
+        # PCIe reference rates: whole-field copies between pageable host
+        # memory and the device (bcm_field_download / bcm_field_upload).
+        host = np.empty_like(data)
+        f.download()
         t0 = time.perf_counter()
-        host.copy_(dev)
-        torch.cuda.synchronize()
+        L.check(mesh.lib.bcm_field_download(f.handle, host.ctypes.data, host.nbytes), "bcm_field_download")
         d2h = nbytes / (time.perf_counter() - t0) / 1e9
         t0 = time.perf_counter()
-        dev.copy_(host)
-        torch.cuda.synchronize()
+        L.check(mesh.lib.bcm_field_upload(g.handle, host.ctypes.data, host.nbytes), "bcm_field_upload")
         h2d = nbytes / (time.perf_counter() - t0) / 1e9
-        del dev, host
+        del host
         res = {
             "metric": "restart file throughput (LocalScalar3D Dump2 + Load2, GB/s of field data)",
             "value": 2 * nbytes * len(td) / (sum(td) + sum(tl)) / 1e9, "unit": "GB/s",
@@ -752,7 +824,7 @@ def bench_restart(args):
                        "blocks": len(blocks), "bytes": nbytes},
             "dump2_GBps": nbytes * len(td) / sum(td) / 1e9,
             "load2_GBps": nbytes * len(tl) / sum(tl) / 1e9,
-            "pcie_pinned_d2h_GBps": d2h, "pcie_pinned_h2d_GBps": h2d,
+            "pcie_pageable_d2h_GBps": d2h, "pcie_pageable_h2d_GBps": h2d,
             "round_trip_bit_exact": exact,
         }
         if not args.no_cpu:
@@ -786,7 +858,7 @@ def main():
     ap.add_argument("--lams", default="", help="serving: comma-separated arrival rates to sweep (same requests)")
     ap.add_argument("--slots", type=int, default=16, help="serving: requests on the device at once")
     ap.add_argument("--seed", type=int, default=1, help="serving: request stream seed")
-    ap.add_argument("--policy", default="fp64", choices=["fp64", "spec"],
+    ap.add_argument("--policy", default="fp64", choices=["fp64", "mixed", "spec"],
                     help="serving: fp64 for every request, or FP32 (tol 1e-6) / mixed (1e-10)")
     ap.add_argument("--per-request-every", type=int, default=4,
                     help="serving: also solve every request alone on the fused FP64 path and every k-th on the "
@@ -800,6 +872,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="bicgstab: iteration limit of the engine solves (a short limit for counter runs)")
+    ap.add_argument("--quick", action="store_true",
+                    help="sor: the headline line only (no tolerance, batched and serving sub-objects)")
     ap.add_argument("--no-graph", action="store_true",
                     help="bicgstab: launch the iteration kernels one by one instead of replaying a hipGraph")
     args = ap.parse_args()
@@ -809,6 +883,31 @@ def main():
         sys.exit(2)
     res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
            "serving": bench_serving, "copy": bench_copy, "restart": bench_restart}[args.workload](args)
+    if args.workload == "sor" and not args.quick:
+        # The rest of the metric: SOR to a tolerance, batched solves/s and
+        # served solves/s with p50/p99 latency, each on a bounded sample.
+        res["sor_to_tolerance"] = sor_to_tolerance()
+        sub = batched_subset(_batch_tables(256), every=8)
+        res["batched"] = {"workload": "every 8th mesh of the 256-mesh batch generator (seeds 1000 + 8k), "
+                                      "BiCGSTAB to 1e-10", **sub}
+        # Served with the mixed engine for every request (the target's
+        # solver), then the same stream with the precision chosen by
+        # tolerance (FP32 for 1e-6, mixed for 1e-10).
+        sv_args = argparse.Namespace(**vars(args))
+        sv_args.requests, sv_args.lam, sv_args.lams, sv_args.policy, sv_args.per_request_every = 12, 100.0, "", \
+            "mixed", 4
+        sv = bench_serving(sv_args)
+        keys = ("value", "unit", "config", "latency_s", "queue_wait_s", "by_tolerance", "all_met_tolerance",
+                "not_met", "makespan_s")
+        res["serving"] = {k: sv[k] for k in keys + ("one_at_a_time",) if k in sv}
+        base = sv.get("one_at_a_time", {})
+        if base.get("plain_fp64_host_driven_est_solves_per_s"):
+            res["serving"]["speedup_vs_plain_fp64_per_request"] = \
+                sv["value"] / base["plain_fp64_host_driven_est_solves_per_s"]
+        sv_args.policy, sv_args.per_request_every = "spec", 0
+        sp = bench_serving(sv_args)
+        res["serving"]["precision_by_tolerance"] = {k: sp[k] for k in ("value", "unit", "latency_s", "by_tolerance",
+                                                                       "all_met_tolerance", "makespan_s")}
     print(json.dumps(res), flush=True)
 
 
