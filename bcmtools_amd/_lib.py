@@ -22,8 +22,9 @@ STATUS_NAMES = {
 F64, F32 = 0, 1
 X_M, X_P, Y_M, Y_P, Z_M, Z_P = range(6)
 BC_INNER, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = range(4)
-DIV_FLAT, DIV_SIMPLE, DIV_SHELL = range(3)
-ORDER_Z, ORDER_HILBERT = range(2)
+DIV_FLAT, DIV_SIMPLE, DIV_SHELL, DIV_USER = range(4)
+ORDER_Z, ORDER_HILBERT, ORDER_RANDOM = range(3)
+NODE_BRANCH, NODE_LEAF_ACTIVE, NODE_LEAF_INACTIVE = range(3)
 BCSET_ALL_DIRICHLET, BCSET_SOR1_X, BCSET_SOR1_Y, BCSET_SOR1_Z, BCSET_PER_FACE = range(5)
 GF_FLAT, GF_F2C, GF_C2F, GF_ALL = 1, 2, 4, 7
 GF_X, GF_Y, GF_Z = 8, 16, 32
@@ -162,9 +163,13 @@ _D = ctypes.c_double
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 
+# Divider callback of bcm_tree_build_user: (user, pedigree) -> NODE_* kind.
+DIVIDER_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64)
+
 # name: (restype, argtypes)
 SIGNATURES = {
     "bcm_tree_build": (_I, [ctypes.POINTER(TreeParams), ctypes.POINTER(_P)]),
+    "bcm_tree_build_user": (_I, [ctypes.POINTER(TreeParams), DIVIDER_FN, _P, ctypes.POINTER(_P)]),
     "bcm_tree_num_blocks": (_I, [_P]),
     "bcm_tree_get_blocks": (_I, [_P, _P, _I]),
     "bcm_tree_destroy": (_I, [_P]),
@@ -194,6 +199,15 @@ SIGNATURES = {
     "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
+    "bcm_ghost_fill_face": (_I, [_P, _I, _I, _I]),
+    "bcm_vfield_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "bcm_vfield_destroy": (_I, [_P]),
+    "bcm_vfield_upload": (_I, [_P, _P, ctypes.c_size_t]),
+    "bcm_vfield_download": (_I, [_P, _P, ctypes.c_size_t]),
+    "bcm_vfield_upload_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),
+    "bcm_vfield_download_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),
+    "bcm_vfield_ghost_fill": (_I, [_P, _I]),
+    "bcm_vfield_ghost_fill_face": (_I, [_P, _I, _I, _I]),
     "bcm_sor_residual_noalloc": (_I, [_P, _P, _DP]),
     "bcm_sor_set_variant": (_I, [_P, _I]),
     "bcm_sor_get_variant": (_I, [_P]),

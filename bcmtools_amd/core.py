@@ -20,7 +20,10 @@ class Tree:
 
     def __init__(self, root_n=(1, 1, 1), periodic=(0, 0, 0), root_origin=(0.0, 0.0, 0.0), root_length=1.0,
                  divider=L.DIV_FLAT, min_level=0, max_level=0, shells=(), ordering=L.ORDER_Z, balance=1,
-                 bc_setter=L.BCSET_ALL_DIRICHLET, shell_mode=0, shell_margin=0.0, face_bc_type=None):
+                 bc_setter=L.BCSET_ALL_DIRICHLET, shell_mode=0, shell_margin=0.0, face_bc_type=None, divide=None):
+        """`divide` (with divider=DIV_USER): a callable taking a node's
+        pedigree word and returning NODE_BRANCH, NODE_LEAF_ACTIVE or
+        NODE_LEAF_INACTIVE, as Divider::operator() (Utils/include/Divider.h)."""
         lib = L.load()
         p = L.TreeParams()
         for a in range(3):
@@ -50,7 +53,11 @@ class Tree:
                 p.face_bc_type[f] = int(face_bc_type[f])
         self.params = p
         h = ctypes.c_void_p()
-        L.check(lib.bcm_tree_build(ctypes.byref(p), ctypes.byref(h)), "bcm_tree_build")
+        if divide is not None:
+            fn = L.DIVIDER_FN(lambda user, ped: int(divide(int(ped))))
+            L.check(lib.bcm_tree_build_user(ctypes.byref(p), fn, None, ctypes.byref(h)), "bcm_tree_build_user")
+        else:
+            L.check(lib.bcm_tree_build(ctypes.byref(p), ctypes.byref(h)), "bcm_tree_build")
         try:
             nb = lib.bcm_tree_num_blocks(h)
             if nb < 0:

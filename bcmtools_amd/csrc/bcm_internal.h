@@ -60,6 +60,7 @@ struct bcm_mesh {
 
   // Ghost-fill tasks ordered by [axis][kind]; off[a][k] .. off[a][k+1].
   int4* d_tasks = nullptr;
+  std::vector<int4> h_tasks;      // the same table on the host
   int ntask_total = 0;
   int off[3][4] = {};
   int kind_count[3] = {};
@@ -135,6 +136,11 @@ struct bcm_field {
   void* data = nullptr;
   size_t count = 0;  // values
   size_t bytes = 0;
+};
+
+// A field of three interleaved components per cell (Vector3D's layout).
+struct bcm_vfield {
+  bcm_field base;  // count = 3 * nblocks * (n+2vc)^3
 };
 
 namespace bcm {

@@ -112,24 +112,28 @@ struct Geo {
   }
 };
 
-template <typename T>
-__device__ __forceinline__ T f2c_value(const T* __restrict__ f, const Geo& g, int I, int J, int K) {
-  return f2c_sum<T>([&](int i, int j, int k) { return f[g.idx(i, j, k)]; }, I, J, K);
+// Component q of a cell of a field with NC interleaved components per
+// cell (NC = 1: a scalar field; NC = 3: Vector3D's layout).
+template <typename T, int NC = 1>
+__device__ __forceinline__ T f2c_value(const T* __restrict__ f, const Geo& g, int I, int J, int K, int q = 0) {
+  return f2c_sum<T>([&](int i, int j, int k) { return f[g.idx(i, j, k) * NC + q]; }, I, J, K);
 }
 
-template <typename T>
-__device__ __forceinline__ T c2f_value(const T* __restrict__ c, const Geo& g, int i, int j, int k) {
-  return c2f_interp<T>([&](int a, int b, int d) { return c[g.idx(a, b, d)]; }, g.n, i, j, k);
+template <typename T, int NC = 1>
+__device__ __forceinline__ T c2f_value(const T* __restrict__ c, const Geo& g, int i, int j, int k, int q = 0) {
+  return c2f_interp<T>([&](int a, int b, int d) { return c[g.idx(a, b, d) * NC + q]; }, g.n, i, j, k);
 }
 
-template <typename T>
+// One work-group per face task; every component of a ghost cell gets the
+// scalar rule applied to that component.
+template <typename T, int NC = 1>
 __global__ __launch_bounds__(256) void k_ghost_fill(T* __restrict__ d, const int4* __restrict__ tasks, Geo g) {
   const int4 tk = tasks[blockIdx.x];
   const int face = tk.z & 7, sub = (tk.z >> 3) & 3, kind = (tk.z >> 5) & 3;
   const int axis = face >> 1;
   const bool plus = face & 1;
-  T* __restrict__ dst = d + (long long)tk.x * g.bvol;
-  const T* __restrict__ src = d + (long long)tk.y * g.bvol;
+  T* __restrict__ dst = d + (long long)tk.x * g.bvol * NC;
+  const T* __restrict__ src = d + (long long)tk.y * g.bvol * NC;
   const int n = g.n, vc = g.vc, h = n / 2;
   const int tn = (kind == KIND_F2C) ? h : n;  // tangential extent
   // Loop space (x fastest): depth vc along the face normal.
@@ -148,7 +152,9 @@ __global__ __launch_bounds__(256) void k_ghost_fill(T* __restrict__ d, const int
     if (kind == KIND_COPY) {
       dc[axis] = plus ? n + dpt : dpt - vc;
       sc[axis] = plus ? dpt : n - vc + dpt;
-      dst[g.idx(dc[0], dc[1], dc[2])] = src[g.idx(sc[0], sc[1], sc[2])];
+#pragma unroll
+      for (int q = 0; q < NC; ++q)
+        dst[g.idx(dc[0], dc[1], dc[2]) * NC + q] = src[g.idx(sc[0], sc[1], sc[2]) * NC + q];
     } else if (kind == KIND_F2C) {
       // Coarse ghost at the subface's quarter (copyFromNeighborF2C_0).
       const int t0 = (axis + 1) % 3, t1 = (axis + 2) % 3;  // X:(y,z) Y:(z,x) Z:(x,y)
@@ -156,7 +162,9 @@ __global__ __launch_bounds__(256) void k_ghost_fill(T* __restrict__ d, const int
       dc[t0] += h * so0;
       dc[t1] += h * so1;
       sc[axis] = plus ? dpt : dpt + h - vc;
-      dst[g.idx(dc[0], dc[1], dc[2])] = f2c_value(src, g, sc[0], sc[1], sc[2]);
+#pragma unroll
+      for (int q = 0; q < NC; ++q)
+        dst[g.idx(dc[0], dc[1], dc[2]) * NC + q] = f2c_value<T, NC>(src, g, sc[0], sc[1], sc[2], q);
     } else {
       // Fine ghost interpolated from the coarse neighbour (copyFromNeighborC2F_0).
       const int t0 = (axis + 1) % 3, t1 = (axis + 2) % 3;
@@ -164,13 +172,15 @@ __global__ __launch_bounds__(256) void k_ghost_fill(T* __restrict__ d, const int
       sc[axis] = plus ? dpt : dpt + 2 * n - vc;
       sc[t0] += n * so0;
       sc[t1] += n * so1;
-      dst[g.idx(dc[0], dc[1], dc[2])] = c2f_value(src, g, sc[0], sc[1], sc[2]);
+#pragma unroll
+      for (int q = 0; q < NC; ++q)
+        dst[g.idx(dc[0], dc[1], dc[2]) * NC + q] = c2f_value<T, NC>(src, g, sc[0], sc[1], sc[2], q);
     }
   }
 }
 
-template <typename T>
-int ghost_fill_raw(bcm_mesh* m, T* data, int mask) {
+template <typename T, int NC>
+static int ghost_fill_nc(bcm_mesh* m, T* data, int mask) {
   const int kinds = mask & BCM_GF_ALL;
   int axes = (mask >> 3) & 7;
   if (axes == 0) axes = 7;
@@ -180,7 +190,7 @@ int ghost_fill_raw(bcm_mesh* m, T* data, int mask) {
   BCM_TRY(timer_begin(m, BCM_K_GHOST, &ev));
   auto launch = [&](int lo, int hi) -> int {
     if (hi <= lo) return BCM_OK;
-    hipLaunchKernelGGL(k_ghost_fill<T>, dim3(hi - lo), dim3(256), 0, m->stream, data, m->d_tasks + lo, g);
+    hipLaunchKernelGGL((k_ghost_fill<T, NC>), dim3(hi - lo), dim3(256), 0, m->stream, data, m->d_tasks + lo, g);
     BCM_HIP_TRY(hipGetLastError());
     return BCM_OK;
   };
@@ -200,8 +210,31 @@ int ghost_fill_raw(bcm_mesh* m, T* data, int mask) {
   return timer_end(m, &ev);
 }
 
+template <typename T>
+int ghost_fill_raw(bcm_mesh* m, T* data, int mask) {
+  return ghost_fill_nc<T, 1>(m, data, mask);
+}
+
 template int ghost_fill_raw<double>(bcm_mesh*, double*, int);
 template int ghost_fill_raw<float>(bcm_mesh*, float*, int);
+
+// The face tasks whose destination is (block, face) -- and, for a finer
+// neighbour, subface -- one work-group each (the per-face exchange of
+// VCUpdater::copyFromNeighbor).
+template <typename T, int NC>
+static int ghost_fill_face_nc(bcm_mesh* m, T* data, int block, int face, int subface) {
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  int found = 0;
+  for (int t = 0; t < int(m->h_tasks.size()); ++t) {
+    const int4 tk = m->h_tasks[t];
+    const int tf = tk.z & 7, ts = (tk.z >> 3) & 3, kind = (tk.z >> 5) & 3;
+    if (tk.x != block || tf != face || (kind == KIND_F2C && ts != subface)) continue;
+    hipLaunchKernelGGL((k_ghost_fill<T, NC>), dim3(1), dim3(256), 0, m->stream, data, m->d_tasks + t, g);
+    BCM_HIP_TRY(hipGetLastError());
+    ++found;
+  }
+  return found ? BCM_OK : BCM_ERR_ARG;
+}
 
 // ---------------------------------------------------------------------
 // Dirichlet mirror: g = 2v - u0 on the first ghost layer
@@ -492,6 +525,7 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
       m->off[a][3] = int(all.size());
     }
     m->ntask_total = int(all.size());
+    m->h_tasks = all;
     if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_tasks), sizeof(int4) * all.size())) != BCM_OK) break;
     if (!all.empty() &&
         hipMemcpy(m->d_tasks, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) != hipSuccess) {
@@ -700,11 +734,15 @@ extern "C" int bcm_field_download(bcm_field_t f, void* host, size_t bytes) {
 }
 
 // Blocks first_block .. first_block+nblocks-1 (e.g. one segment of a batch).
+// Byte range of blocks first_block .. first_block+nblocks-1 (a block holds
+// count / nblocks values: (n+2vc)^3 per component).
 static int block_range(bcm_field_t f, int first_block, int nblocks, size_t bytes, size_t* off, size_t* len) {
-  if (!f || first_block < 0 || nblocks < 0 || first_block + nblocks > f->mesh->nblocks) return BCM_ERR_ARG;
+  if (!f || !f->mesh || first_block < 0 || nblocks < 0 || first_block + nblocks > f->mesh->nblocks)
+    return BCM_ERR_ARG;
   const size_t es = f->dtype == BCM_F64 ? 8 : 4;
-  *off = size_t(first_block) * f->mesh->bvol * es;
-  *len = size_t(nblocks) * f->mesh->bvol * es;
+  const size_t per = f->count / size_t(f->mesh->nblocks);
+  *off = size_t(first_block) * per * es;
+  *len = size_t(nblocks) * per * es;
   return bytes == *len ? BCM_OK : BCM_ERR_ARG;
 }
 
@@ -748,6 +786,90 @@ extern "C" int bcm_ghost_fill(bcm_field_t f, int mask) {
   }
   if (f->dtype == BCM_F64) return ghost_fill_raw<double>(f->mesh, static_cast<double*>(f->data), mask);
   return ghost_fill_raw<float>(f->mesh, static_cast<float*>(f->data), mask);
+}
+
+extern "C" int bcm_ghost_fill_face(bcm_field_t f, int block, int face, int subface) {
+  if (!f || !f->mesh || block < 0 || block >= f->mesh->nblocks || face < 0 || face >= 6 || subface < 0 ||
+      subface >= 4)
+    return BCM_ERR_ARG;
+  if (f->dtype == BCM_F64)
+    return ghost_fill_face_nc<double, 1>(f->mesh, static_cast<double*>(f->data), block, face, subface);
+  return ghost_fill_face_nc<float, 1>(f->mesh, static_cast<float*>(f->data), block, face, subface);
+}
+
+// ---------------------------------------------------------------------
+// Vector fields: three interleaved components per cell.
+// ---------------------------------------------------------------------
+extern "C" int bcm_vfield_create(bcm_mesh_t m, int dtype, bcm_vfield_t* out) {
+  if (!m || !out || (dtype != BCM_F64 && dtype != BCM_F32)) return BCM_ERR_ARG;
+  *out = nullptr;
+  bcm_vfield* v = new (std::nothrow) bcm_vfield();
+  if (!v) return BCM_ERR_NOMEM;
+  bcm_field& f = v->base;
+  f.mesh = m;
+  f.dtype = dtype;
+  f.count = 3 * size_t(m->nblocks) * m->bvol;
+  f.bytes = f.count * (dtype == BCM_F64 ? 8 : 4);
+  int rc = dev_alloc(&f.data, f.bytes);
+  if (rc == BCM_OK && hipMemsetAsync(f.data, 0, f.bytes, m->stream) != hipSuccess) rc = BCM_ERR_HIP;
+  if (rc != BCM_OK) {
+    dev_free(f.data);
+    delete v;
+    return rc;
+  }
+  m->fields.push_back(&f);
+  *out = v;
+  return BCM_OK;
+}
+
+extern "C" int bcm_vfield_destroy(bcm_vfield_t v) {
+  if (!v) return BCM_OK;
+  bcm_field* f = &v->base;
+  if (bcm_mesh* m = f->mesh) {
+    (void)hipStreamSynchronize(m->stream);
+    m->fields.erase(std::remove(m->fields.begin(), m->fields.end(), f), m->fields.end());
+  }
+  dev_free(f->data);
+  delete v;
+  return BCM_OK;
+}
+
+extern "C" int bcm_vfield_upload(bcm_vfield_t v, const void* host, size_t bytes) {
+  if (!v || !v->base.mesh) return BCM_ERR_ARG;
+  return bcm_field_upload(&v->base, host, bytes);
+}
+
+extern "C" int bcm_vfield_download(bcm_vfield_t v, void* host, size_t bytes) {
+  if (!v || !v->base.mesh) return BCM_ERR_ARG;
+  return bcm_field_download(&v->base, host, bytes);
+}
+
+extern "C" int bcm_vfield_upload_blocks(bcm_vfield_t v, int first_block, int nblocks, const void* host,
+                                        size_t bytes) {
+  if (!v) return BCM_ERR_ARG;
+  return bcm_field_upload_blocks(&v->base, first_block, nblocks, host, bytes);
+}
+
+extern "C" int bcm_vfield_download_blocks(bcm_vfield_t v, int first_block, int nblocks, void* host, size_t bytes) {
+  if (!v) return BCM_ERR_ARG;
+  return bcm_field_download_blocks(&v->base, first_block, nblocks, host, bytes);
+}
+
+extern "C" int bcm_vfield_ghost_fill(bcm_vfield_t v, int mask) {
+  if (!v || !v->base.mesh || (mask & ~63)) return BCM_ERR_ARG;
+  bcm_field& f = v->base;
+  if (f.dtype == BCM_F64) return ghost_fill_nc<double, 3>(f.mesh, static_cast<double*>(f.data), mask);
+  return ghost_fill_nc<float, 3>(f.mesh, static_cast<float*>(f.data), mask);
+}
+
+extern "C" int bcm_vfield_ghost_fill_face(bcm_vfield_t v, int block, int face, int subface) {
+  if (!v || !v->base.mesh) return BCM_ERR_ARG;
+  bcm_field& f = v->base;
+  if (block < 0 || block >= f.mesh->nblocks || face < 0 || face >= 6 || subface < 0 || subface >= 4)
+    return BCM_ERR_ARG;
+  if (f.dtype == BCM_F64)
+    return ghost_fill_face_nc<double, 3>(f.mesh, static_cast<double*>(f.data), block, face, subface);
+  return ghost_fill_face_nc<float, 3>(f.mesh, static_cast<float*>(f.data), block, face, subface);
 }
 
 extern "C" int bcm_impose_bc(bcm_field_t f, const double* values, int nvalues) {

@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <random>
 #include <vector>
 
 namespace {
@@ -75,6 +76,8 @@ enum NodeKind { BRANCH, LEAF_ACTIVE, LEAF_INACTIVE };
 
 struct bcm_tree {
   bcm_tree_params p;
+  bcm_divider_fn user_divide = nullptr;  // BCM_DIV_USER
+  void* user = nullptr;
   std::vector<TNode> nodes;
   std::vector<int32_t> root_node;   // node index of each root
   std::vector<int32_t> leaf_nodes;  // node index of each block ID
@@ -113,6 +116,10 @@ struct bcm_tree {
   NodeKind classify(const TNode& nd) const {
     const int level = int(nd.level);
     switch (p.divider) {
+      case BCM_DIV_USER: {  // the caller's Divider::operator() (Utils/include/Divider.h:40-46)
+        const int k = user_divide(user, make_pedigree(nd.level, nd.x, nd.y, nd.z, nd.root));
+        return k == BCM_NODE_BRANCH ? BRANCH : (k == BCM_NODE_LEAF_ACTIVE ? LEAF_ACTIVE : LEAF_INACTIVE);
+      }
       case BCM_DIV_FLAT:  // FlatDivider (Utils/include/SimpleDivider.h:35-50)
         return level >= p.max_level ? LEAF_ACTIVE : BRANCH;
       case BCM_DIV_SIMPLE: {  // SimpleDivider, all six directions (:53-151)
@@ -364,18 +371,29 @@ struct bcm_tree {
   }
 };
 
-extern "C" int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out) {
+// Leaves in random order (BCMOctree::randomShuffle, Utils/src/BCMOctree.cpp:
+// 400-404, which calls std::random_shuffle and so is not reproducible across
+// platforms): a shuffle of the Z order by a fixed-seed Mersenne twister.
+static void shuffle_leaves(bcm_tree* t) {
+  std::mt19937 rng(20111213u);
+  std::shuffle(t->leaf_nodes.begin(), t->leaf_nodes.end(), rng);
+  for (size_t b = 0; b < t->leaf_nodes.size(); ++b) t->nodes[t->leaf_nodes[b]].id = int32_t(b);
+}
+
+static int tree_build(const bcm_tree_params* params, bcm_divider_fn divide, void* user, bcm_tree_t* out) {
   if (!params || !out) return BCM_ERR_ARG;
   *out = nullptr;
   const bcm_tree_params& p = *params;
+  if ((p.divider == BCM_DIV_USER) != (divide != nullptr)) return BCM_ERR_ARG;
   for (int a = 0; a < 3; ++a)
     if (p.root_n[a] < 1) return BCM_ERR_ARG;
   if (p.root_n[0] * p.root_n[1] * p.root_n[2] > 0xfff + 1) return BCM_ERR_ARG;
   if (p.max_level < 0 || p.max_level > 15 || p.min_level < 0 || p.min_level > p.max_level)
     return BCM_ERR_ARG;
-  if (p.divider < BCM_DIV_FLAT || p.divider > BCM_DIV_SHELL) return BCM_ERR_ARG;
+  if (p.divider < BCM_DIV_FLAT || p.divider > BCM_DIV_USER) return BCM_ERR_ARG;
   if (p.divider == BCM_DIV_SHELL && (p.n_shells < 1 || p.n_shells > BCM_MAX_SHELLS)) return BCM_ERR_ARG;
-  if (p.ordering != BCM_ORDER_Z && p.ordering != BCM_ORDER_HILBERT) return BCM_ERR_ARG;
+  if (p.ordering != BCM_ORDER_Z && p.ordering != BCM_ORDER_HILBERT && p.ordering != BCM_ORDER_RANDOM)
+    return BCM_ERR_ARG;
   if (p.bc_setter < BCM_BCSET_ALL_DIRICHLET || p.bc_setter > BCM_BCSET_PER_FACE) return BCM_ERR_ARG;
   if (p.bc_setter == BCM_BCSET_PER_FACE)
     for (int f = 0; f < 6; ++f)
@@ -386,6 +404,8 @@ extern "C" int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out) {
   if (!t) return BCM_ERR_NOMEM;
   try {
     t->p = p;
+    t->user_divide = divide;
+    t->user = user;
     const int nr = t->nroot();
     t->root_node.resize(nr);
     for (int r = 0; r < nr; ++r) {
@@ -404,6 +424,7 @@ extern "C" int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out) {
       if (p.ordering == BCM_ORDER_HILBERT) t->pick_hilbert(t->root_node[r], 0);
       else t->pick_z(t->root_node[r]);
     }
+    if (p.ordering == BCM_ORDER_RANDOM) shuffle_leaves(t);
     const int rc = t->make_blocks();
     if (rc != BCM_OK) {
       delete t;
@@ -415,6 +436,16 @@ extern "C" int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out) {
   }
   *out = t;
   return BCM_OK;
+}
+
+extern "C" int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out) {
+  return tree_build(params, nullptr, nullptr, out);
+}
+
+extern "C" int bcm_tree_build_user(const bcm_tree_params* params, bcm_divider_fn divide, void* user,
+                                   bcm_tree_t* out) {
+  if (!divide) return BCM_ERR_ARG;
+  return tree_build(params, divide, user, out);
 }
 
 extern "C" int bcm_tree_num_blocks(bcm_tree_t tree) {

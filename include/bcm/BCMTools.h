@@ -1,5 +1,6 @@
 // Drop-in facade of the reference's include/BCMTools.h (face/subface
-// enums, exit codes, Exit macro) for drivers built against libbcm_hip.
+// enums, exit codes, Exit macro, assert with NDEBUG unless DEBUG, the MPI
+// header) for drivers built against libbcm_hip.
 #ifndef BCM_FACADE_BCMTOOLS_H
 #define BCM_FACADE_BCMTOOLS_H
 
@@ -7,6 +8,15 @@
 #include <cstdlib>
 
 #include "bcm_hip.h"
+#include "mpi.h"
+
+// As the reference: assertions are off unless DEBUG is defined.
+#ifndef DEBUG
+#ifndef NDEBUG
+#define NDEBUG
+#endif
+#endif
+#include <cassert>
 
 // Exit(x): print the call site and leave (include/BCMTools.h:36-37).
 #define Exit(x) ((void)std::printf("exit at %s:%u\n", __FILE__, __LINE__), std::exit((x)))

@@ -1,5 +1,5 @@
-// BlockBase (include/BlockBase.h:40-138) plus the BoundaryInfo / Block
-// pair of Utils/include/{BoundaryInfo,Block}.h.
+// BlockBase (include/BlockBase.h:40-138): a block's cell count, origin,
+// size, level, neighbour table and registered data classes.
 #ifndef BCM_FACADE_BLOCKBASE_H
 #define BCM_FACADE_BLOCKBASE_H
 
@@ -56,35 +56,6 @@ class BlockBase {
  private:
   BlockBase(const BlockBase&);
   BlockBase& operator=(const BlockBase&);
-};
-
-class BoundaryInfo {
- public:
-  enum Type { INNER = BCM_BC_INNER, DIRICHLET = BCM_BC_DIRICHLET, NEUMANN = BCM_BC_NEUMANN, PERIODIC = BCM_BC_PERIODIC };
-
- private:
-  Type type;
-  int id;
-
- public:
-  BoundaryInfo() : type(INNER), id(-1) {}
-  void setType(Type t) { type = t; }
-  void setID(int i) { id = i; }
-  Type getType() const { return type; }
-  int getID() const { return id; }
-};
-
-class Block : public BlockBase {
-  BoundaryInfo* boundaryInfo;
-
- public:
-  Block(const Vec3i& size_, const Vec3r& origin_, const Vec3r& blockSize_, int level_, NeighborInfo* ni,
-        BoundaryInfo* bi)
-      : BlockBase(size_, origin_, blockSize_, level_, ni), boundaryInfo(bi) {}
-  ~Block() { delete[] boundaryInfo; }
-  const BoundaryInfo* getBoundaryInfo() const { return boundaryInfo; }
-  int boundaryType(int f) const { return boundaryInfo ? int(boundaryInfo[f].getType()) : BCM_BC_INNER; }
-  int boundaryID(int f) const { return boundaryInfo ? boundaryInfo[f].getID() : -1; }
 };
 
 #endif

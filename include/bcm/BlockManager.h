@@ -2,10 +2,12 @@
 // the C ABI of include/bcm_hip.h.
 //
 // Same calls as the reference: registerBlock / endRegisterBlock,
-// setDataClass<D[,U[,T]]>(vc), prepareForVCUpdate(id, tag, method) and
-// every updateVC* / beginUpdateVC* / endUpdateVC* form.  Each data class
-// registered with an updater lives in one contiguous host arena (the
-// reference's setDataClass<D,U,T> layout) mirrored by one device field.
+// setDataClass<D[,U[,T]]>(vc), prepareForVCUpdate(id, tag, method), every
+// updateVC* / beginUpdateVC* / endUpdateVC* form, set/getCommunicator and
+// printBlockLayoutInfo([file]).  Each data class registered with an updater
+// (Scalar3D<T> / Scalar3DUpdater<T>, Vector3D<T> / Vector3DUpdater<T>)
+// lives in one contiguous host arena (the reference's setDataClass<D,U,T>
+// layout) mirrored by one device field.
 //
 // Residency:
 //  * host-resident (default, drop-in): the host arena is authoritative;
@@ -16,44 +18,56 @@
 //    the device kernels (deviceField(id) with the bcm_* entry points);
 //    syncToHost / syncToDevice move the data explicitly.
 //
-// Single GPU, single process: getStartID() is 0, ranks are 0, and there
-// is no communicator (MPI is not linked).  beginUpdateVC* performs the
-// whole exchange and endUpdateVC* returns immediately, which gives the
-// reference's results because the exchange writes face ghosts only.
+// One process drives one GPU: the communicator has one rank (include/bcm/
+// mpi.h), getStartID() is 0, every neighbour is local.  beginUpdateVC*
+// performs the whole exchange and endUpdateVC* returns immediately, which
+// gives the reference's results because the exchange writes face ghosts
+// only.  A per-face VCUpdater call (copyFromNeighbor(face, subface)) comes
+// back here as exchangeFace and runs that face's rule on the device.
 #ifndef BCM_FACADE_BLOCKMANAGER_H
 #define BCM_FACADE_BLOCKMANAGER_H
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <map>
+#include <ostream>
 #include <vector>
 
 #include "BCMTools.h"
 #include "BlockBase.h"
 #include "Scalar3D.h"
 #include "Scalar3DUpdater.h"
+#include "Vector3D.h"
+#include "Vector3DUpdater.h"
 #include "bcm_hip.h"
+#include "mpi.h"
 
 class VCUpdateMethod {
  public:
   enum Type { AtOnce, SeparateXYZ, SeparateXYZ_SeparateLevelDiff };
 };
 
-class BlockManager {
+class BlockManager : public FaceExchanger {
   struct FieldRec {
     int vc = 0;
     int dtype = BCM_F64;
-    bcm_field_t dev = nullptr;
+    int ncomp = 1;                 // values per cell: 1 (Scalar3D) or 3 (Vector3D)
+    bcm_field_t dev = nullptr;     // scalar classes
+    bcm_vfield_t vdev = nullptr;   // vector classes
     void* host = nullptr;
     size_t bytes = 0;
+    size_t block_bytes = 0;
     bool updatable = false;
     bool prepared = false;
     bool device_resident = false;
     void (*free_host)(void*) = nullptr;
   };
 
+  MPI::Intracomm comm;
   std::vector<BlockBase*> blockList;
   Vec3i size;
   int numBlock = 0;
@@ -95,16 +109,24 @@ class BlockManager {
     return it->second;
   }
 
-  template <typename D, typename T>
+  // The updater type U of a data class D (Scalar3DUpdater<T> or
+  // Vector3DUpdater<T>, or none).
+  template <typename D, typename U, typename T>
   int add_contiguous(int vc, bool updatable) {
+    const int nc = D::ncomp;
     const int nx0 = size.x + 2 * vc, ny0 = size.y + 2 * vc, nz0 = size.z + 2 * vc;
-    const size_t bvol = size_t(nx0) * ny0 * nz0;
+    const size_t bvol = size_t(nx0) * ny0 * nz0 * nc;
     T* data = new T[bvol * numBlock]();
     int id = -1;
+    std::vector<VCUpdater*> updaters;
     for (int b = 0; b < numBlock; ++b) {
       BlockBase* block = blockList[b];
       D* dc = new D(block->getSize(), vc, &data[bvol * b]);
-      if (updatable) dc->setUpdater(new Scalar3DUpdater<T>(block->getNeighborInfo()));
+      if (updatable) {
+        VCUpdater* u = new U(block->getNeighborInfo(), comm);
+        dc->setUpdater(u);
+        updaters.push_back(u);
+      }
       const int id0 = block->setDataClass(dc);
       if (b == 0) id = id0;
       else if (id0 != id) {
@@ -112,40 +134,108 @@ class BlockManager {
         Exit(EX_FAILURE);
       }
     }
+    for (int b = 0; b < int(updaters.size()); ++b) updaters[b]->bindExchange(this, id, b);
     FieldRec r;
     r.vc = vc;
     r.dtype = sizeof(T) == 8 ? BCM_F64 : BCM_F32;
+    r.ncomp = nc;
     r.host = data;
-    r.bytes = bvol * numBlock * sizeof(T);
+    r.block_bytes = bvol * sizeof(T);
+    r.bytes = r.block_bytes * numBlock;
     r.updatable = updatable;
     r.free_host = &free_array<T>;
-    if (updatable) check(bcm_field_create(mesh_for(vc), r.dtype, &r.dev), "bcm_field_create");
+    if (updatable) {
+      if (nc == 1) check(bcm_field_create(mesh_for(vc), r.dtype, &r.dev), "bcm_field_create");
+      else check(bcm_vfield_create(mesh_for(vc), r.dtype, &r.vdev), "bcm_vfield_create");
+    }
     fields[id] = r;
     return id;
   }
 
-  void exchange(int id, int mask) {
+  void upload(FieldRec& r) {
+    if (r.dev) check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
+    else check(bcm_vfield_upload(r.vdev, r.host, r.bytes), "bcm_vfield_upload");
+  }
+  void download(FieldRec& r) {
+    if (r.dev) check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
+    else check(bcm_vfield_download(r.vdev, r.host, r.bytes), "bcm_vfield_download");
+  }
+
+  FieldRec& prepared(int id) {
     FieldRec& r = rec(id);
     if (!r.updatable || !r.prepared) {
       std::cout << "*** error: data class " << id << " is not prepared for VC update" << std::endl;
       Exit(EX_FAILURE);
     }
-    if (!r.device_resident) check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
-    check(bcm_ghost_fill(r.dev, mask), "bcm_ghost_fill");
-    if (!r.device_resident) check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
+    return r;
+  }
+
+  void exchange(int id, int mask) {
+    FieldRec& r = prepared(id);
+    if (!r.device_resident) upload(r);
+    if (r.dev) check(bcm_ghost_fill(r.dev, mask), "bcm_ghost_fill");
+    else check(bcm_vfield_ghost_fill(r.vdev, mask), "bcm_vfield_ghost_fill");
+    if (!r.device_resident) download(r);
   }
 
   static int axis_bit(int xyz) { return BCM_GF_X << xyz; }
 
+  // src/BlockManager.cpp:118-287 with one rank: every face is intra-node.
+  void layoutReport(std::ostream& os) const {
+    int lmin = INT_MAX, lmax = 0;
+    for (int b = 0; b < numBlock; ++b) {
+      lmin = std::min(lmin, blockList[b]->getLevel());
+      lmax = std::max(lmax, blockList[b]->getLevel());
+    }
+    os << "Block Layout Information" << std::endl;
+    os << "  Min level: " << lmin << std::endl << "  Max level: " << lmax << std::endl;
+    os << "  Number of blocks" << std::endl;
+    for (int l = lmin; l <= lmax; ++l) {
+      int c = 0;
+      for (int b = 0; b < numBlock; ++b) c += blockList[b]->getLevel() == l;
+      os << "    L=" << l << ": " << c << std::endl;
+    }
+    os << "    total: " << numBlock << std::endl;
+    os << "  Number of total blocks / node" << std::endl;
+    os << "    ave: " << REAL_TYPE(numBlock) << std::endl << "    min: " << numBlock << std::endl
+       << "    max: " << numBlock << std::endl << "     sd: " << 0 << std::endl;
+    int faces[3] = {0, 0, 0};  // dLevel -1, 0, +1
+    for (int b = 0; b < numBlock; ++b) {
+      const NeighborInfo* ni = blockList[b]->getNeighborInfo();
+      for (int f = 0; f < NUM_FACE; ++f) {
+        if (!ni[f].exists()) continue;
+        const int ld = ni[f].getLevelDifference();
+        if (ld == 1) {
+          for (int q = 0; q < NUM_SUBFACE; ++q) faces[2] += ni[f].getID(Subface(q)) >= 0;
+        } else {
+          faces[ld + 1] += 1;
+        }
+      }
+    }
+    os << "  Number of faces" << std::endl << "    intra-node " << std::endl;
+    os << "        dLevel=-1: " << faces[0] << std::endl << "        dLevel= 0: " << faces[1] << std::endl
+       << "        dLevel=+1: " << faces[2] << std::endl << "            total: " << faces[0] + faces[1] + faces[2]
+       << std::endl;
+    os << "    inter-node " << std::endl << "        dLevel=-1: 0" << std::endl << "        dLevel= 0: 0" << std::endl
+       << "        dLevel=+1: 0" << std::endl << "            total: 0" << std::endl;
+    os << "  Number of total inter-node faces / node" << std::endl << "    ave: 0" << std::endl << "    min: 0"
+       << std::endl << "    max: 0" << std::endl << "     sd: 0" << std::endl;
+  }
+
  public:
-  BlockManager() {}
+  BlockManager() : comm(MPI::COMM_WORLD) {}
   ~BlockManager() {
     for (auto& kv : fields) {
       if (kv.second.dev) bcm_field_destroy(kv.second.dev);
+      if (kv.second.vdev) bcm_vfield_destroy(kv.second.vdev);
       // Host arenas stay valid while blocks may still reference them.
     }
     for (auto& kv : meshes) bcm_mesh_destroy(kv.second);
   }
+
+  // include/BlockManager.h:159-162 (one rank).
+  void setCommunicator(const MPI::Intracomm& c) { comm = c; }
+  const MPI::Intracomm& getCommunicator() { return comm; }
 
   static BlockManager& getInstance() {
     static BlockManager instance;
@@ -200,50 +290,58 @@ class BlockManager {
     return blockList[localID];
   }
 
-  // src/BlockManager.cpp:118-287, one rank.
-  void printBlockLayoutInfo() {
-    int lmin = INT_MAX, lmax = 0;
-    for (int b = 0; b < numBlock; ++b) {
-      lmin = std::min(lmin, blockList[b]->getLevel());
-      lmax = std::max(lmax, blockList[b]->getLevel());
+  // include/BlockManager.h:195-196.
+  void printBlockLayoutInfo() { layoutReport(std::cout); }
+  void printBlockLayoutInfo(const char* filename) {
+    std::ofstream ofs(filename, std::ios::out);
+    if (!ofs) {
+      std::cout << "*** error: cannot open " << filename << std::endl;
+      Exit(EX_FAILURE);
     }
-    std::cout << "Block Layout Information" << std::endl;
-    std::cout << "  Min level: " << lmin << std::endl << "  Max level: " << lmax << std::endl;
-    std::cout << "  Number of blocks" << std::endl;
-    for (int l = lmin; l <= lmax; ++l) {
-      int c = 0;
-      for (int b = 0; b < numBlock; ++b) c += blockList[b]->getLevel() == l;
-      std::cout << "    L=" << l << ": " << c << std::endl;
-    }
-    std::cout << "    total: " << numBlock << std::endl;
-    int faces[3] = {0, 0, 0};
-    for (int b = 0; b < numBlock; ++b) {
-      const NeighborInfo* ni = blockList[b]->getNeighborInfo();
-      for (int f = 0; f < NUM_FACE; ++f) {
-        const int ld = ni[f].getLevelDifference();
-        if (ld == 1) {
-          for (int s = 0; s < NUM_SUBFACE; ++s) faces[2] += ni[f].getID(Subface(s)) >= 0;
-        } else if (ni[f].getID() >= 0) {
-          faces[ld + 1] += 1;
-        }
-      }
-    }
-    std::cout << "  Number of faces (dLevel -1/0/+1): " << faces[0] << "/" << faces[1] << "/" << faces[2]
-              << std::endl;
+    layoutReport(ofs);
   }
 
   // include/BlockManager.h:202-283.  A class without an updater is host-only.
   template <typename D>
   int setDataClass(int vc) {
-    return add_contiguous<D, typename D::value_type>(vc, false);
+    return add_contiguous<D, Scalar3DUpdater<typename D::value_type>, typename D::value_type>(vc, false);
   }
   template <typename D, typename U>
   int setDataClass(int vc) {
-    return add_contiguous<D, typename D::value_type>(vc, true);
+    return add_contiguous<D, U, typename D::value_type>(vc, true);
   }
   template <typename D, typename U, typename T>
   int setDataClass(int vc) {
-    return add_contiguous<D, T>(vc, true);
+    return add_contiguous<D, U, T>(vc, true);
+  }
+
+  // FaceExchanger: one face of one block, for VCUpdater::copyFromNeighbor.
+  // A host-resident class moves the blocks the rule reads and writes.
+  void exchangeFace(int id, int block, Face face, Subface subface) {
+    FieldRec& r = prepared(id);
+    const NeighborInfo& ni = blockList[block]->getNeighborInfo()[face];
+    const int src = ni.getLevelDifference() == 1 ? ni.getID(subface) : ni.getID();
+    if (src < 0) {
+      std::cout << "*** error: face " << int(face) << " of block " << block << " has no neighbour" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    char* host = static_cast<char*>(r.host);
+    auto up = [&](int b) {
+      char* p = host + r.block_bytes * b;
+      if (r.dev) check(bcm_field_upload_blocks(r.dev, b, 1, p, r.block_bytes), "bcm_field_upload_blocks");
+      else check(bcm_vfield_upload_blocks(r.vdev, b, 1, p, r.block_bytes), "bcm_vfield_upload_blocks");
+    };
+    if (!r.device_resident) {
+      up(block);
+      if (src != block) up(src);
+    }
+    if (r.dev) check(bcm_ghost_fill_face(r.dev, block, int(face), int(subface)), "bcm_ghost_fill_face");
+    else check(bcm_vfield_ghost_fill_face(r.vdev, block, int(face), int(subface)), "bcm_vfield_ghost_fill_face");
+    if (!r.device_resident) {
+      char* p = host + r.block_bytes * block;
+      if (r.dev) check(bcm_field_download_blocks(r.dev, block, 1, p, r.block_bytes), "bcm_field_download_blocks");
+      else check(bcm_vfield_download_blocks(r.vdev, block, 1, p, r.block_bytes), "bcm_vfield_download_blocks");
+    }
   }
 
   void prepareForVCUpdate(int id, int /*tag*/, VCUpdateMethod::Type /*method*/) { rec(id).prepared = true; }
@@ -306,21 +404,20 @@ class BlockManager {
 
   // Device path.
   bcm_field_t deviceField(int id) { return rec(id).dev; }
+  bcm_vfield_t deviceVectorField(int id) { return rec(id).vdev; }
   bcm_mesh_t deviceMesh(int id) { return mesh_for(rec(id).vc); }
   void setDeviceResident(int id, bool on) {
     FieldRec& r = rec(id);
-    if (on && !r.device_resident) check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
-    if (!on && r.device_resident) check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
+    if (!r.dev && !r.vdev) {
+      std::cout << "*** error: data class " << id << " has no device copy (no updater)" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    if (on && !r.device_resident) upload(r);
+    if (!on && r.device_resident) download(r);
     r.device_resident = on;
   }
-  void syncToHost(int id) {
-    FieldRec& r = rec(id);
-    check(bcm_field_download(r.dev, r.host, r.bytes), "bcm_field_download");
-  }
-  void syncToDevice(int id) {
-    FieldRec& r = rec(id);
-    check(bcm_field_upload(r.dev, r.host, r.bytes), "bcm_field_upload");
-  }
+  void syncToHost(int id) { download(rec(id)); }
+  void syncToDevice(int id) { upload(rec(id)); }
   const std::vector<bcm_block_info>& blockTable() const { return table; }
 
  private:

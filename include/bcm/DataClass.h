@@ -1,6 +1,6 @@
-// DataClass / UpdatableDataClass (include/DataClass.h:38-178).  The ghost
-// exchange of an updatable class runs on the device through its
-// BlockManager, never per face on the host.
+// DataClass / UpdatableDataClass (include/DataClass.h:38-178): a per-block
+// array with vc ghost layers; an updatable one carries the VCUpdater that
+// exchanges its ghosts.
 #ifndef BCM_FACADE_DATACLASS_H
 #define BCM_FACADE_DATACLASS_H
 

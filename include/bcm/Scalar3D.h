@@ -7,6 +7,7 @@
 
 #include "DataClass.h"
 #include "Index3D.h"
+#include "VCUpdater.h"
 
 template <typename T>
 class Scalar3D : public UpdatableDataClass {
@@ -17,6 +18,7 @@ class Scalar3D : public UpdatableDataClass {
 
  public:
   typedef T value_type;
+  static const int ncomp = 1;  // values per cell
   Scalar3D(const Vec3class::Vec3i& size_, int vc_)
       : UpdatableDataClass(size_, vc_), owner(true), index(size_, vc_) {
     nx0 = size_[0] + 2 * vc_;
@@ -38,6 +40,28 @@ class Scalar3D : public UpdatableDataClass {
   Index3DS getIndex() const { return index; }
   T& operator()(int i, int j, int k) { return data[index(i, j, k)]; }
   const T& operator()(int i, int j, int k) const { return data[index(i, j, k)]; }
+
+  // Box copies between the block and a dense buffer (x fastest), and from
+  // another Scalar3D (include/Scalar3D.h:120-137).
+  void copyToBuffer(int i0, int j0, int k0, int nx, int ny, int nz, T* buffer) const {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) buffer[i + nx * (j + ny * k)] = data[index(i0 + i, j0 + j, k0 + k)];
+  }
+  void copyFromBuffer(int i0, int j0, int k0, int nx, int ny, int nz, const T* buffer) {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) data[index(i0 + i, j0 + j, k0 + k)] = buffer[i + nx * (j + ny * k)];
+  }
+  void copyFromDataClass(int i0, int j0, int k0, int i1, int j1, int k1, int nx, int ny, int nz,
+                         const DataClass* dataClass) {
+    const Scalar3D<T>* s = dynamic_cast<const Scalar3D<T>*>(dataClass);
+    const T* sData = s->getData();
+    const Index3DS sIndex = s->getIndex();
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) data[index(i0 + i, j0 + j, k0 + k)] = sData[sIndex(i1 + i, j1 + j, k1 + k)];
+  }
 
  private:
   Scalar3D(const Scalar3D&);

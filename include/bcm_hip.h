@@ -74,8 +74,13 @@ typedef struct bcm_block_info {
 /* Utils/include/BlockFactory.h:89-96) plus an explicit 2:1 balancer,  */
 /* which the reference lacks.                                           */
 /* ------------------------------------------------------------------ */
-enum { BCM_DIV_FLAT = 0, BCM_DIV_SIMPLE = 1, BCM_DIV_SHELL = 2 };
-enum { BCM_ORDER_Z = 0, BCM_ORDER_HILBERT = 1 };
+/* BCM_DIV_USER: the caller's divider, called once per node (see
+ * bcm_tree_build_user).  BCM_ORDER_RANDOM: the leaves in a pseudo-random
+ * order, the same on every platform (the reference's BCMOctree::RANDOM
+ * shuffles with std::random_shuffle, Utils/src/BCMOctree.cpp:400-404,
+ * whose order is platform-dependent). */
+enum { BCM_DIV_FLAT = 0, BCM_DIV_SIMPLE = 1, BCM_DIV_SHELL = 2, BCM_DIV_USER = 3 };
+enum { BCM_ORDER_Z = 0, BCM_ORDER_HILBERT = 1, BCM_ORDER_RANDOM = 2 };
 /* Boundary setters: all outer faces Dirichlet id 0
  * (examples/sor2/BoundaryConditionSetter.h), or the sor1 'x'/'y'/'z'
  * setter (examples/sor1/BoundaryConditionSetter.h): outer faces periodic
@@ -121,6 +126,15 @@ typedef struct bcm_tree* bcm_tree_t;
 /* Replaces `new BCMOctree(rootGrid, divider, ordering)` + the
  * BlockFactory::makeBlock loop of examples/sor1/main.cpp:88-123. */
 int bcm_tree_build(const bcm_tree_params* params, bcm_tree_t* out);
+/* Divider::operator() (Utils/include/Divider.h:40-46): the node kinds of
+ * Divider::NodeType, in its order. */
+enum { BCM_NODE_BRANCH = 0, BCM_NODE_LEAF_ACTIVE = 1, BCM_NODE_LEAF_INACTIVE = 2 };
+typedef int (*bcm_divider_fn)(void* user, uint64_t pedigree);
+/* `new BCMOctree(rootGrid, divider, ordering)` with the caller's divider
+ * (Utils/src/BCMOctree.cpp:23-48, 111-124): params->divider must be
+ * BCM_DIV_USER; divide(user, pedigree) is called for every node, parents
+ * before children, and returns a BCM_NODE_* kind. */
+int bcm_tree_build_user(const bcm_tree_params* params, bcm_divider_fn divide, void* user, bcm_tree_t* out);
 /* BCMOctree::getNumLeafNode (Utils/include/BCMOctree.h:111). Returns the count (>=0) or -BCM_ERR_ARG. */
 int bcm_tree_num_blocks(bcm_tree_t tree);
 /* Copies the block table (one record per leaf in block-ID order). */
@@ -225,6 +239,30 @@ size_t bcm_field_bytes(bcm_field_t field);
  * copyFromNeighborF2C_0, copyFromNeighborC2F_0
  * (include/Scalar3DUpdater.h:321-347, 519-708). Face ghosts only. */
 int bcm_ghost_fill(bcm_field_t field, int mask);
+
+/* VCUpdater::copyFromNeighbor(face, subface) for one block
+ * (include/VCUpdater.h:65-82, include/Scalar3DUpdater.h:321-347, 519-708):
+ * the ghosts of `face` of `block` from its neighbour -- same level, the
+ * finer neighbour on `subface` (F2C), or the coarser one (C2F).
+ * BCM_ERR_ARG when that face has no such neighbour. */
+int bcm_ghost_fill_face(bcm_field_t field, int block, int face, int subface);
+
+/* Vector3D<T> (include/Vector3D.h:36-110): three components per cell,
+ * interleaved (component l of cell (i,j,k) at 3 Index3DS(i,j,k) + l, the
+ * reference's Index3DV), nblocks*3*(n+2vc)^3 values, block-major.  The
+ * reference's Vector3DUpdater is a stub (include/Vector3DUpdater.h:35-45);
+ * here the exchange applies the scalar rules (copy, F2C, C2F) to each
+ * component. */
+typedef struct bcm_vfield* bcm_vfield_t;
+int bcm_vfield_create(bcm_mesh_t mesh, int dtype, bcm_vfield_t* out);
+int bcm_vfield_destroy(bcm_vfield_t field);
+int bcm_vfield_upload(bcm_vfield_t field, const void* host, size_t bytes);
+int bcm_vfield_download(bcm_vfield_t field, void* host, size_t bytes);
+int bcm_vfield_upload_blocks(bcm_vfield_t field, int first_block, int nblocks, const void* host, size_t bytes);
+int bcm_vfield_download_blocks(bcm_vfield_t field, int first_block, int nblocks, void* host, size_t bytes);
+/* kind and axis bits as bcm_ghost_fill (BCM_GF_BLOCKWISE not accepted). */
+int bcm_vfield_ghost_fill(bcm_vfield_t field, int mask);
+int bcm_vfield_ghost_fill_face(bcm_vfield_t field, int block, int face, int subface);
 
 /* VCUpdateMethod (include/BlockManager.h:39-46). */
 #define BCM_VC_AT_ONCE 0

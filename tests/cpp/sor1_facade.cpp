@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <vector>
 
+#include "Block.h"
 #include "BlockManager.h"
 #include "Scalar3D.h"
 #include "Scalar3DUpdater.h"

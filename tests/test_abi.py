@@ -63,7 +63,7 @@ def test_field_api_rejects_bad_arguments_without_gpu():
 # the destroy calls accept NULL like free(); every other int entry point
 # returns BCM_ERR_ARG.
 _COUNTS = {"bcm_tree_num_blocks", "bcm_sor_get_variant", "bcm_server_poll"}
-_DESTROYS = {"bcm_tree_destroy", "bcm_mesh_destroy", "bcm_field_destroy", "bcm_server_destroy"}
+_DESTROYS = {"bcm_tree_destroy", "bcm_mesh_destroy", "bcm_field_destroy", "bcm_server_destroy", "bcm_vfield_destroy"}
 
 
 def test_every_entry_point_rejects_null_handles_without_gpu():
@@ -74,7 +74,8 @@ def test_every_entry_point_rejects_null_handles_without_gpu():
     for name, (res, args) in sorted(L.SIGNATURES.items()):
         if res is not ctypes.c_int:
             continue
-        vals = [0 if a in ints else 0.0 if a in (ctypes.c_double, ctypes.c_float) else None for a in args]
+        vals = [0 if a in ints else 0.0 if a in (ctypes.c_double, ctypes.c_float) else
+                ctypes.cast(None, a) if a is L.DIVIDER_FN else None for a in args]
         got = getattr(lib, name)(*vals)
         if name in _DESTROYS:
             assert got == L.OK, name
