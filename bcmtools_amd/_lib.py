@@ -266,6 +266,38 @@ SIGNATURES = {
 _lib = None
 
 
+# The Fortran-convention per-block entry points (include/bcm_fortran.h):
+# every argument is a pointer (device arrays; host scalars, sizes and
+# results); none returns a value.  Name -> argument count.
+FORTRAN_ARGC = {
+    "fill_": 4, "fill_vf3d_": 5, "copy_": 4, "copy_integer_": 4, "add_": 5, "triad_": 6, "scal_": 4,
+    "axpy_": 5, "xpay_": 5, "axpyz_": 6, "axpbypz_": 7, "dot_": 5, "dotx2_": 7,
+    "jacobi_smoother_": 13, "jacobi_smoother2_": 7, "calc_ax_": 11, "calc_r_": 12, "calc_r2_": 12,
+    "setup_mask_": 4, "copy_mask_": 5, "dot_mask_": 6, "calc_ax_mask_": 12, "jacobi_smoother_mask_": 14,
+    "allreduce_": 2, "allreduce_max_": 2, "allreduce_min_": 2, "dot_all_": 5,
+    "sup_copy_from_neighbor_": 7, "sup_copy_from_neighbor_c2f_": 7, "sup_copy_from_neighbor_f2c_": 7,
+    "sup_copy_to_buffer_": 6, "sup_copy_to_buffer_c2f_": 6, "sup_copy_to_buffer_f2c_": 6,
+    "sup_copy_from_buffer_": 6,
+    "bc_x3_poiseuille_u_": 8, "bc_aw_poiseuille_u_": 10, "bc_x3_poiseuille_p_": 8, "bc_aw_poiseuille_p_": 10,
+}
+for _f in ("1", "2", "3", "4", "5", "6"):
+    for _k in ("d", "d_f", "n"):
+        FORTRAN_ARGC[f"bc_x{_f}_{_k}_"] = 4
+for _f in ("1", "2", "5"):
+    FORTRAN_ARGC[f"bc_x{_f}_p_"] = 4
+for _f in ("w", "e", "s", "n", "b", "t"):
+    for _k in ("d", "n"):
+        FORTRAN_ARGC[f"bc_a{_f}_{_k}_"] = 6
+
+
+def fortran(name):
+    """A Fortran-convention entry point of libbcm_hip.so, typed."""
+    fn = getattr(load(), name)
+    fn.restype = None
+    fn.argtypes = [_P] * FORTRAN_ARGC[name]
+    return fn
+
+
 def load():
     """Load libbcm_hip.so (built in-tree by __graft_entry__.build())."""
     global _lib

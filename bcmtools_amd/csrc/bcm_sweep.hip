@@ -558,7 +558,6 @@ template <typename T>
 int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout) {
   EventPair ev;
   BCM_TRY(timer_begin(m, gout ? BCM_K_SOR : BCM_K_SOR_HALO, &ev));
-  const int nb = m->nblocks;
   if (m->n == 32) {
     launch_halo<T, 32>(m, A, B, s, omega, xin, xout, gout);
   } else if (m->n == 16) {

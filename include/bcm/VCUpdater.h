@@ -115,6 +115,17 @@ class VCUpdater {
     if (!exchanger) fail("VCUpdater: the data class was not registered through BlockManager::setDataClass");
     exchanger->exchangeFace(dataClassID, blockID, face, subface);
   }
+  // Where a subface starts on the face's first and second tangential axis
+  // (1 = from the midpoint), for user updaters that index subfaces
+  // themselves (include/VCUpdater.h:251-273).
+  static int subfaceOrigin0(Subface subface) {
+    if (subface < SF_00 || subface > SF_11) Exit(EX_FAILURE);
+    return int(subface) & 1;
+  }
+  static int subfaceOrigin1(Subface subface) {
+    if (subface < SF_00 || subface > SF_11) Exit(EX_FAILURE);
+    return int(subface) >> 1;
+  }
   // A buffer rule: a second rank's data.
   static void needsSecondRank(const char* what) {
     std::cout << "*** error: " << what << " moves data between MPI ranks; this build runs one rank on one GPU"

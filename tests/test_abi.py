@@ -85,3 +85,41 @@ def test_every_entry_point_rejects_null_handles_without_gpu():
             assert got == L.ERR_ARG, (name, got)
         checked += 1
     assert checked >= 70
+
+
+def fortran_declarations():
+    """name -> argument count of every declaration in include/bcm_fortran.h."""
+    text = open(os.path.join(ROOT, "include", "bcm_fortran.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*void\s+(\w+_)\s*\(([^;]*)\)\s*;", text, flags=re.M):
+        out[m.group(1)] = len([a for a in m.group(2).split(",") if a.strip()])
+    return out
+
+
+def test_fortran_header_bindings_and_exports_agree():
+    decl = fortran_declarations()
+    assert len(decl) == 71
+    assert decl == L.FORTRAN_ARGC
+    lib = ctypes.CDLL(L.LIB_PATH)
+    missing = [n for n in decl if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_fortran_entry_points_exit_on_bad_sizes_without_gpu():
+    """Sizes are checked before anything touches the device; a bad one
+    prints the entry point's name and exits 1, as the reference's Exit()."""
+    import subprocess
+    import sys
+    code = ("import ctypes; from bcmtools_amd import _lib as L; "
+            "sz=(ctypes.c_int*3)(4,0,4); g=ctypes.c_int(1); a=ctypes.c_double(1.0); "
+            "L.fortran('fill_')(None, ctypes.byref(a), sz, ctypes.byref(g))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1
+    assert "bcm: fill_: block size must be positive" in out.stderr
+    code = ("import ctypes; from bcmtools_amd import _lib as L; "
+            "sz=(ctypes.c_int*3)(4,4,4); g=ctypes.c_int(1); a=ctypes.c_double(1.0); "
+            "L.fortran('bc_x1_d_')(None, ctypes.byref(a), sz, ctypes.byref(g))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1
+    assert "bcm: bc_x1_d_: needs at least 2 ghost layers (g = 1)" in out.stderr

@@ -1,0 +1,288 @@
+"""The oracle of SolverTest's per-block Fortran kernels (oracle/solvertest.py),
+pinned without a Fortran compiler:
+
+- the 7-point statements against a sparse matrix assembled independently
+  from the coefficient arrays (neighbour offsets, signs, the order of Ap);
+- Jacobi's fixed point at the solution and r = b - Ax;
+- the masked forms against the unmasked ones under setup_mask, and against
+  a cell-by-cell loop over blas.f90's flat range for a mask with ones in
+  the ghost cells inside that range;
+- sup.f90's boxes, laid out as examples/SolverTest/Scalar3DUpdater1.cpp
+  lays them, against the compiled reference library's ghost fill on an
+  adaptive mesh (the tests/golden ghost fixtures): same-level copies and
+  C2F bit for bit, F2C (its own summation order) to rounding; C2F
+  reproduces trilinear fields, F2C averages linear ones;
+- bca.f90's folded coefficients against bcx.f90's ghost values: a residual
+  evaluated with the folded system equals the residual with the ghost
+  layer bcx writes, on every face, Dirichlet and Neumann;
+- the periodic copies against a wrap-around gather, and the Poiseuille
+  profile against its closed form.
+
+Runs on the CPU (no GPU)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import solvertest as S
+
+from conftest import load_golden
+
+SZ = (6, 5, 4)
+
+
+def rand_field(rng, sz, g):
+    return rng.standard_normal(S.shape(sz, g))
+
+
+def rand_coef(rng, sz, g):
+    A = [rng.uniform(0.1, 1.0, S.shape(sz, g)) for _ in range(7)]
+    A[0] = -(2.0 + rng.uniform(0.5, 1.0, S.shape(sz, g)))  # Ap, away from zero
+    return A
+
+
+def matrix_from_coefficients(A, sz, g):
+    """Rows of the interior cells over all (sz+2g)^3 unknowns, built cell by
+    cell from the meaning of each coefficient."""
+    cx, cy = sz[0] + 2 * g, sz[1] + 2 * g
+    vol = cx * cy * (sz[2] + 2 * g)
+    rows, cols, vals = [], [], []
+    flat = [a.reshape(-1) for a in A]
+    nb = {1: (-1, 0, 0), 2: (1, 0, 0), 3: (0, -1, 0), 4: (0, 1, 0), 5: (0, 0, -1), 6: (0, 0, 1)}
+    for k in range(sz[2]):
+        for j in range(sz[1]):
+            for i in range(sz[0]):
+                o = (i + g) + cx * ((j + g) + cy * (k + g))
+                rows.append(o)
+                cols.append(o)
+                vals.append(flat[0][o])
+                for q, (di, dj, dk) in nb.items():
+                    rows.append(o)
+                    cols.append(o + di + cx * (dj + cy * dk))
+                    vals.append(flat[q][o])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(vol, vol))
+
+
+@pytest.mark.parametrize("g", [1, 2])
+def test_stencils_match_an_independently_assembled_matrix(g):
+    rng = np.random.default_rng(3 + g)
+    A, x, b = rand_coef(rng, SZ, g), rand_field(rng, SZ, g), rand_field(rng, SZ, g)
+    M = matrix_from_coefficients(A, SZ, g)
+    I = S.interior(SZ, g)
+    want = (M @ x.reshape(-1)).reshape(x.shape)[I]
+    ax = np.zeros_like(x)
+    S.calc_ax(ax, A, x, SZ, g)
+    np.testing.assert_allclose(ax[I], want, rtol=1e-13, atol=1e-13)
+    r = np.zeros_like(x)
+    S.calc_r(r, A, x, b, SZ, g)
+    np.testing.assert_allclose(r[I], b[I] - want, rtol=1e-12, atol=1e-12)
+    assert S.calc_r2(A, x, b, SZ, g) == pytest.approx(float(np.sum((b[I] - want) ** 2)), rel=1e-12)
+    assert S.dot(x, b, SZ, g) == pytest.approx(float(np.dot(x[I].ravel(), b[I].ravel())), rel=1e-12)
+
+
+def test_jacobi_fixed_point_and_packed_coefficients():
+    rng = np.random.default_rng(11)
+    g = 1
+    A, xs = rand_coef(rng, SZ, g), rand_field(rng, SZ, g)
+    b = np.zeros_like(xs)
+    S.calc_ax(b, A, xs, SZ, g)  # b = A x*, ghosts of x* held
+    x1 = xs.copy()
+    S.jacobi_smoother(x1, xs, A, b, 0.8, SZ, g)
+    np.testing.assert_allclose(x1, xs, rtol=1e-12, atol=1e-12)
+    # Away from the fixed point the update is x0 + omega (D^-1 r).
+    x0 = rand_field(rng, SZ, g)
+    x1 = np.zeros_like(x0)
+    S.jacobi_smoother(x1, x0, A, b, 0.7, SZ, g)
+    r = np.zeros_like(x0)
+    S.calc_r(r, A, x0, b, SZ, g)
+    I = S.interior(SZ, g)
+    np.testing.assert_allclose(x1[I], x0[I] + 0.7 * r[I] / A[0][I], rtol=1e-11, atol=1e-11)
+    assert S.pack7(A).shape == S.shape(SZ, g) + (7,)
+
+
+def test_masked_forms():
+    rng = np.random.default_rng(5)
+    g = 1
+    A, x0, b = rand_coef(rng, SZ, g), rand_field(rng, SZ, g), rand_field(rng, SZ, g)
+    m, mask = S.setup_mask(SZ, g)
+    I = S.interior(SZ, g)
+    assert m[I].min() == 1.0 and m.sum() == np.prod(SZ) and mask.sum() == np.prod(SZ)
+    # Under setup_mask the masked forms are the unmasked ones.
+    u, v = np.zeros_like(x0), np.zeros_like(x0)
+    S.jacobi_smoother(u, x0, A, b, 0.9, SZ, g)
+    S.jacobi_smoother_mask(v, x0, A, b, mask, 0.9, SZ, g)
+    np.testing.assert_array_equal(u, v)
+    S.calc_ax(u, A, x0, SZ, g)
+    S.calc_ax_mask(v, A, x0, mask, SZ, g)
+    np.testing.assert_array_equal(u, v)
+    assert S.dot_mask(x0, b, mask, SZ, g) == pytest.approx(S.dot(x0, b, SZ, g), rel=1e-14)
+    # A mask with ones in ghost cells inside the flat range: cell-by-cell loop.
+    mk = (rng.uniform(size=x0.shape) < 0.5).astype(np.int32)
+    cx, cy, cz = SZ[0] + 2 * g, SZ[1] + 2 * g, SZ[2] + 2 * g
+    ijk0 = cx * cy * g + cx * g + g + 1
+    ijkx = cx * cy * cz - ijk0 + 1
+    xf, mf = x0.reshape(-1), mk.reshape(-1)
+    F = [a.reshape(-1) for a in A]
+    want_y = np.zeros(x0.size)
+    want_ax = np.zeros(x0.size)
+    want_dot = 0.0
+    for ijk in range(ijk0, ijkx + 1):
+        o = ijk - 1
+        if mf[o] != 1:
+            continue
+        want_y[o] = xf[o]
+        want_dot += xf[o] * b.reshape(-1)[o]
+        want_ax[o] = (F[1][o] * xf[o - 1] + F[2][o] * xf[o + 1] + F[3][o] * xf[o - cx] + F[4][o] * xf[o + cx] +
+                      F[5][o] * xf[o - cx * cy] + F[6][o] * xf[o + cx * cy] + F[0][o] * xf[o])
+    y = np.zeros_like(x0)
+    S.copy_mask(y, x0, mk, SZ, g)
+    np.testing.assert_array_equal(y.reshape(-1), want_y)
+    ax = np.zeros_like(x0)
+    S.calc_ax_mask(ax, A, x0, mk, SZ, g)
+    np.testing.assert_allclose(ax.reshape(-1), want_ax, rtol=1e-14, atol=1e-14)
+    assert S.dot_mask(x0, b, mk, SZ, g) == pytest.approx(want_dot, rel=1e-13)
+    # Some of those ghost cells really are visited.
+    inner = np.zeros(x0.shape, dtype=bool)
+    inner[I] = True
+    visited = np.zeros(x0.size, dtype=bool)
+    visited[ijk0 - 1:ijkx] = True
+    assert np.any(visited & (mf == 1) & ~inner.reshape(-1))
+
+
+def updater1_fill(blocks, n, vc, fin):
+    """Scalar3DUpdater1's face updates through the sup.f90 restatement."""
+    out = fin.copy()
+    sz = (n, n, n)
+    for b in range(len(blocks)):
+        for face in range(6):
+            d = int(blocks["nb_level_diff"][b][face])
+            ids = blocks["nb_id"][b][face]
+            if d == 1:
+                subs = [(s, int(ids[s])) for s in range(4)]
+            else:
+                subs = [(int(blocks["nb_subface"][b][face]) if d == -1 else 0, int(ids[0]))]
+            for sub, nid in subs:
+                if nid < 0:
+                    continue
+                i1d, i1s, szc, op = S.updater1_boxes(face, d, sub, n, vc)
+                vals = S.region_values(fin[nid], i1s, szc, sz, vc, op)
+                S.region_store(out[b], i1d, vals, vc)
+    return out
+
+
+@pytest.mark.parametrize("name", ["ghost_sphere_atonce_d", "ghost_sphere_vc2_d", "ghost_periodic_x_d",
+                                  "ghost_simple_vc2_d"])
+def test_sup_boxes_reproduce_the_reference_ghost_fill(name):
+    gd = load_golden(name)
+    blocks, n, vc = gd["blocks"], gd["n"], gd["vc"]
+    got = updater1_fill(blocks, n, vc, gd["fin"])
+    want = gd["fout"]
+    f2c = np.zeros(got.shape, dtype=bool)
+    for b in range(len(blocks)):
+        for face in range(6):
+            if int(blocks["nb_level_diff"][b][face]) == 1:
+                for s in range(4):
+                    if blocks["nb_id"][b][face][s] >= 0:
+                        i1d, _, szc, _ = S.updater1_boxes(face, 1, s, n, vc)
+                        i, j, k = [np.arange(szc[a]) + i1d[a] - 1 + vc for a in range(3)]
+                        f2c[b][np.ix_(k, j, i)] = True
+    np.testing.assert_array_equal(got[~f2c], want[~f2c])
+    if f2c.any():
+        np.testing.assert_allclose(got[f2c], want[f2c], rtol=4e-16, atol=4e-16)
+
+
+def test_sup_c2f_reproduces_trilinear_and_f2c_averages_linear():
+    n, vc = 6, 2
+    sz = (n, n, n)
+    idx = np.arange(1 - vc, n + vc + 1, dtype=np.float64)
+    K, J, I = np.meshgrid(idx, idx, idx, indexing="ij")
+    lin = lambda x, y, z: 0.5 + 1.25 * x - 0.75 * y + 2.0 * z + 0.125 * x * y * z
+    coarse = lin(I - 0.5, J - 0.5, K - 0.5)  # coarse cell centres in coarse units
+    fine_pos = [1, 1 + vc]  # fine positions of two boxes, ends included
+    for start in fine_pos:
+        i1s = [start, 1, 2 * n - 3]
+        szc = [vc, 2 * n, 4]
+        got = S.region_values(coarse, i1s, szc, sz, vc, "c2f")
+        fi, fj, fk = [np.arange(szc[a]) + i1s[a] for a in range(3)]
+        Fk, Fj, Fi = np.meshgrid(fk, fj, fi, indexing="ij")
+        want = lin((Fi - 0.5) / 2, (Fj - 0.5) / 2, (Fk - 0.5) / 2)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
+    fine = 3.0 - 0.5 * I + 0.25 * J + 1.5 * K  # linear in fine units
+    got = S.region_values(fine, [1, 2, 1], [n // 2, 2, n // 2], sz, vc, "f2c")
+    ci, cj, ck = [np.arange(s) + a for s, a in zip([n // 2, 2, n // 2], [1, 2, 1])]
+    Ck, Cj, Ci = np.meshgrid(ck, cj, ci, indexing="ij")
+    want = 3.0 - 0.5 * (2 * Ci - 0.5) + 0.25 * (2 * Cj - 0.5) + 1.5 * (2 * Ck - 0.5)
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("face", range(6))
+@pytest.mark.parametrize("kind", ["d", "n"])
+def test_folded_coefficients_equal_the_ghost_values(face, kind):
+    rng = np.random.default_rng(20 + face)
+    g = 2
+    A, x, b = rand_coef(rng, SZ, g), rand_field(rng, SZ, g), rand_field(rng, SZ, g)
+    xc = 0.37
+    # bcx writes the ghost layer the stencil reads ...
+    xg = x.copy()
+    S.bcx(xg, face, kind, xc, SZ, g)
+    r_ghost = np.zeros_like(x)
+    S.calc_r(r_ghost, A, xg, b, SZ, g)
+    # ... bca folds the same condition into the boundary cells' row.
+    Af = [a.copy() for a in A]
+    bf = b.copy()
+    nb = face + 1  # Aw, Ae, As, An, Ab, At follow the face order
+    S.bca(Af[0], Af[nb], bf, face, kind, xc, SZ, g)
+    r_fold = np.zeros_like(x)
+    S.calc_r(r_fold, Af, x, bf, SZ, g)
+    I = S.interior(SZ, g)
+    np.testing.assert_allclose(r_fold[I], r_ghost[I], rtol=1e-12, atol=1e-12)
+    assert not np.array_equal(r_fold[I], np.zeros_like(r_fold[I]))
+
+
+def test_dirichlet_value_face_and_mirror():
+    rng = np.random.default_rng(2)
+    g = 2
+    x = rand_field(rng, SZ, g)
+    for face in range(6):
+        y = x.copy()
+        S.bcx(y, face, "d_f", 1.5, SZ, g)
+        for layer in (1, 2):
+            assert np.all(y[S._face_index(face, SZ, g, layer)] == 1.5)
+        boundary = y[S._face_index(face, SZ, g, 0)]
+        assert np.all(boundary == 1.5) == bool(face % 2)
+        z = x.copy()
+        S.bcx(z, face, "d", 1.5, SZ, g)
+        mean = 0.5 * (z[S._face_index(face, SZ, g, 0)] + z[S._face_index(face, SZ, g, 1)])
+        np.testing.assert_allclose(mean, 1.5, rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_periodic_copies_are_a_wrap_around(axis):
+    rng = np.random.default_rng(30 + axis)
+    g = 2
+    x = rand_field(rng, SZ, g)
+    y = x.copy()
+    S.bcx_periodic(y, axis, SZ, g)
+    n = SZ[axis]
+    src = [n - 2, n - 1, None, 0, 1]  # Fortran -1, 0, n+1, n+2 copy Fortran n-1, n, 1, 2 (0-based below)
+    ax = 2 - axis
+    for pos, s in zip([-1, 0, None, n + 1, n + 2], src):
+        if pos is None:
+            continue
+        got = np.take(y, pos - 1 + g, axis=ax)
+        want = np.take(x, s + g, axis=ax)
+        np.testing.assert_array_equal(got, want)
+
+
+def test_poiseuille_profile_closed_form():
+    sz = (4, 8, 8)
+    center = (0.0, 0.5, 0.5)
+    org = (0.0, 0.0, 0.0)
+    csize = (0.125, 0.125, 0.125)
+    u = S.pois_u(1.0, sz, center, org, csize)
+    k, j = np.meshgrid(np.arange(1, 9), np.arange(1, 9), indexing="ij")
+    r = np.hypot((j - 0.5) * 0.125 - 0.5, (k - 0.5) * 0.125 - 0.5)
+    want = np.where(r > 0.5, 0.0, 2.0 * (1.0 - r * r / 0.25))
+    np.testing.assert_allclose(u, want, rtol=1e-14, atol=1e-14)
+    assert u.max() > 1.9 and (u == 0).any()
+    gp = S.pois_gp(2.0, sz, center, org, csize)
+    np.testing.assert_array_equal(gp, np.where(r > 0.5, 0.0, 0.25))
