@@ -196,6 +196,8 @@ SIGNATURES = {
     "bcm_ghost_fill": (_I, [_P, _I]),
     "bcm_dirichlet_mirror": (_I, [_P, _DP, _I]),
     "bcm_impose_bc": (_I, [_P, _DP, _I]),
+    "bcm_impose_face_bc": (_I, [_P, _IP, _DP]),
+    "bcm_fold_face_bc": (_I, [ctypes.POINTER(_P), _P, _IP, _DP]),
     "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),

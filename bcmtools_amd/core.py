@@ -274,6 +274,21 @@ class Mesh:
         v = np.ascontiguousarray(values, dtype=np.float64)
         L.check(self.lib.bcm_impose_bc(f.handle, L.dptr(v), len(v)), "bcm_impose_bc")
 
+    def impose_face_bc(self, f, types, values):
+        """LocalScalar3D's per-field conditions (BlockScalar3D.h:199-206): one
+        BCM_BC_* type and value per domain face, on every block face without
+        a neighbour."""
+        t = (ctypes.c_int * 6)(*[int(v) for v in types])
+        v = (ctypes.c_double * 6)(*[float(x) for x in values])
+        L.check(self.lib.bcm_impose_face_bc(f.handle, t, v), "bcm_impose_face_bc")
+
+    def fold_face_bc(self, A, b, types, values):
+        """The same conditions folded into A[7] and b (BlockScalar3D.h:207-214,
+        bca.f90)."""
+        t = (ctypes.c_int * 6)(*[int(v) for v in types])
+        v = (ctypes.c_double * 6)(*[float(x) for x in values])
+        L.check(self.lib.bcm_fold_face_bc(_handles(A), b.handle, t, v), "bcm_fold_face_bc")
+
     # -- red-black SOR ------------------------------------------------------
     def rbsor_sweep(self, f, source=None, omega=1.0):
         L.check(self.lib.bcm_rbsor_sweep(f.handle, source.handle if source is not None else None, float(omega)),

@@ -69,6 +69,10 @@ struct bcm_mesh {
   int n_bc_tasks = 0;
   int n_dirichlet = 0;
   int n_neumann = 0;
+  // Blocks with a face on the domain boundary {block, mask of those faces},
+  // built on first use by the per-field face conditions.
+  int2* d_outer = nullptr;
+  int n_outer = -1;
   bcm::BlockMeta* d_meta = nullptr;
 
   // Reduction scratch: one double per reduction work-group, plus results.

@@ -392,6 +392,7 @@ static void mesh_release(bcm_mesh* m) {
   for (auto e : m->event_pool) (void)hipEventDestroy(e);
   dev_free(m->d_tasks);
   dev_free(m->d_bc_tasks);
+  dev_free(m->d_outer);
   dev_free(m->d_meta);
   dev_free(m->d_partials);
   dev_free(m->d_result);
@@ -900,6 +901,208 @@ extern "C" int bcm_update_virtual_cells(bcm_field_t f, int method) {
 extern "C" int bcm_impose_boundary_condition(bcm_field_t f, int method, const double* values, int nvalues) {
   BCM_TRY(bcm_update_virtual_cells(f, method));
   return bcm_impose_bc(f, values, nvalues);
+}
+
+// ---------------------------------------------------------------------
+// Per-field outer-face conditions: SolverTest's LocalScalar3D keeps one
+// boundary type and value per domain face for each field
+// (examples/SolverTest/LocalScalar3D.h:36-100, BlockScalar3D.h:76-198) and
+// applies them to every block face without a neighbour, faces in X_M..Z_P
+// order within a block (BlockScalar3D.h:199-214).  One work-group per block
+// that has such a face, a barrier between its faces.
+// ---------------------------------------------------------------------
+struct FaceBC {
+  int type[6];
+  double value[6];
+};
+
+static int outer_blocks(bcm_mesh* m) {
+  if (m->n_outer >= 0) return BCM_OK;
+  std::vector<int2> list;
+  for (int b = 0; b < m->nblocks; ++b) {
+    int mask = 0;
+    for (int f = 0; f < 6; ++f)
+      if (m->blocks[b].nb_id[f][0] < 0) mask |= 1 << f;
+    if (mask) list.push_back(make_int2(b, mask));
+  }
+  if (!list.empty()) {
+    BCM_TRY(dev_alloc(reinterpret_cast<void**>(&m->d_outer), sizeof(int2) * list.size()));
+    BCM_HIP_TRY(hipMemcpy(m->d_outer, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice));
+  }
+  m->n_outer = int(list.size());
+  return BCM_OK;
+}
+
+// Ghost values (bcx.f90:18-376, 561-652 through BlockScalar3D.h:216-279):
+// Dirichlet g1 = 2v - u, g2 = v; Neumann g1 = u -/+ xc, g2 = u -/+ 2 xc on a
+// minus/plus face, xc = v cellSize.x; periodic (a plus face only) copies
+// the block's own opposite layers over the whole face, ghosts included.
+// The second layer is written when the field has one.  FP64 arithmetic,
+// rounded once to the field type.
+template <typename T>
+__global__ __launch_bounds__(256) void k_face_bc(T* __restrict__ d, const int2* __restrict__ list, FaceBC bc,
+                                                 const BlockMeta* __restrict__ meta, Geo g) {
+  const int2 it = list[blockIdx.x];
+  T* __restrict__ f = d + (long long)it.x * g.bvol;
+  const int n = g.n;
+  const bool two = g.vc >= 2;
+  for (int face = 0; face < 6; ++face) {
+    const int type = bc.type[face];
+    const int axis = face >> 1;
+    const bool plus = face & 1;
+    if (!(it.y >> face & 1) || !(type == BCM_BC_DIRICHLET || type == BCM_BC_NEUMANN ||
+                                 (type == BCM_BC_PERIODIC && plus)))
+      continue;
+    if (type == BCM_BC_PERIODIC) {
+      const int w = n + 2 * g.vc;
+      for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+        int c[3];
+        const int p = e % w - g.vc, q = e / w - g.vc;
+        if (axis == 0) { c[1] = p; c[2] = q; }
+        else if (axis == 1) { c[0] = p; c[2] = q; }
+        else { c[0] = p; c[1] = q; }
+        auto at = [&](int a) { c[axis] = a; return g.idx(c[0], c[1], c[2]); };
+        f[at(n)] = f[at(0)];
+        if (two) f[at(n + 1)] = f[at(1)];
+        f[at(-1)] = f[at(n - 1)];
+        if (two) f[at(-2)] = f[at(n - 2)];
+      }
+    } else {
+      const bool neu = type == BCM_BC_NEUMANN;
+      const double v = bc.value[face];
+      const double xc = neu ? v * meta[it.x].hx : v;
+      for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        int c[3];
+        const int p = e % n, q = e / n;
+        if (axis == 0) { c[1] = p; c[2] = q; }
+        else if (axis == 1) { c[0] = p; c[2] = q; }
+        else { c[0] = p; c[1] = q; }
+        c[axis] = plus ? n - 1 : 0;
+        const double u = double(f[g.idx(c[0], c[1], c[2])]);
+        double g1, g2;
+        if (neu) {
+          g1 = plus ? u + xc : u - xc;
+          g2 = plus ? u + xc * 2.0 : u - xc * 2.0;
+        } else {
+          g1 = 2.0 * xc - u;
+          g2 = xc;
+        }
+        c[axis] = plus ? n : -1;
+        f[g.idx(c[0], c[1], c[2])] = T(g1);
+        if (two) {
+          c[axis] = plus ? n + 1 : -2;
+          f[g.idx(c[0], c[1], c[2])] = T(g2);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// The same conditions folded into the coefficients of the boundary cells
+// (bca.f90:12-394 through BlockScalar3D.h:207-214, 283-345): Dirichlet
+// Ap -= Anb, b -= Anb v 2; Neumann Ap += Anb, b += Anb xc on a minus face
+// and b -= Anb xc on a plus face; Anb = 0.  Periodic faces are unchanged.
+struct FoldPtrs {
+  void* Ap;
+  void* Anb[6];  // Aw, Ae, As, An, Ab, At
+  void* b;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fold_face_bc(FoldPtrs P, const int2* __restrict__ list, FaceBC bc,
+                                                      const BlockMeta* __restrict__ meta, Geo g) {
+  const int2 it = list[blockIdx.x];
+  const long long base = (long long)it.x * g.bvol;
+  T* __restrict__ Ap = static_cast<T*>(P.Ap) + base;
+  T* __restrict__ b = static_cast<T*>(P.b) + base;
+  const int n = g.n;
+  for (int face = 0; face < 6; ++face) {
+    const int type = bc.type[face];
+    if (!(it.y >> face & 1) || !(type == BCM_BC_DIRICHLET || type == BCM_BC_NEUMANN)) continue;
+    T* __restrict__ An = static_cast<T*>(P.Anb[face]) + base;
+    const int axis = face >> 1;
+    const bool plus = face & 1;
+    const bool neu = type == BCM_BC_NEUMANN;
+    const double v = bc.value[face];
+    const double xc = neu ? v * meta[it.x].hx : v;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      int c[3];
+      const int p = e % n, q = e / n;
+      if (axis == 0) { c[1] = p; c[2] = q; }
+      else if (axis == 1) { c[0] = p; c[2] = q; }
+      else { c[0] = p; c[1] = q; }
+      c[axis] = plus ? n - 1 : 0;
+      const int o = g.idx(c[0], c[1], c[2]);
+      const double a = double(An[o]);
+      if (neu) {
+        Ap[o] = T(double(Ap[o]) + a);
+        b[o] = T(plus ? double(b[o]) - a * xc : double(b[o]) + a * xc);
+      } else {
+        Ap[o] = T(double(Ap[o]) - a);
+        b[o] = T(double(b[o]) - a * xc * 2.0);
+      }
+      An[o] = T(0);
+    }
+    __syncthreads();
+  }
+}
+
+static int face_bc_args(const int32_t* types, const double* values, FaceBC* bc) {
+  if (!types || !values) return BCM_ERR_ARG;
+  for (int f = 0; f < 6; ++f) {
+    if (types[f] < BCM_BC_INNER || types[f] > BCM_BC_PERIODIC) return BCM_ERR_ARG;
+    bc->type[f] = types[f];
+    bc->value[f] = values[f];
+  }
+  return BCM_OK;
+}
+
+extern "C" int bcm_impose_face_bc(bcm_field_t f, const int32_t types[6], const double values[6]) {
+  if (!f || !f->mesh) return BCM_ERR_ARG;
+  FaceBC bc;
+  BCM_TRY(face_bc_args(types, values, &bc));
+  bcm_mesh* m = f->mesh;
+  BCM_TRY(outer_blocks(m));
+  if (m->n_outer == 0) return BCM_OK;
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_DIRICHLET, &ev));
+  if (f->dtype == BCM_F64)
+    hipLaunchKernelGGL(k_face_bc<double>, dim3(m->n_outer), dim3(256), 0, m->stream, static_cast<double*>(f->data),
+                       (const int2*)m->d_outer, bc, (const BlockMeta*)m->d_meta, g);
+  else
+    hipLaunchKernelGGL(k_face_bc<float>, dim3(m->n_outer), dim3(256), 0, m->stream, static_cast<float*>(f->data),
+                       (const int2*)m->d_outer, bc, (const BlockMeta*)m->d_meta, g);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
+}
+
+extern "C" int bcm_fold_face_bc(const bcm_field_t A[7], bcm_field_t b, const int32_t types[6],
+                                const double values[6]) {
+  if (!A || !b || !b->mesh) return BCM_ERR_ARG;
+  for (int i = 0; i < 7; ++i)
+    if (!A[i] || A[i]->mesh != b->mesh || A[i]->dtype != b->dtype) return BCM_ERR_ARG;
+  FaceBC bc;
+  BCM_TRY(face_bc_args(types, values, &bc));
+  bcm_mesh* m = b->mesh;
+  BCM_TRY(outer_blocks(m));
+  if (m->n_outer == 0) return BCM_OK;
+  FoldPtrs P;
+  P.Ap = A[0]->data;
+  for (int i = 0; i < 6; ++i) P.Anb[i] = A[1 + i]->data;
+  P.b = b->data;
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_ASSEMBLE, &ev));
+  if (b->dtype == BCM_F64)
+    hipLaunchKernelGGL(k_fold_face_bc<double>, dim3(m->n_outer), dim3(256), 0, m->stream, P,
+                       (const int2*)m->d_outer, bc, (const BlockMeta*)m->d_meta, g);
+  else
+    hipLaunchKernelGGL(k_fold_face_bc<float>, dim3(m->n_outer), dim3(256), 0, m->stream, P,
+                       (const int2*)m->d_outer, bc, (const BlockMeta*)m->d_meta, g);
+  BCM_HIP_TRY(hipGetLastError());
+  return timer_end(m, &ev);
 }
 
 extern "C" int bcm_dirichlet_mirror(bcm_field_t f, const double* values, int nvalues) {

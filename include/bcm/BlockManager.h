@@ -418,6 +418,12 @@ class BlockManager : public FaceExchanger {
   }
   void syncToHost(int id) { download(rec(id)); }
   void syncToDevice(int id) { upload(rec(id)); }
+  // The local index of a registered block (-1 if it is not one).
+  int indexOf(const BlockBase* block) const {
+    for (int b = 0; b < numBlock; ++b)
+      if (blockList[b] == block) return b;
+    return -1;
+  }
   const std::vector<bcm_block_info>& blockTable() const { return table; }
 
  private:

@@ -291,6 +291,29 @@ int bcm_dirichlet_mirror(bcm_field_t field, const double* values, int nvalues);
  * FP64 arithmetic, rounded once for FP32 fields. */
 int bcm_impose_bc(bcm_field_t field, const double* values, int nvalues);
 
+/* Per-field outer-face conditions of SolverTest's LocalScalar3D, whose
+ * fields each carry one boundary type and value per domain face
+ * (examples/SolverTest/LocalScalar3D.h:36-100, BlockScalar3D.h:76-214).
+ * types[f] is BCM_BC_DIRICHLET, BCM_BC_NEUMANN, BCM_BC_PERIODIC or
+ * BCM_BC_INNER (nothing) for domain face f = X_M..Z_P; values[f] its
+ * value, a Neumann value scaled by each block's cellSize.x
+ * (BlockScalar3D.h:98-100).  They apply to every block face without a
+ * neighbour, in X_M..Z_P order within a block.
+ * bcm_impose_face_bc (BlockScalar3D::ImposeBoundaryCondition(), bcx.f90):
+ * Dirichlet g1 = 2v - u, g2 = v; Neumann g1 = u -/+ xc, g2 = u -/+ 2 xc on a
+ * minus/plus face; periodic (plus faces: bc_x1_p, bc_x2_p, bc_x5_p,
+ * bcx.f90:561-652) copies the block's own opposite layers across the whole
+ * face, ghost rows included.  g2 and the second periodic layer are written
+ * when vc >= 2.
+ * bcm_fold_face_bc (BlockScalar3D::ImposeBoundaryCondition(Ap..b),
+ * bca.f90:12-394): Dirichlet Ap -= Anb, b -= Anb v 2; Neumann Ap += Anb,
+ * b += Anb xc on a minus face, b -= Anb xc on a plus face; Anb = 0;
+ * periodic faces unchanged.  A[7] = {Ap, Aw, Ae, As, An, Ab, At}.
+ * The Poiseuille inflow (bcax.f90) is available per block through
+ * include/bcm_fortran.h. */
+int bcm_impose_face_bc(bcm_field_t field, const int32_t types[6], const double values[6]);
+int bcm_fold_face_bc(const bcm_field_t A[7], bcm_field_t b, const int32_t types[6], const double values[6]);
+
 /* ------------------------------------------------------------------ */
 /* Red-black SOR (examples/sor1/Solver.cpp:132-153,                     */
 /* examples/sor2/Solver.cpp:136-160).  Both colours, ghosts frozen.      */

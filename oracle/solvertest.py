@@ -246,33 +246,40 @@ def _face_index(face, sz, g, layer):
     return (idx[2], idx[1], idx[0])
 
 
-def bcx(x, face, kind, xc, sz, g):
-    """bc_x?_d / _n / _d_f (bcx.f90:18-559)."""
+def bcx(x, face, kind, xc, sz, g, layers=2):
+    """bc_x?_d / _n / _d_f (bcx.f90:18-559).  layers=1 writes the first
+    ghost layer only (a field with one ghost layer)."""
     plus = face % 2
     c = x[_face_index(face, sz, g, 0)].copy()
-    g1, g2 = _face_index(face, sz, g, 1), _face_index(face, sz, g, 2)
+    g1 = _face_index(face, sz, g, 1)
+    g2 = _face_index(face, sz, g, 2) if layers >= 2 else None
     if kind == "d":
         x[g1] = 2.0 * xc - c
-        x[g2] = xc
+        if g2 is not None:
+            x[g2] = xc
     elif kind == "n":
         if plus:
             x[g1] = c + xc
-            x[g2] = c + xc * 2.0
+            if g2 is not None:
+                x[g2] = c + xc * 2.0
         else:
             x[g1] = c - xc
-            x[g2] = c - xc * 2.0
+            if g2 is not None:
+                x[g2] = c - xc * 2.0
     elif kind == "d_f":
         if plus:
             x[_face_index(face, sz, g, 0)] = xc
         x[g1] = xc
-        x[g2] = xc
+        if g2 is not None:
+            x[g2] = xc
     else:
         raise ValueError(kind)
 
 
-def bcx_periodic(x, axis, sz, g):
+def bcx_periodic(x, axis, sz, g, layers=2):
     """bc_x1_p / bc_x2_p / bc_x5_p (bcx.f90:561-652): the statements in
-    order, over the full tangential range."""
+    order, over the full tangential range (layers=1: the first ghost layer
+    on each side only)."""
     n = sz[axis]
     ax = 2 - axis  # numpy axis of Fortran axis `axis`
 
@@ -282,9 +289,55 @@ def bcx_periodic(x, axis, sz, g):
         return tuple(idx)
 
     x[sl(n + 1)] = x[sl(1)]
-    x[sl(n + 2)] = x[sl(2)]
+    if layers >= 2:
+        x[sl(n + 2)] = x[sl(2)]
     x[sl(0)] = x[sl(n)]
-    x[sl(-1)] = x[sl(n - 1)]
+    if layers >= 2:
+        x[sl(-1)] = x[sl(n - 1)]
+
+
+# LocalScalar3D's conditions over a whole mesh: BCM_BC_* codes.
+BC_INNER, BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC = 0, 1, 2, 3
+
+
+def mesh_face_bc(blocks, n, vc, data, types, values):
+    """BlockScalar3D::ImposeBoundaryCondition() on every block
+    (BlockScalar3D.h:199-206): for each face without a neighbour, in X_M..Z_P
+    order, Dirichlet / Neumann ghosts (a Neumann value times the block's
+    cellSize.x, BlockScalar3D.h:98-100) or, on a plus face, the periodic
+    self copy; data is (nblocks, n+2vc, n+2vc, n+2vc)."""
+    sz = (n, n, n)
+    layers = min(vc, 2)
+    for b in range(len(blocks)):
+        hx = blocks["block_size"][b][0] / n
+        for face in range(6):
+            if blocks["nb_id"][b][face][0] >= 0:
+                continue
+            t, v = types[face], values[face]
+            if t == BC_DIRICHLET:
+                bcx(data[b], face, "d", v, sz, vc, layers)
+            elif t == BC_NEUMANN:
+                bcx(data[b], face, "n", v * hx, sz, vc, layers)
+            elif t == BC_PERIODIC and face % 2:
+                bcx_periodic(data[b], face // 2, sz, vc, layers)
+
+
+def mesh_fold_face_bc(blocks, n, vc, A, b, types, values):
+    """BlockScalar3D::ImposeBoundaryCondition(Ap..b) on every block
+    (BlockScalar3D.h:207-214): bca.f90's folds on each face without a
+    neighbour, in X_M..Z_P order.  A is the list (Ap, Aw, Ae, As, An, Ab, At)
+    of mesh arrays."""
+    sz = (n, n, n)
+    for k in range(len(blocks)):
+        hx = blocks["block_size"][k][0] / n
+        for face in range(6):
+            if blocks["nb_id"][k][face][0] >= 0:
+                continue
+            t, v = types[face], values[face]
+            if t == BC_DIRICHLET:
+                bca(A[0][k], A[1 + face][k], b[k], face, "d", v, sz, vc)
+            elif t == BC_NEUMANN:
+                bca(A[0][k], A[1 + face][k], b[k], face, "n", v * hx, sz, vc)
 
 
 # ------------------------------------------------------------------ bca.f90

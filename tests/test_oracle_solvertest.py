@@ -286,3 +286,27 @@ def test_poiseuille_profile_closed_form():
     assert u.max() > 1.9 and (u == 0).any()
     gp = S.pois_gp(2.0, sz, center, org, csize)
     np.testing.assert_array_equal(gp, np.where(r > 0.5, 0.0, 0.25))
+
+
+def test_mesh_face_conditions_match_the_mesh_oracle():
+    """The per-field composition equals the C oracle's BlockScalar3D
+    conditions when the block table carries the same per-face types
+    (BCM_BCSET_PER_FACE, value id = face), for one and two ghost layers."""
+    import oracle
+    from bcmtools_amd import Tree
+    from bcmtools_amd import _lib as L
+    types = [S.BC_DIRICHLET, S.BC_NEUMANN, S.BC_NEUMANN, S.BC_DIRICHLET, S.BC_DIRICHLET, S.BC_NEUMANN]
+    values = [0.5, -1.25, 0.75, 2.0, -0.25, 0.125]
+    t = Tree(divider=L.DIV_SIMPLE, min_level=1, max_level=2, bc_setter=L.BCSET_PER_FACE, face_bc_type=types)
+    blocks = t.blocks
+    n = 4
+    rng = np.random.default_rng(12)
+    for vc in (1, 2):
+        w = n + 2 * vc
+        x = rng.standard_normal((len(blocks), w, w, w))
+        want = x.copy()
+        oracle.impose_bc(blocks, n, vc, want, values)
+        got = x.copy()
+        S.mesh_face_bc(blocks, n, vc, got, types, values)
+        np.testing.assert_array_equal(got, want)
+        assert not np.array_equal(got, x)
