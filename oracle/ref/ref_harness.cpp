@@ -1,14 +1,20 @@
 // Fixture writer linked against the reference BCMTools sources (built by
-// oracle/ref/Makefile into oracle/_ref/; never shipped, never run on the
-// GPU box).  Builds meshes with the reference's own octree, block factory
-// and BlockManager, runs the reference ghost fill (BlockManager::updateVC*)
-// and the reference sor1 solver (examples/sor1/Solver.cpp), and dumps
-// block tables and fields in the bcm_block_info layout of include/bcm_hip.h.
+// oracle/ref/Makefile into oracle/_ref/, which git ignores; the binary
+// travels with the working tree but only tests/golden/make_golden.py runs
+// it, where the reference tree is).  Builds meshes with the reference's
+// own octree, block factory and BlockManager, runs the reference ghost
+// fill (BlockManager::updateVC*), the reference sor1 solver
+// (examples/sor1/Solver.cpp) and SolverTest's VTK writer
+// (examples/SolverTest/VtkWriter.h), and dumps block tables and fields in
+// the bcm_block_info layout of include/bcm_hip.h.
 //
 // usage:
 //   ref_harness mesh  <out> <mesh args>
 //   ref_harness ghost <out> <method 0|1|2> <dtype d|f> <mesh args>
 //   ref_harness sor1  <out> <config file> <root_n> <ordering>
+//   ref_harness vtk   <out> <step> <difflevel> <mesh args>
+//       (VtkWriter::writeScalar of a double field into ./VTK; <out> gets
+//        the block table and the field)
 // mesh args: divider(flat|simple|sphere) minLevel maxLevel ordering(Z|Hilbert)
 //            rnx rny rnz bc(dir|x|y|z) n vc [cx cy cz r]
 #include "mpi.h"
@@ -35,6 +41,7 @@
 #include "Config.h"
 #include "Solver.h"
 #include "BoundaryConditionSetter.h"  // examples/sor1 setter (type x/y/z)
+#include "VtkWriter.h"                // examples/SolverTest
 
 #include "bcm_hip.h"
 
@@ -110,6 +117,8 @@ int parse_mesh(char** a, int na, MeshArgs& m) {
 }
 
 BCMOctree* g_tree = 0;
+RootGrid* g_root = 0;
+Partition* g_partition = 0;
 
 // Build the mesh as examples/sor1/main.cpp:76-123 does, one rank.
 void build_mesh(const MeshArgs& a) {
@@ -130,6 +139,8 @@ void build_mesh(const MeshArgs& a) {
   BCMOctree::Ordering ord = a.ordering == "Hilbert" ? BCMOctree::HILBERT : BCMOctree::Z;
   g_tree = new BCMOctree(rootGrid, divider, ord);
   Partition* partition = new Partition(1, g_tree->getNumLeafNode());
+  g_root = rootGrid;
+  g_partition = partition;
   Vec3i size(a.n, a.n, a.n);
   BoundaryConditionSetterBase* setter;
   if (axis >= 0) setter = new AxisSetter(axis);
@@ -258,6 +269,30 @@ int run_sor1(const char* out, const char* conf_file, int rn, const char* orderin
   return 0;
 }
 
+// SolverTest's VTK output (examples/SolverTest/VtkWriter.h:733-889) of a
+// double field holding cell_value, root origin 0 and root length 1.
+int run_vtk(const char* out, int step, int difflevel, const MeshArgs& a) {
+  build_mesh(a);
+  BlockManager& bm = BlockManager::getInstance();
+  int id = bm.setDataClass<Scalar3D<double>, Scalar3DUpdater<double> >(a.vc);
+  const int nx0 = a.n + 2 * a.vc;
+  const size_t bvol = size_t(nx0) * nx0 * nx0;
+  std::vector<double> in(bvol * bm.getNumBlock());
+  for (int b = 0; b < bm.getNumBlock(); ++b) {
+    double* d = dynamic_cast<Scalar3D<double>*>(bm.getBlock(b)->getDataClass(id))->getData();
+    for (size_t c = 0; c < bvol; ++c) d[c] = cell_value(b, c);
+    std::memcpy(&in[b * bvol], d, bvol * sizeof(double));
+  }
+  VtkWriter writer;
+  writer.writeScalar<double>(id, a.vc, "p", step, difflevel, g_root, g_tree, g_partition, Vec3d(0.0, 0.0, 0.0), 1.0);
+  FILE* fp = fopen(out, "wb");
+  if (!fp) return 2;
+  write_header(fp, block_table(), a.n, a.vc);
+  fwrite(in.data(), sizeof(double), in.size(), fp);
+  fclose(fp);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -278,6 +313,8 @@ int main(int argc, char** argv) {
       rc = argv[4][0] == 'f' ? run_ghost<float>(argv[2], atoi(argv[3]), a) : run_ghost<double>(argv[2], atoi(argv[3]), a);
     } else if (cmd == "sor1" && argc >= 6) {
       rc = run_sor1(argv[2], argv[3], atoi(argv[4]), argv[5]);
+    } else if (cmd == "vtk" && argc >= 5 && parse_mesh(argv + 5, argc - 5, a) == 0) {
+      rc = run_vtk(argv[2], atoi(argv[3]), atoi(argv[4]), a);
     }
   }
   if (rc == 1) fprintf(stderr, "usage: see oracle/ref/ref_harness.cpp\n");

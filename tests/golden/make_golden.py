@@ -47,6 +47,12 @@ GHOSTS = {
     "ghost_sphere_vc2_leveldiff_f": (2, "f", "sphere 1 3 Hilbert 1 1 1 dir 4 2 0.5 0.5 0.5 0.3"),
 }
 
+VTK = {
+    # name: (step, difflevel, mesh args); the .vthb index and every 16th
+    # block's .vti (plus the last) are kept.
+    "vtk_sphere": (7, 2, "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
+}
+
 SOR1 = {
     # name: (config text, roots, ordering)
     "sor1_log4": (None, 1, "Z"),
@@ -103,6 +109,27 @@ def main(only=()):
             np.savez_compressed(os.path.join(HERE, name + ".npz"), blocks=blocks.view(np.uint8), n=n, vc=vc,
                                 method=method, fin=fin, fout=fout, args=margs)
             print(name, len(blocks), fin.shape)
+        for name, (step, difflevel, margs) in VTK.items():
+            if not wanted(name):
+                continue
+            d = os.path.join(tmp, name)
+            os.makedirs(d)
+            out = os.path.join(d, name + ".bin")
+            run(["vtk", out, str(step), str(difflevel)] + margs.split(), d)
+            blocks, n, vc, (fin,) = read_dump(out, 1, np.float64)
+            vtkdir = os.path.join(d, "VTK")
+            with open(os.path.join(vtkdir, f"data-p-{step:010d}.vthb"), "rb") as fp:
+                vthb = np.frombuffer(fp.read(), dtype=np.uint8)
+            keep = sorted(set(range(0, len(blocks), 16)) | {len(blocks) - 1})
+            vti = {}
+            for b in keep:
+                with open(os.path.join(vtkdir, f"{step:010d}", f"data-p-00000-{b:05d}-{step:010d}.vti"), "rb") as fp:
+                    vti[f"vti_{b}"] = np.frombuffer(fp.read(), dtype=np.uint8)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), blocks=blocks.view(np.uint8), n=n, vc=vc,
+                                fin_kept=fin[keep], step=step, difflevel=difflevel, vthb=vthb, kept=np.array(keep),
+                                args=margs,
+                                **vti)
+            print(name, len(blocks), "vti kept", len(keep))
         for name, (conf, rn, ordering) in SOR1.items():
             if not wanted(name):
                 continue

@@ -1,4 +1,6 @@
-"""Pin the VTK restatement (oracle/vtk.py) by what the files must hold:
+"""Pin the VTK restatement (oracle/vtk.py) against the reference writer's
+own bytes (tests/golden/vtk_sphere.npz, written by the compiled
+examples/SolverTest/VtkWriter.h), and by what the files must hold:
 the .vti XML parses, its extents and spacing describe the block, and the
 appended raw data (after "_" and an int32 byte count) decodes to the
 block's interior in float32; the .vthb index parses and each amr_box is the
@@ -48,3 +50,22 @@ def test_vthb_boxes_in_closed_form():
         for a in range(3):
             assert box[2 * a] == ijk[a] * n and box[2 * a + 1] == ijk[a] * n + n - 1
         assert ds.get("file") == f"./0000000003/data-p-00000-{i:05d}-0000000003.vti"
+
+
+def test_restatement_reproduces_the_reference_writer():
+    """tests/golden/vtk_sphere: examples/SolverTest/VtkWriter.h's own
+    writeScalar, compiled from the reference (oracle/ref), on a 344-block
+    adaptive mesh with two root cubes; the restatement must give the same
+    bytes for the .vthb index and for every kept block's .vti."""
+    import os
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, "vtk_sphere.npz"), allow_pickle=False)
+    blocks = z["blocks"].view(L.BLOCK_DTYPE).copy()
+    n, vc, step, difflevel = int(z["n"]), int(z["vc"]), int(z["step"]), int(z["difflevel"])
+    text = V.vthb_text(blocks, n, "p", step, difflevel, (0.0, 0.0, 0.0), 1.0)
+    assert text.encode() == z["vthb"].tobytes()
+    for b, fin in zip(z["kept"], z["fin_kept"]):
+        bi = blocks[int(b)]
+        interior = fin[vc:vc + n, vc:vc + n, vc:vc + n]
+        raw = V.vti_bytes(interior, n, bi["origin"], bi["block_size"][0] / n, "p")
+        assert raw == z[f"vti_{int(b)}"].tobytes(), int(b)
