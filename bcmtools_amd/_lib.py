@@ -244,6 +244,7 @@ SIGNATURES = {
                                _I, _IP, _DP]),
     "bcm_poisson_assemble": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _I]),
     "bcm_jacobi7_poisson": (_I, [_P, _P, _P, _D]),
+    "bcm_jacobi_poisson_iterate": (_I, [_P, _P, _D, _I]),
     "bcm_calc_ax_poisson": (_I, [_P, _P]),
     "bcm_calc_r_poisson": (_I, [_P, _P, _P]),
     "bcm_jacobi_solve": (_I, [_P, ctypes.POINTER(_P), _P, _P, ctypes.POINTER(KrylovParams), _DP, _I, _IP, _DP]),

@@ -333,6 +333,11 @@ class Mesh:
     def jacobi7_poisson(self, x1, x0, b, omega):
         L.check(self.lib.bcm_jacobi7_poisson(x1.handle, x0.handle, b.handle, float(omega)), "bcm_jacobi7_poisson")
 
+    def jacobi_poisson_iterate(self, x, b, omega, nsweeps):
+        """nsweeps x (jacobi7_poisson, then the ghost exchange) in one call."""
+        L.check(self.lib.bcm_jacobi_poisson_iterate(x.handle, b.handle, float(omega), int(nsweeps)),
+                "bcm_jacobi_poisson_iterate")
+
     def calc_ax_poisson(self, ax, x):
         L.check(self.lib.bcm_calc_ax_poisson(ax.handle, x.handle), "bcm_calc_ax_poisson")
 

@@ -224,7 +224,11 @@ size_t krylov_bytes(int nblocks, int n, int precision, int count_pre);
 // Field-sized workspace buffer kept on the mesh (bcm_solver.hip).
 void* workspace(bcm_mesh* m, int slot, size_t bytes, int* rc);
 // Compact iterates A, B and source of the halo-gathering SOR schedule.
-enum WorkspaceSlot { WS_SOR_CA_F64 = 30, WS_SOR_CB_F64, WS_SOR_CS_F64, WS_SOR_CA_F32, WS_SOR_CB_F32, WS_SOR_CS_F32 };
+enum WorkspaceSlot {
+  WS_SOR_CA_F64 = 30, WS_SOR_CB_F64, WS_SOR_CS_F64, WS_SOR_CA_F32, WS_SOR_CB_F32, WS_SOR_CS_F32,
+  WS_RES_PUB,   // bcm_jacobi_poisson_iterate: published interiors, two per block
+  WS_RES_PING   // bcm_jacobi_poisson_iterate: the second ghosted buffer of the two-launch form
+};
 
 enum BlasOp { OP_AXPY = 0, OP_XPAY = 1, OP_AXPYZ = 2, OP_AXPBYPZ = 3, OP_COPY_INTERIOR = 4 };
 enum ConvOp { CONV_D2F = 0, CONV_ADD_F2D = 1 };

@@ -221,30 +221,30 @@ def bench_jacobi(args):
     rhs = mesh.field(np.float64, -f)
     gb = mesh.field(np.float64)
     mesh.poisson_assemble(None, gb, rhs, [0.0])
-    gx, g0 = mesh.field(np.float64), mesh.field(np.float64)
-    for _ in range(args.warmup):
-        mesh.jacobi7_poisson(g0, gx, gb, 1.0)
-        gx, g0 = g0, gx
-        mesh.update_vc(gx)
+    gx = mesh.field(np.float64)
+    # Every step (one jacobi_smoother sweep, then the ghost exchange) of the
+    # run in one call: the 64 blocks stay in LDS for the whole run.
+    mesh.jacobi_poisson_iterate(gx, gb, 1.0, max(args.warmup, 1))
     mesh.sync()
+    mesh.reset_timing()
     mesh.set_timing(True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mesh.jacobi7_poisson(g0, gx, gb, 1.0)
-        gx, g0 = g0, gx
-        mesh.update_vc(gx)
+    mesh.jacobi_poisson_iterate(gx, gb, 1.0, args.steps)
     mesh.sync()
     el = time.perf_counter() - t0
     jms, jn = mesh.kernel_time(L.K_JACOBI)
-    javg = jms / max(jn, 1) * 1e-3
+    javg = jms / max(args.steps, 1) * 1e-3  # per step, from the call's HIP events
     achieved = 24.0 * ncell / javg / 1e9
     res = {"metric": "stencil GCUPS (FP64 Jacobi7 + ghost fill, config 1)", "value": ncell * args.steps / el / 1e9,
            "unit": "GCUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "uniform level-2 octree, 64 x 16^3, matrix-free Jacobi7 + ghost fill"},
+           "config": {"workload": "uniform level-2 octree, 64 x 16^3, matrix-free Jacobi7 + ghost fill, "
+                                  "all steps in one bcm_jacobi_poisson_iterate call (blocks resident in LDS)"},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "note": "algorithmic 24 B per cell update; the blocks live in LDS and the exchange in L2, "
+                                "so the step is bound by the grid barrier, not by HBM"}}
     if not args.no_cpu:
         import oracle
 

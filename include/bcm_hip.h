@@ -429,6 +429,17 @@ int bcm_poisson_assemble(const bcm_field_t A[7], bcm_field_t b, bcm_field_t rhs,
  * to the stored-coefficient kernels on bcm_poisson_assemble's output while
  * reading 3 arrays instead of 10 (b must be the folded right-hand side). */
 int bcm_jacobi7_poisson(bcm_field_t x1, bcm_field_t x0, bcm_field_t b, double omega);
+/* nsweeps steps of ILS3D::Jacobi_Smoother on the Poisson operator
+ * (ILS3D.h:65-126 without the boundary step: the operator carries the
+ * folded conditions): x <- jacobi_smoother(x) with b, then the ghost
+ * exchange (updateVC), in one call.  The result equals that many
+ * bcm_jacobi7_poisson + bcm_ghost_fill(BCM_GF_ALL) pairs bit for bit on
+ * every cell they write; cells the exchange never writes (edges, corners,
+ * faces without a neighbour) keep x's values.  A mesh of vc = 1, n = 8 or
+ * 16, whose blocks can all be resident on the GPU at once runs it as one
+ * launch with every block in LDS; any other mesh as two launches per step.
+ * x and b FP64. */
+int bcm_jacobi_poisson_iterate(bcm_field_t x, bcm_field_t b, double omega, int nsweeps);
 int bcm_calc_ax_poisson(bcm_field_t ax, bcm_field_t x);
 int bcm_calc_r_poisson(bcm_field_t r, bcm_field_t x, bcm_field_t b);
 
