@@ -134,6 +134,7 @@ class ServerParams(ctypes.Structure):
         ("count_pre", ctypes.c_int32),
         ("omega", ctypes.c_double),
         ("check_every", ctypes.c_int32),
+        ("batch_blocks", ctypes.c_int32),
     ]
 
 

@@ -511,10 +511,13 @@ class Server:
 
     PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "fp32": L.PREC_FP32}
 
-    def __init__(self, hbm_budget, max_active=8, n=16, max_iter=20000, count_pre=2, omega=0.8, check_every=8):
+    def __init__(self, hbm_budget, max_active=8, n=16, max_iter=20000, count_pre=2, omega=0.8, check_every=8,
+                 batch_blocks=0):
+        """batch_blocks > 0 merges requests of at most that many blocks into
+        batched groups of at most that many blocks (0: one request per group)."""
         self.lib = L.load()
         p = L.ServerParams(float(hbm_budget), int(max_active), int(n), int(max_iter), int(count_pre), float(omega),
-                           int(check_every))
+                           int(check_every), int(batch_blocks))
         h = ctypes.c_void_p()
         L.check(self.lib.bcm_server_create(ctypes.byref(p), ctypes.byref(h)), "bcm_server_create")
         self.handle = h

@@ -1,16 +1,23 @@
 // Request server: Poisson solves of independent meshes arriving over time,
 // run with continuous batching on one GPU.
 //
-// Every admitted request owns a device mesh (its own HIP stream), its
-// fields and its fused-solver state; the server loop keeps every admitted
-// request's stream fed with hipGraph-replayed blocks of iterations and
-// polls completion events without blocking, so the work of all admitted
-// requests overlaps on the device.  Requests wait in FIFO order and are
-// admitted (first fit) while the device bytes they need fit in the HBM
-// budget and a slot is free; a finished request's memory is released at
-// once, which admits the next ones.  Each request is computed exactly as it
-// would be alone (its kernels read only its own mesh), so its result does
-// not depend on what else is running.
+// Admitted requests run in groups.  A group owns a device mesh (its own
+// HIP stream), its fields and its fused-solver state; the server loop keeps
+// every group's stream fed with hipGraph-replayed blocks of iterations and
+// polls completion events without blocking, so the work of all groups
+// overlaps on the device.  With batch_blocks = 0 every request is a group
+// of its own.  With batch_blocks > 0, requests of at most that many blocks
+// are merged at admission, in arrival order and with the same precision,
+// into one group of at most batch_blocks blocks: one batched mesh whose
+// segments are the requests (bcm_mesh_create_batch), so one launch of each
+// solver kernel serves all of them while their scalars and tolerances stay
+// per segment.  Requests wait in FIFO order and are admitted (first fit)
+// while the device bytes they need fit in the HBM budget and a group slot
+// is free; a finished group's memory is released at once, which admits the
+// next ones.  Each request is computed exactly as it would be alone (a
+// segment's kernels read only its own blocks and its reductions are its
+// own), so its result does not depend on what else is running or on its
+// group.
 //
 // The reference has no counterpart (one BlockManager per process,
 // include/BlockManager.h:153-156); a request's solve is bcm_solve's fused
@@ -19,6 +26,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <new>
 #include <thread>
 #include <vector>
@@ -78,7 +86,14 @@ struct Request {
   double* x_out = nullptr;
   double t_submit = 0.0, t_admit = 0.0;
   size_t bytes = 0;
-  bool counted = false;  // bytes charged to the budget
+};
+
+// Requests sharing one device mesh (one segment each, in order).
+struct Group {
+  std::vector<Request*> reqs;
+  std::vector<int> first;  // first block of each request's segment
+  size_t bytes = 0;        // charged to the budget
+  int nblocks = 0;
   bcm_mesh_t mesh = nullptr;
   bcm_field_t x = nullptr, rhs = nullptr;
 };
@@ -88,8 +103,8 @@ struct Request {
 
 struct bcm_server {
   bcm_server_params p{};
-  std::deque<bcm::Request*> queue;
-  std::vector<bcm::Request*> active;
+  std::list<bcm::Request*> queue;
+  std::vector<bcm::Group*> active;
   std::deque<bcm_server_result> done;
   size_t in_use = 0;
   bcm::Clock::time_point t0;
@@ -106,17 +121,16 @@ size_t request_bytes(const bcm_server* sv, int nblocks, int precision) {
   return size_t(nblocks) * 640 + 2 * ng * 8 + krylov_bytes(nblocks, n, precision, sv->p.count_pre);
 }
 
-void release(bcm_server* sv, Request* r) {
-  if (r->x) bcm_field_destroy(r->x);
-  if (r->rhs) bcm_field_destroy(r->rhs);
-  if (r->mesh) bcm_mesh_destroy(r->mesh);
-  r->x = r->rhs = nullptr;
-  r->mesh = nullptr;
-  if (r->counted) sv->in_use -= r->bytes;
-  delete r;
+void release(bcm_server* sv, Group* g) {
+  if (g->x) bcm_field_destroy(g->x);
+  if (g->rhs) bcm_field_destroy(g->rhs);
+  if (g->mesh) bcm_mesh_destroy(g->mesh);
+  sv->in_use -= g->bytes;
+  for (Request* r : g->reqs) delete r;
+  delete g;
 }
 
-void finish_result(bcm_server* sv, Request* r, int status, const bcm_solve_stats* st) {
+void finish_result(bcm_server* sv, Request* r, int status, const bcm_solve_stats* st, double t_done = -1.0) {
   bcm_server_result o;
   std::memset(&o, 0, sizeof(o));
   o.tag = r->tag;
@@ -125,7 +139,7 @@ void finish_result(bcm_server* sv, Request* r, int status, const bcm_solve_stats
   o.precision = r->precision;
   o.t_submit = r->t_submit;
   o.t_admit = r->t_admit;
-  o.t_done = sv->now();
+  o.t_done = t_done >= 0.0 ? t_done : sv->now();
   if (st) {
     o.iters = st->iters;
     o.relres = st->relres;
@@ -133,27 +147,44 @@ void finish_result(bcm_server* sv, Request* r, int status, const bcm_solve_stats
   sv->done.push_back(o);
 }
 
-int admit(bcm_server* sv, Request* r) {
+int admit(bcm_server* sv, Group* g) {
   const int n = sv->p.n;
-  BCM_TRY(bcm_mesh_create(r->blocks.data(), int(r->blocks.size()), n, 1, &r->mesh));
-  BCM_TRY(bcm_field_create(r->mesh, BCM_F64, &r->x));
-  BCM_TRY(bcm_field_create(r->mesh, BCM_F64, &r->rhs));
-  bcm_mesh* m = r->mesh;
-  if (r->rhs_host) {
-    BCM_TRY(bcm_field_upload(r->rhs, r->rhs_host, r->rhs->bytes));
-  } else {
-    std::vector<BlockGeo> geo(r->blocks.size());
-    for (size_t b = 0; b < geo.size(); ++b) {
-      const bcm_block_info& bi = r->blocks[b];
-      geo[b] = BlockGeo{bi.origin[0], bi.origin[1], bi.origin[2], bi.block_size[0] / n, bi.block_size[1] / n,
-                        bi.block_size[2] / n, (unsigned long long)bi.pedigree};
+  const int nseg = int(g->reqs.size());
+  std::vector<bcm_block_info> blocks;
+  std::vector<int> counts;
+  blocks.reserve(size_t(g->nblocks));
+  for (Request* r : g->reqs) {
+    g->first.push_back(int(blocks.size()));
+    counts.push_back(int(r->blocks.size()));
+    blocks.insert(blocks.end(), r->blocks.begin(), r->blocks.end());
+  }
+  if (nseg == 1) BCM_TRY(bcm_mesh_create(blocks.data(), int(blocks.size()), n, 1, &g->mesh));
+  else BCM_TRY(bcm_mesh_create_batch(blocks.data(), counts.data(), nseg, n, 1, &g->mesh));
+  BCM_TRY(bcm_field_create(g->mesh, BCM_F64, &g->x));
+  BCM_TRY(bcm_field_create(g->mesh, BCM_F64, &g->rhs));
+  bcm_mesh* m = g->mesh;
+  const size_t bvol = size_t(n + 2) * (n + 2) * (n + 2);
+  std::vector<BlockGeo> geo;
+  for (int s = 0; s < nseg; ++s) {
+    Request* r = g->reqs[size_t(s)];
+    const int nb = int(r->blocks.size());
+    if (r->rhs_host) {
+      BCM_TRY(bcm_field_upload_blocks(g->rhs, g->first[size_t(s)], nb, r->rhs_host, sizeof(double) * bvol * nb));
+      continue;
+    }
+    geo.resize(size_t(nb));
+    for (int b = 0; b < nb; ++b) {
+      const bcm_block_info& bi = r->blocks[size_t(b)];
+      geo[size_t(b)] = BlockGeo{bi.origin[0], bi.origin[1], bi.origin[2], bi.block_size[0] / n,
+                                bi.block_size[1] / n, bi.block_size[2] / n, (unsigned long long)bi.pedigree};
     }
     void* d = nullptr;
     BCM_TRY(dev_alloc(&d, sizeof(BlockGeo) * geo.size()));
     hipError_t e = hipMemcpyAsync(d, geo.data(), sizeof(BlockGeo) * geo.size(), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_rhs_synth, dim3(unsigned(geo.size())), dim3(256), 0, m->stream,
-                         static_cast<double*>(r->rhs->data), static_cast<const BlockGeo*>(d), n, r->seed, 0.01);
+      hipLaunchKernelGGL(k_rhs_synth, dim3(unsigned(nb)), dim3(256), 0, m->stream,
+                         static_cast<double*>(g->rhs->data) + bvol * size_t(g->first[size_t(s)]),
+                         static_cast<const BlockGeo*>(d), n, r->seed, 0.01);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
@@ -164,29 +195,43 @@ int admit(bcm_server* sv, Request* r) {
   BCM_TRY(set_bvalues(m, &zero, 1));
   bcm_solve_params P;
   std::memset(&P, 0, sizeof(P));
-  P.precision = r->precision;
-  P.tol = r->tol;
+  P.precision = g->reqs[0]->precision;
+  P.tol = g->reqs[0]->tol;
   P.max_iter = sv->p.max_iter;
   P.count_pre = sv->p.count_pre;
   P.omega = sv->p.omega;
   P.check_every = sv->p.check_every;
   P.use_graph = 1;
-  BCM_TRY(krylov_check(m, &P, nullptr));
-  return krylov_begin(m, static_cast<double*>(r->x->data), static_cast<const double*>(r->rhs->data), P, nullptr);
+  std::vector<double> tol;
+  for (Request* r : g->reqs) tol.push_back(r->tol);
+  BCM_TRY(krylov_check(m, &P, tol.data()));
+  return krylov_begin(m, static_cast<double*>(g->x->data), static_cast<const double*>(g->rhs->data), P,
+                      tol.data());
 }
 
-// Finish a request whose solve is over: outcome, solution to the caller.
-int complete(bcm_server* sv, Request* r) {
-  bcm_solve_stats st;
-  BCM_TRY(krylov_finish(r->mesh, &st));
-  if (r->x_out) BCM_TRY(bcm_field_download(r->x, r->x_out, r->x->bytes));
-  finish_result(sv, r, st.status, &st);
+// Finish a group whose solve is over: each request's outcome and solution.
+int complete(bcm_server* sv, Group* g) {
+  const int nseg = int(g->reqs.size());
+  std::vector<bcm_solve_stats> st(static_cast<size_t>(nseg));
+  BCM_TRY(krylov_finish(g->mesh, st.data()));
+  const size_t bvol = size_t(sv->p.n + 2) * (sv->p.n + 2) * (sv->p.n + 2);
+  const double t_done = sv->now();  // the group's solve is over for every member
+  for (int s = 0; s < nseg; ++s) {
+    Request* r = g->reqs[size_t(s)];
+    const int nb = int(r->blocks.size());
+    if (r->x_out)
+      BCM_TRY(bcm_field_download_blocks(g->x, g->first[size_t(s)], nb, r->x_out, sizeof(double) * bvol * nb));
+    finish_result(sv, r, st[size_t(s)].status, &st[size_t(s)], t_done);
+  }
   return BCM_OK;
 }
 
-// Admit queued requests (first fit in arrival order) while slots and the
-// HBM budget allow.  A request larger than the whole budget is refused.
+// Admit queued requests (first fit in arrival order) while group slots and
+// the HBM budget allow; with batch_blocks > 0 a small request takes the
+// later queued requests that fit with it (same precision) into its group.
+// A request larger than the whole budget is refused.
 void admit_ready(bcm_server* sv) {
+  const int cap = sv->p.batch_blocks;
   for (auto it = sv->queue.begin(); it != sv->queue.end() && int(sv->active.size()) < sv->p.max_active;) {
     Request* r = *it;
     if (double(r->bytes) > sv->p.hbm_budget) {
@@ -200,16 +245,36 @@ void admit_ready(bcm_server* sv) {
       continue;
     }
     it = sv->queue.erase(it);
-    r->t_admit = sv->now();
-    sv->in_use += r->bytes;
-    r->counted = true;
-    const int rc = admit(sv, r);
+    Group* g = new Group();
+    const double now = sv->now();
+    auto take = [&](Request* q) {
+      q->t_admit = now;
+      g->reqs.push_back(q);
+      g->nblocks += int(q->blocks.size());
+      g->bytes += q->bytes;
+      sv->in_use += q->bytes;
+    };
+    take(r);
+    if (cap > 0 && g->nblocks <= cap) {
+      for (auto jt = it; jt != sv->queue.end();) {
+        Request* q = *jt;
+        if (q->precision == r->precision && g->nblocks + int(q->blocks.size()) <= cap &&
+            double(sv->in_use + q->bytes) <= sv->p.hbm_budget) {
+          if (jt == it) ++it;
+          jt = sv->queue.erase(jt);
+          take(q);
+        } else {
+          ++jt;
+        }
+      }
+    }
+    const int rc = admit(sv, g);
     if (rc != BCM_OK) {
-      finish_result(sv, r, -rc, nullptr);
-      release(sv, r);
+      for (Request* q : g->reqs) finish_result(sv, q, -rc, nullptr);
+      release(sv, g);
       continue;
     }
-    sv->active.push_back(r);
+    sv->active.push_back(g);
   }
 }
 
@@ -218,11 +283,17 @@ void admit_ready(bcm_server* sv) {
 
 using namespace bcm;
 
+static int active_requests(const bcm_server* sv) {
+  int k = 0;
+  for (const Group* g : sv->active) k += int(g->reqs.size());
+  return k;
+}
+
 extern "C" int bcm_server_create(const bcm_server_params* p, bcm_server_t* out) {
   if (!p || !out) return BCM_ERR_ARG;
   *out = nullptr;
   if (!(p->hbm_budget > 0.0) || p->max_active < 1 || (p->n != 8 && p->n != 16 && p->n != 32) || p->max_iter < 1 ||
-      p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0)
+      p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0 || p->batch_blocks < 0)
     return BCM_ERR_ARG;
   bcm_server* sv = new (std::nothrow) bcm_server();
   if (!sv) return BCM_ERR_NOMEM;
@@ -234,7 +305,7 @@ extern "C" int bcm_server_create(const bcm_server_params* p, bcm_server_t* out) 
 
 extern "C" int bcm_server_destroy(bcm_server_t sv) {
   if (!sv) return BCM_OK;
-  for (Request* r : sv->active) release(sv, r);
+  for (Group* g : sv->active) release(sv, g);
   for (Request* r : sv->queue) delete r;
   delete sv;
   return BCM_OK;
@@ -253,8 +324,9 @@ extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, 
   // A tag names one pending request: the caller's buffers are found by it.
   for (const Request* q : sv->queue)
     if (q->tag == tag) return BCM_ERR_ARG;
-  for (const Request* q : sv->active)
-    if (q->tag == tag) return BCM_ERR_ARG;
+  for (const Group* g : sv->active)
+    for (const Request* q : g->reqs)
+      if (q->tag == tag) return BCM_ERR_ARG;
   Request* r = new (std::nothrow) Request();
   if (!r) return BCM_ERR_NOMEM;
   r->tag = tag;
@@ -277,24 +349,25 @@ extern "C" int bcm_server_progress(bcm_server_t sv, double max_seconds, int* n_a
   do {
     bool moved = false;
     for (size_t i = 0; i < sv->active.size();) {
-      Request* r = sv->active[i];
+      Group* g = sv->active[i];
       bool act = true;
-      int rc = krylov_advance(r->mesh, false, &act);
+      int rc = krylov_advance(g->mesh, false, &act);
       if (rc == BCM_OK && act) {
         ++i;
         continue;
       }
       moved = true;
-      if (rc == BCM_OK) rc = complete(sv, r);
-      if (rc != BCM_OK) finish_result(sv, r, -rc, nullptr);
+      if (rc == BCM_OK) rc = complete(sv, g);
+      if (rc != BCM_OK)
+        for (Request* q : g->reqs) finish_result(sv, q, -rc, nullptr);
       sv->active.erase(sv->active.begin() + long(i));
-      release(sv, r);
+      release(sv, g);
     }
     if (moved) admit_ready(sv);
     if (sv->active.empty() && sv->queue.empty()) break;
     if (!moved) std::this_thread::sleep_for(std::chrono::microseconds(20));
   } while (sv->now() < t_end);
-  if (n_active) *n_active = int(sv->active.size());
+  if (n_active) *n_active = active_requests(sv);
   if (n_queued) *n_queued = int(sv->queue.size());
   return BCM_OK;
 }
@@ -312,7 +385,7 @@ extern "C" int bcm_server_poll(bcm_server_t sv, bcm_server_result* out, int capa
 extern "C" int bcm_server_stats(bcm_server_t sv, double* bytes_in_use, int* n_active, int* n_queued) {
   if (!sv) return BCM_ERR_ARG;
   if (bytes_in_use) *bytes_in_use = double(sv->in_use);
-  if (n_active) *n_active = int(sv->active.size());
+  if (n_active) *n_active = active_requests(sv);
   if (n_queued) *n_queued = int(sv->queue.size());
   return BCM_OK;
 }

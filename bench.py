@@ -582,7 +582,8 @@ def serve_stream(args, lam, trees, tols, gaps):
 
     R, n = len(trees), 16
     arrivals = np.cumsum(gaps / lam)
-    sv = Server(200e9, max_active=args.slots, n=n, max_iter=20000, check_every=8)
+    sv = Server(200e9, max_active=args.slots, n=n, max_iter=20000, check_every=8,
+                batch_blocks=getattr(args, "batch_blocks", 0))
     done, k = {}, 0
     t0 = sv.clock()
     last_print = t0
@@ -637,7 +638,7 @@ def bench_serving(args):
     R = args.requests
     lams = [float(v) for v in args.lams.split(",")] if args.lams else [args.lam]
     rng = np.random.default_rng(args.seed)
-    targets = np.exp(rng.uniform(np.log(64), np.log(50000), size=R)).astype(int)
+    targets = np.exp(rng.uniform(np.log(64), np.log(args.max_blocks), size=R)).astype(int)
     tols = rng.choice([1e-6, 1e-10], size=R)
     gaps = rng.exponential(1.0, size=R)
     tb = time.perf_counter()
@@ -657,12 +658,16 @@ def bench_serving(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.policy == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"{R} requests, Poisson arrivals at {lams[0]:g}/s, meshes of 1-3 random sphere "
-                               "surfaces with block counts log-uniform in [64, 50k] (16^3 blocks), tol 1e-6 or "
-                               "1e-10, "
+                               f"surfaces with block counts log-uniform in [64, {args.max_blocks}] (16^3 blocks), "
+                               "tol 1e-6 or 1e-10, "
                                + {"spec": "FP32 path for 1e-6 and FP32 + FP64 refinement for 1e-10",
                                   "mixed": "mixed FP32/FP64 BiCGSTAB (FP64 reliable updates) for every request",
                                   "fp64": "fused FP64 BiCGSTAB for every request"}[args.policy]
-                               + f", up to {args.slots} requests on the device, HBM budget 200 GB",
+                               + f", up to {args.slots} groups on the device, HBM budget 200 GB"
+                               + (f", requests of at most {args.batch_blocks} blocks merged into batched groups of "
+                                  f"at most {args.batch_blocks} blocks" if getattr(args, "batch_blocks", 0) else
+                                  ", one request per group"),
+                   "batch_blocks": int(getattr(args, "batch_blocks", 0)),
                    "precision_policy": args.policy,
                    "requests": R, "lambda_per_s": lams[0], "blocks_total": int(sum(counts)),
                    "blocks_per_request": [int(min(counts)), int(np.median(counts)), int(max(counts))],
@@ -863,6 +868,11 @@ def main():
     ap.add_argument("--per-request-every", type=int, default=4,
                     help="serving: also solve every request alone on the fused FP64 path and every k-th on the "
                          "plain host-driven path (0: off)")
+    ap.add_argument("--max-blocks", type=int, default=50000,
+                    help="serving: block counts are log-uniform in [64, max-blocks]")
+    ap.add_argument("--batch-blocks", type=int, default=0,
+                    help="serving: merge requests of at most this many blocks into batched groups of at most "
+                         "this many blocks (0: every request on its own mesh)")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="batched: precision of the full batch (auto: the faster on the subset); "

@@ -590,13 +590,19 @@ int bcm_mesh_set_progress(bcm_mesh_t mesh, double seconds);
  * include/BlockManager.h:153-156).  Requests (a block table, a right-hand
  * side, a tolerance and a precision) queue in arrival order and are
  * admitted first-fit while their device bytes fit in hbm_budget and fewer
- * than max_active run; each admitted request gets its own device mesh,
- * HIP stream and fused-solver state (bcm_solve's BiCGSTAB, hipGraph
- * replayed), and bcm_server_progress keeps every admitted stream fed
- * without blocking, so admitted requests overlap on the device.  Memory is
- * released as soon as a request finishes.  A request's result does not
- * depend on the other requests.  Single-threaded: one host thread calls
- * every bcm_server_* function of a server. */
+ * than max_active groups run.  A group owns a device mesh, a HIP stream and
+ * a fused-solver state (bcm_solve's BiCGSTAB, hipGraph replayed), and
+ * bcm_server_progress keeps every group's stream fed without blocking, so
+ * groups overlap on the device.  batch_blocks = 0: every request is a group
+ * of its own.  batch_blocks > 0: a request of at most batch_blocks blocks
+ * takes, at admission, the later queued requests of the same precision
+ * that fit with it (arrival order, group total <= batch_blocks blocks, HBM
+ * budget) into one batched mesh (bcm_mesh_create_batch), one segment per
+ * request; the group's results are reported when its last segment
+ * finishes.  Memory is released as soon as a group finishes.  A request's
+ * result does not depend on the other requests or on its group.
+ * Single-threaded: one host thread calls every bcm_server_* function of a
+ * server. */
 typedef struct bcm_server* bcm_server_t;
 typedef struct bcm_server_params {
   double hbm_budget;          /* device bytes all admitted requests may hold */
@@ -606,6 +612,7 @@ typedef struct bcm_server_params {
   int32_t count_pre;          /* Jacobi preconditioner sweeps */
   double omega;               /* Jacobi weight */
   int32_t check_every;        /* iterations per graph replay (0: 8) */
+  int32_t batch_blocks;       /* merge small requests into groups of at most this many blocks (0: off) */
 } bcm_server_params;
 typedef struct bcm_server_result {
   int64_t tag;

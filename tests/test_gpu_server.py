@@ -38,8 +38,8 @@ def _requests():
     return reqs
 
 
-def _run(reqs, max_active, budget=64e9):
-    sv = Server(budget, max_active=max_active, n=16, check_every=4)
+def _run(reqs, max_active, budget=64e9, batch_blocks=0):
+    sv = Server(budget, max_active=max_active, n=16, check_every=4, batch_blocks=batch_blocks)
     for tag, blocks, tol, prec, rhs in reqs:
         sv.submit(blocks, tol, prec, rhs=rhs, tag=tag, want_x=True)
     out = {}
@@ -64,6 +64,56 @@ def test_server_results_same_alone_and_in_the_mix():
         assert m["iters"] == alone["iters"] and m["relres"] == alone["relres"]
         np.testing.assert_array_equal(m["x"], alone["x"])
         assert oracle_relres(blocks, 16, m["x"], rhs) <= tol * (1 + 1e-6)
+
+
+def test_batched_groups_give_each_request_its_result_alone():
+    """batch_blocks merges small same-precision requests into one batched
+    mesh; each request's solution, iterations and residual equal those of
+    the request solved alone; a request above batch_blocks runs alone; the
+    synthetic right-hand side (seeded on the device) is the same in a group."""
+    reqs = _requests()
+    # The three mixed-precision requests fit one group; the FP32 request
+    # above the cap runs alone.
+    cap = sum(len(r[1]) for r in reqs if r[3] == "mixed")
+    assert cap < max(len(r[1]) for r in reqs if r[3] == "fp32")
+    case(f"server batched groups cap={cap}, blocks {[len(r[1]) for r in reqs]}")
+    grouped = _run(reqs, max_active=2, batch_blocks=cap)
+    # Members of one group are admitted and finished together.
+    tags = sorted(grouped)
+    assert any(grouped[a]["t_done"] == grouped[b]["t_done"] and grouped[a]["t_admit"] == grouped[b]["t_admit"]
+               for i, a in enumerate(tags) for b in tags[i + 1:])
+    for req in reqs:
+        tag = req[0]
+        alone = _run([req], max_active=1)[tag]
+        g = grouped[tag]
+        assert g["status"] == alone["status"] == 1, (g, alone)
+        assert g["iters"] == alone["iters"] and g["relres"] == alone["relres"]
+        np.testing.assert_array_equal(g["x"], alone["x"])
+    # Seeded synthetic right-hand sides, three requests of one precision.
+    seeded = []
+    for i in range(3):
+        blocks = W.random_tree(20 + i, max_level=3).blocks
+        seeded.append((100 + i, blocks))
+    sv = Server(64e9, max_active=1, n=16, check_every=4, batch_blocks=100000)
+    for tag, blocks in seeded:
+        sv.submit(blocks, 1e-9, "fp64", seed=tag, tag=tag, want_x=True)
+    out = {}
+    while len(out) < len(seeded):
+        sv.progress(0.05)
+        for r in sv.poll():
+            out[r["tag"]] = r
+    sv.close()
+    for tag, blocks in seeded:
+        one = Server(64e9, max_active=1, n=16, check_every=4)
+        one.submit(blocks, 1e-9, "fp64", seed=tag, tag=tag, want_x=True)
+        res = None
+        while res is None:
+            one.progress(0.05)
+            got = one.poll()
+            res = got[0] if got else None
+        one.close()
+        assert out[tag]["iters"] == res["iters"] and out[tag]["relres"] == res["relres"]
+        np.testing.assert_array_equal(out[tag]["x"], res["x"])
 
 
 def test_server_budget_admission_and_refusal():
