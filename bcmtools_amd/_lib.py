@@ -34,7 +34,7 @@ SOR_AUTO, SOR_FUSED_TASKS, SOR_FUSED_GATHER, SOR_HALO, SOR_COLOUR_TASKS, SOR_NUM
 PREC_FP64, PREC_MIXED, PREC_FP32 = range(3)
 SOLVE_ACTIVE, SOLVE_CONVERGED, SOLVE_NOCONV, SOLVE_FAILED = range(4)
 (K_GHOST, K_DIRICHLET, K_SOR, K_JACOBI, K_SPMV, K_RESID, K_DOT, K_BLAS1, K_REDUCE,
- K_ASSEMBLE, K_SOR_HALO) = range(11)
+ K_ASSEMBLE, K_SOR_HALO, K_JACOBI_HALO) = range(12)
 MAX_SHELLS = 8
 
 

@@ -111,6 +111,9 @@ struct bcm_mesh {
   // X-face column exports (columns 0, 1, n-2, n-1 of every block), two
   // buffers for the ping-pong of iterates; bytes sized for FP64.
   void* d_xexp[2] = {nullptr, nullptr};
+  // The Jacobi form of the halo schedule: every face has a neighbour on
+  // every subface (copy, F2C with all four, C2F) or none at all.
+  bool jhalo_ok = false;
 
   // Segments: independent meshes sharing the arena, segment s owning blocks
   // seg_first[s] .. seg_first[s+1]-1 (one segment for a plain mesh).
@@ -207,6 +210,14 @@ int dirichlet_device(bcm_mesh* m, bcm_field* f);
 // column export it reads.
 template <typename T>
 int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, const T* xin, T* xout, T* gout);
+// Halo-gathering Jacobi of the matrix-free Poisson operator (bcm_sweep.hip):
+// step 1 from the field x with its stored ghosts into compact buffers,
+// then steps on compact iterates; the last one (gout) writes x's rows.
+template <typename T>
+int jacobi_first_raw(bcm_mesh* m, const T* x, const T* rhs, double omega, T* out, T* xout, T* rout);
+template <typename T>
+int jacobi_halo_raw(bcm_mesh* m, const T* A, T* B, const T* rhs, double omega, const T* xin, T* xout, T* x,
+                    bool gout);
 // Block-wise ghost fill (all faces, all kinds; n in {8, 16, 32}, vc = 1).
 template <typename T> int ghost_gather_raw(bcm_mesh* m, T* data);
 // Frees the fused solver's state (bcm_krylov.hip).

@@ -67,19 +67,25 @@ __device__ __forceinline__ double fold_rhs(const BlockMeta& me, int i, int j, in
   return bv;
 }
 
-// jacobi_smoother (blas.f90:485-493).
+// jacobi_smoother (blas.f90:485-493) on the cell's value xc and its six
+// neighbours' values.
+template <typename T>
+__device__ __forceinline__ T jacobi_values(const Coef7<T>& a, T xw, T xe, T xs, T xn, T xb, T xt, T xc, T bv, T om) {
+  T r = bv - a.w * xw;
+  r = r - a.e * xe;
+  r = r - a.s * xs;
+  r = r - a.n * xn;
+  r = r - a.b * xb;
+  r = r - a.t * xt;
+  const T xtent = r / a.p;
+  return xc - om * (xc - xtent);
+}
+
+// The same on a ghosted array (cell offset o, row and plane strides).
 template <typename T>
 __device__ __forceinline__ T jacobi_cell(const Coef7<T>& a, const T* __restrict__ x, long long o, int sy, int sz,
                                          T bv, T om) {
-  T r = bv - a.w * x[o - 1];
-  r = r - a.e * x[o + 1];
-  r = r - a.s * x[o - sy];
-  r = r - a.n * x[o + sy];
-  r = r - a.b * x[o - sz];
-  r = r - a.t * x[o + sz];
-  const T xt = r / a.p;
-  const T xc = x[o];
-  return xc - om * (xc - xt);
+  return jacobi_values(a, x[o - 1], x[o + 1], x[o - sy], x[o + sy], x[o - sz], x[o + sz], x[o], bv, om);
 }
 
 // calc_ax (blas.f90:584-591): neighbours first, centre last.

@@ -266,6 +266,41 @@ int resident_launch(bcm_mesh* m, double* x, const double* b, double omega, int n
   return BCM_OK;
 }
 
+// Meshes too large to be resident: the halo-gathering schedule.  Step 1
+// reads x with its stored ghosts and writes the compact iterate, its X
+// export and the compact right-hand side; steps 2..nsweeps build each
+// block's ghost frame from the previous compact iterate (ping-pong in the
+// SOR schedule's workspace buffers); the last writes x's rows; then one
+// exchange completes x's ghosts.
+int halo_iterate(bcm_mesh* m, double* x, const double* b, double omega, int nsweeps) {
+  int rc = BCM_OK;
+  const size_t xbytes = size_t(m->nblocks) * 4 * m->n * m->n * sizeof(double);
+  for (int i = 0; i < 2; ++i)
+    if (!m->d_xexp[i]) BCM_TRY(dev_alloc(&m->d_xexp[i], xbytes));
+  const size_t cbytes = m->ncell * sizeof(double);
+  double* A = static_cast<double*>(workspace(m, WS_SOR_CA_F64, cbytes, &rc));
+  if (rc != BCM_OK) return rc;
+  double* B = static_cast<double*>(workspace(m, WS_SOR_CB_F64, cbytes, &rc));
+  if (rc != BCM_OK) return rc;
+  double* R = static_cast<double*>(workspace(m, WS_SOR_CS_F64, cbytes, &rc));
+  if (rc != BCM_OK) return rc;
+  double* xe[2] = {static_cast<double*>(m->d_xexp[0]), static_cast<double*>(m->d_xexp[1])};
+  EventPair ev;
+  BCM_TRY(timer_begin(m, BCM_K_JACOBI, &ev));
+  BCM_TRY(jacobi_first_raw<double>(m, x, b, omega, A, xe[0], R));
+  BCM_TRY(timer_end(m, &ev));
+  int cur = 0;
+  for (int it = 1; it < nsweeps; ++it) {
+    const bool last = it == nsweeps - 1;
+    BCM_TRY(timer_begin(m, last ? BCM_K_JACOBI : BCM_K_JACOBI_HALO, &ev));
+    BCM_TRY(jacobi_halo_raw<double>(m, A, B, R, omega, xe[cur], xe[cur ^ 1], x, last));
+    BCM_TRY(timer_end(m, &ev));
+    std::swap(A, B);
+    cur ^= 1;
+  }
+  return ghost_fill_raw<double>(m, x, BCM_GF_ALL);
+}
+
 }  // namespace
 }  // namespace bcm
 
@@ -286,6 +321,8 @@ extern "C" int bcm_jacobi_poisson_iterate(bcm_field_t x, bcm_field_t b, double o
     BCM_TRY(resident_ok<8>(m, &ok));
     if (ok) return resident_launch<8>(m, xd, bd, omega, nsweeps);
   }
+  if (m->vc == 1 && (m->n == 16 || m->n == 32) && m->jhalo_ok && nsweeps >= 2)
+    return halo_iterate(m, xd, bd, omega, nsweeps);
   // Two launches per step, ping-pong with a copy of x (so the cells the
   // exchange never writes keep x's values whichever buffer ends last).
   int rc = BCM_OK;

@@ -17,6 +17,7 @@
 
 #include "bcm_internal.h"
 #include "bcm_interp.h"
+#include "bcm_poisson.h"
 
 namespace bcm {
 
@@ -29,7 +30,14 @@ struct HaloSrc {
   const int* desc;   // face descriptors of this work-group's block (LDS copy)
   const double* __restrict__ bval;
   const double* bface;  // Dirichlet value per face of this block (LDS copy)
+  // Ghosted field whose ghost cells stand for the faces without a
+  // neighbour (FD_NONE), read as stored; null where no such face occurs.
+  const T* G = nullptr;
   static constexpr int N3 = N * N * N;
+  static constexpr int NXG = N + 2;
+  __device__ __forceinline__ const T* g_ptr(int blk, int i, int j, int k) const {
+    return G + (long long)blk * NXG * NXG * NXG + (i + 1) + NXG * ((j + 1) + NXG * (k + 1));
+  }
   __device__ __forceinline__ T a_at(int blk, int i, int j, int k) const {
     return A[(long long)blk * N3 + i + N * (j + N * k)];
   }
@@ -97,8 +105,11 @@ struct HaloSrc {
       } else if (kind == FD_DIR) {
         c[a] = plus ? N - 1 : 0;
         base = ptr(a, b, c[0], c[1], c[2]);
+      } else if (G) {
+        // No neighbour: the stored ghost.
+        base = g_ptr(b, ci, cj, ck);
       } else {
-        // No neighbour and no condition: the halo schedule is only used on
+        // No neighbour and no condition: the SOR schedule is only used on
         // meshes without such faces (the stored ghost is not in the compact
         // iterate); an in-block cell keeps the address valid.
         base = a_ptr(b, min(max(ci, 0), N - 1), min(max(cj, 0), N - 1), min(max(ck, 0), N - 1));
@@ -200,6 +211,12 @@ struct HaloSrc {
     fl.base[0] = viax ? x_ptr(blk0, x, y, 0) : a_ptr(blk0, x, y, 0);
     fl.base[1] = viax ? x_ptr(blk1, x, y, 0) : a_ptr(blk1, x, y, 0);
     fl.zs = viax ? N : N * N;
+    if (kind == FD_NONE && G) {
+      // No neighbour: the stored ghost cells of this block, plane by plane.
+      const int gx = a == 0 ? (plus ? N : -1) : inp, gy = a == 0 ? inp : (plus ? N : -1);
+      fl.base[0] = fl.base[1] = g_ptr(b, gx, gy, 0);
+      fl.zs = NXG * NXG;
+    }
     const int xs = viax ? N * N : 1, ys = viax ? 1 : N;
     const bool many = kind == FD_F2C || kind == FD_C2F;
 #pragma unroll
@@ -571,6 +588,289 @@ int halo_sweep_raw(bcm_mesh* m, const T* A, T* B, const T* s, double omega, cons
 template int halo_sweep_raw<double>(bcm_mesh*, const double*, double*, const double*, double, const double*, double*,
                                     double*);
 template int halo_sweep_raw<float>(bcm_mesh*, const float*, float*, const float*, double, const float*, float*, float*);
+
+// ---------------------------------------------------------------------
+// Halo-gathering Jacobi iteration of the matrix-free Poisson operator.
+//
+// The reference's step is jacobi_smoother on every block (blas.f90:452-502,
+// with the Poisson coefficients of bcut.f90:2848-2866 folded at outer
+// faces as bca.f90 folds them) followed by the ghost exchange
+// (include/Scalar3DUpdater.h:190-266, 321-347).  As for SOR above, the
+// ghosts of step k+1 are a function of the interiors after step k, so
+// step k+1 builds each block's ghost frame from the compact iterate of
+// step k (copy / F2C / C2F with the exchange's arithmetic) and no ghost
+// pass runs between steps.  Faces without a neighbour keep the ghost
+// values stored in the field (the exchange never writes them; a folded
+// face multiplies them by a zero coefficient), read from the field as
+// they are.
+// ---------------------------------------------------------------------
+
+// Step 1 reads the field with its stored ghosts, as the reference's first
+// step does, and writes the compact iterate, its four-column X export and
+// the compact right-hand side.  One work-group per block.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_jacobi_first(const T* __restrict__ x, const T* __restrict__ rhs,
+                                                      const BlockMeta* __restrict__ meta, double omega,
+                                                      T* __restrict__ out, T* __restrict__ xout,
+                                                      T* __restrict__ rout) {
+  constexpr int NX = N + 2, GV = NX * NX * NX, N3 = N * N * N;
+  const int b = blockIdx.x;
+  const BlockMeta me = meta[b];
+  const T* __restrict__ xb = x + (long long)b * GV;
+  const T* __restrict__ bb = rhs + (long long)b * GV;
+  const T om = T(omega);
+  for (int e = threadIdx.x; e < N3; e += 256) {
+    const int i = e % N, j = (e / N) % N, k = e / (N * N);
+    const int o = (i + 1) + NX * ((j + 1) + NX * (k + 1));
+    const Coef7<T> a = poisson_coef<T>(me, i, j, k, N);
+    const T bv = bb[o];
+    const T v = jacobi_cell(a, xb, o, NX, NX * NX, bv, om);
+    out[(long long)b * N3 + e] = v;
+    rout[(long long)b * N3 + e] = bv;
+    if (i < 2 || i >= N - 2) {
+      const int col = i < 2 ? i : i - (N - 4);
+      xout[((long long)(b * 4 + col) * N + k) * N + j] = v;
+    }
+  }
+}
+
+// Steps 2..k: one block per work-group, N^2/2 threads, thread t owning the
+// cell pair (2pi..2pi+1, pj) of every z-plane, as the SOR sweep above.
+// Own cells of planes k-1..k+3 sit in registers (vertical neighbours),
+// LDS holds three ghosted planes (in-plane neighbours and the X/Y frame)
+// and two frame staging planes; one barrier per z-step:
+//   step k: commit plane k; park the frame sources of plane k+1; issue the
+//   loads of plane k+3 (own cells, frame sources) and of the right-hand
+//   side of plane k+2; barrier; combine the frame of plane k+1; Jacobi on
+//   plane k's pair; store it (compact iterate and X export, or -- GOUT, the
+//   last step -- the field's rows).
+// Slot k%3 is rewritten at step k+2 (frame) and k+3 (own cells), after the
+// barriers of steps k+1 and k+2.  For 16^3 blocks (four frame values per
+// thread and plane) the kernel needs 132 VGPRs: three waves per SIMD
+// instead of four, without spills (0.550 against 0.556 ms per step on
+// 26,888 blocks).
+template <typename T, int N, bool GOUT>
+__global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(N == 16 ? 3 : 4))) void k_jacobi_halo(
+    const T* __restrict__ A, T* __restrict__ B, const T* __restrict__ rhs, const BlockMeta* __restrict__ meta,
+    double omega, const int* __restrict__ desc, const T* __restrict__ xin, T* __restrict__ xout, T* x) {
+  constexpr int NX = N + 2;
+  constexpr int PL = NX * NX;
+  constexpr int HALF = N / 2;
+  constexpr int NN = N * N;
+  __shared__ __align__(16) T lds[3][PL];
+  __shared__ T stage[2][8][4 * N];
+  __shared__ int bdesc[48];
+  __shared__ double bnone[6];  // no Dirichlet mirror here: zero values
+  const int b = blockIdx.x;
+  // Faces without a neighbour read the stored ghosts: a Dirichlet face is
+  // one of them here (its condition is folded into the coefficients).
+  if (threadIdx.x < 48) {
+    const int v = desc[b * 48 + threadIdx.x];
+    bdesc[threadIdx.x] = (threadIdx.x % 8 == 0 && v == FD_DIR) ? int(FD_NONE) : v;
+  }
+  if (threadIdx.x < 6) bnone[threadIdx.x] = 0.0;
+  __syncthreads();
+  HaloSrc<T, N> src{A, xin, bdesc, nullptr, bnone};
+  src.G = x;
+  const long long base = (long long)b * NN * N;
+  const T* __restrict__ Ab = A + base;
+  T* __restrict__ Bb = B + base;
+  const T* __restrict__ rb = rhs + base;
+  const BlockMeta me = meta[b];
+  const T om = T(omega);
+  const int t = threadIdx.x;
+  const int i0 = 2 * (t % HALF), j0 = t / HALF;
+  const int own = j0 * N + i0;
+  const bool framer = t < 4 * N;
+  const int edge = t / N, u = t % N;
+  const int fci = edge == 0 ? -1 : (edge == 1 ? N : u);
+  const int fcj = edge < 2 ? u : (edge == 2 ? -1 : N);
+  auto L = [](int i, int j) __attribute__((always_inline)) { return (j + 1) * NX + (i + 1); };
+  auto LQ = [&](int q) __attribute__((always_inline)) { return (j0 + 1) * NX + (i0 + q + 1); };
+
+  auto load_own = [&](int p, T* o) __attribute__((always_inline)) {
+    if (p >= 0 && p < N) {
+      const T* row = Ab + (long long)p * NN + own;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) o[q] = row[q];
+    } else {
+#pragma unroll 1
+      for (int q = 0; q < 2; ++q) {
+        T v[8], w;
+        const int i = i0 + q, j = j0;
+        if (p < 0) {
+          const int kd = src.template gather<BCM_Z_M>(b, i, j, p, v);
+          w = src.template combine<BCM_Z_M>(b, i, j, p, kd, v);
+        } else {
+          const int kd = src.template gather<BCM_Z_P>(b, i, j, p, v);
+          w = src.template combine<BCM_Z_P>(b, i, j, p, kd, v);
+        }
+        o[0] = q == 0 ? w : o[0];
+        o[1] = q == 1 ? w : o[1];
+      }
+    }
+  };
+  // The right-hand side is read once per step and by no other work-group.
+  auto load_r = [&](int p, T* rv) __attribute__((always_inline)) {
+    const T* row = rb + (long long)p * NN + own;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) rv[q] = __builtin_nontemporal_load(row + q);
+  };
+  constexpr int VPT = 64 / N;
+  const int gq = t / (4 * N), gc = t % (4 * N);
+  const int gedge = gc / N, gu = gc % N;
+  const int gci = gedge == 0 ? -1 : (gedge == 1 ? N : gu);
+  const int gcj = gedge < 2 ? gu : (gedge == 2 ? -1 : N);
+  const auto fload = src.template frame_setup<VPT>(gedge, b, gci, gcj, VPT * gq);
+  const auto fcomb = src.comb_setup(framer ? edge : 0, fci, fcj);
+  auto gather_frame = [&](int p, T* v) __attribute__((always_inline)) { src.frame_load(fload, p, v); };
+  auto park_frame = [&](int p, const T* v) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < VPT; ++e) stage[p & 1][VPT * gq + e][gc] = v[e];
+  };
+  auto put_frame = [&](int p) __attribute__((always_inline)) {
+    if (framer) {
+      T v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = stage[p & 1][q][t];
+      lds[p % 3][L(fci, fcj)] = src.frame_combine(fcomb, p, v);
+    }
+  };
+  auto commit = [&](int p, const T* o) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) lds[p % 3][LQ(q)] = o[q];
+  };
+  // Export columns the next step reads (as the SOR sweep's export).
+  const bool xlo = i0 == 0, xhi = i0 == N - 2;
+  const int xkind = bdesc[(xlo ? BCM_X_M : BCM_X_P) * 8];
+  const bool lvl = xkind == FD_F2C || xkind == FD_C2F;
+  const bool xo0 = (xlo && (xkind != FD_NONE)) || (xhi && lvl);
+  const bool xo1 = (xlo && lvl) || (xhi && (xkind != FD_NONE));
+  const int xcol = xlo ? 0 : 2;
+  auto store_own = [&](int p, const T* o) __attribute__((always_inline)) {
+    if constexpr (GOUT) {
+      // Whole rows of the field: the interior pair, and at the row ends the
+      // X ghost cells with this step's frame values -- the stored ghost
+      // where the face has no neighbour (so it is written back unchanged),
+      // the exchanged value elsewhere (rewritten by the closing exchange).
+      T* g = x + (long long)b * PL * NX + (long long)(p + 1) * PL + LQ(0);
+      if (xlo) g[-1] = lds[p % 3][L(-1, j0)];
+      g[0] = o[0];
+      g[1] = o[1];
+      if (xhi) g[2] = lds[p % 3][L(N, j0)];
+      return;
+    }
+    store_pair(Bb + (long long)p * NN + own, o[0], o[1]);
+    if (xo0) xout[((long long)(b * 4 + xcol) * N + p) * N + j0] = o[0];
+    if (xo1) xout[((long long)(b * 4 + xcol + 1) * N + p) * N + j0] = o[1];
+  };
+  // Jacobi on plane p's pair: in-plane neighbours from LDS, vertical ones
+  // from the own cells of planes p-1 (dn) and p+1 (up).
+  auto jacobi = [&](int p, const T* cur, const T* dn, const T* up, const T* rv, T* res)
+      __attribute__((always_inline)) {
+    const int S = p % 3;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = LQ(q);
+      const Coef7<T> a = poisson_coef<T>(me, i0 + q, j0, p, N);
+      res[q] = jacobi_values(a, lds[S][c - 1], lds[S][c + 1], lds[S][c - NX], lds[S][c + NX], dn[q], up[q], cur[q],
+                             rv[q], om);
+    }
+  };
+
+  T um1[2], u0[2], up1[2], up2[2], up3[2];
+  T rc[2], rn[2], rnn[2] = {T(0), T(0)};
+  T zp[2];
+  load_own(N, zp);
+  load_own(-1, um1);
+  load_own(0, u0);
+  load_own(1, up1);
+  load_own(2, up2);
+  load_r(0, rc);
+  load_r(1, rn);
+  T fr1[VPT], fr2[VPT], fr3[VPT];
+  gather_frame(0, fr1);
+  park_frame(0, fr1);
+  gather_frame(1, fr1);
+  gather_frame(2, fr2);
+  __syncthreads();
+  put_frame(0);
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    commit(k, u0);
+    if (k + 1 < N) park_frame(k + 1, fr1);
+    if (k + 3 < N) {
+      const T* row = Ab + (long long)(k + 3) * NN + own;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) up3[q] = row[q];
+    } else if (k + 3 == N) {
+      up3[0] = zp[0];
+      up3[1] = zp[1];
+    }
+    if (k + 2 < N) load_r(k + 2, rnn);
+    if (k + 3 < N) gather_frame(k + 3, fr3);
+    __syncthreads();
+    if (k + 1 < N) put_frame(k + 1);
+    T res[2];
+    jacobi(k, u0, um1, up1, rc, res);
+    store_own(k, res);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      um1[q] = u0[q];
+      u0[q] = up1[q];
+      up1[q] = up2[q];
+      up2[q] = up3[q];
+      rc[q] = rn[q];
+      rn[q] = rnn[q];
+    }
+#pragma unroll
+    for (int e = 0; e < VPT; ++e) {
+      fr1[e] = fr2[e];
+      fr2[e] = fr3[e];
+    }
+  }
+}
+
+template <typename T>
+int jacobi_first_raw(bcm_mesh* m, const T* x, const T* rhs, double omega, T* out, T* xout, T* rout) {
+  const dim3 grid(m->nblocks), blk(256);
+  if (m->n == 32)
+    hipLaunchKernelGGL((k_jacobi_first<T, 32>), grid, blk, 0, m->stream, x, rhs, m->d_meta, omega, out, xout, rout);
+  else if (m->n == 16)
+    hipLaunchKernelGGL((k_jacobi_first<T, 16>), grid, blk, 0, m->stream, x, rhs, m->d_meta, omega, out, xout, rout);
+  else
+    return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+template int jacobi_first_raw<double>(bcm_mesh*, const double*, const double*, double, double*, double*, double*);
+
+template <typename T, int N>
+static void launch_jacobi_halo(bcm_mesh* m, const T* A, T* B, const T* rhs, double omega, const T* xin, T* xout,
+                               T* x, bool gout) {
+  const dim3 grid(m->nblocks), blk(N * N / 2);
+  if (gout)
+    hipLaunchKernelGGL((k_jacobi_halo<T, N, true>), grid, blk, 0, m->stream, A, B, rhs, m->d_meta, omega, m->d_desc,
+                       xin, xout, x);
+  else
+    hipLaunchKernelGGL((k_jacobi_halo<T, N, false>), grid, blk, 0, m->stream, A, B, rhs, m->d_meta, omega,
+                       m->d_desc, xin, xout, x);
+}
+
+template <typename T>
+int jacobi_halo_raw(bcm_mesh* m, const T* A, T* B, const T* rhs, double omega, const T* xin, T* xout, T* x,
+                    bool gout) {
+  if (m->n == 32)
+    launch_jacobi_halo<T, 32>(m, A, B, rhs, omega, xin, xout, x, gout);
+  else if (m->n == 16)
+    launch_jacobi_halo<T, 16>(m, A, B, rhs, omega, xin, xout, x, gout);
+  else
+    return BCM_ERR_ARG;
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
+}
+template int jacobi_halo_raw<double>(bcm_mesh*, const double*, double*, const double*, double, const double*, double*,
+                                     double*, bool);
 
 
 }  // namespace bcm

@@ -245,6 +245,8 @@ def bench_jacobi(args):
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                         "note": "algorithmic 24 B per cell update; the blocks live in LDS and the exchange in L2, "
                                 "so the step is bound by the grid barrier, not by HBM"}}
+    mesh.close()
+    res["level7"] = jacobi_level7(args)
     if not args.no_cpu:
         import oracle
 
@@ -264,6 +266,64 @@ def bench_jacobi(args):
                                "sample": f"same mesh, {sweeps} sweeps of jacobi_smoother (stored A) + ghost fill "
                                          f"({dt:.2f} s) on {cores} threads of {cpu_model()}"}
     return res
+
+
+def jacobi_level7(args, steps=20):
+    """The same iteration at HBM scale: the level-7 shell mesh (26,888 x
+    16^3, Dirichlet outer faces), too large to stay in LDS, so every step
+    streams the compact iterate (halo-gathering schedule).  Also times the
+    same steps made as one jacobi7_poisson and one ghost fill call each."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    t = W.config3_tree()
+    blocks, n = t.blocks, 16
+    ncell = len(blocks) * n ** 3
+    mesh = Mesh(blocks, n, 1)
+    rhs = mesh.field(np.float64, -W.rhs_field_fast(blocks, n, 1, seed=1))
+    gb = mesh.field(np.float64)
+    mesh.poisson_assemble(None, gb, rhs, [0.0])
+    gx = mesh.field(np.float64)
+    mesh.jacobi_poisson_iterate(gx, gb, 1.0, 3)
+    mesh.sync()
+    mesh.reset_timing()
+    mesh.set_timing(True)
+    t0 = time.perf_counter()
+    mesh.jacobi_poisson_iterate(gx, gb, 1.0, steps)
+    mesh.sync()
+    el = time.perf_counter() - t0
+    hms, hn = mesh.kernel_time(L.K_JACOBI_HALO)
+    ems, en = mesh.kernel_time(L.K_JACOBI)
+    gms, gn = mesh.kernel_time(L.K_GHOST)
+    mesh.set_timing(False)
+    havg = hms / max(hn, 1)
+    achieved = 24.0 * ncell / (havg * 1e-3) / 1e9
+    # The reference's step as two calls (sweep, then exchange), for comparison.
+    gy = mesh.field(np.float64)
+    k = 5
+    mesh.sync()
+    t1 = time.perf_counter()
+    for _ in range(k):
+        mesh.jacobi7_poisson(gy, gx, gb, 1.0)
+        gx, gy = gy, gx
+        mesh.update_vc(gx)
+    mesh.sync()
+    el2 = time.perf_counter() - t1
+    mesh.close()
+    out = {"workload": f"level-7 shell mesh, {len(blocks)} x 16^3 ({ncell} cells), Dirichlet outer faces, "
+                       f"{steps} Jacobi steps (sweep + ghost exchange) in one bcm_jacobi_poisson_iterate call",
+           "GCUPS": ncell * steps / el / 1e9, "ms_per_step": el / steps * 1e3,
+           "halo_step_ms": havg, "first_and_last_step_ms": ems / max(en, 1), "closing_ghost_fill_ms": gms / max(gn, 1),
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "note": "k_jacobi_halo<double,16,false>: 24 B per cell (compact iterate read, compact "
+                                "right-hand side read, compact iterate written)"},
+           "two_calls_per_step": {"ms_per_step": el2 / k * 1e3, "GCUPS": ncell * k / el2 / 1e9,
+                                  "note": "bcm_jacobi7_poisson then bcm_ghost_fill per step on the ghosted field"}}
+    print(f"[bench] jacobi level 7: {out['GCUPS']:.1f} GCUPS, halo step {havg:.3f} ms "
+          f"({achieved:.0f} GB/s), two calls per step {el2 / k * 1e3:.3f} ms", file=sys.stderr, flush=True)
+    return out
 
 
 def true_relres(mesh, x, rhs_field, bvalues):
@@ -769,6 +829,60 @@ def bench_copy(args):
             "config": {"workload": "copy of one 3816 x 34^3 FP64 field", "bytes_per_copy": nbytes}}
 
 
+def ghost_fill_bytes(counts, n):
+    """Algorithmic bytes of one complete ghost fill (one ghost layer, FP64):
+    every ghost cell written once and every source cell it needs read once.
+    Same-level face: n^2 ghosts, one source each.  F2C subface: (n/2)^2
+    coarse ghosts, eight fine sources each.  C2F face: n^2 fine ghosts from
+    the (n/2+1)^2 x 2 coarse cells the trilinear stencils touch."""
+    copy, f2c, c2f = counts["flat"], counts["f2c"], counts["c2f"]
+    return 8 * (copy * 2 * n * n + f2c * 9 * (n // 2) ** 2 + c2f * (n * n + 2 * (n // 2 + 1) ** 2))
+
+
+def bench_ghost(args):
+    """The reference's updateVC on its own (include/BlockManager.h:311-726,
+    include/Scalar3DUpdater.h:190-347): the complete ghost fill of an FP64
+    field with one ghost layer, on the 3,816 x 32^3 shell mesh and the
+    26,888 x 16^3 shell mesh, the whole fill and each axis alone."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    out = {}
+    for name, tree, n in (("shell3816x32", W.config2_tree(), 32), ("shell26888x16", W.config3_tree(), 16)):
+        mesh = Mesh(tree.blocks, n, 1)
+        f = mesh.field(np.float64, W.rhs_field_fast(tree.blocks, n, 1, seed=3))
+        counts = mesh.task_counts()
+        nbytes = ghost_fill_bytes(counts, n)
+        row = {"blocks": len(tree.blocks), "n": n, "tasks": counts,
+               "algorithmic_bytes": nbytes}
+        for label, mask in (("all", L.GF_ALL), ("x", L.GF_ALL | L.GF_X), ("y", L.GF_ALL | L.GF_Y),
+                            ("z", L.GF_ALL | L.GF_Z), ("blockwise", L.GF_ALL | L.GF_BLOCKWISE)):
+            for _ in range(args.warmup):
+                mesh.update_vc(f, mask)
+            mesh.sync()
+            mesh.set_timing(True)
+            mesh.reset_timing()
+            for _ in range(args.steps):
+                mesh.update_vc(f, mask)
+            mesh.sync()
+            ms, nl = mesh.kernel_time(L.K_GHOST)
+            mesh.set_timing(False)
+            row[f"ms_{label}"] = ms / max(nl, 1)
+        row["algorithmic_GBps_all"] = nbytes / (row["ms_all"] * 1e-3) / 1e9
+        print(f"[bench] ghost fill {name}: {row}", file=sys.stderr, flush=True)
+        out[name] = row
+        mesh.close()
+    first = out["shell3816x32"]
+    return {"metric": "ghost fill (updateVC) throughput, algorithmic bytes", "value": first["algorithmic_GBps_all"],
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": first["ms_all"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "complete ghost fill of one FP64 field (one ghost layer) on the level-5 shell "
+                                   "mesh (3,816 x 32^3) and the level-7 shell mesh (26,888 x 16^3)"},
+            "meshes": out}
+
+
 def bench_restart(args):
     """Restart files of SolverTest (LocalScalar3D::Dump2 / Load2) of an FP64
     field on the level-7 shell mesh, written from HBM and read back into it."""
@@ -857,7 +971,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sor",
-                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart"])
+                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart", "ghost"])
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
     ap.add_argument("--lams", default="", help="serving: comma-separated arrival rates to sweep (same requests)")
@@ -892,7 +1006,8 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
-           "serving": bench_serving, "copy": bench_copy, "restart": bench_restart}[args.workload](args)
+           "serving": bench_serving, "copy": bench_copy, "restart": bench_restart,
+           "ghost": bench_ghost}[args.workload](args)
     if args.workload == "sor" and not args.quick:
         # The rest of the metric: SOR to a tolerance, batched solves/s and
         # served solves/s with p50/p99 latency, each on a bounded sample.

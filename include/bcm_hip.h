@@ -160,10 +160,13 @@ int bcm_mesh_task_counts(bcm_mesh_t mesh, int counts[4]);
 int bcm_mesh_sync(bcm_mesh_t mesh);
 /* Per-kernel HIP-event timing on the mesh stream (off by default).
  * BCM_K_SOR_HALO: the halo-gathering sweeps between a bcm_sor_iterate
- * call's first sweep and its last; BCM_K_SOR: every other SOR sweep. */
+ * call's first sweep and its last; BCM_K_SOR: every other SOR sweep.
+ * BCM_K_JACOBI_HALO: the same for bcm_jacobi_poisson_iterate's
+ * halo-gathering steps; BCM_K_JACOBI: every other Jacobi sweep. */
 enum {
   BCM_K_GHOST = 0, BCM_K_DIRICHLET, BCM_K_SOR, BCM_K_JACOBI, BCM_K_SPMV, BCM_K_RESID,
-  BCM_K_DOT, BCM_K_BLAS1, BCM_K_REDUCE, BCM_K_ASSEMBLE, BCM_K_SOR_HALO, BCM_K_COUNT
+  BCM_K_DOT, BCM_K_BLAS1, BCM_K_REDUCE, BCM_K_ASSEMBLE, BCM_K_SOR_HALO, BCM_K_JACOBI_HALO,
+  BCM_K_COUNT
 };
 int bcm_mesh_set_timing(bcm_mesh_t mesh, int enable);
 /* Sum of event-measured durations (ms) and launch count since the last reset. */

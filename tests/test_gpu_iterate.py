@@ -1,8 +1,12 @@
 """bcm_jacobi_poisson_iterate: nsweeps of (jacobi7_poisson, ghost exchange)
 in one call equal the same steps made one call at a time, bit for bit on
 every cell -- the one-launch LDS-resident form (config 1's 64 x 16^3, an
-adaptive 456 x 8^3 mesh with level changes) and the two-launch form (a
-mesh too large to be resident, 2,528 x 16^3; 32^3 blocks)."""
+adaptive 456 x 8^3 mesh with level changes) and the halo-gathering form on
+compact iterates for meshes too large to be resident (2,528 x 16^3 and
+32^3 blocks without boundary conditions, whose stored random ghosts enter
+the unfolded stencil; shell meshes whose outer faces are Dirichlet, folded
+into the coefficients; two and three steps, where the first and the last
+step are the only halo steps or the only middle one)."""
 import numpy as np
 import pytest
 
@@ -29,7 +33,8 @@ def stepwise(mesh, x, b, omega, nsweeps):
 
 
 @pytest.mark.parametrize("name,n,nsweeps", [("config1", 16, 100), ("config1", 16, 37), ("simple23", 8, 25),
-                                            ("simple24", 16, 6), ("simple12_n32", 32, 5)])
+                                            ("simple24", 16, 6), ("simple12_n32", 32, 5), ("simple12_n32", 32, 2),
+                                            ("shell5", 16, 7), ("shell3", 32, 3)])
 def test_iterate_equals_stepwise_calls(name, n, nsweeps):
     if name == "config1":
         blocks = W.config1_tree().blocks
@@ -37,6 +42,10 @@ def test_iterate_equals_stepwise_calls(name, n, nsweeps):
         blocks = Tree(divider=L.DIV_SIMPLE, min_level=2, max_level=3).blocks
     elif name == "simple24":
         blocks = Tree(divider=L.DIV_SIMPLE, min_level=2, max_level=4).blocks
+    elif name == "shell5":
+        blocks = W.shell_tree(5).blocks
+    elif name == "shell3":
+        blocks = W.shell_tree(3).blocks
     else:
         blocks = Tree(divider=L.DIV_SIMPLE, min_level=1, max_level=2).blocks
     case(f"jacobi_poisson_iterate {name} nb={len(blocks)} n={n} sweeps={nsweeps}")
