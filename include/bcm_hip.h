@@ -440,8 +440,12 @@ int bcm_jacobi7_poisson(bcm_field_t x1, bcm_field_t x0, bcm_field_t b, double om
  * every cell they write; cells the exchange never writes (edges, corners,
  * faces without a neighbour) keep x's values.  A mesh of vc = 1, n = 8 or
  * 16, whose blocks can all be resident on the GPU at once runs it as one
- * launch with every block in LDS; any other mesh as two launches per step.
- * x and b FP64. */
+ * launch with every block in LDS.  A mesh of vc = 1, n = 16 or 32, each of
+ * whose faces is connected on every subface or not at all, runs one launch
+ * per step on compact copies of x and b, building the ghosts each step
+ * needs from the previous step's interiors (nsweeps >= 2; the reads of
+ * unconnected faces' ghosts come from x, unchanged until the last step).
+ * Any other mesh runs two launches per step.  x and b FP64. */
 int bcm_jacobi_poisson_iterate(bcm_field_t x, bcm_field_t b, double omega, int nsweeps);
 int bcm_calc_ax_poisson(bcm_field_t ax, bcm_field_t x);
 int bcm_calc_r_poisson(bcm_field_t r, bcm_field_t x, bcm_field_t b);
