@@ -335,6 +335,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   constexpr int NN = N * N;
   __shared__ __align__(16) T lds[3][PL];
   __shared__ T stage[2][8][4 * N];
+  __shared__ T xstage[3][4][N];  // X-export values of a plane, [slot][column][j]
   __shared__ int bdesc[48];
   __shared__ double bface[6];
   const int b = blockIdx.x;
@@ -459,6 +460,20 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   const bool xo0 = (xlo && (xkind != FD_NONE)) || (xhi && lvl);   // cell i0 (column 0 or N-2)
   const bool xo1 = (xlo && lvl) || (xhi && (xkind != FD_NONE));   // cell i0+1 (column 1 or N-1)
   const int xcol = xlo ? 0 : 2;
+  // Exported columns 0, 1, N-2, N-1 (bits 0..3), as xo0 / xo1 above.
+  const int kxm = bdesc[BCM_X_M * 8], kxp = bdesc[BCM_X_P * 8];
+  const int xcols = (kxm != FD_NONE ? 1 : 0) | (kxm == FD_F2C || kxm == FD_C2F ? 2 : 0) |
+                    (kxp == FD_F2C || kxp == FD_C2F ? 4 : 0) | (kxp != FD_NONE ? 8 : 0);
+  // The export of plane p is staged in LDS by the threads owning the cells
+  // (step p+1) and written at step p+2 by the last wave, two values per
+  // lane, so each column row of N values leaves as whole cache lines.
+  constexpr int XT0 = N * N / 2 - 64;
+  auto export_plane = [&](int p) __attribute__((always_inline)) {
+    const int l = t - XT0, col = l / (N / 2), j2 = 2 * (l % (N / 2));
+    if (l >= 0 && l < 2 * N && ((xcols >> col) & 1))
+      store_pair(xout + ((long long)(b * 4 + col) * N + p) * N + j2, xstage[p % 3][col][j2],
+                 xstage[p % 3][col][j2 + 1]);
+  };
   // Plane p's final pair, once black(p) has run: B and the export, or
   // (GOUT: the last sweep of a call) the ghosted field's interior.  GOUT is
   // a template parameter so that no store sits under a run-time branch:
@@ -482,8 +497,8 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
       return;
     }
     store_pair(Bb + (long long)p * NN + own, o[0], o[1]);
-    if (xo0) xout[((long long)(b * 4 + xcol) * N + p) * N + j0] = o[0];
-    if (xo1) xout[((long long)(b * 4 + xcol + 1) * N + p) * N + j0] = o[1];
+    if (xo0) xstage[p % 3][xcol][j0] = o[0];
+    if (xo1) xstage[p % 3][xcol + 1][j0] = o[1];
   };
 
   // Register window: own cells of planes k-2 .. k+3 (loads of plane k+3
@@ -523,6 +538,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
     if (k + 3 < N) gather_frame(k + 3, fr3);
     __syncthreads();
     if (k + 1 < N) put_frame(k + 1);
+    if (!GOUT && k >= 2) export_plane(k - 2);
     red(std::integral_constant<int, PAR>(), k, u0, um1, up1, sc);
     if (k >= 1) {
       black(std::integral_constant<int, 1 - PAR>(), k - 1, um1, um2, u0, sp);
@@ -554,8 +570,13 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(4))) v
   }
   // um1 = plane N-1, um2 = plane N-2, u0 = plane N (ghost); sp = s(N-1).
   __syncthreads();
+  if (!GOUT) export_plane(N - 2);
   black(std::integral_constant<int, (N - 1) & 1>(), N - 1, um1, um2, u0, sp);
   store_own(N - 1, um1);
+  if (!GOUT) {
+    __syncthreads();
+    export_plane(N - 1);
+  }
 }
 
 template <typename T, int N>
@@ -659,6 +680,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(N == 1
   constexpr int NN = N * N;
   __shared__ __align__(16) T lds[3][PL];
   __shared__ T stage[2][8][4 * N];
+  __shared__ T xstage[3][4][N];  // X-export values of a plane, [slot][column][j]
   __shared__ int bdesc[48];
   __shared__ double bnone[6];  // no Dirichlet mirror here: zero values
   const int b = blockIdx.x;
@@ -747,6 +769,19 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(N == 1
   const bool xo0 = (xlo && (xkind != FD_NONE)) || (xhi && lvl);
   const bool xo1 = (xlo && lvl) || (xhi && (xkind != FD_NONE));
   const int xcol = xlo ? 0 : 2;
+  // The export of plane p is staged in LDS by the threads owning the cells
+  // (step p) and written at step p+1 by the last wave, two values per
+  // lane, as whole cache lines (as the SOR sweep does).
+  const int kxm = bdesc[BCM_X_M * 8], kxp = bdesc[BCM_X_P * 8];
+  const int xcols = (kxm != FD_NONE ? 1 : 0) | (kxm == FD_F2C || kxm == FD_C2F ? 2 : 0) |
+                    (kxp == FD_F2C || kxp == FD_C2F ? 4 : 0) | (kxp != FD_NONE ? 8 : 0);
+  constexpr int XT0 = N * N / 2 - 64;
+  auto export_plane = [&](int p) __attribute__((always_inline)) {
+    const int l = t - XT0, col = l / (N / 2), j2 = 2 * (l % (N / 2));
+    if (l >= 0 && l < 2 * N && ((xcols >> col) & 1))
+      store_pair(xout + ((long long)(b * 4 + col) * N + p) * N + j2, xstage[p % 3][col][j2],
+                 xstage[p % 3][col][j2 + 1]);
+  };
   auto store_own = [&](int p, const T* o) __attribute__((always_inline)) {
     if constexpr (GOUT) {
       // Whole rows of the field: the interior pair, and at the row ends the
@@ -761,8 +796,8 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(N == 1
       return;
     }
     store_pair(Bb + (long long)p * NN + own, o[0], o[1]);
-    if (xo0) xout[((long long)(b * 4 + xcol) * N + p) * N + j0] = o[0];
-    if (xo1) xout[((long long)(b * 4 + xcol + 1) * N + p) * N + j0] = o[1];
+    if (xo0) xstage[p % 3][xcol][j0] = o[0];
+    if (xo1) xstage[p % 3][xcol + 1][j0] = o[1];
   };
   // Jacobi on plane p's pair: in-plane neighbours from LDS, vertical ones
   // from the own cells of planes p-1 (dn) and p+1 (up).
@@ -811,6 +846,7 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(N == 1
     if (k + 3 < N) gather_frame(k + 3, fr3);
     __syncthreads();
     if (k + 1 < N) put_frame(k + 1);
+    if (!GOUT && k >= 1) export_plane(k - 1);
     T res[2];
     jacobi(k, u0, um1, up1, rc, res);
     store_own(k, res);
@@ -828,6 +864,10 @@ __global__ __launch_bounds__(N* N / 2) __attribute__((amdgpu_waves_per_eu(N == 1
       fr1[e] = fr2[e];
       fr2[e] = fr3[e];
     }
+  }
+  if (!GOUT) {
+    __syncthreads();
+    export_plane(N - 1);
   }
 }
 
