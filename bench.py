@@ -316,7 +316,7 @@ def jacobi_level7(args, steps=20):
            "GCUPS": ncell * steps / el / 1e9, "ms_per_step": el / steps * 1e3,
            "halo_step_ms": havg, "first_and_last_step_ms": ems / max(en, 1), "closing_ghost_fill_ms": gms / max(gn, 1),
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS,
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("jacobi_halo_n16"),
                         "note": "k_jacobi_halo<double,16,false>: 24 B per cell (compact iterate read, compact "
                                 "right-hand side read, compact iterate written)"},
            "two_calls_per_step": {"ms_per_step": el2 / k * 1e3, "GCUPS": ncell * k / el2 / 1e9,
