@@ -171,6 +171,7 @@ DIVIDER_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64)
 SIGNATURES = {
     "bcm_tree_build": (_I, [ctypes.POINTER(TreeParams), ctypes.POINTER(_P)]),
     "bcm_tree_build_user": (_I, [ctypes.POINTER(TreeParams), DIVIDER_FN, _P, ctypes.POINTER(_P)]),
+    "bcm_tree_set_threads": (_I, [_I]),
     "bcm_tree_num_blocks": (_I, [_P]),
     "bcm_tree_get_blocks": (_I, [_P, _P, _I]),
     "bcm_tree_destroy": (_I, [_P]),
@@ -261,6 +262,9 @@ SIGNATURES = {
     "bcm_server_destroy": (_I, [_P]),
     "bcm_server_clock": (_D, [_P]),
     "bcm_server_submit": (_I, [_P, _P, _I, _P, ctypes.c_uint64, _D, _I, _P, ctypes.c_int64]),
+    "bcm_server_checkpoint": (_I, [_P, ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p, _I, _I, _IP]),
+    "bcm_server_submit_restart": (_I, [_P, _P, _I, _P, ctypes.c_uint64, _D, _I, ctypes.c_char_p, ctypes.c_char_p,
+                                       _I, _I, _P, ctypes.c_int64]),
     "bcm_server_progress": (_I, [_P, _D, _IP, _IP]),
     "bcm_server_poll": (_I, [_P, ctypes.POINTER(ServerResult), _I]),
     "bcm_server_stats": (_I, [_P, _DP, _IP, _IP]),

@@ -539,20 +539,39 @@ class Server:
     def clock(self):
         return self.lib.bcm_server_clock(self.handle)
 
-    def submit(self, blocks, tol, precision="mixed", rhs=None, seed=0, tag=0, want_x=False):
+    def submit(self, blocks, tol, precision="mixed", rhs=None, seed=0, tag=0, want_x=False, restart=None):
         """Queue a request; `tag` must not name a request still pending (the
-        server reads rhs and writes x of the request it names)."""
+        server reads rhs and writes x of the request it names).  restart =
+        (dir, label, step[, rank]): start from the iterate in that restart
+        file (bcm_server_submit_restart) instead of zero."""
         if int(tag) in self._keep:
             raise L.BCMError(L.ERR_ARG, f"bcm_server_submit (tag {int(tag)} already pending)")
         blocks = np.ascontiguousarray(blocks, dtype=L.BLOCK_DTYPE)
         nx = self.n + 2
         r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64).reshape(len(blocks), nx, nx, nx)
         x = np.zeros((len(blocks), nx, nx, nx)) if want_x else None
-        L.check(self.lib.bcm_server_submit(self.handle, blocks.ctypes.data, len(blocks),
-                                           None if r is None else r.ctypes.data, int(seed), float(tol),
-                                           self.PRECISIONS[precision], None if x is None else x.ctypes.data,
-                                           int(tag)), "bcm_server_submit")
+        rp = None if r is None else r.ctypes.data
+        xp = None if x is None else x.ctypes.data
+        if restart is None:
+            L.check(self.lib.bcm_server_submit(self.handle, blocks.ctypes.data, len(blocks), rp, int(seed),
+                                               float(tol), self.PRECISIONS[precision], xp, int(tag)),
+                    "bcm_server_submit")
+        else:
+            d, label, step = restart[0], restart[1], restart[2]
+            rank = restart[3] if len(restart) > 3 else 0
+            L.check(self.lib.bcm_server_submit_restart(self.handle, blocks.ctypes.data, len(blocks), rp, int(seed),
+                                                       float(tol), self.PRECISIONS[precision], str(d).encode(),
+                                                       str(label).encode(), int(rank), int(step), xp, int(tag)),
+                    "bcm_server_submit_restart")
         self._keep[int(tag)] = (blocks, r, x)
+
+    def checkpoint(self, tag, dir, label, step, rank=0):
+        """Write the running request `tag`'s current iterate as a restart
+        file (Dump2 format); returns the iterations it has done."""
+        it = ctypes.c_int()
+        L.check(self.lib.bcm_server_checkpoint(self.handle, int(tag), str(dir).encode(), str(label).encode(),
+                                               int(rank), int(step), ctypes.byref(it)), "bcm_server_checkpoint")
+        return it.value
 
     def progress(self, seconds):
         a, q = ctypes.c_int(), ctypes.c_int()

@@ -230,6 +230,20 @@ int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg
 int krylov_begin(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& p, const double* seg_tol);
 int krylov_advance(bcm_mesh* m, bool wait, bool* active);
 int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats);
+// The current iterate of a running fused solve (x plus the pending FP64
+// correction on the interior, connected faces' ghosts gathered) into y,
+// one ghosted array of the whole mesh (device); waits for the iterations
+// already queued.  *iters: iterations segment seg has done.
+int krylov_snapshot(bcm_mesh* m, double* y, int seg, int* iters);
+// Restart files (bcm_io.hip).  dump2_raw: the Dump2 file of nblk ghosted
+// blocks of the mesh's edge and ghost width held at dev (device memory,
+// esize bytes per value).  load2_header: checks a Dump2 file against edge
+// n (or n/2), ghost width vc and nblk blocks; *nfile = its edge.
+// load2_into: blocks first.. of f from such a file.
+int dump2_raw(bcm_mesh* m, const void* dev, int nblk, int esize, const char* dir, const char* label, int rank,
+              int step);
+int load2_header(const char* dir, const char* label, int rank, int step, int n, int vc, int nblk, int* nfile);
+int load2_into(bcm_field* f, int first, int nblk, const char* dir, const char* label, int rank, int step);
 // Device bytes the fused solver allocates for a mesh of nblocks n^3 blocks.
 size_t krylov_bytes(int nblocks, int n, int precision, int count_pre);
 // Field-sized workspace buffer kept on the mesh (bcm_solver.hip).

@@ -224,23 +224,20 @@ int check_args(bcm_field* f, const char* dir, const char* label) {
 }
 
 }  // namespace
-}  // namespace bcm
 
-using namespace bcm;
-
-extern "C" int bcm_field_dump2(bcm_field_t f, const char* dir, const char* label, int rank, int step) {
-  BCM_TRY(check_args(f, dir, label));
-  bcm_mesh* m = f->mesh;
+int dump2_raw(bcm_mesh* m, const void* dev, int nblk, int esize, const char* dir, const char* label, int rank,
+              int step) {
+  if (!m || !dev || nblk < 0 || !dir || !label) return BCM_ERR_ARG;
   if (::mkdir(dir, 0755) != 0 && errno != EEXIST) return BCM_ERR_IO;
   Stage st(m->stream);
   BCM_TRY(st.init());
   File out;
   out.fp = std::fopen(join(dir, label, rank, -1, step).c_str(), "wb");
   if (!out.fp) return BCM_ERR_IO;
-  const int32_t h[6] = {m->n, m->n, m->n, m->vc, 1, m->nblocks};
+  const int32_t h[6] = {m->n, m->n, m->n, m->vc, 1, nblk};
   if (std::fwrite(h, sizeof h, 1, out.fp) != 1) return BCM_ERR_IO;
-  BCM_HIP_TRY(hipStreamSynchronize(m->stream));  // the field's producers
-  BCM_TRY(stream_out(m, static_cast<const char*>(f->data), f->bytes, out.fp, st));
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));  // the data's producers
+  BCM_TRY(stream_out(m, static_cast<const char*>(dev), size_t(nblk) * m->bvol * size_t(esize), out.fp, st));
   if (std::fclose(out.fp) != 0) {
     out.fp = nullptr;
     return BCM_ERR_IO;
@@ -249,21 +246,48 @@ extern "C" int bcm_field_dump2(bcm_field_t f, const char* dir, const char* label
   return BCM_OK;
 }
 
-extern "C" int bcm_field_load2(bcm_field_t f, const char* dir, const char* label, int rank, int step) {
-  BCM_TRY(check_args(f, dir, label));
-  bcm_mesh* m = f->mesh;
+int load2_header(const char* dir, const char* label, int rank, int step, int n, int vc, int nblk, int* nfile) {
+  if (!dir || !label) return BCM_ERR_ARG;
   File in;
   in.fp = std::fopen(join(dir, label, rank, -1, step).c_str(), "rb");
   if (!in.fp) return BCM_ERR_IO;
   int32_t h[6];
   BCM_TRY(read_header(in.fp, h, 6));
-  const bool same = h[0] == m->n && h[1] == m->n && h[2] == m->n;
-  const bool half = 2 * h[0] == m->n && 2 * h[1] == m->n && 2 * h[2] == m->n;
-  if (!(same || half) || h[3] != m->vc || h[4] != 1 || h[5] != m->nblocks) return BCM_ERR_ARG;
+  const bool same = h[0] == n && h[1] == n && h[2] == n;
+  const bool half = 2 * h[0] == n && 2 * h[1] == n && 2 * h[2] == n;
+  if (!(same || half) || h[3] != vc || h[4] != 1 || h[5] != nblk) return BCM_ERR_ARG;
+  if (nfile) *nfile = h[0];
+  return BCM_OK;
+}
+
+int load2_into(bcm_field* f, int first, int nblk, const char* dir, const char* label, int rank, int step) {
+  bcm_mesh* m = f->mesh;
+  if (first < 0 || nblk < 0 || first + nblk > m->nblocks) return BCM_ERR_ARG;
+  int nfile = 0;
+  BCM_TRY(load2_header(dir, label, rank, step, m->n, m->vc, nblk, &nfile));
+  File in;
+  in.fp = std::fopen(join(dir, label, rank, -1, step).c_str(), "rb");
+  if (!in.fp) return BCM_ERR_IO;
+  int32_t h[6];
+  BCM_TRY(read_header(in.fp, h, 6));
   Stage st(m->stream);
   BCM_TRY(st.init());
   BCM_HIP_TRY(hipStreamSynchronize(m->stream));
-  return load_blocks(f, 0, m->nblocks, h[0], in.fp, st);
+  return load_blocks(f, first, nblk, nfile, in.fp, st);
+}
+
+}  // namespace bcm
+
+using namespace bcm;
+
+extern "C" int bcm_field_dump2(bcm_field_t f, const char* dir, const char* label, int rank, int step) {
+  BCM_TRY(check_args(f, dir, label));
+  return dump2_raw(f->mesh, f->data, f->mesh->nblocks, f->dtype == BCM_F64 ? 8 : 4, dir, label, rank, step);
+}
+
+extern "C" int bcm_field_load2(bcm_field_t f, const char* dir, const char* label, int rank, int step) {
+  BCM_TRY(check_args(f, dir, label));
+  return load2_into(f, 0, f->mesh->nblocks, dir, label, rank, step);
 }
 
 extern "C" int bcm_field_dump(bcm_field_t f, const char* dir, const char* label, int rank, int step) {

@@ -1665,6 +1665,49 @@ int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats) {
   }
 }
 
+// The current iterate of a running solve: y = x + dx on the interior of
+// every block (the sum the next reliable update forms), x's stored ghost
+// cells elsewhere; the connected faces' ghosts are then gathered, as the
+// end of a solve leaves them.
+template <int N>
+__global__ __launch_bounds__(KWG) void k_cur_x(double* __restrict__ y, const double* __restrict__ x,
+                                               const double* __restrict__ dx) {
+  using B = Blk<N>;
+  const long long b = blockIdx.x;
+  const double* xb = x + b * B::GV;
+  const double* db = dx + b * B::N3;
+  double* yb = y + b * B::GV;
+  for (int e = threadIdx.x; e < B::GV; e += KWG) {
+    const int i = e % B::NX - 1, j = (e / B::NX) % B::NX - 1, k = e / (B::NX * B::NX) - 1;
+    const bool in = i >= 0 && i < N && j >= 0 && j < N && k >= 0 && k < N;
+    yb[e] = in ? xb[e] + db[i + N * (j + N * k)] : xb[e];
+  }
+}
+
+template <int N>
+static int snapshot_impl(bcm_mesh* m, double* y) {
+  Krylov* K = m->kry;
+  hipLaunchKernelGGL(k_cur_x<N>, dim3(unsigned(m->nblocks)), dim3(KWG), 0, m->stream, y, K->run.x, K->dx);
+  BCM_HIP_TRY(hipGetLastError());
+  return gather<double, N>(m, y, 0);
+}
+
+int krylov_snapshot(bcm_mesh* m, double* y, int seg, int* iters) {
+  Krylov* K = m->kry;
+  if (!K || !K->ev || !K->dx || !K->run.x || seg < 0 || seg >= m->nseg) return BCM_ERR_STATE;
+  // Iterations already queued finish first, so x and dx belong together.
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  switch (m->n) {
+    case 8: BCM_TRY(snapshot_impl<8>(m, y)); break;
+    case 16: BCM_TRY(snapshot_impl<16>(m, y)); break;
+    case 32: BCM_TRY(snapshot_impl<32>(m, y)); break;
+    default: return BCM_ERR_ARG;
+  }
+  BCM_HIP_TRY(hipStreamSynchronize(m->stream));
+  if (iters) *iters = K->h_st[seg].iters;
+  return BCM_OK;
+}
+
 int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg_tol) {
   if (m->vc != 1 || (m->n != 8 && m->n != 16 && m->n != 32)) return BCM_ERR_ARG;
   if (p->precision < BCM_PREC_FP64 || p->precision > BCM_PREC_FP32 || !(p->tol > 0.0) || p->max_iter < 1 ||

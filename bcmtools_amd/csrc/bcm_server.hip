@@ -28,6 +28,7 @@
 #include <deque>
 #include <list>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -86,6 +87,10 @@ struct Request {
   double* x_out = nullptr;
   double t_submit = 0.0, t_admit = 0.0;
   size_t bytes = 0;
+  // Initial guess from a restart file (bcm_server_submit_restart).
+  bool restart = false;
+  std::string r_dir, r_label;
+  int r_rank = 0, r_step = 0;
 };
 
 // Requests sharing one device mesh (one segment each, in order).
@@ -190,6 +195,13 @@ int admit(bcm_server* sv, Group* g) {
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     dev_free(d);
     if (e != hipSuccess) return BCM_ERR_HIP;
+  }
+  // Initial guesses of resumed requests; the others start from zero.
+  for (int s = 0; s < nseg; ++s) {
+    const Request* r = g->reqs[size_t(s)];
+    if (r->restart)
+      BCM_TRY(load2_into(g->x, g->first[size_t(s)], int(r->blocks.size()), r->r_dir.c_str(), r->r_label.c_str(),
+                         r->r_rank, r->r_step));
   }
   const double zero = 0.0;
   BCM_TRY(set_bvalues(m, &zero, 1));
@@ -313,8 +325,8 @@ extern "C" int bcm_server_destroy(bcm_server_t sv) {
 
 extern "C" double bcm_server_clock(bcm_server_t sv) { return sv ? sv->now() : -1.0; }
 
-extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, int nblocks, const double* rhs,
-                                 uint64_t seed, double tol, int precision, double* x_out, int64_t tag) {
+static int submit(bcm_server_t sv, const bcm_block_info* blocks, int nblocks, const double* rhs, uint64_t seed,
+                  double tol, int precision, double* x_out, int64_t tag, Request* proto) {
   if (!sv || !blocks || nblocks <= 0 || !(tol > 0.0) || precision < BCM_PREC_FP64 || precision > BCM_PREC_FP32)
     return BCM_ERR_ARG;
   for (int b = 0; b < nblocks; ++b)
@@ -327,8 +339,12 @@ extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, 
   for (const Group* g : sv->active)
     for (const Request* q : g->reqs)
       if (q->tag == tag) return BCM_ERR_ARG;
+  if (proto && proto->restart)
+    BCM_TRY(load2_header(proto->r_dir.c_str(), proto->r_label.c_str(), proto->r_rank, proto->r_step, sv->p.n, 1,
+                         nblocks, nullptr));
   Request* r = new (std::nothrow) Request();
   if (!r) return BCM_ERR_NOMEM;
+  if (proto) *r = *proto;
   r->tag = tag;
   r->blocks.assign(blocks, blocks + nblocks);
   r->rhs_host = rhs;
@@ -341,6 +357,47 @@ extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, 
   sv->queue.push_back(r);
   admit_ready(sv);
   return BCM_OK;
+}
+
+extern "C" int bcm_server_submit(bcm_server_t sv, const bcm_block_info* blocks, int nblocks, const double* rhs,
+                                 uint64_t seed, double tol, int precision, double* x_out, int64_t tag) {
+  return submit(sv, blocks, nblocks, rhs, seed, tol, precision, x_out, tag, nullptr);
+}
+
+extern "C" int bcm_server_submit_restart(bcm_server_t sv, const bcm_block_info* blocks, int nblocks,
+                                         const double* rhs, uint64_t seed, double tol, int precision,
+                                         const char* dir, const char* label, int rank, int step, double* x_out,
+                                         int64_t tag) {
+  if (!dir || !label) return BCM_ERR_ARG;
+  Request proto;
+  proto.restart = true;
+  proto.r_dir = dir;
+  proto.r_label = label;
+  proto.r_rank = rank;
+  proto.r_step = step;
+  return submit(sv, blocks, nblocks, rhs, seed, tol, precision, x_out, tag, &proto);
+}
+
+extern "C" int bcm_server_checkpoint(bcm_server_t sv, int64_t tag, const char* dir, const char* label, int rank,
+                                     int step, int* iters) {
+  if (!sv || !dir || !label) return BCM_ERR_ARG;
+  for (const Request* q : sv->queue)
+    if (q->tag == tag) return BCM_ERR_STATE;  // not admitted: no iterate yet
+  for (Group* g : sv->active)
+    for (size_t s = 0; s < g->reqs.size(); ++s) {
+      if (g->reqs[s]->tag != tag) continue;
+      bcm_mesh* m = g->mesh;
+      const int nb = int(g->reqs[s]->blocks.size());
+      void* y = nullptr;
+      BCM_TRY(dev_alloc(&y, sizeof(double) * m->bvol * size_t(m->nblocks)));
+      int rc = krylov_snapshot(m, static_cast<double*>(y), int(s), iters);
+      if (rc == BCM_OK)
+        rc = dump2_raw(m, static_cast<double*>(y) + m->bvol * size_t(g->first[s]), nb, 8, dir, label, rank, step);
+      (void)hipStreamSynchronize(m->stream);
+      dev_free(y);
+      return rc;
+    }
+  return BCM_ERR_ARG;
 }
 
 extern "C" int bcm_server_progress(bcm_server_t sv, double max_seconds, int* n_active, int* n_queued) {

@@ -10,14 +10,61 @@
 #include "bcm_hip.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <new>
 #include <random>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 namespace {
+
+// Host threads of the builder (bcm_tree_set_threads; 0: one per hardware
+// thread, at most 64).
+std::atomic<int> g_threads{0};
+
+int host_threads() {
+  const int want = g_threads.load();
+  const int t = want > 0 ? want : int(std::thread::hardware_concurrency());
+  return std::max(1, std::min(t, 64));
+}
+
+// f(begin, end) over [0, n) in one contiguous chunk per thread; serial when
+// n is below two grains.  A thread that cannot be started runs its chunk
+// on the calling thread afterwards.  The chunks partition [0, n), so the
+// outcome does not depend on the thread count as long as f writes only
+// its own items (or per-chunk outputs merged in chunk order).
+template <typename F>
+void parallel_chunks(size_t n, size_t grain, int nchunk, F&& f) {
+  if (nchunk <= 1) {
+    f(size_t(0), n, 0);
+    return;
+  }
+  const size_t per = (n + size_t(nchunk) - 1) / size_t(nchunk);
+  std::vector<std::thread> th;
+  std::vector<int> inline_chunks;
+  for (int c = 1; c < nchunk; ++c) {
+    const size_t b = size_t(c) * per, e = std::min(n, b + per);
+    if (b >= e) continue;
+    try {
+      th.emplace_back([&f, b, e, c] { f(b, e, c); });
+    } catch (const std::system_error&) {
+      inline_chunks.push_back(c);
+    }
+  }
+  f(size_t(0), std::min(n, per), 0);
+  for (auto& t : th) t.join();
+  for (int c : inline_chunks) f(size_t(c) * per, std::min(n, size_t(c) * per + per), c);
+  (void)grain;
+}
+
+int chunks_for(size_t n, size_t grain) {
+  if (n < 2 * grain) return 1;
+  return int(std::min<size_t>(size_t(host_threads()), n / grain));
+}
 
 // Pedigree word: x16 | y16 | z16 | root12 | level4 (Utils/include/Pedigree.h:42-73).
 inline uint64_t make_pedigree(unsigned level, unsigned x, unsigned y, unsigned z, unsigned root) {
@@ -27,6 +74,7 @@ inline uint64_t make_pedigree(unsigned level, unsigned x, unsigned y, unsigned z
 
 struct TNode {
   uint32_t level, x, y, z, root;
+  int32_t parent;  // index of the parent node; -1 for a root
   int32_t child0;  // index of the first of 8 consecutive children; -1 for a leaf
   int32_t id;      // block ID of an active leaf, else -1
   bool active;
@@ -72,6 +120,14 @@ const int kSubfaceOfChild[3][8] = {
 
 enum NodeKind { BRANCH, LEAF_ACTIVE, LEAF_INACTIVE };
 
+// A block record whose default construction leaves the bytes alone: the
+// table is sized once and every record is written in full by the thread
+// that owns it, so a serial zero fill would only add a pass.
+struct BlockRec {
+  bcm_block_info v;
+  BlockRec() {}
+};
+
 }  // namespace
 
 struct bcm_tree {
@@ -81,7 +137,7 @@ struct bcm_tree {
   std::vector<TNode> nodes;
   std::vector<int32_t> root_node;   // node index of each root
   std::vector<int32_t> leaf_nodes;  // node index of each block ID
-  std::vector<bcm_block_info> blocks;
+  std::vector<BlockRec> blocks;
 
   int nroot() const { return p.root_n[0] * p.root_n[1] * p.root_n[2]; }
   int rix(int r) const { return r % p.root_n[0]; }
@@ -184,6 +240,7 @@ struct bcm_tree {
       ch.y = parent.y * 2 + ((c >> 1) & 1);
       ch.z = parent.z * 2 + ((c >> 2) & 1);
       ch.root = parent.root;
+      ch.parent = idx;
       ch.child0 = -1;
       ch.id = -1;
       ch.active = true;
@@ -207,54 +264,141 @@ struct bcm_tree {
     }
   }
 
+  // The same tree as grow() on every root, built level by level: the
+  // nodes of one level are classified in parallel, then their children
+  // appended.  Node indices differ from the depth-first build, which no
+  // output depends on (block IDs follow the child links).  Used for the
+  // built-in dividers; a caller's divider keeps grow()'s order of calls.
+  void grow_levels() {
+    std::vector<int> front(root_node.begin(), root_node.end());
+    std::vector<uint8_t> kind;
+    while (!front.empty()) {
+      kind.assign(front.size(), 0);
+      const int nc = chunks_for(front.size(), 2048);
+      parallel_chunks(front.size(), 2048, nc, [&](size_t b, size_t e, int) {
+        for (size_t i = b; i < e; ++i) kind[i] = uint8_t(classify(nodes[size_t(front[i])]));
+      });
+      size_t nbr = 0;
+      for (size_t i = 0; i < front.size(); ++i) {
+        if (kind[i] == BRANCH) ++nbr;
+        else if (kind[i] == LEAF_INACTIVE) nodes[size_t(front[i])].active = false;
+      }
+      const size_t base = nodes.size();
+      nodes.resize(base + 8 * nbr);
+      std::vector<int> next(8 * nbr);
+      size_t k = 0;
+      for (size_t i = 0; i < front.size(); ++i) {
+        if (kind[i] != BRANCH) continue;
+        const TNode parent = nodes[size_t(front[i])];
+        const int32_t c0 = int32_t(base + 8 * k);
+        nodes[size_t(front[i])].child0 = c0;
+        for (int c = 0; c < 8; ++c) {
+          TNode& ch = nodes[size_t(c0 + c)];
+          ch.level = parent.level + 1;
+          ch.x = parent.x * 2 + (c & 1);
+          ch.y = parent.y * 2 + ((c >> 1) & 1);
+          ch.z = parent.z * 2 + ((c >> 2) & 1);
+          ch.root = parent.root;
+          ch.parent = front[i];
+          ch.child0 = -1;
+          ch.id = -1;
+          ch.active = true;
+          next[8 * k + size_t(c)] = c0 + c;
+        }
+        ++k;
+      }
+      front.swap(next);
+    }
+  }
+
   // BCMOctree::findNeighborNode (Utils/src/BCMOctree.cpp:277-364): the
   // deepest node at or above my level covering my face neighbour; -1 at
-  // a non-periodic outer face.
+  // a non-periodic outer face.  Inside a root the search climbs to the
+  // lowest ancestor that also covers the neighbour cell and descends from
+  // there (the reference descends from the root; the node reached is the
+  // same, since the root's path to it passes through that ancestor).
   int find_neighbor(int idx, int face) const {
-    const TNode& nd = nodes[idx];
+    const TNode& nd = nodes[size_t(idx)];
     const int level = int(nd.level);
     const int m = (1 << level) - 1;
     int c[3] = {int(nd.x), int(nd.y), int(nd.z)};
-    int root = int(nd.root);
     const int a = face / 2;
+    const int own = c[a];
     c[a] += (face & 1) ? 1 : -1;
+    int cur, from;
     if (c[a] < 0 || c[a] > m) {
-      root = root_neighbor(root, face);
+      const int root = root_neighbor(int(nd.root), face);
       if (root < 0) return -1;
       c[a] = (face & 1) ? 0 : m;
+      cur = root_node[size_t(root)];
+      from = 1;
+    } else {
+      cur = idx;
+      int sh = 0;
+      while ((c[a] >> sh) != (own >> sh)) {
+        cur = nodes[size_t(cur)].parent;
+        ++sh;
+      }
+      from = level - sh + 1;
     }
-    int cur = root_node[root];
-    for (int l = 1; l <= level; ++l) {
-      if (nodes[cur].child0 < 0) break;
+    for (int l = from; l <= level; ++l) {
+      if (nodes[size_t(cur)].child0 < 0) break;
       const int sh = level - l;
       const int ijk = ((c[0] >> sh) & 1) + 2 * ((c[1] >> sh) & 1) + 4 * ((c[2] >> sh) & 1);
-      cur = nodes[cur].child0 + ijk;
+      cur = nodes[size_t(cur)].child0 + ijk;
     }
     return cur;
   }
 
   // Split leaves until every face neighbour of an active leaf is at most
-  // one level coarser.
+  // one level coarser.  Splitting only ever makes neighbours finer, so a
+  // leaf that passed a check keeps passing it: after the first pass over
+  // every leaf, a pass checks only the leaves that failed the previous one
+  // (their split neighbour may still be two levels coarser) and the
+  // children just created.  The splits reach the same fixed point as
+  // repeated passes over all leaves.
   void balance() {
+    std::vector<int> check;
+    for (int idx = 0; idx < int(nodes.size()); ++idx) check.push_back(idx);
     for (;;) {
-      std::vector<int> to_split;
-      for (int idx = 0; idx < int(nodes.size()); ++idx) {
-        const TNode& nd = nodes[idx];
-        if (nd.child0 >= 0 || !nd.active || nd.level < 2) continue;
-        for (int f = 0; f < 6; ++f) {
-          const int nb = find_neighbor(idx, f);
-          if (nb < 0) continue;
-          const TNode& nn = nodes[nb];
-          if (nn.child0 < 0 && int(nn.level) + 1 < int(nd.level)) to_split.push_back(nb);
+      const int nc = chunks_for(check.size(), 4096);
+      std::vector<std::vector<int>> split_part(static_cast<size_t>(nc)), fail_part(static_cast<size_t>(nc));
+      parallel_chunks(check.size(), 4096, nc, [&](size_t b, size_t e, int c) {
+        std::vector<int>& sp = split_part[size_t(c)];
+        std::vector<int>& fl = fail_part[size_t(c)];
+        for (size_t i = b; i < e; ++i) {
+          const int idx = check[i];
+          const TNode& nd = nodes[size_t(idx)];
+          if (nd.child0 >= 0 || !nd.active || nd.level < 2) continue;
+          bool failed = false;
+          for (int f = 0; f < 6; ++f) {
+            const int nb = find_neighbor(idx, f);
+            if (nb < 0) continue;
+            const TNode& nn = nodes[size_t(nb)];
+            if (nn.child0 < 0 && int(nn.level) + 1 < int(nd.level)) {
+              sp.push_back(nb);
+              failed = true;
+            }
+          }
+          if (failed) fl.push_back(idx);
         }
+      });
+      std::vector<int> to_split;
+      check.clear();
+      for (int c = 0; c < nc; ++c) {
+        to_split.insert(to_split.end(), split_part[size_t(c)].begin(), split_part[size_t(c)].end());
+        check.insert(check.end(), fail_part[size_t(c)].begin(), fail_part[size_t(c)].end());
       }
       if (to_split.empty()) return;
       std::sort(to_split.begin(), to_split.end());
       to_split.erase(std::unique(to_split.begin(), to_split.end()), to_split.end());
+      nodes.reserve(nodes.size() + 8 * to_split.size());
       for (int idx : to_split) {
-        if (nodes[idx].child0 < 0) split(idx);
-        if (!nodes[idx].active)
-          for (int c = 0; c < 8; ++c) nodes[nodes[idx].child0 + c].active = false;
+        if (nodes[size_t(idx)].child0 < 0) split(idx);
+        const int c0 = nodes[size_t(idx)].child0;
+        if (!nodes[size_t(idx)].active)
+          for (int c = 0; c < 8; ++c) nodes[size_t(c0 + c)].active = false;
+        for (int c = 0; c < 8; ++c) check.push_back(c0 + c);
       }
     }
   }
@@ -295,11 +439,22 @@ struct bcm_tree {
     return c[face / 2] == want && root_outer(int(nd.root), face);
   }
 
+  // One record per leaf; the leaves are independent, so chunks of them
+  // are filled in parallel.
   int make_blocks() {
     blocks.resize(leaf_nodes.size());
-    for (size_t b = 0; b < leaf_nodes.size(); ++b) {
+    std::atomic<int> status{BCM_OK};
+    const int nc = chunks_for(leaf_nodes.size(), 2048);
+    parallel_chunks(leaf_nodes.size(), 2048, nc, [&](size_t b0, size_t b1, int) {
+      if (make_block_range(b0, b1) != BCM_OK) status.store(BCM_ERR_BALANCE);
+    });
+    return status.load();
+  }
+
+  int make_block_range(size_t b0, size_t b1) {
+    for (size_t b = b0; b < b1; ++b) {
       const TNode& nd = nodes[leaf_nodes[b]];
-      bcm_block_info& bi = blocks[b];
+      bcm_block_info& bi = blocks[b].v;
       std::memset(&bi, 0, sizeof(bi));
       bi.pedigree = make_pedigree(nd.level, nd.x, nd.y, nd.z, nd.root);
       bi.level = int32_t(nd.level);
@@ -412,13 +567,15 @@ static int tree_build(const bcm_tree_params* params, bcm_divider_fn divide, void
       TNode root;
       root.level = root.x = root.y = root.z = 0;
       root.root = uint32_t(r);
+      root.parent = -1;
       root.child0 = -1;
       root.id = -1;
       root.active = true;
       t->root_node[r] = int32_t(t->nodes.size());
       t->nodes.push_back(root);
-      t->grow(t->root_node[r]);
+      if (divide) t->grow(t->root_node[r]);
     }
+    if (!divide) t->grow_levels();
     if (p.balance) t->balance();
     for (int r = 0; r < nr; ++r) {
       if (p.ordering == BCM_ORDER_HILBERT) t->pick_hilbert(t->root_node[r], 0);
@@ -448,6 +605,12 @@ extern "C" int bcm_tree_build_user(const bcm_tree_params* params, bcm_divider_fn
   return tree_build(params, divide, user, out);
 }
 
+extern "C" int bcm_tree_set_threads(int nthreads) {
+  if (nthreads < 0) return BCM_ERR_ARG;
+  g_threads.store(nthreads);
+  return BCM_OK;
+}
+
 extern "C" int bcm_tree_num_blocks(bcm_tree_t tree) {
   if (!tree) return -BCM_ERR_ARG;
   return int(tree->blocks.size());
@@ -455,6 +618,7 @@ extern "C" int bcm_tree_num_blocks(bcm_tree_t tree) {
 
 extern "C" int bcm_tree_get_blocks(bcm_tree_t tree, bcm_block_info* out, int capacity) {
   if (!tree || !out || capacity < int(tree->blocks.size())) return BCM_ERR_ARG;
+  static_assert(sizeof(BlockRec) == sizeof(bcm_block_info), "records are copied as one array");
   std::memcpy(out, tree->blocks.data(), tree->blocks.size() * sizeof(bcm_block_info));
   return BCM_OK;
 }

@@ -883,6 +883,65 @@ def bench_ghost(args):
             "meshes": out}
 
 
+def bench_mesh(args):
+    """Mesh construction on the host (BCMOctree + BlockFactory, Utils/src/
+    BCMOctree.cpp:23-478, as bcm_tree_build) on one thread and on all host
+    threads, and the device tables of bcm_mesh_create (BlockManager::
+    prepareForVCUpdate's face lists, src/BlockManager.cpp:63-114, 471-648),
+    for the largest request of the served stream and two large shell meshes."""
+    import ctypes
+    import time
+
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    lib = L.load()
+
+    def build_s(params):
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        L.check(lib.bcm_tree_build(ctypes.byref(params), ctypes.byref(h)), "bcm_tree_build")
+        dt = time.perf_counter() - t0
+        nb = lib.bcm_tree_num_blocks(h)
+        lib.bcm_tree_destroy(h)
+        return dt, nb
+
+    threads = os.cpu_count() or 1
+    cases = (("serving_request", W.serving_tree(7000 + 7, 50000)), ("shell_level7", W.shell_tree(7)),
+             ("shell_level8", W.shell_tree(8)))
+    out = {}
+    for name, tree in cases:
+        row = {"blocks": len(tree.blocks)}
+        for label, nt in (("one_thread", 1), ("all_threads", 0)):
+            lib.bcm_tree_set_threads(nt)
+            for _ in range(max(1, args.warmup)):
+                build_s(tree.params)
+            row[f"tree_ms_{label}"] = 1e3 * min(build_s(tree.params)[0] for _ in range(max(1, args.steps)))
+        lib.bcm_tree_set_threads(0)
+        row["speedup"] = row["tree_ms_one_thread"] / row["tree_ms_all_threads"]
+        if len(tree.blocks) <= 200000:
+            times = []
+            for _ in range(max(1, min(args.steps, 3))):
+                t0 = time.perf_counter()
+                m = Mesh(tree.blocks, 16, 1)
+                m.sync()
+                times.append(time.perf_counter() - t0)
+                m.close()
+            row["mesh_create_ms"] = 1e3 * min(times)
+        print(f"[bench] mesh {name}: {row}", file=sys.stderr, flush=True)
+        out[name] = row
+    first = out["serving_request"]
+    return {"metric": "host mesh build time (octree, 2:1 balance, block table)", "value": first["tree_ms_all_threads"],
+            "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": first["tree_ms_all_threads"], "higher_is_better": False, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "bcm_tree_build of the served stream's largest request and of the level-7 and "
+                                   "level-8 shell meshes, on one host thread and on all of them",
+                       "host_threads": threads},
+            "meshes": out}
+
+
 def bench_restart(args):
     """Restart files of SolverTest (LocalScalar3D::Dump2 / Load2) of an FP64
     field on the level-7 shell mesh, written from HBM and read back into it."""
@@ -971,7 +1030,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sor",
-                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart", "ghost"])
+                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart", "ghost", "mesh"])
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
     ap.add_argument("--lams", default="", help="serving: comma-separated arrival rates to sweep (same requests)")
@@ -1007,7 +1066,7 @@ def main():
         sys.exit(2)
     res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
            "serving": bench_serving, "copy": bench_copy, "restart": bench_restart,
-           "ghost": bench_ghost}[args.workload](args)
+           "ghost": bench_ghost, "mesh": bench_mesh}[args.workload](args)
     if args.workload == "sor" and not args.quick:
         # The rest of the metric: SOR to a tolerance, batched solves/s and
         # served solves/s with p50/p99 latency, each on a bounded sample.
@@ -1015,12 +1074,13 @@ def main():
         sub = batched_subset(_batch_tables(256), every=8)
         res["batched"] = {"workload": "every 8th mesh of the 256-mesh batch generator (seeds 1000 + 8k), "
                                       "BiCGSTAB to 1e-10", **sub}
-        # Served with the mixed engine for every request (the target's
-        # solver), then the same stream with the precision chosen by
+        # Served with the fused FP64 engine for every request (the fastest
+        # policy on these meshes, and the precision of the one-at-a-time
+        # comparison), then the same stream with the precision chosen by
         # tolerance (FP32 for 1e-6, mixed for 1e-10).
         sv_args = argparse.Namespace(**vars(args))
         sv_args.requests, sv_args.lam, sv_args.lams, sv_args.policy, sv_args.per_request_every = 12, 100.0, "", \
-            "mixed", 4
+            "fp64", 4
         sv = bench_serving(sv_args)
         keys = ("value", "unit", "config", "latency_s", "queue_wait_s", "by_tolerance", "all_met_tolerance",
                 "not_met", "makespan_s")

@@ -135,6 +135,14 @@ typedef int (*bcm_divider_fn)(void* user, uint64_t pedigree);
  * BCM_DIV_USER; divide(user, pedigree) is called for every node, parents
  * before children, and returns a BCM_NODE_* kind. */
 int bcm_tree_build_user(const bcm_tree_params* params, bcm_divider_fn divide, void* user, bcm_tree_t* out);
+/* Host threads of bcm_tree_build (0, the default: one per hardware thread,
+ * at most 64).  The reference builds its octree on one thread
+ * (Utils/src/BCMOctree.cpp:23-48); here the levels of the tree, the 2:1
+ * balance checks and the block records are split over threads, and the
+ * block table is the same for every thread count.  A caller's divider
+ * (bcm_tree_build_user) is still called on one thread, parents before
+ * children. */
+int bcm_tree_set_threads(int nthreads);
 /* BCMOctree::getNumLeafNode (Utils/include/BCMOctree.h:111). Returns the count (>=0) or -BCM_ERR_ARG. */
 int bcm_tree_num_blocks(bcm_tree_t tree);
 /* Copies the block table (one record per leaf in block-ID order). */
@@ -645,6 +653,32 @@ double bcm_server_clock(bcm_server_t server);
  * solution (NULL: not returned); all outer faces Dirichlet u = 0. */
 int bcm_server_submit(bcm_server_t server, const bcm_block_info* blocks, int nblocks, const double* rhs,
                       uint64_t seed, double tol, int precision, double* x_out, int64_t tag);
+/* Checkpoint of a running request in SolverTest's restart format
+ * (LocalScalar3D::Dump2, examples/SolverTest/LocalScalar3D.cpp:227-281):
+ * waits for the iterations already queued for the request's group, then
+ * writes the request's current iterate -- x plus the pending FP64
+ * correction on the interior, the ghost cells of connected faces gathered
+ * from it as the end of a solve leaves them (Dump2 imposes the boundary
+ * conditions before it writes) -- to
+ * dir/dump-<label>-<rank %05d>-<step %010d>.bin, header int32
+ * {n, n, n, 1, 1, nblocks}, the request's blocks in its block-ID order.
+ * *iters (may be NULL): iterations done so far.  The request keeps
+ * running.  A queued request has no iterate yet: BCM_ERR_STATE; no
+ * pending request with this tag: BCM_ERR_ARG. */
+int bcm_server_checkpoint(bcm_server_t server, int64_t tag, const char* dir, const char* label, int rank, int step,
+                          int* iters);
+/* bcm_server_submit starting from an initial guess read, when the request
+ * is admitted, from the restart file dir/dump-<label>-<rank>-<step>.bin
+ * (written by bcm_server_checkpoint or bcm_field_dump2; Load2's rules,
+ * LocalScalar3D.cpp:283-357, a file of half-edge blocks prolonged by
+ * sf3d_copy_x2).  Resuming a checkpointed request on a new server
+ * continues its solve from the checkpointed iterate with a fresh Krylov
+ * space.  The header is checked at submission: BCM_ERR_IO for a missing
+ * or short file, BCM_ERR_ARG for another block edge, ghost width or block
+ * count. */
+int bcm_server_submit_restart(bcm_server_t server, const bcm_block_info* blocks, int nblocks, const double* rhs,
+                              uint64_t seed, double tol, int precision, const char* dir, const char* label,
+                              int rank, int step, double* x_out, int64_t tag);
 /* Drive the admitted requests for up to max_seconds (returns early when
  * nothing is left); finished requests move to the result list. */
 int bcm_server_progress(bcm_server_t server, double max_seconds, int* n_active, int* n_queued);

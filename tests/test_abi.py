@@ -61,7 +61,9 @@ def test_field_api_rejects_bad_arguments_without_gpu():
 
 # Entry points that report a count return -BCM_ERR_ARG for a NULL handle;
 # the destroy calls accept NULL like free(); every other int entry point
-# returns BCM_ERR_ARG.
+# returns BCM_ERR_ARG, except the process-wide setting that takes no
+# handle (0 is its default).
+_SETTINGS = {"bcm_tree_set_threads"}
 _COUNTS = {"bcm_tree_num_blocks", "bcm_sor_get_variant", "bcm_server_poll"}
 _DESTROYS = {"bcm_tree_destroy", "bcm_mesh_destroy", "bcm_field_destroy", "bcm_server_destroy", "bcm_vfield_destroy"}
 
@@ -77,7 +79,7 @@ def test_every_entry_point_rejects_null_handles_without_gpu():
         vals = [0 if a in ints else 0.0 if a in (ctypes.c_double, ctypes.c_float) else
                 ctypes.cast(None, a) if a is L.DIVIDER_FN else None for a in args]
         got = getattr(lib, name)(*vals)
-        if name in _DESTROYS:
+        if name in _DESTROYS or name in _SETTINGS:
             assert got == L.OK, name
         elif name in _COUNTS:
             assert got == -L.ERR_ARG, name
@@ -123,3 +125,39 @@ def test_fortran_entry_points_exit_on_bad_sizes_without_gpu():
     out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
     assert out.returncode == 1
     assert "bcm: bc_x1_d_: needs at least 2 ghost layers (g = 1)" in out.stderr
+
+
+def test_server_checkpoint_and_restart_checks_without_gpu(tmp_path):
+    """bcm_server_submit_restart reads the restart header at submission
+    (missing file: BCM_ERR_IO; other edge, ghost width or block count:
+    BCM_ERR_ARG), and a checkpoint names a pending request; none of these
+    checks touches the device."""
+    from oracle import restart as R
+    lib = L.load()
+    p = L.ServerParams(1e9, 1, 16, 100, 2, 0.8, 4, 0)
+    h = ctypes.c_void_p()
+    assert lib.bcm_server_create(ctypes.byref(p), ctypes.byref(h)) == L.OK
+    blocks = np.zeros(3, dtype=L.BLOCK_DTYPE)
+    for b in range(3):
+        blocks["nb_id"][b] = -1
+    d = str(tmp_path)
+
+    def submit(label):
+        return lib.bcm_server_submit_restart(h, blocks.ctypes.data, 3, None, 1, ctypes.c_double(1e-8), 0, d.encode(),
+                                             label.encode(), 0, 7, None, 1)
+
+    assert submit("missing") == L.ERR_IO
+    R.dump2(np.zeros((2, 18, 18, 18)), 16, 1, R.name2(d, "two", 0, 7))  # 2 blocks, not 3
+    assert submit("two") == L.ERR_ARG
+    R.dump2(np.zeros((3, 8, 8, 8)), 6, 1, R.name2(d, "other", 0, 7))  # edge 6: neither 16 nor 16/2
+    assert submit("other") == L.ERR_ARG
+    with open(R.name2(d, "short", 0, 7), "wb") as fp:
+        fp.write(np.array([16, 16], dtype=np.int32).tobytes())
+    assert submit("short") == L.ERR_IO
+    it = ctypes.c_int(-1)
+    assert lib.bcm_server_checkpoint(h, 1, d.encode(), b"x", 0, 0, ctypes.byref(it)) == L.ERR_ARG
+    assert lib.bcm_server_checkpoint(h, 1, None, b"x", 0, 0, ctypes.byref(it)) == L.ERR_ARG
+    n_active, n_queued = ctypes.c_int(-1), ctypes.c_int(-1)
+    assert lib.bcm_server_stats(h, None, ctypes.byref(n_active), ctypes.byref(n_queued)) == L.OK
+    assert (n_active.value, n_queued.value) == (0, 0)
+    assert lib.bcm_server_destroy(h) == L.OK

@@ -123,3 +123,29 @@ def test_rhs_noise_independent_of_block_order():
     np.testing.assert_array_equal(a[perm], b)
     noise = W.normal_noise(1, t.blocks["pedigree"], 8).ravel()
     assert abs(noise.mean()) < 0.02 and abs(noise.std() - 1.0) < 0.02
+
+
+@pytest.mark.parametrize("name,make,digest,count", [
+    ("shell5", lambda: W.shell_tree(5), "3037731f804aa7db", 3816),
+    ("shell7", lambda: W.shell_tree(7), "e13c1c73b6e59c91", 181000),
+    ("random11", lambda: W.random_tree(11, max_level=6), "76d6b4b034dec832", 1289),
+    ("simple_periodic", lambda: B.Tree(root_n=(3, 2, 2), periodic=(1, 0, 1), divider=L.DIV_SIMPLE, min_level=1,
+                                     max_level=4, ordering=L.ORDER_HILBERT), "f894ae14512d7e9a", 16336),
+])
+def test_parallel_tree_build_gives_the_serial_block_table(name, make, digest, count):
+    """bcm_tree_build splits the levels, the 2:1 balance checks and the
+    block records over host threads.  The digests were recorded from the
+    single-threaded depth-first builder (one balance pass over every leaf
+    until none fails); every thread count gives the same table byte for
+    byte."""
+    import hashlib
+    lib = L.load()
+    try:
+        for threads in (1, 3, 8):
+            assert lib.bcm_tree_set_threads(threads) == L.OK
+            t = make()
+            assert len(t.blocks) == count, (name, threads)
+            assert hashlib.sha256(t.blocks.tobytes()).hexdigest()[:16] == digest, (name, threads)
+    finally:
+        lib.bcm_tree_set_threads(0)
+    assert lib.bcm_tree_set_threads(-1) == L.ERR_ARG
