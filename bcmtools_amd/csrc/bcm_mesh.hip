@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 
+#include "bcm_host_par.h"
 #include "bcm_internal.h"
 #include "bcm_interp.h"
 
@@ -437,20 +438,24 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
   if (!out) return BCM_ERR_ARG;
   *out = nullptr;
   if (!blocks || nblocks <= 0 || n < 2 || (n % 2) != 0 || vc < 1 || n / 2 < vc || n > 256) return BCM_ERR_ARG;
-  // Validate the neighbour table.
-  for (int b = 0; b < nblocks; ++b) {
-    const bcm_block_info& bi = blocks[b];
-    for (int f = 0; f < 6; ++f) {
-      const int ld = bi.nb_level_diff[f];
-      if (ld < -1 || ld > 1) return BCM_ERR_ARG;
-      for (int s = 0; s < 4; ++s)
-        if (bi.nb_id[f][s] < -1 || bi.nb_id[f][s] >= nblocks) return BCM_ERR_ARG;
-      if (ld == -1 && (bi.nb_subface[f] < 0 || bi.nb_subface[f] > 3)) return BCM_ERR_ARG;
-      if (bi.bc_type[f] < BCM_BC_INNER || bi.bc_type[f] > BCM_BC_PERIODIC) return BCM_ERR_ARG;
-      if ((bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) && bi.bc_id[f] < 0)
-        return BCM_ERR_ARG;
+  // Validate the neighbour table (chunks of blocks on the host threads).
+  const int nchunk = chunks_for(size_t(nblocks), 4096);
+  std::atomic<bool> bad{false};
+  parallel_chunks(size_t(nblocks), nchunk, [&](size_t b0, size_t b1, int) {
+    for (size_t b = b0; b < b1 && !bad.load(std::memory_order_relaxed); ++b) {
+      const bcm_block_info& bi = blocks[b];
+      for (int f = 0; f < 6; ++f) {
+        const int ld = bi.nb_level_diff[f];
+        bool ok = ld >= -1 && ld <= 1;
+        for (int s = 0; s < 4; ++s) ok = ok && bi.nb_id[f][s] >= -1 && bi.nb_id[f][s] < nblocks;
+        ok = ok && !(ld == -1 && (bi.nb_subface[f] < 0 || bi.nb_subface[f] > 3));
+        ok = ok && bi.bc_type[f] >= BCM_BC_INNER && bi.bc_type[f] <= BCM_BC_PERIODIC;
+        ok = ok && !((bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) && bi.bc_id[f] < 0);
+        if (!ok) bad.store(true);
+      }
     }
-  }
+  });
+  if (bad.load()) return BCM_ERR_ARG;
   bcm_mesh* m = new (std::nothrow) bcm_mesh();
   if (!m) return BCM_ERR_NOMEM;
   m->n = n;
@@ -465,92 +470,108 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
     if (hipGetDevice(&m->device) != hipSuccess) { rc = BCM_ERR_HIP; break; }
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { rc = BCM_ERR_HIP; break; }
     // Face tasks, as BlockManager::setFaceLists (src/BlockManager.cpp:471-530)
-    // walks them: per block, per face, per subface.
-    std::vector<int4> lists[3][3];
-    std::vector<int4> bc;
+    // walks them: per block, per face, per subface.  Chunks of blocks build
+    // their own lists on the host threads, merged in chunk order, so every
+    // list is in block order whatever the chunk count.
+    struct Part {
+      std::vector<int4> lists[3][3];
+      std::vector<int4> bc;
+      int n_dirichlet = 0, n_neumann = 0, need_bvalues = 0;
+      bool halo_ok = true, jhalo_ok = true;
+    };
+    std::vector<Part> parts(static_cast<size_t>(nchunk));
     m->meta.resize(nblocks);
-    for (int b = 0; b < nblocks; ++b) {
-      const bcm_block_info& bi = blocks[b];
-      BlockMeta& me = m->meta[b];
-      const double hx = bi.block_size[0] / n, hy = bi.block_size[1] / n, hz = bi.block_size[2] / n;
-      me.cx = 1.0 / (hx * hx);
-      me.cy = 1.0 / (hy * hy);
-      me.cz = 1.0 / (hz * hz);
-      me.hx = hx;
-      me.dir_mask = 0;
-      me.neu_mask = 0;
-      for (int f = 0; f < 6; ++f) {
-        me.bc_id[f] = bi.bc_id[f];
-        const int axis = f / 2;
-        const int ld = bi.nb_level_diff[f];
-        if (ld == 0 || ld == -1) {
-          const int nb = bi.nb_id[f][0];
-          if (nb >= 0) {
-            const int kind = ld == 0 ? KIND_COPY : KIND_C2F;
-            const int sub = ld == 0 ? 0 : bi.nb_subface[f];
-            lists[axis][kind].push_back(make_int4(b, nb, f | (sub << 3) | (kind << 5), 0));
-          }
-        } else {
-          for (int s = 0; s < 4; ++s) {
-            const int nb = bi.nb_id[f][s];
-            if (nb >= 0) lists[axis][KIND_F2C].push_back(make_int4(b, nb, f | (s << 3) | (KIND_F2C << 5), 0));
-          }
-        }
-        if (bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) {
-          bc.push_back(make_int4(b, f, bi.bc_id[f], bi.bc_type[f]));
-          if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
-            me.dir_mask |= 1 << f;
-            m->n_dirichlet++;
-          } else {
-            me.neu_mask |= 1 << f;
-            m->n_neumann++;
-          }
-          m->need_bvalues = std::max(m->need_bvalues, bi.bc_id[f] + 1);
-        }
-      }
-    }
     // Face descriptors of the halo-gathering sweep.
     std::vector<int> desc(size_t(nblocks) * 48, 0);
+    parallel_chunks(size_t(nblocks), nchunk, [&](size_t b0, size_t b1, int c) {
+      Part& P = parts[size_t(c)];
+      for (size_t bb = b0; bb < b1; ++bb) {
+        const int b = int(bb);
+        const bcm_block_info& bi = blocks[b];
+        BlockMeta& me = m->meta[b];
+        const double hx = bi.block_size[0] / n, hy = bi.block_size[1] / n, hz = bi.block_size[2] / n;
+        me.cx = 1.0 / (hx * hx);
+        me.cy = 1.0 / (hy * hy);
+        me.cz = 1.0 / (hz * hz);
+        me.hx = hx;
+        me.dir_mask = 0;
+        me.neu_mask = 0;
+        for (int f = 0; f < 6; ++f) {
+          me.bc_id[f] = bi.bc_id[f];
+          const int axis = f / 2;
+          const int ld = bi.nb_level_diff[f];
+          if (ld == 0 || ld == -1) {
+            const int nb = bi.nb_id[f][0];
+            if (nb >= 0) {
+              const int kind = ld == 0 ? KIND_COPY : KIND_C2F;
+              const int sub = ld == 0 ? 0 : bi.nb_subface[f];
+              P.lists[axis][kind].push_back(make_int4(b, nb, f | (sub << 3) | (kind << 5), 0));
+            }
+          } else {
+            for (int q = 0; q < 4; ++q) {
+              const int nb = bi.nb_id[f][q];
+              if (nb >= 0) P.lists[axis][KIND_F2C].push_back(make_int4(b, nb, f | (q << 3) | (KIND_F2C << 5), 0));
+            }
+          }
+          if (bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) {
+            P.bc.push_back(make_int4(b, f, bi.bc_id[f], bi.bc_type[f]));
+            if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
+              me.dir_mask |= 1 << f;
+              P.n_dirichlet++;
+            } else {
+              me.neu_mask |= 1 << f;
+              P.n_neumann++;
+            }
+            P.need_bvalues = std::max(P.need_bvalues, bi.bc_id[f] + 1);
+          }
+          int* d = &desc[size_t(b) * 48 + f * 8];
+          for (int q = 0; q < 4; ++q) d[1 + q] = bi.nb_id[f][q];
+          d[5] = bi.nb_subface[f];
+          d[6] = bi.bc_id[f];
+          if (ld == 0 && bi.nb_id[f][0] >= 0) d[0] = FD_COPY;
+          else if (ld == -1 && bi.nb_id[f][0] >= 0) d[0] = FD_C2F;
+          else if (ld == 1 && bi.nb_id[f][0] >= 0 && bi.nb_id[f][1] >= 0 && bi.nb_id[f][2] >= 0 &&
+                   bi.nb_id[f][3] >= 0) d[0] = FD_F2C;
+          else if (bi.bc_type[f] == BCM_BC_DIRICHLET) d[0] = FD_DIR;
+          else {
+            d[0] = FD_NONE;
+            P.halo_ok = false;
+          }
+          if (d[0] == FD_DIR || d[0] == FD_NONE)
+            for (int q = 0; q < 4; ++q)
+              if (bi.nb_id[f][q] >= 0) P.jhalo_ok = false;
+        }
+      }
+    });
     m->halo_ok = true;
     m->jhalo_ok = true;
-    for (int b = 0; b < nblocks; ++b) {
-      const bcm_block_info& bi = blocks[b];
-      for (int f = 0; f < 6; ++f) {
-        int* d = &desc[size_t(b) * 48 + f * 8];
-        const int ld = bi.nb_level_diff[f];
-        for (int s = 0; s < 4; ++s) d[1 + s] = bi.nb_id[f][s];
-        d[5] = bi.nb_subface[f];
-        d[6] = bi.bc_id[f];
-        if (ld == 0 && bi.nb_id[f][0] >= 0) d[0] = FD_COPY;
-        else if (ld == -1 && bi.nb_id[f][0] >= 0) d[0] = FD_C2F;
-        else if (ld == 1 && bi.nb_id[f][0] >= 0 && bi.nb_id[f][1] >= 0 && bi.nb_id[f][2] >= 0 &&
-                 bi.nb_id[f][3] >= 0) d[0] = FD_F2C;
-        else if (bi.bc_type[f] == BCM_BC_DIRICHLET) d[0] = FD_DIR;
-        else {
-          d[0] = FD_NONE;
-          m->halo_ok = false;
-        }
-        if (d[0] == FD_DIR || d[0] == FD_NONE)
-          for (int s = 0; s < 4; ++s)
-            if (bi.nb_id[f][s] >= 0) m->jhalo_ok = false;
-      }
+    std::vector<int4> bc;
+    for (const Part& P : parts) {
+      bc.insert(bc.end(), P.bc.begin(), P.bc.end());
+      m->n_dirichlet += P.n_dirichlet;
+      m->n_neumann += P.n_neumann;
+      m->need_bvalues = std::max(m->need_bvalues, P.need_bvalues);
+      m->halo_ok = m->halo_ok && P.halo_ok;
+      m->jhalo_ok = m->jhalo_ok && P.jhalo_ok;
     }
     if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_desc), sizeof(int) * desc.size())) != BCM_OK) break;
     if (hipMemcpy(m->d_desc, desc.data(), sizeof(int) * desc.size(), hipMemcpyHostToDevice) != hipSuccess) {
       rc = BCM_ERR_HIP;
       break;
     }
-    std::vector<int4> all;
+    std::vector<int4>& all = m->h_tasks;
+    all.clear();
     for (int a = 0; a < 3; ++a) {
       for (int k = 0; k < 3; ++k) {
         m->off[a][k] = int(all.size());
-        all.insert(all.end(), lists[a][k].begin(), lists[a][k].end());
-        m->kind_count[k] += int(lists[a][k].size());
+        for (const Part& P : parts) {
+          all.insert(all.end(), P.lists[a][k].begin(), P.lists[a][k].end());
+          m->kind_count[k] += int(P.lists[a][k].size());
+        }
       }
       m->off[a][3] = int(all.size());
     }
     m->ntask_total = int(all.size());
-    m->h_tasks = all;
     if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_tasks), sizeof(int4) * all.size())) != BCM_OK) break;
     if (!all.empty() &&
         hipMemcpy(m->d_tasks, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) != hipSuccess) {

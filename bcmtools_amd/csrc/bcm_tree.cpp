@@ -8,6 +8,7 @@
 // Utils/src/BCMOctree.cpp:462-467, and silently drops neighbours two
 // levels coarser, :441-474).
 #include "bcm_hip.h"
+#include "bcm_host_par.h"
 
 #include <algorithm>
 #include <atomic>
@@ -18,52 +19,21 @@
 #include <random>
 #include <system_error>
 #include <thread>
+#include <utility>
 #include <vector>
+
+namespace bcm {
+std::atomic<int> g_host_threads{0};
+}  // namespace bcm
 
 namespace {
 
-// Host threads of the builder (bcm_tree_set_threads; 0: one per hardware
-// thread, at most 64).
-std::atomic<int> g_threads{0};
+using bcm::chunks_for;
 
-int host_threads() {
-  const int want = g_threads.load();
-  const int t = want > 0 ? want : int(std::thread::hardware_concurrency());
-  return std::max(1, std::min(t, 64));
-}
-
-// f(begin, end) over [0, n) in one contiguous chunk per thread; serial when
-// n is below two grains.  A thread that cannot be started runs its chunk
-// on the calling thread afterwards.  The chunks partition [0, n), so the
-// outcome does not depend on the thread count as long as f writes only
-// its own items (or per-chunk outputs merged in chunk order).
+// Same-signature wrapper of bcm::parallel_chunks for the loops below.
 template <typename F>
-void parallel_chunks(size_t n, size_t grain, int nchunk, F&& f) {
-  if (nchunk <= 1) {
-    f(size_t(0), n, 0);
-    return;
-  }
-  const size_t per = (n + size_t(nchunk) - 1) / size_t(nchunk);
-  std::vector<std::thread> th;
-  std::vector<int> inline_chunks;
-  for (int c = 1; c < nchunk; ++c) {
-    const size_t b = size_t(c) * per, e = std::min(n, b + per);
-    if (b >= e) continue;
-    try {
-      th.emplace_back([&f, b, e, c] { f(b, e, c); });
-    } catch (const std::system_error&) {
-      inline_chunks.push_back(c);
-    }
-  }
-  f(size_t(0), std::min(n, per), 0);
-  for (auto& t : th) t.join();
-  for (int c : inline_chunks) f(size_t(c) * per, std::min(n, size_t(c) * per + per), c);
-  (void)grain;
-}
-
-int chunks_for(size_t n, size_t grain) {
-  if (n < 2 * grain) return 1;
-  return int(std::min<size_t>(size_t(host_threads()), n / grain));
+void parallel_chunks(size_t n, size_t /*grain*/, int nchunk, F&& f) {
+  bcm::parallel_chunks(n, nchunk, std::forward<F>(f));
 }
 
 // Pedigree word: x16 | y16 | z16 | root12 | level4 (Utils/include/Pedigree.h:42-73).
@@ -607,7 +577,7 @@ extern "C" int bcm_tree_build_user(const bcm_tree_params* params, bcm_divider_fn
 
 extern "C" int bcm_tree_set_threads(int nthreads) {
   if (nthreads < 0) return BCM_ERR_ARG;
-  g_threads.store(nthreads);
+  bcm::g_host_threads.store(nthreads);
   return BCM_OK;
 }
 

@@ -884,15 +884,18 @@ def bench_ghost(args):
 
 
 def bench_mesh(args):
-    """Mesh construction on the host (BCMOctree + BlockFactory, Utils/src/
-    BCMOctree.cpp:23-478, as bcm_tree_build) on one thread and on all host
-    threads, and the device tables of bcm_mesh_create (BlockManager::
-    prepareForVCUpdate's face lists, src/BlockManager.cpp:63-114, 471-648),
-    for the largest request of the served stream and two large shell meshes."""
+    """Mesh construction: the octree, the block table with its neighbour
+    information (BCMOctree + BlockFactory, Utils/src/BCMOctree.cpp:23-478,
+    as bcm_tree_build) on one host thread and on all of them, and the
+    device mesh (bcm_mesh_create: the face lists of BlockManager::
+    prepareForVCUpdate, src/BlockManager.cpp:63-114, 471-648, and the device
+    tables).  The reference's own construction (compiled in oracle/_ref)
+    is timed on the same SimpleDivider trees as the CPU baseline."""
     import ctypes
+    import subprocess
     import time
 
-    from bcmtools_amd import Mesh
+    from bcmtools_amd import Mesh, Tree
     from bcmtools_amd import _lib as L
     from bcmtools_amd import workloads as W
 
@@ -903,43 +906,73 @@ def bench_mesh(args):
         t0 = time.perf_counter()
         L.check(lib.bcm_tree_build(ctypes.byref(params), ctypes.byref(h)), "bcm_tree_build")
         dt = time.perf_counter() - t0
-        nb = lib.bcm_tree_num_blocks(h)
         lib.bcm_tree_destroy(h)
-        return dt, nb
+        return dt
 
+    def simple(level):
+        return Tree(root_n=(2, 2, 2), divider=L.DIV_SIMPLE, min_level=1, max_level=level, ordering=L.ORDER_Z,
+                    balance=0)
+
+    ref = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "ref_harness")
     threads = os.cpu_count() or 1
-    cases = (("serving_request", W.serving_tree(7000 + 7, 50000)), ("shell_level7", W.shell_tree(7)),
-             ("shell_level8", W.shell_tree(8)))
+    cases = (("serving_request", W.serving_tree(7000 + 7, 50000), None),
+             ("shell_level7", W.shell_tree(7), None),
+             ("simple_level6", simple(6), "simple 1 6 Z 2 2 2 dir 16 1"),
+             ("simple_level7", simple(7), "simple 1 7 Z 2 2 2 dir 16 1"))
     out = {}
-    for name, tree in cases:
+    for name, tree, ref_args in cases:
         row = {"blocks": len(tree.blocks)}
         for label, nt in (("one_thread", 1), ("all_threads", 0)):
             lib.bcm_tree_set_threads(nt)
             for _ in range(max(1, args.warmup)):
                 build_s(tree.params)
-            row[f"tree_ms_{label}"] = 1e3 * min(build_s(tree.params)[0] for _ in range(max(1, args.steps)))
+            row[f"tree_ms_{label}"] = 1e3 * min(build_s(tree.params) for _ in range(max(1, args.steps)))
         lib.bcm_tree_set_threads(0)
         row["speedup"] = row["tree_ms_one_thread"] / row["tree_ms_all_threads"]
-        if len(tree.blocks) <= 200000:
-            times = []
-            for _ in range(max(1, min(args.steps, 3))):
-                t0 = time.perf_counter()
-                m = Mesh(tree.blocks, 16, 1)
-                m.sync()
-                times.append(time.perf_counter() - t0)
-                m.close()
-            row["mesh_create_ms"] = 1e3 * min(times)
+        times = []
+        for _ in range(max(1, min(args.steps, 3))):
+            t0 = time.perf_counter()
+            m = Mesh(tree.blocks, 16, 1)
+            m.sync()
+            times.append(time.perf_counter() - t0)
+            m.close()
+        row["mesh_create_ms"] = 1e3 * min(times)
+        if ref_args and os.path.exists(ref) and not args.no_cpu:
+            # The reference's construction, one rank, best of three processes
+            # (its BlockManager is a process-wide singleton).
+            runs = []
+            for _ in range(3):
+                r = subprocess.run([ref, "meshtime"] + ref_args.split(), capture_output=True, text=True, timeout=300)
+                if r.returncode != 0:
+                    break
+                f = r.stdout.split()
+                runs.append({f[i]: float(f[i + 1]) for i in range(0, len(f) - 1, 2)})
+            if len(runs) == 3:
+                assert int(runs[0]["blocks"]) == len(tree.blocks)
+                best = {k: min(x[k] for x in runs) for k in ("octree_ms", "blocks_ms", "facelists_ms")}
+                row["reference"] = {**best, "tree_and_blocks_ms": best["octree_ms"] + best["blocks_ms"]}
+                row["speedup_tree_vs_reference"] = row["reference"]["tree_and_blocks_ms"] / row["tree_ms_all_threads"]
+                row["speedup_mesh_vs_reference"] = (row["reference"]["tree_and_blocks_ms"] + best["facelists_ms"]) / (
+                    row["tree_ms_all_threads"] + row["mesh_create_ms"])
         print(f"[bench] mesh {name}: {row}", file=sys.stderr, flush=True)
         out[name] = row
     first = out["serving_request"]
-    return {"metric": "host mesh build time (octree, 2:1 balance, block table)", "value": first["tree_ms_all_threads"],
-            "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": first["tree_ms_all_threads"], "higher_is_better": False, "scaling": "strong",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "bcm_tree_build of the served stream's largest request and of the level-7 and "
-                                   "level-8 shell meshes, on one host thread and on all of them",
-                       "host_threads": threads},
-            "meshes": out}
+    res = {"metric": "host mesh build time (octree, block table with neighbours)", "value": first["tree_ms_all_threads"],
+           "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": first["tree_ms_all_threads"], "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "bcm_tree_build of the served stream's largest request, the level-7 shell mesh and "
+                                  "two SimpleDivider meshes, on one host thread and on all of them; bcm_mesh_create",
+                      "host_threads": threads},
+           "meshes": out}
+    refrow = out["simple_level6"].get("reference")
+    if refrow:
+        res["cpu_baseline"] = {"value": refrow["tree_and_blocks_ms"], "unit": "ms", "cores": 1, "kind": "reference",
+                               "sample": "the reference's BCMOctree + BlockFactory::makeBlock loop on the SimpleDivider "
+                                         "tree of levels 1-6 (219,080 blocks), best of 3 processes; its "
+                                         "prepareForVCUpdate face lists took a further "
+                                         f"{refrow['facelists_ms']:.0f} ms"}
+    return res
 
 
 def bench_restart(args):

@@ -15,10 +15,15 @@
 //   ref_harness vtk   <out> <step> <difflevel> <mesh args>
 //       (VtkWriter::writeScalar of a double field into ./VTK; <out> gets
 //        the block table and the field)
+//   ref_harness meshtime <mesh args>
+//       (prints the wall time of the reference's mesh construction, one
+//        rank: the octree, the blocks with their neighbour information,
+//        and the face lists of prepareForVCUpdate)
 // mesh args: divider(flat|simple|sphere) minLevel maxLevel ordering(Z|Hilbert)
 //            rnx rny rnz bc(dir|x|y|z) n vc [cx cy cz r]
 #include "mpi.h"
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -293,6 +298,38 @@ int run_vtk(const char* out, int step, int difflevel, const MeshArgs& a) {
   return 0;
 }
 
+// Wall time of examples/sor1/main.cpp:76-123's mesh construction and of
+// the face lists of the first exchange (BlockManager::prepareForVCUpdate,
+// src/BlockManager.cpp:63-114), as build_mesh does it.
+int run_meshtime(const MeshArgs& a) {
+  using clk = std::chrono::steady_clock;
+  auto ms = [](clk::time_point x, clk::time_point y) { return std::chrono::duration<double, std::milli>(y - x).count(); };
+  const clk::time_point t0 = clk::now();
+  RootGrid* rootGrid = new RootGrid(a.rn[0], a.rn[1], a.rn[2]);
+  Divider* divider;
+  if (a.divider == "flat") divider = new FlatDivider(rootGrid, a.maxLevel);
+  else if (a.divider == "simple") divider = new SimpleDivider(rootGrid, a.minLevel, a.maxLevel);
+  else divider = new SphereDivider(rootGrid, a.minLevel, a.maxLevel, a.sph[0], a.sph[1], a.sph[2], a.sph[3]);
+  BCMOctree::Ordering ord = a.ordering == "Hilbert" ? BCMOctree::HILBERT : BCMOctree::Z;
+  BCMOctree* tree = new BCMOctree(rootGrid, divider, ord);
+  const clk::time_point t1 = clk::now();
+  Partition* partition = new Partition(1, tree->getNumLeafNode());
+  AllDirichletSetter setter;
+  Vec3i size(a.n, a.n, a.n);  // the factory keeps a reference to it
+  BlockFactory factory(tree, partition, &setter, size);
+  BlockManager& bm = BlockManager::getInstance();
+  std::vector<Node*>& leaves = tree->getLeafNodeArray();
+  for (int id = 0; id < tree->getNumLeafNode(); id++) bm.registerBlock(factory.makeBlock(leaves[id]));
+  bm.endRegisterBlock();
+  const clk::time_point t2 = clk::now();
+  const int id = bm.setDataClass<Scalar3D<double>, Scalar3DUpdater<double> >(a.vc);
+  bm.prepareForVCUpdate(id, 100, VCUpdateMethod::AtOnce);
+  const clk::time_point t3 = clk::now();
+  printf("blocks %d octree_ms %.3f blocks_ms %.3f facelists_ms %.3f\n", tree->getNumLeafNode(), ms(t0, t1), ms(t1, t2),
+         ms(t2, t3));
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -313,6 +350,8 @@ int main(int argc, char** argv) {
       rc = argv[4][0] == 'f' ? run_ghost<float>(argv[2], atoi(argv[3]), a) : run_ghost<double>(argv[2], atoi(argv[3]), a);
     } else if (cmd == "sor1" && argc >= 6) {
       rc = run_sor1(argv[2], argv[3], atoi(argv[4]), argv[5]);
+    } else if (cmd == "meshtime" && parse_mesh(argv + 2, argc - 2, a) == 0) {
+      rc = run_meshtime(a);
     } else if (cmd == "vtk" && argc >= 5 && parse_mesh(argv + 5, argc - 5, a) == 0) {
       rc = run_vtk(argv[2], atoi(argv[3]), atoi(argv[4]), a);
     }
