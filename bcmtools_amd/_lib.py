@@ -184,6 +184,8 @@ SIGNATURES = {
     "bcm_mesh_reset_timing": (_I, [_P]),
     "bcm_field_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "bcm_field_destroy": (_I, [_P]),
+    "bcm_host_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "bcm_host_free": (_I, [_P]),
     "bcm_field_upload": (_I, [_P, _P, ctypes.c_size_t]),
     "bcm_field_download": (_I, [_P, _P, ctypes.c_size_t]),
     "bcm_field_upload_blocks": (_I, [_P, _I, _I, _P, ctypes.c_size_t]),

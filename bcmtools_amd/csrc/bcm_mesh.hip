@@ -772,6 +772,23 @@ extern "C" int bcm_field_upload(bcm_field_t f, const void* host, size_t bytes) {
   return BCM_OK;
 }
 
+extern "C" int bcm_host_alloc(size_t bytes, void** out) {
+  if (!out) return BCM_ERR_ARG;
+  *out = nullptr;
+  if (bytes == 0) return BCM_ERR_ARG;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) return BCM_ERR_NOMEM;
+  std::memset(p, 0, bytes);
+  *out = p;
+  return BCM_OK;
+}
+
+extern "C" int bcm_host_free(void* p) {
+  if (!p) return BCM_OK;
+  BCM_HIP_TRY(hipHostFree(p));
+  return BCM_OK;
+}
+
 extern "C" int bcm_field_download(bcm_field_t f, void* host, size_t bytes) {
   if (!f || !host || bytes != f->bytes) return BCM_ERR_ARG;
   BCM_HIP_TRY(hipMemcpyAsync(host, f->data, bytes, hipMemcpyDeviceToHost, f->mesh->stream));

@@ -839,6 +839,38 @@ def ghost_fill_bytes(counts, n):
     return 8 * (copy * 2 * n * n + f2c * 9 * (n // 2) ** 2 + c2f * (n * n + 2 * (n // 2 + 1) ** 2))
 
 
+def host_exchange(mesh, f, args):
+    """The facade's host-resident updateVC (the driver's arena is
+    authoritative): upload the whole field, fill the ghosts, download it,
+    from ordinary (pageable) host memory and from page-locked memory
+    (bcm_host_alloc, what the facade allocates)."""
+    import ctypes
+    import time
+
+    from bcmtools_amd import _lib as L
+
+    lib = L.load()
+    nbytes = int(np.prod(f.shape)) * f.dtype.itemsize
+    out = {}
+    pinned = ctypes.c_void_p()
+    L.check(lib.bcm_host_alloc(nbytes, ctypes.byref(pinned)), "bcm_host_alloc")
+    try:
+        pageable = np.zeros(nbytes // 8)
+        for label, ptr in (("pageable", pageable.ctypes.data), ("pinned", pinned.value)):
+            times = []
+            for _ in range(max(1, args.warmup) + max(1, min(args.steps, 5))):
+                t0 = time.perf_counter()
+                L.check(lib.bcm_field_upload(f.handle, ptr, nbytes), "bcm_field_upload")
+                L.check(lib.bcm_ghost_fill(f.handle, L.GF_ALL), "bcm_ghost_fill")
+                L.check(lib.bcm_field_download(f.handle, ptr, nbytes), "bcm_field_download")
+                times.append(time.perf_counter() - t0)
+            out[label] = 1e3 * min(times[max(1, args.warmup):])
+    finally:
+        lib.bcm_host_free(pinned)
+    out["field_bytes"] = nbytes
+    return out
+
+
 def bench_ghost(args):
     """The reference's updateVC on its own (include/BlockManager.h:311-726,
     include/Scalar3DUpdater.h:190-347): the complete ghost fill of an FP64
@@ -870,6 +902,8 @@ def bench_ghost(args):
             mesh.set_timing(False)
             row[f"ms_{label}"] = ms / max(nl, 1)
         row["algorithmic_GBps_all"] = nbytes / (row["ms_all"] * 1e-3) / 1e9
+        if n == 32:
+            row["host_resident_exchange_ms"] = host_exchange(mesh, f, args)
         print(f"[bench] ghost fill {name}: {row}", file=sys.stderr, flush=True)
         out[name] = row
         mesh.close()

@@ -13,6 +13,8 @@
 //  * host-resident (default, drop-in): the host arena is authoritative;
 //    updateVC* uploads it, fills the face ghosts on the GPU and downloads
 //    it, so a driver that computes on getData() pointers runs unchanged.
+//    The arena is page-locked (bcm_host_alloc), so these copies run at the
+//    full PCIe rate.
 //  * device-resident (setDeviceResident(id, true)): the device field is
 //    authoritative, updateVC* is one kernel launch, and the driver calls
 //    the device kernels (deviceField(id) with the bcm_* entry points);
@@ -86,6 +88,7 @@ class BlockManager : public FaceExchanger {
   static void free_array(void* p) {
     delete[] static_cast<T*>(p);
   }
+  static void free_pinned(void* p) { bcm_host_free(p); }
 
   bcm_mesh_t mesh_for(int vc) {
     auto it = meshes.find(vc);
@@ -116,7 +119,17 @@ class BlockManager : public FaceExchanger {
     const int nc = D::ncomp;
     const int nx0 = size.x + 2 * vc, ny0 = size.y + 2 * vc, nz0 = size.z + 2 * vc;
     const size_t bvol = size_t(nx0) * ny0 * nz0 * nc;
-    T* data = new T[bvol * numBlock]();
+    // A class mirrored on the device gets a page-locked arena, so its
+    // whole-field copies run at the full PCIe rate; a host-only class an
+    // ordinary one.
+    T* data = nullptr;
+    if (updatable) {
+      void* p = nullptr;
+      check(bcm_host_alloc(sizeof(T) * bvol * size_t(numBlock), &p), "bcm_host_alloc");
+      data = static_cast<T*>(p);
+    } else {
+      data = new T[bvol * numBlock]();
+    }
     int id = -1;
     std::vector<VCUpdater*> updaters;
     for (int b = 0; b < numBlock; ++b) {
@@ -143,7 +156,7 @@ class BlockManager : public FaceExchanger {
     r.block_bytes = bvol * sizeof(T);
     r.bytes = r.block_bytes * numBlock;
     r.updatable = updatable;
-    r.free_host = &free_array<T>;
+    r.free_host = updatable ? &free_pinned : &free_array<T>;
     if (updatable) {
       if (nc == 1) check(bcm_field_create(mesh_for(vc), r.dtype, &r.dev), "bcm_field_create");
       else check(bcm_vfield_create(mesh_for(vc), r.dtype, &r.vdev), "bcm_vfield_create");

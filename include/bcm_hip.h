@@ -188,6 +188,13 @@ int bcm_field_create(bcm_mesh_t mesh, int dtype, bcm_field_t* out);
 int bcm_field_destroy(bcm_field_t field);
 /* Host <-> device copies of the whole field (nblocks*(n+2vc)^3 values). */
 int bcm_field_upload(bcm_field_t field, const void* host, size_t bytes);
+/* Page-locked host memory, zero-filled (no counterpart in the reference,
+ * whose fields live in ordinary host memory).  Copies between a field and
+ * such memory run at the full PCIe rate; from ordinary memory they are
+ * staged by the runtime.  The C++ facade allocates the host arena of every
+ * class it mirrors on the device with it.  bcm_host_free accepts NULL. */
+int bcm_host_alloc(size_t bytes, void** out);
+int bcm_host_free(void* ptr);
 int bcm_field_download(bcm_field_t field, void* host, size_t bytes);
 /* The same for blocks first_block .. first_block+nblocks-1 only (one
  * segment of a batch, a block range of one mesh); bytes must match. */
