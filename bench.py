@@ -908,13 +908,32 @@ def bench_ghost(args):
         out[name] = row
         mesh.close()
     first = out["shell3816x32"]
-    return {"metric": "ghost fill (updateVC) throughput, algorithmic bytes", "value": first["algorithmic_GBps_all"],
+    res_cpu = None
+    if not args.no_cpu:
+        # The oracle's fill (OpenMP over the face tasks, the reference's
+        # copyVCFromNeighbor loop) on the same field, a few repetitions.
+        import oracle
+        tree = W.config2_tree()
+        host = W.rhs_field_fast(tree.blocks, 32, 1, seed=3)
+        oracle.ghost_fill(tree.blocks, 32, 1, host)
+        reps, t0 = 3, time.perf_counter()
+        for _ in range(reps):
+            oracle.ghost_fill(tree.blocks, 32, 1, host)
+        dt = (time.perf_counter() - t0) / reps
+        res_cpu = {"value": first["algorithmic_bytes"] / dt / 1e9, "unit": "GB/s", "cores": os.cpu_count() or 1,
+                   "kind": "port", "ms": 1e3 * dt,
+                   "sample": f"the oracle's complete fill of the same 3,816 x 32^3 FP64 field, {reps} repetitions"}
+        print(f"[bench] ghost fill on the host (oracle): {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+    res = {"metric": "ghost fill (updateVC) throughput, algorithmic bytes", "value": first["algorithmic_GBps_all"],
             "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": first["ms_all"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic",
             "config": {"workload": "complete ghost fill of one FP64 field (one ghost layer) on the level-5 shell "
                                    "mesh (3,816 x 32^3) and the level-7 shell mesh (26,888 x 16^3)"},
             "meshes": out}
+    if res_cpu:
+        res["cpu_baseline"] = res_cpu
+    return res
 
 
 def bench_mesh(args):
