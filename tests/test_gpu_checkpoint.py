@@ -26,7 +26,7 @@ from oracle import restart as R
 
 pytestmark = pytest.mark.gpu
 
-N = 16
+N = 16  # block edge of the resume test
 
 
 def case(msg):
@@ -34,12 +34,12 @@ def case(msg):
     sys.stdout.flush()
 
 
-def oracle_relres(blocks, x, rhs):
+def oracle_relres(blocks, x, rhs, n=N):
     A = [np.zeros_like(rhs) for _ in range(7)]
     b = np.zeros_like(rhs)
-    oracle.poisson_assemble(blocks, N, 1, A, b, rhs, [0.0])
+    oracle.poisson_assemble(blocks, n, 1, A, b, rhs, [0.0])
     r = np.zeros_like(rhs)
-    oracle.calc_r(len(blocks), N, 1, r, A, x, b)
+    oracle.calc_r(len(blocks), n, 1, r, A, x, b)
     ri, bi = W.interior(r, 1), W.interior(b, 1)
     return float(np.sqrt(np.sum(ri * ri)) / np.sqrt(np.sum(bi * bi)))
 
@@ -53,39 +53,41 @@ def drive(sv, tags):
     return out
 
 
-def run(reqs, max_iter=20000, batch_blocks=0, restart=None):
-    sv = Server(64e9, max_active=4, n=N, check_every=4, max_iter=max_iter, batch_blocks=batch_blocks)
+def run(reqs, max_iter=20000, batch_blocks=0, restart=None, n=N, precision="fp64"):
+    sv = Server(64e9, max_active=4, n=n, check_every=4, max_iter=max_iter, batch_blocks=batch_blocks)
     for tag, blocks, rhs in reqs:
-        sv.submit(blocks, 1e-10, "fp64", rhs=rhs, tag=tag, want_x=True, restart=restart)
+        sv.submit(blocks, 1e-10, precision, rhs=rhs, tag=tag, want_x=True, restart=restart)
     out = drive(sv, [r[0] for r in reqs])
     sv.close()
     return out
 
 
-def read_ckpt(path, nblocks):
-    return R.load2(np.zeros((nblocks, N + 2, N + 2, N + 2)), N, 1, path)
+def read_ckpt(path, nblocks, n=N):
+    return R.load2(np.zeros((nblocks, n + 2, n + 2, n + 2)), n, 1, path)
 
 
-def requests():
+def requests(n=N):
     out = []
     for tag, seed, lv in ((1, 21, 3), (2, 24, 2)):
         blocks = W.random_tree(seed, max_level=lv).blocks
-        out.append((tag, blocks, -W.rhs_field(blocks, N, 1, seed=70 + tag)))
+        out.append((tag, blocks, -W.rhs_field(blocks, n, 1, seed=70 + tag)))
     return out
 
 
-def test_checkpoints_of_a_batched_group_are_the_iterates(tmp_path):
-    reqs = requests()
-    case(f"checkpoint: group of {[len(r[1]) for r in reqs]} blocks, FP64, checkpoints after 0/4/8/12 iterations")
-    plain = run(reqs)
+@pytest.mark.parametrize("precision,n", [("fp64", 16), ("mixed", 16), ("fp64", 32)])
+def test_checkpoints_of_a_batched_group_are_the_iterates(tmp_path, precision, n):
+    reqs = requests(n)
+    case(f"checkpoint: group of {[len(r[1]) for r in reqs]} blocks of {n}^3, {precision}, "
+         "checkpoints after 0/4/8/12 iterations")
+    plain = run(reqs, n=n, precision=precision)
     d = str(tmp_path / "BIN")
     # A small request holds the only slot while the two others queue, so
     # they are admitted together as one batched group when it finishes.
-    sv = Server(64e9, max_active=1, n=N, check_every=4, batch_blocks=4096)
+    sv = Server(64e9, max_active=1, n=n, check_every=4, batch_blocks=4096)
     blocker = W.random_tree(23, max_level=2).blocks
     sv.submit(blocker, 1e-6, "fp64", seed=5, tag=0)
     for tag, blocks, rhs in reqs:
-        sv.submit(blocks, 1e-10, "fp64", rhs=rhs, tag=tag, want_x=True)
+        sv.submit(blocks, 1e-10, precision, rhs=rhs, tag=tag, want_x=True)
     first = []
     while not first:
         sv.progress(0.0)  # one pass: at most one more block of iterations
@@ -106,17 +108,17 @@ def test_checkpoints_of_a_batched_group_are_the_iterates(tmp_path):
         assert done[tag]["iters"] == plain[tag]["iters"]
         np.testing.assert_array_equal(done[tag]["x"], plain[tag]["x"])
     # The checkpoint after 8 iterations is the iterate of a solve stopped there.
-    stopped = run(reqs, max_iter=8)
+    stopped = run(reqs, max_iter=8, n=n, precision=precision)
     for tag, blocks, rhs in reqs:
-        ck = read_ckpt(R.name2(d, f"x{tag}", 0, 2), len(blocks))
+        ck = read_ckpt(R.name2(d, f"x{tag}", 0, 2), len(blocks), n)
         assert stopped[tag]["iters"] == 8
         np.testing.assert_array_equal(ck, stopped[tag]["x"])
         # The stopped solve's true residual, recomputed by the oracle from the
         # checkpoint (the residual of Jacobi-BiCGSTAB first rises above 1).
-        rel = oracle_relres(blocks, ck, rhs)
+        rel = oracle_relres(blocks, ck, rhs, n)
         assert abs(rel - stopped[tag]["relres"]) <= 1e-9 * rel, (rel, stopped[tag]["relres"])
         # Before any iteration the iterate is the zero initial guess.
-        ck0 = read_ckpt(R.name2(d, f"x{tag}", 0, 0), len(blocks))
+        ck0 = read_ckpt(R.name2(d, f"x{tag}", 0, 0), len(blocks), n)
         assert not np.any(ck0)
 
 
