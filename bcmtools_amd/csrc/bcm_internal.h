@@ -230,11 +230,11 @@ int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg
 int krylov_begin(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& p, const double* seg_tol);
 int krylov_advance(bcm_mesh* m, bool wait, bool* active);
 int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats);
-// The current iterate of a running fused solve (x plus the pending FP64
-// correction on the interior, connected faces' ghosts gathered) into y,
-// one ghosted array of the whole mesh (device); waits for the iterations
-// already queued.  *iters: iterations segment seg has done.
-int krylov_snapshot(bcm_mesh* m, double* y, int seg, int* iters);
+// The current iterate of segment seg of a running fused solve (x plus the
+// pending FP64 correction on the interior, the segment's connected faces'
+// ghosts gathered) into y, the segment's ghosted blocks (device); waits
+// for the iterations already queued.  *iters: iterations done.
+int krylov_snapshot(bcm_mesh* m, int seg, double* y, int* iters);
 // Restart files (bcm_io.hip).  dump2_raw: the Dump2 file of nblk ghosted
 // blocks of the mesh's edge and ghost width held at dev (device memory,
 // esize bytes per value).  load2_header: checks a Dump2 file against edge

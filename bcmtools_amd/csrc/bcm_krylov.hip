@@ -146,19 +146,21 @@ __device__ __forceinline__ bool gated_out(const int* __restrict__ blk_seg, const
 // six faces from the neighbours' interiors (copy, F2C, C2F as
 // include/Scalar3DUpdater.h:190-266, 321-347, 519-708), the same values as
 // the per-face task kernel.  Dirichlet / unconnected faces are left as
-// they are.
+// they are.  `a` holds blocks first, first+1, ... (first = 0: the whole
+// mesh; one segment's blocks otherwise: a segment's faces connect only
+// blocks of that segment).
 // ---------------------------------------------------------------------
 template <typename T, int N, int GATE>
 __global__ __launch_bounds__(KWG) void k_gather(T* __restrict__ a, const int* __restrict__ desc,
                                                 const int* __restrict__ blk_seg, const SegState* __restrict__ st,
-    const int* __restrict__ blist) {
+    const int* __restrict__ blist, int first) {
   using B = Blk<N>;
-  const int b = blist ? blist[blockIdx.x] : int(blockIdx.x);
+  const int b = blist ? blist[blockIdx.x] : first + int(blockIdx.x);
   if (gated_out<GATE>(blk_seg, st, b)) return;
   __shared__ int d[48];
   if (threadIdx.x < 48) d[threadIdx.x] = desc[b * 48 + threadIdx.x];
   __syncthreads();
-  T* __restrict__ dst = a + b * B::GV;
+  T* __restrict__ dst = a + (long long)(b - first) * B::GV;
   constexpr int h = N / 2;
   for (int e = threadIdx.x; e < 6 * N * N; e += KWG) {
     const int f = e / (N * N), uv = e % (N * N), u = uv % N, v = uv / N;
@@ -174,13 +176,13 @@ __global__ __launch_bounds__(KWG) void k_gather(T* __restrict__ a, const int* __
     c[t1] = v;
     T val;
     if (kind == FD_COPY) {
-      const T* __restrict__ src = a + df[1] * B::GV;
+      const T* __restrict__ src = a + (long long)(df[1] - first) * B::GV;
       int sc[3] = {c[0], c[1], c[2]};
       sc[ax] = plus ? 0 : N - 1;
       val = src[B::g(sc[0], sc[1], sc[2])];
     } else if (kind == FD_F2C) {
       const int s0 = u >= h, s1 = v >= h;
-      const T* __restrict__ src = a + df[1 + s0 + 2 * s1] * B::GV;
+      const T* __restrict__ src = a + (long long)(df[1 + s0 + 2 * s1] - first) * B::GV;
       int I[3];
       I[ax] = plus ? 0 : h - 1;
       I[t0] = u - h * s0;
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(KWG) void k_gather(T* __restrict__ a, const int* __
       val = f2c_sum<T>([&](int i, int j, int k) { return src[B::g(i, j, k)]; }, I[0], I[1], I[2]);
     } else {
       const int sf = df[5];
-      const T* __restrict__ src = a + df[1] * B::GV;
+      const T* __restrict__ src = a + (long long)(df[1] - first) * B::GV;
       int p[3];
       p[ax] = plus ? 0 : 2 * N - 1;
       p[t0] = u + N * (sf & 1);
@@ -1244,14 +1246,14 @@ static int gather(bcm_mesh* m, T* a, int gate) {
   const SegState* st = m->kry ? m->kry->st : nullptr;
   if (gate == 0) {
     hipLaunchKernelGGL((k_gather<T, N, 0>), dim3(m->nblocks), blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st,
-                       (const int*)nullptr);
+                       (const int*)nullptr, 0);
   } else {
     const dim3 grid(m->kry->n_list);
     const int* bl = m->kry->list;
     if (gate == 1)
-      hipLaunchKernelGGL((k_gather<T, N, 1>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st, bl);
+      hipLaunchKernelGGL((k_gather<T, N, 1>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st, bl, 0);
     else
-      hipLaunchKernelGGL((k_gather<T, N, 2>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st, bl);
+      hipLaunchKernelGGL((k_gather<T, N, 2>), grid, blk, 0, m->stream, a, m->d_desc, m->d_blk_seg, st, bl, 0);
   }
   BCM_HIP_TRY(hipGetLastError());
   return BCM_OK;
@@ -1665,18 +1667,19 @@ int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats) {
   }
 }
 
-// The current iterate of a running solve: y = x + dx on the interior of
-// every block (the sum the next reliable update forms), x's stored ghost
-// cells elsewhere; the connected faces' ghosts are then gathered, as the
-// end of a solve leaves them.
+// The current iterate of a running solve, one segment: y = x + dx on the
+// interior of its blocks (the sum the next reliable update forms), x's
+// stored ghost cells elsewhere; then the segment's connected faces' ghosts
+// are gathered, as the end of a solve leaves them.  y holds the segment's
+// blocks only.
 template <int N>
 __global__ __launch_bounds__(KWG) void k_cur_x(double* __restrict__ y, const double* __restrict__ x,
-                                               const double* __restrict__ dx) {
+                                               const double* __restrict__ dx, int first) {
   using B = Blk<N>;
-  const long long b = blockIdx.x;
+  const long long b = first + (long long)blockIdx.x;
   const double* xb = x + b * B::GV;
   const double* db = dx + b * B::N3;
-  double* yb = y + b * B::GV;
+  double* yb = y + (long long)blockIdx.x * B::GV;
   for (int e = threadIdx.x; e < B::GV; e += KWG) {
     const int i = e % B::NX - 1, j = (e / B::NX) % B::NX - 1, k = e / (B::NX * B::NX) - 1;
     const bool in = i >= 0 && i < N && j >= 0 && j < N && k >= 0 && k < N;
@@ -1685,22 +1688,26 @@ __global__ __launch_bounds__(KWG) void k_cur_x(double* __restrict__ y, const dou
 }
 
 template <int N>
-static int snapshot_impl(bcm_mesh* m, double* y) {
+static int snapshot_impl(bcm_mesh* m, double* y, int first, int nb) {
   Krylov* K = m->kry;
-  hipLaunchKernelGGL(k_cur_x<N>, dim3(unsigned(m->nblocks)), dim3(KWG), 0, m->stream, y, K->run.x, K->dx);
+  hipLaunchKernelGGL(k_cur_x<N>, dim3(unsigned(nb)), dim3(KWG), 0, m->stream, y, K->run.x, K->dx, first);
   BCM_HIP_TRY(hipGetLastError());
-  return gather<double, N>(m, y, 0);
+  hipLaunchKernelGGL((k_gather<double, N, 0>), dim3(unsigned(nb)), dim3(KWG), 0, m->stream, y, m->d_desc,
+                     m->d_blk_seg, K->st, (const int*)nullptr, first);
+  BCM_HIP_TRY(hipGetLastError());
+  return BCM_OK;
 }
 
-int krylov_snapshot(bcm_mesh* m, double* y, int seg, int* iters) {
+int krylov_snapshot(bcm_mesh* m, int seg, double* y, int* iters) {
   Krylov* K = m->kry;
   if (!K || !K->ev || !K->dx || !K->run.x || seg < 0 || seg >= m->nseg) return BCM_ERR_STATE;
+  const int first = m->seg_first[size_t(seg)], nb = m->seg_first[size_t(seg) + 1] - first;
   // Iterations already queued finish first, so x and dx belong together.
   BCM_HIP_TRY(hipStreamSynchronize(m->stream));
   switch (m->n) {
-    case 8: BCM_TRY(snapshot_impl<8>(m, y)); break;
-    case 16: BCM_TRY(snapshot_impl<16>(m, y)); break;
-    case 32: BCM_TRY(snapshot_impl<32>(m, y)); break;
+    case 8: BCM_TRY(snapshot_impl<8>(m, y, first, nb)); break;
+    case 16: BCM_TRY(snapshot_impl<16>(m, y, first, nb)); break;
+    case 32: BCM_TRY(snapshot_impl<32>(m, y, first, nb)); break;
     default: return BCM_ERR_ARG;
   }
   BCM_HIP_TRY(hipStreamSynchronize(m->stream));

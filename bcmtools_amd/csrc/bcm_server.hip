@@ -389,10 +389,9 @@ extern "C" int bcm_server_checkpoint(bcm_server_t sv, int64_t tag, const char* d
       bcm_mesh* m = g->mesh;
       const int nb = int(g->reqs[s]->blocks.size());
       void* y = nullptr;
-      BCM_TRY(dev_alloc(&y, sizeof(double) * m->bvol * size_t(m->nblocks)));
-      int rc = krylov_snapshot(m, static_cast<double*>(y), int(s), iters);
-      if (rc == BCM_OK)
-        rc = dump2_raw(m, static_cast<double*>(y) + m->bvol * size_t(g->first[s]), nb, 8, dir, label, rank, step);
+      BCM_TRY(dev_alloc(&y, sizeof(double) * m->bvol * size_t(nb)));
+      int rc = krylov_snapshot(m, int(s), static_cast<double*>(y), iters);
+      if (rc == BCM_OK) rc = dump2_raw(m, y, nb, 8, dir, label, rank, step);
       (void)hipStreamSynchronize(m->stream);
       dev_free(y);
       return rc;
