@@ -135,6 +135,8 @@ class ServerParams(ctypes.Structure):
         ("omega", ctypes.c_double),
         ("check_every", ctypes.c_int32),
         ("batch_blocks", ctypes.c_int32),
+        ("window_blocks", ctypes.c_int32),
+        ("max_wait", ctypes.c_double),
     ]
 
 

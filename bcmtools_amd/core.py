@@ -512,12 +512,15 @@ class Server:
     PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED, "fp32": L.PREC_FP32}
 
     def __init__(self, hbm_budget, max_active=8, n=16, max_iter=20000, count_pre=2, omega=0.8, check_every=8,
-                 batch_blocks=0):
+                 batch_blocks=0, window_blocks=0, max_wait=0.0):
         """batch_blocks > 0 merges requests of at most that many blocks into
-        batched groups of at most that many blocks (0: one request per group)."""
+        batched groups of at most that many blocks (0: one request per group).
+        window_blocks > 0 launches the smallest groups' iterations first,
+        within that many blocks in flight (max_wait > 0: a group waits at
+        most that many seconds); 0: every group as soon as it is ready."""
         self.lib = L.load()
         p = L.ServerParams(float(hbm_budget), int(max_active), int(n), int(max_iter), int(count_pre), float(omega),
-                           int(check_every), int(batch_blocks))
+                           int(check_every), int(batch_blocks), int(window_blocks), float(max_wait))
         h = ctypes.c_void_p()
         L.check(self.lib.bcm_server_create(ctypes.byref(p), ctypes.byref(h)), "bcm_server_create")
         self.handle = h

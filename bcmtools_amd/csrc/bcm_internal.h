@@ -230,6 +230,9 @@ int krylov_check(const bcm_mesh* m, const bcm_solve_params* p, const double* seg
 int krylov_begin(bcm_mesh* m, double* x, const double* rhs, const bcm_solve_params& p, const double* seg_tol);
 int krylov_advance(bcm_mesh* m, bool wait, bool* active);
 int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats);
+// Without waiting: *state = 0 while the last block of iterations runs, 1
+// when it is done and another block is due, 2 when the solve is over.
+int krylov_state(bcm_mesh* m, int* state);
 // The current iterate of segment seg of a running fused solve (x plus the
 // pending FP64 correction on the interior, the segment's connected faces'
 // ghosts gathered) into y, the segment's ghosted blocks (device); waits

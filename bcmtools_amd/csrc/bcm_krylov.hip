@@ -1658,6 +1658,19 @@ int krylov_advance(bcm_mesh* m, bool wait, bool* active) {
   }
 }
 
+int krylov_state(bcm_mesh* m, int* state) {
+  Krylov* K = m->kry;
+  if (!K || !K->ev) return BCM_ERR_STATE;
+  const hipError_t q = hipEventQuery(K->ev);
+  if (q == hipErrorNotReady) {
+    *state = 0;
+    return BCM_OK;
+  }
+  BCM_HIP_TRY(q);
+  *state = (!any_active(m) || K->run.blocks >= K->run.max_blocks) ? 2 : 1;
+  return BCM_OK;
+}
+
 int krylov_finish(bcm_mesh* m, bcm_solve_stats* stats) {
   switch (m->n) {
     case 8: return finish_impl<8>(m, stats);

@@ -22,6 +22,7 @@
 // The reference has no counterpart (one BlockManager per process,
 // include/BlockManager.h:153-156); a request's solve is bcm_solve's fused
 // BiCGSTAB (examples/SolverTest/ILS3D.h:663-844).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -101,6 +102,7 @@ struct Group {
   int nblocks = 0;
   bcm_mesh_t mesh = nullptr;
   bcm_field_t x = nullptr, rhs = nullptr;
+  double ready_since = -1.0;  // when its next block of iterations became due and was held back
 };
 
 }  // namespace
@@ -290,6 +292,74 @@ void admit_ready(bcm_server* sv) {
   }
 }
 
+// Ends the solve of active group i (over, or failed with rc).
+void finish_group(bcm_server* sv, size_t i, int rc) {
+  Group* g = sv->active[i];
+  if (rc == BCM_OK) rc = complete(sv, g);
+  if (rc != BCM_OK)
+    for (Request* q : g->reqs) finish_result(sv, q, -rc, nullptr);
+  sv->active.erase(sv->active.begin() + long(i));
+  release(sv, g);
+}
+
+// One pass, window_blocks = 0: every group whose previous block of
+// iterations is done gets its next one.  Returns whether a group finished.
+bool pass_all(bcm_server* sv) {
+  bool moved = false;
+  for (size_t i = 0; i < sv->active.size();) {
+    bool act = true;
+    const int rc = krylov_advance(sv->active[i]->mesh, false, &act);
+    if (rc == BCM_OK && act) {
+      ++i;
+      continue;
+    }
+    moved = true;
+    finish_group(sv, i, rc);
+  }
+  return moved;
+}
+
+// One pass, window_blocks > 0: the groups whose previous block is done are
+// served smallest first, each while the blocks in flight stay within the
+// window (or nothing is in flight, or it has waited max_wait seconds).
+bool pass_windowed(bcm_server* sv) {
+  bool moved = false;
+  const double now = sv->now();
+  long long inflight = 0;
+  std::vector<Group*> ready;
+  for (size_t i = 0; i < sv->active.size();) {
+    Group* g = sv->active[i];
+    int state = 0;
+    const int rc = krylov_state(g->mesh, &state);
+    if (rc == BCM_OK && state == 0) {
+      inflight += g->nblocks;
+      ++i;
+    } else if (rc == BCM_OK && state == 1) {
+      ready.push_back(g);
+      ++i;
+    } else {
+      moved = true;
+      finish_group(sv, i, rc);
+    }
+  }
+  std::stable_sort(ready.begin(), ready.end(), [](const Group* a, const Group* b) { return a->nblocks < b->nblocks; });
+  for (Group* g : ready) {
+    if (g->ready_since < 0.0) g->ready_since = now;
+    const bool aged = sv->p.max_wait > 0.0 && now - g->ready_since >= sv->p.max_wait;
+    if (!(inflight == 0 || inflight + g->nblocks <= sv->p.window_blocks || aged)) continue;
+    bool act = true;
+    const int rc = krylov_advance(g->mesh, false, &act);
+    if (rc == BCM_OK && act) {
+      inflight += g->nblocks;
+      g->ready_since = -1.0;
+      continue;
+    }
+    moved = true;
+    finish_group(sv, size_t(std::find(sv->active.begin(), sv->active.end(), g) - sv->active.begin()), rc);
+  }
+  return moved;
+}
+
 }  // namespace
 }  // namespace bcm
 
@@ -305,7 +375,8 @@ extern "C" int bcm_server_create(const bcm_server_params* p, bcm_server_t* out) 
   if (!p || !out) return BCM_ERR_ARG;
   *out = nullptr;
   if (!(p->hbm_budget > 0.0) || p->max_active < 1 || (p->n != 8 && p->n != 16 && p->n != 32) || p->max_iter < 1 ||
-      p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0 || p->batch_blocks < 0)
+      p->count_pre < 0 || p->count_pre > 16 || p->check_every < 0 || p->batch_blocks < 0 || p->window_blocks < 0 ||
+      !(p->max_wait >= 0.0))
     return BCM_ERR_ARG;
   bcm_server* sv = new (std::nothrow) bcm_server();
   if (!sv) return BCM_ERR_NOMEM;
@@ -403,22 +474,7 @@ extern "C" int bcm_server_progress(bcm_server_t sv, double max_seconds, int* n_a
   if (!sv || !(max_seconds >= 0.0)) return BCM_ERR_ARG;
   const double t_end = sv->now() + max_seconds;
   do {
-    bool moved = false;
-    for (size_t i = 0; i < sv->active.size();) {
-      Group* g = sv->active[i];
-      bool act = true;
-      int rc = krylov_advance(g->mesh, false, &act);
-      if (rc == BCM_OK && act) {
-        ++i;
-        continue;
-      }
-      moved = true;
-      if (rc == BCM_OK) rc = complete(sv, g);
-      if (rc != BCM_OK)
-        for (Request* q : g->reqs) finish_result(sv, q, -rc, nullptr);
-      sv->active.erase(sv->active.begin() + long(i));
-      release(sv, g);
-    }
+    const bool moved = sv->p.window_blocks > 0 ? pass_windowed(sv) : pass_all(sv);
     if (moved) admit_ready(sv);
     if (sv->active.empty() && sv->queue.empty()) break;
     if (!moved) std::this_thread::sleep_for(std::chrono::microseconds(20));

@@ -643,7 +643,8 @@ def serve_stream(args, lam, trees, tols, gaps):
     R, n = len(trees), 16
     arrivals = np.cumsum(gaps / lam)
     sv = Server(200e9, max_active=args.slots, n=n, max_iter=20000, check_every=8,
-                batch_blocks=getattr(args, "batch_blocks", 0))
+                batch_blocks=getattr(args, "batch_blocks", 0), window_blocks=getattr(args, "window_blocks", 0),
+                max_wait=getattr(args, "max_wait", 0.0))
     done, k = {}, 0
     t0 = sv.clock()
     last_print = t0
@@ -726,8 +727,13 @@ def bench_serving(args):
                                + f", up to {args.slots} groups on the device, HBM budget 200 GB"
                                + (f", requests of at most {args.batch_blocks} blocks merged into batched groups of "
                                   f"at most {args.batch_blocks} blocks" if getattr(args, "batch_blocks", 0) else
-                                  ", one request per group"),
+                                  ", one request per group")
+                               + (f", smallest groups first within {args.window_blocks} blocks in flight "
+                                  f"(a group held back at most {args.max_wait:g} s)"
+                                  if getattr(args, "window_blocks", 0) else ", every group as soon as it is ready"),
                    "batch_blocks": int(getattr(args, "batch_blocks", 0)),
+                   "window_blocks": int(getattr(args, "window_blocks", 0)),
+                   "max_wait_s": float(getattr(args, "max_wait", 0.0)),
                    "precision_policy": args.policy,
                    "requests": R, "lambda_per_s": lams[0], "blocks_total": int(sum(counts)),
                    "blocks_per_request": [int(min(counts)), int(np.median(counts)), int(max(counts))],
@@ -1132,6 +1138,12 @@ def main():
     ap.add_argument("--batch-blocks", type=int, default=0,
                     help="serving: merge requests of at most this many blocks into batched groups of at most "
                          "this many blocks (0: every request on its own mesh)")
+    ap.add_argument("--window-blocks", type=int, default=8192,
+                    help="serving: launch the smallest groups' iterations first, within this many blocks in flight "
+                         "(0: every group as soon as it is ready)")
+    ap.add_argument("--max-wait", type=float, default=5.0,
+                    help="serving: with --window-blocks, the longest a group's iterations are held back (s; 0: no "
+                         "limit)")
     ap.add_argument("--meshes", type=int, default=256, help="batched: number of meshes")
     ap.add_argument("--precision", default="auto", choices=["auto", "mixed", "fp64"],
                     help="batched: precision of the full batch (auto: the faster on the subset); "

@@ -635,6 +635,15 @@ typedef struct bcm_server_params {
   double omega;               /* Jacobi weight */
   int32_t check_every;        /* iterations per graph replay (0: 8) */
   int32_t batch_blocks;       /* merge small requests into groups of at most this many blocks (0: off) */
+  /* Launch order of the groups' blocks of iterations.  window_blocks = 0:
+   * every group's next block as soon as its previous one is done.
+   * window_blocks > 0: smallest groups first -- a ready group launches
+   * while the blocks of the groups with work in flight, its own included,
+   * stay within window_blocks, or when nothing is in flight, or once it
+   * has waited max_wait seconds (max_wait <= 0: no such limit); small
+   * requests then do not queue behind the kernels of large ones. */
+  int32_t window_blocks;
+  double max_wait;
 } bcm_server_params;
 typedef struct bcm_server_result {
   int64_t tag;

@@ -135,7 +135,7 @@ def test_server_checkpoint_and_restart_checks_without_gpu(tmp_path):
     checks touches the device."""
     from oracle import restart as R
     lib = L.load()
-    p = L.ServerParams(1e9, 1, 16, 100, 2, 0.8, 4, 0)
+    p = L.ServerParams(1e9, 1, 16, 100, 2, 0.8, 4, 0, 0, 0.0)
     h = ctypes.c_void_p()
     assert lib.bcm_server_create(ctypes.byref(p), ctypes.byref(h)) == L.OK
     blocks = np.zeros(3, dtype=L.BLOCK_DTYPE)

@@ -38,8 +38,9 @@ def _requests():
     return reqs
 
 
-def _run(reqs, max_active, budget=64e9, batch_blocks=0):
-    sv = Server(budget, max_active=max_active, n=16, check_every=4, batch_blocks=batch_blocks)
+def _run(reqs, max_active, budget=64e9, batch_blocks=0, window_blocks=0, max_wait=0.0):
+    sv = Server(budget, max_active=max_active, n=16, check_every=4, batch_blocks=batch_blocks,
+                window_blocks=window_blocks, max_wait=max_wait)
     for tag, blocks, tol, prec, rhs in reqs:
         sv.submit(blocks, tol, prec, rhs=rhs, tag=tag, want_x=True)
     out = {}
@@ -52,10 +53,15 @@ def _run(reqs, max_active, budget=64e9, batch_blocks=0):
     return out
 
 
-def test_server_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("window_blocks,max_wait", [(0, 0.0), (64, 0.0), (64, 0.001)])
+def test_server_results_same_alone_and_in_the_mix(window_blocks, max_wait):
+    """Every launch order (each group as soon as it is ready; smallest
+    groups first within a window of blocks in flight, with and without a
+    wait limit) gives each request its result alone."""
     reqs = _requests()
-    case(f"server mix of {len(reqs)} requests, blocks {[len(r[1]) for r in reqs]}")
-    mixed = _run(reqs, max_active=6)
+    case(f"server mix of {len(reqs)} requests, blocks {[len(r[1]) for r in reqs]}, window {window_blocks}, "
+         f"max_wait {max_wait}")
+    mixed = _run(reqs, max_active=6, window_blocks=window_blocks, max_wait=max_wait)
     for req in reqs:
         tag, blocks, tol, prec, rhs = req
         alone = _run([req], max_active=1)[tag]
