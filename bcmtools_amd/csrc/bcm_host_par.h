@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstddef>
+#include <exception>
 #include <system_error>
 #include <thread>
 #include <vector>
@@ -33,6 +34,8 @@ inline int chunks_for(size_t n, size_t grain) {
 // the calling thread afterwards.  The chunks partition [0, n), so the
 // outcome does not depend on the chunk count as long as f writes only its
 // own items (or per-chunk outputs that the caller merges in chunk order).
+// An exception thrown by f in any chunk (std::bad_alloc, say) is rethrown
+// on the calling thread once every chunk has ended.
 template <typename F>
 void parallel_chunks(size_t n, int nchunk, F&& f) {
   if (nchunk <= 1) {
@@ -40,20 +43,30 @@ void parallel_chunks(size_t n, int nchunk, F&& f) {
     return;
   }
   const size_t per = (n + size_t(nchunk) - 1) / size_t(nchunk);
+  std::vector<std::exception_ptr> err(static_cast<size_t>(nchunk));
+  auto run = [&f, &err](size_t b, size_t e, int c) {
+    try {
+      f(b, e, c);
+    } catch (...) {
+      err[size_t(c)] = std::current_exception();
+    }
+  };
   std::vector<std::thread> th;
   std::vector<int> inline_chunks;
   for (int c = 1; c < nchunk; ++c) {
     const size_t b = size_t(c) * per, e = std::min(n, b + per);
     if (b >= e) continue;
     try {
-      th.emplace_back([&f, b, e, c] { f(b, e, c); });
-    } catch (const std::system_error&) {
+      th.emplace_back(run, b, e, c);
+    } catch (...) {
       inline_chunks.push_back(c);
     }
   }
-  f(size_t(0), std::min(n, per), 0);
+  run(size_t(0), std::min(n, per), 0);
   for (auto& t : th) t.join();
-  for (int c : inline_chunks) f(size_t(c) * per, std::min(n, size_t(c) * per + per), c);
+  for (int c : inline_chunks) run(size_t(c) * per, std::min(n, size_t(c) * per + per), c);
+  for (const std::exception_ptr& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 }  // namespace bcm

@@ -464,164 +464,168 @@ static int mesh_create(const bcm_block_info* blocks, int nblocks, int n, int vc,
   m->nx0 = n + 2 * vc;
   m->bvol = size_t(m->nx0) * m->nx0 * m->nx0;
   m->ncell = size_t(nblocks) * n * n * n;
-  m->blocks.assign(blocks, blocks + nblocks);
   int rc = BCM_OK;
-  do {
-    if (hipGetDevice(&m->device) != hipSuccess) { rc = BCM_ERR_HIP; break; }
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { rc = BCM_ERR_HIP; break; }
-    // Face tasks, as BlockManager::setFaceLists (src/BlockManager.cpp:471-530)
-    // walks them: per block, per face, per subface.  Chunks of blocks build
-    // their own lists on the host threads, merged in chunk order, so every
-    // list is in block order whatever the chunk count.
-    struct Part {
-      std::vector<int4> lists[3][3];
-      std::vector<int4> bc;
-      int n_dirichlet = 0, n_neumann = 0, need_bvalues = 0;
-      bool halo_ok = true, jhalo_ok = true;
-    };
-    std::vector<Part> parts(static_cast<size_t>(nchunk));
-    m->meta.resize(nblocks);
-    // Face descriptors of the halo-gathering sweep.
-    std::vector<int> desc(size_t(nblocks) * 48, 0);
-    parallel_chunks(size_t(nblocks), nchunk, [&](size_t b0, size_t b1, int c) {
-      Part& P = parts[size_t(c)];
-      for (size_t bb = b0; bb < b1; ++bb) {
-        const int b = int(bb);
-        const bcm_block_info& bi = blocks[b];
-        BlockMeta& me = m->meta[b];
-        const double hx = bi.block_size[0] / n, hy = bi.block_size[1] / n, hz = bi.block_size[2] / n;
-        me.cx = 1.0 / (hx * hx);
-        me.cy = 1.0 / (hy * hy);
-        me.cz = 1.0 / (hz * hz);
-        me.hx = hx;
-        me.dir_mask = 0;
-        me.neu_mask = 0;
-        for (int f = 0; f < 6; ++f) {
-          me.bc_id[f] = bi.bc_id[f];
-          const int axis = f / 2;
-          const int ld = bi.nb_level_diff[f];
-          if (ld == 0 || ld == -1) {
-            const int nb = bi.nb_id[f][0];
-            if (nb >= 0) {
-              const int kind = ld == 0 ? KIND_COPY : KIND_C2F;
-              const int sub = ld == 0 ? 0 : bi.nb_subface[f];
-              P.lists[axis][kind].push_back(make_int4(b, nb, f | (sub << 3) | (kind << 5), 0));
-            }
-          } else {
-            for (int q = 0; q < 4; ++q) {
-              const int nb = bi.nb_id[f][q];
-              if (nb >= 0) P.lists[axis][KIND_F2C].push_back(make_int4(b, nb, f | (q << 3) | (KIND_F2C << 5), 0));
-            }
-          }
-          if (bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) {
-            P.bc.push_back(make_int4(b, f, bi.bc_id[f], bi.bc_type[f]));
-            if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
-              me.dir_mask |= 1 << f;
-              P.n_dirichlet++;
+  try {
+    m->blocks.assign(blocks, blocks + nblocks);
+    do {
+      if (hipGetDevice(&m->device) != hipSuccess) { rc = BCM_ERR_HIP; break; }
+      if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { rc = BCM_ERR_HIP; break; }
+      // Face tasks, as BlockManager::setFaceLists (src/BlockManager.cpp:471-530)
+      // walks them: per block, per face, per subface.  Chunks of blocks build
+      // their own lists on the host threads, merged in chunk order, so every
+      // list is in block order whatever the chunk count.
+      struct Part {
+        std::vector<int4> lists[3][3];
+        std::vector<int4> bc;
+        int n_dirichlet = 0, n_neumann = 0, need_bvalues = 0;
+        bool halo_ok = true, jhalo_ok = true;
+      };
+      std::vector<Part> parts(static_cast<size_t>(nchunk));
+      m->meta.resize(nblocks);
+      // Face descriptors of the halo-gathering sweep.
+      std::vector<int> desc(size_t(nblocks) * 48, 0);
+      parallel_chunks(size_t(nblocks), nchunk, [&](size_t b0, size_t b1, int c) {
+        Part& P = parts[size_t(c)];
+        for (size_t bb = b0; bb < b1; ++bb) {
+          const int b = int(bb);
+          const bcm_block_info& bi = blocks[b];
+          BlockMeta& me = m->meta[b];
+          const double hx = bi.block_size[0] / n, hy = bi.block_size[1] / n, hz = bi.block_size[2] / n;
+          me.cx = 1.0 / (hx * hx);
+          me.cy = 1.0 / (hy * hy);
+          me.cz = 1.0 / (hz * hz);
+          me.hx = hx;
+          me.dir_mask = 0;
+          me.neu_mask = 0;
+          for (int f = 0; f < 6; ++f) {
+            me.bc_id[f] = bi.bc_id[f];
+            const int axis = f / 2;
+            const int ld = bi.nb_level_diff[f];
+            if (ld == 0 || ld == -1) {
+              const int nb = bi.nb_id[f][0];
+              if (nb >= 0) {
+                const int kind = ld == 0 ? KIND_COPY : KIND_C2F;
+                const int sub = ld == 0 ? 0 : bi.nb_subface[f];
+                P.lists[axis][kind].push_back(make_int4(b, nb, f | (sub << 3) | (kind << 5), 0));
+              }
             } else {
-              me.neu_mask |= 1 << f;
-              P.n_neumann++;
+              for (int q = 0; q < 4; ++q) {
+                const int nb = bi.nb_id[f][q];
+                if (nb >= 0) P.lists[axis][KIND_F2C].push_back(make_int4(b, nb, f | (q << 3) | (KIND_F2C << 5), 0));
+              }
             }
-            P.need_bvalues = std::max(P.need_bvalues, bi.bc_id[f] + 1);
+            if (bi.bc_type[f] == BCM_BC_DIRICHLET || bi.bc_type[f] == BCM_BC_NEUMANN) {
+              P.bc.push_back(make_int4(b, f, bi.bc_id[f], bi.bc_type[f]));
+              if (bi.bc_type[f] == BCM_BC_DIRICHLET) {
+                me.dir_mask |= 1 << f;
+                P.n_dirichlet++;
+              } else {
+                me.neu_mask |= 1 << f;
+                P.n_neumann++;
+              }
+              P.need_bvalues = std::max(P.need_bvalues, bi.bc_id[f] + 1);
+            }
+            int* d = &desc[size_t(b) * 48 + f * 8];
+            for (int q = 0; q < 4; ++q) d[1 + q] = bi.nb_id[f][q];
+            d[5] = bi.nb_subface[f];
+            d[6] = bi.bc_id[f];
+            if (ld == 0 && bi.nb_id[f][0] >= 0) d[0] = FD_COPY;
+            else if (ld == -1 && bi.nb_id[f][0] >= 0) d[0] = FD_C2F;
+            else if (ld == 1 && bi.nb_id[f][0] >= 0 && bi.nb_id[f][1] >= 0 && bi.nb_id[f][2] >= 0 &&
+                     bi.nb_id[f][3] >= 0) d[0] = FD_F2C;
+            else if (bi.bc_type[f] == BCM_BC_DIRICHLET) d[0] = FD_DIR;
+            else {
+              d[0] = FD_NONE;
+              P.halo_ok = false;
+            }
+            if (d[0] == FD_DIR || d[0] == FD_NONE)
+              for (int q = 0; q < 4; ++q)
+                if (bi.nb_id[f][q] >= 0) P.jhalo_ok = false;
           }
-          int* d = &desc[size_t(b) * 48 + f * 8];
-          for (int q = 0; q < 4; ++q) d[1 + q] = bi.nb_id[f][q];
-          d[5] = bi.nb_subface[f];
-          d[6] = bi.bc_id[f];
-          if (ld == 0 && bi.nb_id[f][0] >= 0) d[0] = FD_COPY;
-          else if (ld == -1 && bi.nb_id[f][0] >= 0) d[0] = FD_C2F;
-          else if (ld == 1 && bi.nb_id[f][0] >= 0 && bi.nb_id[f][1] >= 0 && bi.nb_id[f][2] >= 0 &&
-                   bi.nb_id[f][3] >= 0) d[0] = FD_F2C;
-          else if (bi.bc_type[f] == BCM_BC_DIRICHLET) d[0] = FD_DIR;
-          else {
-            d[0] = FD_NONE;
-            P.halo_ok = false;
+        }
+      });
+      m->halo_ok = true;
+      m->jhalo_ok = true;
+      std::vector<int4> bc;
+      for (const Part& P : parts) {
+        bc.insert(bc.end(), P.bc.begin(), P.bc.end());
+        m->n_dirichlet += P.n_dirichlet;
+        m->n_neumann += P.n_neumann;
+        m->need_bvalues = std::max(m->need_bvalues, P.need_bvalues);
+        m->halo_ok = m->halo_ok && P.halo_ok;
+        m->jhalo_ok = m->jhalo_ok && P.jhalo_ok;
+      }
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_desc), sizeof(int) * desc.size())) != BCM_OK) break;
+      if (hipMemcpy(m->d_desc, desc.data(), sizeof(int) * desc.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+      std::vector<int4>& all = m->h_tasks;
+      all.clear();
+      for (int a = 0; a < 3; ++a) {
+        for (int k = 0; k < 3; ++k) {
+          m->off[a][k] = int(all.size());
+          for (const Part& P : parts) {
+            all.insert(all.end(), P.lists[a][k].begin(), P.lists[a][k].end());
+            m->kind_count[k] += int(P.lists[a][k].size());
           }
-          if (d[0] == FD_DIR || d[0] == FD_NONE)
-            for (int q = 0; q < 4; ++q)
-              if (bi.nb_id[f][q] >= 0) P.jhalo_ok = false;
         }
+        m->off[a][3] = int(all.size());
       }
-    });
-    m->halo_ok = true;
-    m->jhalo_ok = true;
-    std::vector<int4> bc;
-    for (const Part& P : parts) {
-      bc.insert(bc.end(), P.bc.begin(), P.bc.end());
-      m->n_dirichlet += P.n_dirichlet;
-      m->n_neumann += P.n_neumann;
-      m->need_bvalues = std::max(m->need_bvalues, P.need_bvalues);
-      m->halo_ok = m->halo_ok && P.halo_ok;
-      m->jhalo_ok = m->jhalo_ok && P.jhalo_ok;
-    }
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_desc), sizeof(int) * desc.size())) != BCM_OK) break;
-    if (hipMemcpy(m->d_desc, desc.data(), sizeof(int) * desc.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      rc = BCM_ERR_HIP;
-      break;
-    }
-    std::vector<int4>& all = m->h_tasks;
-    all.clear();
-    for (int a = 0; a < 3; ++a) {
-      for (int k = 0; k < 3; ++k) {
-        m->off[a][k] = int(all.size());
-        for (const Part& P : parts) {
-          all.insert(all.end(), P.lists[a][k].begin(), P.lists[a][k].end());
-          m->kind_count[k] += int(P.lists[a][k].size());
-        }
+      m->ntask_total = int(all.size());
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_tasks), sizeof(int4) * all.size())) != BCM_OK) break;
+      if (!all.empty() &&
+          hipMemcpy(m->d_tasks, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
       }
-      m->off[a][3] = int(all.size());
-    }
-    m->ntask_total = int(all.size());
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_tasks), sizeof(int4) * all.size())) != BCM_OK) break;
-    if (!all.empty() &&
-        hipMemcpy(m->d_tasks, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      rc = BCM_ERR_HIP;
-      break;
-    }
-    m->n_bc_tasks = int(bc.size());
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_bc_tasks), sizeof(int4) * bc.size())) != BCM_OK) break;
-    if (!bc.empty() &&
-        hipMemcpy(m->d_bc_tasks, bc.data(), sizeof(int4) * bc.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      rc = BCM_ERR_HIP;
-      break;
-    }
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_meta), sizeof(BlockMeta) * size_t(nblocks))) != BCM_OK)
-      break;
-    if (hipMemcpy(m->d_meta, m->meta.data(), sizeof(BlockMeta) * size_t(nblocks), hipMemcpyHostToDevice) !=
-        hipSuccess) {
-      rc = BCM_ERR_HIP;
-      break;
-    }
-    // One partial per 256-cell work-group of a cell sweep.
-    m->n_partials = (m->ncell + 255) / 256 + 1024;
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_partials), sizeof(double) * m->n_partials)) != BCM_OK)
-      break;
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_result), sizeof(double) * 16)) != BCM_OK) break;
-    if (hipHostMalloc(reinterpret_cast<void**>(&m->h_result), sizeof(double) * 16, hipHostMallocDefault) !=
-        hipSuccess) {
-      m->h_result = nullptr;
-      rc = BCM_ERR_NOMEM;
-      break;
-    }
-    // Segment tables.
-    m->nseg = nseg;
-    m->seg_first.assign(size_t(nseg) + 1, 0);
-    std::vector<int> blk_seg(size_t(nblocks), 0);
-    for (int s = 0, b = 0; s < nseg; ++s) {
-      m->seg_first[s] = b;
-      for (int e = 0; e < seg_counts[s]; ++e) blk_seg[size_t(b++)] = s;
-      m->seg_first[s + 1] = b;
-    }
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_seg_first), sizeof(int) * (size_t(nseg) + 1))) != BCM_OK)
-      break;
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_blk_seg), sizeof(int) * size_t(nblocks))) != BCM_OK) break;
-    if (hipMemcpy(m->d_seg_first, m->seg_first.data(), sizeof(int) * (size_t(nseg) + 1), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(m->d_blk_seg, blk_seg.data(), sizeof(int) * size_t(nblocks), hipMemcpyHostToDevice) != hipSuccess) {
-      rc = BCM_ERR_HIP;
-      break;
-    }
-  } while (false);
+      m->n_bc_tasks = int(bc.size());
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_bc_tasks), sizeof(int4) * bc.size())) != BCM_OK) break;
+      if (!bc.empty() &&
+          hipMemcpy(m->d_bc_tasks, bc.data(), sizeof(int4) * bc.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_meta), sizeof(BlockMeta) * size_t(nblocks))) != BCM_OK)
+        break;
+      if (hipMemcpy(m->d_meta, m->meta.data(), sizeof(BlockMeta) * size_t(nblocks), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+      // One partial per 256-cell work-group of a cell sweep.
+      m->n_partials = (m->ncell + 255) / 256 + 1024;
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_partials), sizeof(double) * m->n_partials)) != BCM_OK)
+        break;
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_result), sizeof(double) * 16)) != BCM_OK) break;
+      if (hipHostMalloc(reinterpret_cast<void**>(&m->h_result), sizeof(double) * 16, hipHostMallocDefault) !=
+          hipSuccess) {
+        m->h_result = nullptr;
+        rc = BCM_ERR_NOMEM;
+        break;
+      }
+      // Segment tables.
+      m->nseg = nseg;
+      m->seg_first.assign(size_t(nseg) + 1, 0);
+      std::vector<int> blk_seg(size_t(nblocks), 0);
+      for (int s = 0, b = 0; s < nseg; ++s) {
+        m->seg_first[s] = b;
+        for (int e = 0; e < seg_counts[s]; ++e) blk_seg[size_t(b++)] = s;
+        m->seg_first[s + 1] = b;
+      }
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_seg_first), sizeof(int) * (size_t(nseg) + 1))) != BCM_OK)
+        break;
+      if ((rc = dev_alloc(reinterpret_cast<void**>(&m->d_blk_seg), sizeof(int) * size_t(nblocks))) != BCM_OK) break;
+      if (hipMemcpy(m->d_seg_first, m->seg_first.data(), sizeof(int) * (size_t(nseg) + 1), hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(m->d_blk_seg, blk_seg.data(), sizeof(int) * size_t(nblocks), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = BCM_ERR_HIP;
+        break;
+      }
+    } while (false);
+  } catch (const std::bad_alloc&) {  // host tables (also from the threads that build them)
+    rc = BCM_ERR_NOMEM;
+  }
   if (rc != BCM_OK) {
     mesh_release(m);
     return rc;
@@ -648,7 +652,12 @@ extern "C" int bcm_mesh_create_batch(const bcm_block_info* blocks, const int* co
   if (total > (1LL << 30)) return BCM_ERR_ARG;
   // Shift each segment's neighbour ids (local to the segment) into the
   // concatenated table.
-  std::vector<bcm_block_info> all(blocks, blocks + total);
+  std::vector<bcm_block_info> all;
+  try {
+    all.assign(blocks, blocks + total);
+  } catch (const std::bad_alloc&) {
+    return BCM_ERR_NOMEM;
+  }
   for (int s = 0, b0 = 0; s < nseg; b0 += counts[s], ++s) {
     for (int e = 0; e < counts[s]; ++e) {
       bcm_block_info& bi = all[size_t(b0 + e)];
