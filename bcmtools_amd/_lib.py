@@ -125,6 +125,19 @@ class SolveStats(ctypes.Structure):
     ]
 
 
+class CommFace(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int32),
+        ("face", ctypes.c_int32),
+        ("subface", ctypes.c_int32),
+        ("kind", ctypes.c_int32),
+        ("offset", ctypes.c_int64),
+    ]
+
+
+COMM_SAME, COMM_F2C, COMM_C2F = 0, 1, 2
+
+
 class ServerParams(ctypes.Structure):
     _fields_ = [
         ("hbm_budget", ctypes.c_double),
@@ -208,6 +221,11 @@ SIGNATURES = {
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
     "bcm_ghost_fill_face": (_I, [_P, _I, _I, _I]),
+    "bcm_comm_face_size": (ctypes.c_int64, [_P, _I]),
+    "bcm_comm_pack": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),
+    "bcm_comm_unpack": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),
+    "bcm_comm_pack_host": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),
+    "bcm_comm_unpack_host": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),
     "bcm_vfield_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "bcm_vfield_destroy": (_I, [_P]),
     "bcm_vfield_upload": (_I, [_P, _P, ctypes.c_size_t]),

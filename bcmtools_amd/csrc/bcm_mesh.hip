@@ -862,6 +862,140 @@ extern "C" int bcm_ghost_fill_face(bcm_field_t f, int block, int face, int subfa
 }
 
 // ---------------------------------------------------------------------
+// Face buffers (Scalar3DUpdater::copyToCommBuffer* / copyFromCommBuffer*,
+// include/Scalar3DUpdater.h:377-515, 711-901): one face per work-group,
+// its box enumerated x fastest (element m = i + w (j + h k)).
+// ---------------------------------------------------------------------
+namespace bcm {
+namespace {
+
+template <typename T, bool PACK>
+__global__ __launch_bounds__(256) void k_comm(T* __restrict__ d, T* __restrict__ buf,
+                                              const bcm_comm_face* __restrict__ faces, Geo g) {
+  const bcm_comm_face fc = faces[blockIdx.x];
+  const int axis = fc.face >> 1;
+  const bool plus = fc.face & 1;
+  const int n = g.n, vc = g.vc, h = n / 2;
+  const int t0 = (axis + 1) % 3, t1 = (axis + 2) % 3;  // X:(y,z) Y:(z,x) Z:(x,y)
+  const int so0 = fc.subface & 1, so1 = fc.subface >> 1;
+  const int tn = fc.kind == BCM_COMM_F2C ? h : n;
+  int ext[3];
+  ext[axis] = vc;
+  ext[t0] = tn;
+  ext[t1] = tn;
+  const int count = ext[0] * ext[1] * ext[2];
+  T* __restrict__ blk = d + (long long)fc.block * g.bvol;
+  T* __restrict__ fb = buf + fc.offset;
+  for (int m = threadIdx.x; m < count; m += 256) {
+    int c[3] = {m % ext[0], (m / ext[0]) % ext[1], m / (ext[0] * ext[1])};
+    if (PACK) {
+      T v;
+      if (fc.kind == BCM_COMM_SAME) {
+        c[axis] += plus ? n - vc : 0;
+        v = blk[g.idx(c[0], c[1], c[2])];
+      } else if (fc.kind == BCM_COMM_F2C) {
+        // copyToCommBufferF2C_0: coarse-cell coordinates of the fine block.
+        c[axis] += plus ? h - vc : 0;
+        v = f2c_value<T>(blk, g, c[0], c[1], c[2]);
+      } else {
+        // copyToCommBufferC2F_0: fine-cell coordinates of the coarse block.
+        c[axis] += plus ? 2 * n - vc : 0;
+        c[t0] += n * so0;
+        c[t1] += n * so1;
+        v = c2f_value<T>(blk, g, c[0], c[1], c[2]);
+      }
+      fb[m] = v;
+    } else {
+      c[axis] += plus ? n : -vc;
+      if (fc.kind == BCM_COMM_F2C) {
+        c[t0] += h * so0;
+        c[t1] += h * so1;
+      }
+      blk[g.idx(c[0], c[1], c[2])] = fb[m];
+    }
+  }
+}
+
+template <typename T, bool PACK>
+int comm_run(bcm_field* f, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes) {
+  bcm_mesh* m = f->mesh;
+  const long long cap = (long long)(bytes / sizeof(T));
+  const long long full = (long long)m->n * m->n * m->vc;
+  for (int i = 0; i < nfaces; ++i) {
+    const bcm_comm_face& c = faces[i];
+    if (c.block < 0 || c.block >= m->nblocks || c.face < 0 || c.face >= BCM_NUM_FACE || c.subface < 0 ||
+        c.subface > 3 || c.kind < BCM_COMM_SAME || c.kind > BCM_COMM_C2F || c.offset < 0)
+      return BCM_ERR_ARG;
+    const long long need = c.kind == BCM_COMM_F2C ? full / 4 : full;
+    if (c.offset > cap - need) return BCM_ERR_ARG;
+  }
+  if (nfaces == 0) return BCM_OK;
+  void* df = nullptr;
+  BCM_TRY(dev_alloc(&df, sizeof(bcm_comm_face) * size_t(nfaces)));
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  hipError_t e = hipMemcpyAsync(df, faces, sizeof(bcm_comm_face) * size_t(nfaces), hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((k_comm<T, PACK>), dim3(unsigned(nfaces)), dim3(256), 0, m->stream, static_cast<T*>(f->data),
+                       static_cast<T*>(buffer), static_cast<const bcm_comm_face*>(df), g);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  dev_free(df);
+  return e == hipSuccess ? BCM_OK : BCM_ERR_HIP;
+}
+
+}  // namespace
+}  // namespace bcm
+
+extern "C" int64_t bcm_comm_face_size(bcm_field_t f, int kind) {
+  if (!f || !f->mesh || kind < BCM_COMM_SAME || kind > BCM_COMM_C2F) return -BCM_ERR_ARG;
+  const int64_t full = int64_t(f->mesh->n) * f->mesh->n * f->mesh->vc;
+  return kind == BCM_COMM_F2C ? full / 4 : full;
+}
+
+extern "C" int bcm_comm_pack(bcm_field_t f, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes) {
+  if (!f || !f->mesh || nfaces < 0 || (nfaces > 0 && (!faces || !buffer))) return BCM_ERR_ARG;
+  if (f->dtype == BCM_F64) return comm_run<double, true>(f, faces, nfaces, buffer, bytes);
+  return comm_run<float, true>(f, faces, nfaces, buffer, bytes);
+}
+
+extern "C" int bcm_comm_unpack(bcm_field_t f, const bcm_comm_face* faces, int nfaces, const void* buffer,
+                               size_t bytes) {
+  if (!f || !f->mesh || nfaces < 0 || (nfaces > 0 && (!faces || !buffer))) return BCM_ERR_ARG;
+  if (f->dtype == BCM_F64) return comm_run<double, false>(f, faces, nfaces, const_cast<void*>(buffer), bytes);
+  return comm_run<float, false>(f, faces, nfaces, const_cast<void*>(buffer), bytes);
+}
+
+static int comm_host(bcm_field_t f, const bcm_comm_face* faces, int nfaces, void* host, size_t bytes, bool pack) {
+  if (!f || !f->mesh || nfaces < 0 || (nfaces > 0 && (!faces || !host))) return BCM_ERR_ARG;
+  if (nfaces == 0) return BCM_OK;
+  if (bytes == 0) return BCM_ERR_ARG;
+  bcm_mesh* m = f->mesh;
+  void* d = nullptr;
+  BCM_TRY(dev_alloc(&d, bytes));
+  int rc = BCM_OK;
+  if (!pack && hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, m->stream) != hipSuccess) rc = BCM_ERR_HIP;
+  if (rc == BCM_OK) rc = pack ? bcm_comm_pack(f, faces, nfaces, d, bytes) : bcm_comm_unpack(f, faces, nfaces, d, bytes);
+  if (rc == BCM_OK && pack) {
+    if (hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+      rc = BCM_ERR_HIP;
+  }
+  (void)hipStreamSynchronize(m->stream);
+  dev_free(d);
+  return rc;
+}
+
+extern "C" int bcm_comm_pack_host(bcm_field_t f, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes) {
+  return comm_host(f, faces, nfaces, buffer, bytes, true);
+}
+
+extern "C" int bcm_comm_unpack_host(bcm_field_t f, const bcm_comm_face* faces, int nfaces, const void* buffer,
+                                    size_t bytes) {
+  return comm_host(f, faces, nfaces, const_cast<void*>(buffer), bytes, false);
+}
+
+// ---------------------------------------------------------------------
 // Vector fields: three interleaved components per cell.
 // ---------------------------------------------------------------------
 extern "C" int bcm_vfield_create(bcm_mesh_t m, int dtype, bcm_vfield_t* out) {

@@ -193,6 +193,30 @@ class BlockManager : public FaceExchanger {
 
   static int axis_bit(int xyz) { return BCM_GF_X << xyz; }
 
+  void commFace(int id, int block, Face face, Subface subface, int kind, void* buffer, size_t bytes, bool pack) {
+    FieldRec& r = rec(id);
+    if (!r.dev) {
+      std::cout << "*** error: buffer rules need a scalar class with an updater" << std::endl;
+      Exit(EX_FAILURE);
+    }
+    char* p = static_cast<char*>(r.host) + r.block_bytes * block;
+    if (!r.device_resident)
+      check(bcm_field_upload_blocks(r.dev, block, 1, p, r.block_bytes), "bcm_field_upload_blocks");
+    bcm_comm_face fc;
+    fc.block = block;
+    fc.face = int(face);
+    fc.subface = int(subface);
+    fc.kind = kind;
+    fc.offset = 0;
+    if (pack) {
+      check(bcm_comm_pack_host(r.dev, &fc, 1, buffer, bytes), "bcm_comm_pack_host");
+    } else {
+      check(bcm_comm_unpack_host(r.dev, &fc, 1, buffer, bytes), "bcm_comm_unpack_host");
+      if (!r.device_resident)
+        check(bcm_field_download_blocks(r.dev, block, 1, p, r.block_bytes), "bcm_field_download_blocks");
+    }
+  }
+
   // src/BlockManager.cpp:118-287 with one rank: every face is intra-node.
   void layoutReport(std::ostream& os) const {
     int lmin = INT_MAX, lmax = 0;
@@ -355,6 +379,15 @@ class BlockManager : public FaceExchanger {
       if (r.dev) check(bcm_field_download_blocks(r.dev, block, 1, p, r.block_bytes), "bcm_field_download_blocks");
       else check(bcm_vfield_download_blocks(r.vdev, block, 1, p, r.block_bytes), "bcm_vfield_download_blocks");
     }
+  }
+
+  // FaceExchanger: one face of one block into or out of a host buffer, for
+  // the VCUpdater buffer rules.  A host-resident class moves the block.
+  void packFace(int id, int block, Face face, Subface subface, int kind, void* buffer, size_t bytes) {
+    commFace(id, block, face, subface, kind, buffer, bytes, true);
+  }
+  void unpackFace(int id, int block, Face face, Subface subface, int kind, const void* buffer, size_t bytes) {
+    commFace(id, block, face, subface, kind, const_cast<void*>(buffer), bytes, false);
   }
 
   void prepareForVCUpdate(int id, int /*tag*/, VCUpdateMethod::Type /*method*/) { rec(id).prepared = true; }

@@ -1,7 +1,8 @@
 // Scalar3DUpdater<T> (include/Scalar3DUpdater.h:64-904): the VCUpdater of a
 // Scalar3D<T>.  Its local rules -- same-level copy, fine-to-coarse average
 // and coarse-to-fine trilinear interpolation of one face -- run on the GPU
-// (bcm_ghost_fill_face, the kernels of the whole-mesh exchange); the buffer
+// (bcm_ghost_fill_face, the kernels of the whole-mesh exchange), and so do
+// its buffer rules (bcm_comm_pack_host / bcm_comm_unpack_host); the buffer
 // sizes follow include/Scalar3DUpdater.h:84-177.
 #ifndef BCM_FACADE_SCALAR3DUPDATER_H
 #define BCM_FACADE_SCALAR3DUPDATER_H
@@ -55,12 +56,27 @@ class Scalar3DUpdater : public VCUpdater {
   void copyFromNeighbor(Face face) { exchangeOnDevice(face, Subface(0)); }
   void copyFromNeighborF2C(Face face, Subface subface) { exchangeOnDevice(face, subface); }
   void copyFromNeighborC2F(Face face, Subface) { exchangeOnDevice(face, Subface(0)); }
-  void copyToCommBuffer(Face) { needsSecondRank("Scalar3DUpdater::copyToCommBuffer"); }
-  void copyToCommBufferF2C(Face, Subface) { needsSecondRank("Scalar3DUpdater::copyToCommBufferF2C"); }
-  void copyToCommBufferC2F(Face, Subface) { needsSecondRank("Scalar3DUpdater::copyToCommBufferC2F"); }
-  void copyFromCommBuffer(Face) { needsSecondRank("Scalar3DUpdater::copyFromCommBuffer"); }
-  void copyFromCommBufferF2C(Face, Subface) { needsSecondRank("Scalar3DUpdater::copyFromCommBufferF2C"); }
-  void copyFromCommBufferC2F(Face, Subface) { needsSecondRank("Scalar3DUpdater::copyFromCommBufferC2F"); }
+  // The buffer rules (include/Scalar3DUpdater.h:377-515, 711-901), on the
+  // device: the buffers the communication layer attached, in the
+  // reference's element order.  A face without a buffer is left alone.
+  void copyToCommBuffer(Face face) {
+    if (T* b = sendBuffer[face][0]) packOnDevice(face, SF_00, BCM_COMM_SAME, b, getSendBufferByteSize(face));
+  }
+  void copyToCommBufferF2C(Face face, Subface subface) {
+    if (T* b = sendBuffer[face][0]) packOnDevice(face, subface, BCM_COMM_F2C, b, getSendBufferByteSizeF2C(face, subface));
+  }
+  void copyToCommBufferC2F(Face face, Subface subface) {
+    if (T* b = sendBuffer[face][subface])
+      packOnDevice(face, subface, BCM_COMM_C2F, b, getSendBufferByteSizeC2F(face, subface));
+  }
+  void copyFromCommBuffer(Face face) {
+    if (T* b = recvBuffer[face][0]) unpackOnDevice(face, SF_00, BCM_COMM_SAME, b, getRecvBufferByteSize(face));
+  }
+  void copyFromCommBufferF2C(Face face, Subface subface) {
+    if (T* b = recvBuffer[face][subface])
+      unpackOnDevice(face, subface, BCM_COMM_F2C, b, getRecvBufferByteSizeF2C(face, subface));
+  }
+  void copyFromCommBufferC2F(Face face, Subface) { copyFromCommBuffer(face); }
   size_t getSendBufferByteSize(Face face) const { return sizeof(T) * getCommBufferSize(face); }
   size_t getSendBufferByteSizeF2C(Face face, Subface) const { return sizeof(T) * getCommBufferSize(face) / 4; }
   size_t getSendBufferByteSizeC2F(Face face, Subface) const { return sizeof(T) * getCommBufferSize(face); }

@@ -5,10 +5,11 @@
 // reference does: same level -> copyFromNeighbor(face), finer neighbour ->
 // ...F2C(face, subface), coarser neighbour -> ...C2F(face, my subface on
 // it).  A concrete updater (Scalar3DUpdater, Vector3DUpdater) implements
-// the private rules.  In this build a local rule runs on the GPU through
-// the BlockManager that registered the data class (bindExchange); the
-// communication-buffer rules and sizes exist for the interface, and the
-// buffer copies need a second rank, which a one-GPU build does not have.
+// the private rules.  In this build the rules run on the GPU through the
+// BlockManager that registered the data class (bindExchange): a local rule
+// fills the face from the neighbour's block, a buffer rule packs the face
+// into, or unpacks it from, the host buffer a communication layer
+// attached (getSend/RecvBufferPointerSetter) in the reference's order.
 //
 // BlockManager::updateVC* never goes through here: it exchanges every face
 // of every block in one device launch (bcm_ghost_fill).
@@ -30,6 +31,12 @@ class FaceExchanger {
  public:
   virtual ~FaceExchanger() {}
   virtual void exchangeFace(int dataClassID, int blockID, Face face, Subface subface) = 0;
+  // One face into / out of a host communication buffer (bcm_comm_pack_host,
+  // bcm_comm_unpack_host; kind BCM_COMM_SAME, _F2C or _C2F).
+  virtual void packFace(int dataClassID, int blockID, Face face, Subface subface, int kind, void* buffer,
+                        size_t bytes) = 0;
+  virtual void unpackFace(int dataClassID, int blockID, Face face, Subface subface, int kind, const void* buffer,
+                          size_t bytes) = 0;
 };
 
 class VCUpdater {
@@ -114,6 +121,15 @@ class VCUpdater {
   void exchangeOnDevice(Face face, Subface subface) {
     if (!exchanger) fail("VCUpdater: the data class was not registered through BlockManager::setDataClass");
     exchanger->exchangeFace(dataClassID, blockID, face, subface);
+  }
+  // A buffer rule: the face to or from a host buffer, on the device.
+  void packOnDevice(Face face, Subface subface, int kind, void* buffer, size_t bytes) {
+    if (!exchanger) fail("VCUpdater: the data class was not registered through BlockManager::setDataClass");
+    exchanger->packFace(dataClassID, blockID, face, subface, kind, buffer, bytes);
+  }
+  void unpackOnDevice(Face face, Subface subface, int kind, const void* buffer, size_t bytes) {
+    if (!exchanger) fail("VCUpdater: the data class was not registered through BlockManager::setDataClass");
+    exchanger->unpackFace(dataClassID, blockID, face, subface, kind, buffer, bytes);
   }
   // Where a subface starts on the face's first and second tangential axis
   // (1 = from the midpoint), for user updaters that index subfaces

@@ -265,6 +265,43 @@ int bcm_ghost_fill(bcm_field_t field, int mask);
  * BCM_ERR_ARG when that face has no such neighbour. */
 int bcm_ghost_fill_face(bcm_field_t field, int block, int face, int subface);
 
+/* Face buffers of an exchange between GPUs (Scalar3DUpdater::
+ * copyToCommBuffer, copyToCommBufferF2C, copyToCommBufferC2F and
+ * copyFromCommBuffer, copyFromCommBufferF2C, copyFromCommBufferC2F,
+ * include/Scalar3DUpdater.h:377-515, 711-901).  One GPU holds every block
+ * here, so the exchange never needs them; they are the pack and unpack
+ * steps a multi-GPU exchange would run around its transfers, with the
+ * reference's buffer layout: a face's box is enumerated x fastest, element
+ * m = i + w (j + h k) for a box of w x h x d cells.
+ *   pack, BCM_COMM_SAME: the block's interior slab of depth vc at `face`
+ *     (w x h x d = vc x n x n for X faces, n x vc x n for Y, n x n x vc for Z);
+ *   pack, BCM_COMM_F2C: the fine sender's restriction toward its coarser
+ *     neighbour across `face` (a quarter box: vc x n/2 x n/2 for X faces);
+ *   pack, BCM_COMM_C2F: the coarse sender's prolongation for the finer
+ *     neighbour on quarter `subface` of `face` (vc x n x n for X faces);
+ *   unpack, BCM_COMM_SAME or BCM_COMM_C2F: the block's ghost slab at `face`;
+ *   unpack, BCM_COMM_F2C: the quarter `subface` of the ghost slab at `face`.
+ * offset: the face's first element in `buffer` (elements of the field's
+ * type).  `buffer` is device memory of `bytes` bytes; every face must fit. */
+enum { BCM_COMM_SAME = 0, BCM_COMM_F2C = 1, BCM_COMM_C2F = 2 };
+typedef struct bcm_comm_face {
+  int32_t block;
+  int32_t face;     /* BCM_X_M .. BCM_Z_P, the block's face toward its neighbour */
+  int32_t subface;  /* 0..3: quarter of `face` (pack C2F, unpack F2C); otherwise ignored */
+  int32_t kind;     /* BCM_COMM_* */
+  int64_t offset;
+} bcm_comm_face;
+/* Elements one face of `kind` takes in a buffer (n^2 vc, or n^2 vc / 4 for F2C). */
+int64_t bcm_comm_face_size(bcm_field_t field, int kind);
+int bcm_comm_pack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes);
+int bcm_comm_unpack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, const void* buffer, size_t bytes);
+/* The same with `buffer` in host memory (staged through a device buffer):
+ * what the C++ facade's Scalar3DUpdater uses for a driver's send and
+ * receive buffers. */
+int bcm_comm_pack_host(bcm_field_t field, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes);
+int bcm_comm_unpack_host(bcm_field_t field, const bcm_comm_face* faces, int nfaces, const void* buffer,
+                         size_t bytes);
+
 /* Vector3D<T> (include/Vector3D.h:36-110): three components per cell,
  * interleaved (component l of cell (i,j,k) at 3 Index3DS(i,j,k) + l, the
  * reference's Index3DV), nblocks*3*(n+2vc)^3 values, block-major.  The

@@ -8,7 +8,11 @@
 //     that component;
 //  3. per-face exchange: VCUpdater::copyFromNeighbor(face, subface) called
 //     for every face of every block reproduces updateVC, for the scalar and
-//     the vector field, host-resident and device-resident;
+//     the vector field, host-resident and device-resident; so do the
+//     buffer rules for the scalar field, as two ranks would run them: the
+//     sender's copyToCommBuffer(face, subface) into a host buffer attached
+//     with getSendBufferPointerSetter, the receiver's
+//     copyFromCommBuffer(face, subface) from it (getRecvBufferPointerSetter);
 //  4. Scalar3D / Vector3D box copies (copyToBuffer, copyFromBuffer,
 //     copyFromDataClass) round trip.
 // Prints PASS when every check holds.
@@ -145,7 +149,9 @@ int main() {
 
   // Per-face exchange through the VCUpdater interface, against updateVC.
   for (int resident = 0; resident < 2; ++resident) {
-    for (int id : {sid, vid}) {
+    for (int run = 0; run < 3; ++run) {
+      const int id = run == 1 ? vid : sid;
+      const bool buffers = run == 2;  // the scalar field through the buffer rules
       std::vector<double> want;
       for (int b = 0; b < nb; ++b) {
         UpdatableDataClass* dc = dynamic_cast<UpdatableDataClass*>(bm.getBlock(b)->getDataClass(id));
@@ -168,16 +174,51 @@ int main() {
             }
       }
       if (resident) bm.setDeviceResident(id, true);
+      auto updater = [&](int b) {
+        return dynamic_cast<UpdatableDataClass*>(bm.getBlock(b)->getDataClass(id))->getUpdater();
+      };
+      // Receiver b's face f (subface q of a finer neighbour's face): the
+      // sender packs its face toward b into a host buffer, b unpacks it.
+      auto through_buffer = [&](int b, int f, int q) {
+        const NeighborInfo* ni = bm.getBlock(b)->getNeighborInfo();
+        const int ld = ni[f].getLevelDifference();
+        const int src = ld == 1 ? ni[f].getID(Subface(q)) : ni[f].getID();
+        const int sf = f ^ 1;
+        // The sender's subface argument: my position on a coarse sender's
+        // face (C2F), its own position on my face (F2C), 0 at one level.
+        const int sq = ld == -1 ? ni[f].getNeighborSubface() : (ld == 1 ? int(bm.getBlock(src)->getNeighborInfo()[sf].getNeighborSubface()) : 0);
+        VCUpdater* su = updater(src);
+        VCUpdater* ru = updater(b);
+        std::vector<char> buf(su->getSendBufferByteSize(Face(sf), Subface(sq)));
+        expect(buf.size() == ru->getRecvBufferByteSize(Face(f), Subface(q)), "send and receive buffers agree in size");
+        PointerSetterBase* ps = su->getSendBufferPointerSetter(Face(sf), Subface(ld == -1 ? sq : 0));
+        ps->setPointer(buf.data());
+        delete ps;
+        su->copyToCommBuffer(Face(sf), Subface(sq));
+        PointerSetterBase* pr = ru->getRecvBufferPointerSetter(Face(f), Subface(ld == 1 ? q : 0));
+        pr->setPointer(buf.data());
+        delete pr;
+        ru->copyFromCommBuffer(Face(f), Subface(q));
+        ps = su->getSendBufferPointerSetter(Face(sf), Subface(ld == -1 ? sq : 0));
+        ps->setPointer(nullptr);
+        delete ps;
+        pr = ru->getRecvBufferPointerSetter(Face(f), Subface(ld == 1 ? q : 0));
+        pr->setPointer(nullptr);
+        delete pr;
+      };
       for (int b = 0; b < nb; ++b) {
-        UpdatableDataClass* dc = dynamic_cast<UpdatableDataClass*>(bm.getBlock(b)->getDataClass(id));
-        VCUpdater* u = dc->getUpdater();
+        VCUpdater* u = updater(b);
         const NeighborInfo* ni = bm.getBlock(b)->getNeighborInfo();
         for (int f = 0; f < NUM_FACE; ++f) {
           if (ni[f].getLevelDifference() == 1) {
             for (int q = 0; q < NUM_SUBFACE; ++q)
-              if (ni[f].getID(Subface(q)) >= 0) u->copyFromNeighbor(Face(f), Subface(q));
+              if (ni[f].getID(Subface(q)) >= 0) {
+                if (buffers) through_buffer(b, f, q);
+                else u->copyFromNeighbor(Face(f), Subface(q));
+              }
           } else if (ni[f].getID() >= 0) {
-            u->copyFromNeighbor(Face(f), Subface(0));
+            if (buffers) through_buffer(b, f, 0);
+            else u->copyFromNeighbor(Face(f), Subface(0));
           }
         }
       }
@@ -213,8 +254,10 @@ int main() {
             }
         off += size_t(nc) * w * w * w;
       }
-      expect(same, resident ? "per-face exchange equals updateVC (device-resident)"
-                            : "per-face exchange equals updateVC (host-resident)");
+      expect(same, buffers ? (resident ? "buffer rules equal updateVC (device-resident)"
+                                       : "buffer rules equal updateVC (host-resident)")
+                           : (resident ? "per-face exchange equals updateVC (device-resident)"
+                                       : "per-face exchange equals updateVC (host-resident)"));
     }
   }
 

@@ -15,7 +15,8 @@ from conftest import ROOT
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "bcm_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|void\*|size_t|double)\s+\*?\s*(bcm_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|void\*|size_t|double)\s+\*?\s*(bcm_\w+)\s*\(", text,
+                                 flags=re.M)))
 
 
 def test_header_declares_the_abi():
@@ -64,7 +65,7 @@ def test_field_api_rejects_bad_arguments_without_gpu():
 # returns BCM_ERR_ARG, except the process-wide setting that takes no
 # handle (0 is its default).
 _SETTINGS = {"bcm_tree_set_threads"}
-_COUNTS = {"bcm_tree_num_blocks", "bcm_sor_get_variant", "bcm_server_poll"}
+_COUNTS = {"bcm_tree_num_blocks", "bcm_sor_get_variant", "bcm_server_poll", "bcm_comm_face_size"}
 _DESTROYS = {"bcm_tree_destroy", "bcm_mesh_destroy", "bcm_field_destroy", "bcm_server_destroy", "bcm_vfield_destroy",
              "bcm_host_free"}
 
@@ -75,7 +76,7 @@ def test_every_entry_point_rejects_null_handles_without_gpu():
             ctypes.c_uint64)
     checked = 0
     for name, (res, args) in sorted(L.SIGNATURES.items()):
-        if res is not ctypes.c_int:
+        if res not in (ctypes.c_int, ctypes.c_int64):
             continue
         vals = [0 if a in ints else 0.0 if a in (ctypes.c_double, ctypes.c_float) else
                 ctypes.cast(None, a) if a is L.DIVIDER_FN else None for a in args]

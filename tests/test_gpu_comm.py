@@ -1,0 +1,127 @@
+"""Face buffers of a multi-GPU exchange (Scalar3DUpdater::copyToCommBuffer*
+and copyFromCommBuffer*, include/Scalar3DUpdater.h:377-515, 711-901).
+
+Every face of every block is packed by the block that owns the source cells
+(its face toward the receiver: same-level slab, F2C restriction by the fine
+sender, C2F prolongation by the coarse sender) and unpacked by the
+receiver, as two ranks would route it.  Pinned against the reference's own
+exchange: starting from the fixture's input field, pack + unpack must give
+the reference's updateVC result on every cell, and every buffer segment
+must hold the receiver's ghost box of that result in the reference's
+buffer order (m = i + w (j + h k), x fastest; copyFromBuffer's boxes).
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from bcmtools_amd import Mesh
+from bcmtools_amd import _lib as L
+
+from conftest import load_golden
+from hipmem import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def comm_faces(blocks):
+    """(sender faces, receiver faces) for every neighbour link of the mesh."""
+    send, recv = [], []
+    for b in range(len(blocks)):
+        for f in range(6):
+            ld = int(blocks["nb_level_diff"][b][f])
+            ids = [int(v) for v in blocks["nb_id"][b][f]]
+            if ld == 0 and ids[0] >= 0:
+                send.append((ids[0], f ^ 1, 0, L.COMM_SAME))
+                recv.append((b, f, 0, L.COMM_SAME))
+            elif ld == -1 and ids[0] >= 0:
+                send.append((ids[0], f ^ 1, int(blocks["nb_subface"][b][f]), L.COMM_C2F))
+                recv.append((b, f, 0, L.COMM_C2F))
+            elif ld == 1:
+                for s in range(4):
+                    if ids[s] >= 0:
+                        send.append((ids[s], f ^ 1, 0, L.COMM_F2C))
+                        recv.append((b, f, s, L.COMM_F2C))
+    return send, recv
+
+
+def recv_box(a, n, vc, face, sub, kind):
+    """The ghost box copyFromCommBuffer* fills, x fastest, from one block's
+    ghosted array a[z][y][x]."""
+    axis, plus = face // 2, face % 2
+    t0, t1 = (axis + 1) % 3, (axis + 2) % 3
+    lo, ext = [0, 0, 0], [n, n, n]
+    lo[axis], ext[axis] = (n if plus else -vc), vc
+    if kind == L.COMM_F2C:
+        h = n // 2
+        ext[t0] = ext[t1] = h
+        lo[t0], lo[t1] = h * (sub & 1), h * (sub >> 1)
+    sl = tuple(slice(lo[d] + vc, lo[d] + vc + ext[d]) for d in (2, 1, 0))
+    return a[sl].ravel()
+
+
+def face_array(faces, offsets):
+    return (L.CommFace * len(faces))(*[L.CommFace(b, f, s, k, int(o)) for (b, f, s, k), o in zip(faces, offsets)])
+
+
+@pytest.mark.parametrize("name", ["ghost_sphere_atonce_d", "ghost_sphere_atonce_f", "ghost_periodic_x_d",
+                                  "ghost_simple_vc2_d", "ghost_sphere_vc2_d", "ghost_sphere_vc2_leveldiff_f"])
+def test_pack_unpack_reproduce_the_reference_exchange(name):
+    g = load_golden(name)
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    send, recv = comm_faces(blocks)
+    kinds = [k for _, _, _, k in send]
+    case(f"comm {name}: nb={len(blocks)} n={n} vc={vc}, faces same/F2C/C2F = "
+         f"{kinds.count(L.COMM_SAME)}/{kinds.count(L.COMM_F2C)}/{kinds.count(L.COMM_C2F)}")
+    mesh = Mesh(blocks, n, vc)
+    fld = mesh.field(g["fin"].dtype, g["fin"])
+    lib = mesh.lib
+    sizes = [int(lib.bcm_comm_face_size(fld.handle, k)) for k in kinds]
+    assert sizes[0] > 0
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(np.sum(sizes))
+    buf = DeviceArray(np.zeros(total, dtype=g["fin"].dtype))
+    nbytes = total * g["fin"].dtype.itemsize
+    assert lib.bcm_comm_pack(fld.handle, face_array(send, offsets), len(send), buf.ptr, nbytes) == L.OK
+    host = buf.get()
+    fout = g["fout"]
+    for (rb, rf, rs, k), off, size in zip(recv, offsets, sizes):
+        np.testing.assert_array_equal(host[off:off + size], recv_box(fout[rb], n, vc, rf, rs, k))
+    # Packing reads the field only.
+    np.testing.assert_array_equal(fld.download(), g["fin"])
+    assert lib.bcm_comm_unpack(fld.handle, face_array(recv, offsets), len(recv), buf.ptr, nbytes) == L.OK
+    np.testing.assert_array_equal(fld.download(), fout)
+    buf.free()
+
+
+def test_comm_rejects_faces_outside_the_buffer():
+    g = load_golden("ghost_sphere_atonce_d")
+    mesh = Mesh(g["blocks"], g["n"], g["vc"])
+    fld = mesh.field(np.float64, g["fin"])
+    lib = mesh.lib
+    size = int(lib.bcm_comm_face_size(fld.handle, L.COMM_SAME))
+    assert int(lib.bcm_comm_face_size(fld.handle, L.COMM_F2C)) * 4 == size
+    assert int(lib.bcm_comm_face_size(fld.handle, 7)) == -L.ERR_ARG
+    case(f"comm argument checks, face size {size}")
+    buf = DeviceArray(np.zeros(size, dtype=np.float64))
+    ok = face_array([(0, 1, 0, L.COMM_SAME)], [0])
+    assert lib.bcm_comm_pack(fld.handle, ok, 1, buf.ptr, size * 8) == L.OK
+    for bad, nbytes in (([(0, 1, 0, L.COMM_SAME)], size * 8 - 8),           # one element short
+                        ([(len(g["blocks"]), 1, 0, L.COMM_SAME)], size * 8),  # block out of range
+                        ([(0, 6, 0, L.COMM_SAME)], size * 8),                # face out of range
+                        ([(0, 1, 4, L.COMM_C2F)], size * 8),                 # subface out of range
+                        ([(0, 1, 0, 3)], size * 8)):                         # unknown kind
+        fa = face_array(bad, [0])
+        assert lib.bcm_comm_pack(fld.handle, fa, 1, buf.ptr, nbytes) == L.ERR_ARG
+        assert lib.bcm_comm_unpack(fld.handle, fa, 1, buf.ptr, nbytes) == L.ERR_ARG
+    assert lib.bcm_comm_pack(fld.handle, face_array([(0, 1, 0, L.COMM_SAME)], [-1]), 1, buf.ptr, size * 8) == \
+        L.ERR_ARG
+    assert lib.bcm_comm_pack(fld.handle, None, 0, None, 0) == L.OK
+    assert lib.bcm_comm_pack(fld.handle, ctypes.cast(None, ctypes.POINTER(L.CommFace)), 1, buf.ptr, 8) == L.ERR_ARG
+    buf.free()
