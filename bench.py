@@ -1034,6 +1034,69 @@ def bench_mesh(args):
     return res
 
 
+def bench_sortol(args):
+    """Config 2 at its own size: the sor2 outer iteration (examples/sor2/
+    Solver.cpp:77-133) on the level-5 shell mesh, 3,816 x 32^3, run until
+    relres ||s - lap_h f|| / ||s|| <= 1e-8 at omega = 1.4 (1.6 diverges with
+    the reference's frozen ghosts), checked every --chunk sweeps.  The run
+    can span several processes: --state DIR keeps the field (a Dump2 restart
+    file of LocalScalar3D's format) and the sweep count, time and residual
+    history between them; each process stops after --budget seconds."""
+    import ctypes
+
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import workloads as W
+
+    n, omega, tol = 32, args.omega, 1e-8
+    blocks = W.shell_tree(args.tol_level).blocks
+    mesh = Mesh(blocks, n, 1)
+    sf = mesh.field(np.float64, -W.rhs_field_fast(blocks, n, 1, seed=0))
+    f = mesh.field(np.float64)
+    state = {"sweeps": 0, "time_s": 0.0, "history": []}
+    sdir = args.state
+    if sdir and os.path.exists(os.path.join(sdir, "state.json")):
+        state = json.load(open(os.path.join(sdir, "state.json")))
+        f.load2(state["sweeps"], "f", directory=sdir)
+        print(f"[bench] sor to tolerance: resumed at {state['sweeps']} sweeps", file=sys.stderr, flush=True)
+    else:
+        mesh.dirichlet_mirror(f, [0.0])
+    r = ctypes.c_double()
+    rr = state["history"][-1][1] if state["history"] else float("inf")
+    mesh.sync()
+    t0 = time.perf_counter()
+    while rr > tol and time.perf_counter() - t0 < args.budget:
+        mesh.sor_iterate(f, sf, omega, [0.0], args.chunk)
+        state["sweeps"] += args.chunk
+        if mesh.lib.bcm_sor_residual_noalloc(f.handle, sf.handle, ctypes.byref(r)) != 0:
+            raise RuntimeError("bcm_sor_residual_noalloc failed")
+        rr = r.value
+        state["history"].append([state["sweeps"], rr])
+        print(f"[bench] sor to tolerance: {state['sweeps']} sweeps, relres {rr:.4e}, "
+              f"{state['time_s'] + time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if not np.isfinite(rr):
+            break
+    mesh.sync()
+    state["time_s"] += time.perf_counter() - t0
+    if sdir:
+        os.makedirs(sdir, exist_ok=True)
+        f.dump2(state["sweeps"], "f", directory=sdir)
+        with open(os.path.join(sdir, "state.json"), "w") as fp:
+            json.dump(state, fp)
+    ncell = len(blocks) * n ** 3
+    mesh.close()
+    return {"metric": "time to relres 1e-8, FP64 red-black SOR (sor2 outer iteration)", "value": state["time_s"],
+            "unit": "s", "n_gpus": 1, "steps": state["sweeps"], "warmup": 0,
+            "ms_per_step": 1e3 * state["time_s"] / max(state["sweeps"], 1), "higher_is_better": False,
+            "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"adaptive shell octree level {args.tol_level}, {len(blocks)} x {n}^3 blocks, "
+                                   f"vc=1, sor2 RB-SOR omega={omega} + Dirichlet + ghost fill per sweep, relres "
+                                   f"checked every {args.chunk} sweeps", "blocks": len(blocks), "omega": omega,
+                       "tol": tol},
+            "sweeps": state["sweeps"], "relres": rr, "converged": bool(rr <= tol),
+            "GCUPS": ncell * state["sweeps"] / max(state["time_s"], 1e-9) / 1e9,
+            "history": state["history"][:: max(1, len(state["history"]) // 50)]}
+
+
 def bench_restart(args):
     """Restart files of SolverTest (LocalScalar3D::Dump2 / Load2) of an FP64
     field on the level-7 shell mesh, written from HBM and read back into it."""
@@ -1122,7 +1185,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sor",
-                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart", "ghost", "mesh"])
+                    choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart", "ghost", "mesh",
+                             "sortol"])
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
     ap.add_argument("--lams", default="", help="serving: comma-separated arrival rates to sweep (same requests)")
@@ -1149,6 +1213,11 @@ def main():
                     help="batched: precision of the full batch (auto: the faster on the subset); "
                          "bicgstab: auto times mixed, FP64 and the host-driven path, else that engine only")
     ap.add_argument("--level", type=int, default=7, help="bicgstab: shell refinement level")
+    ap.add_argument("--tol-level", type=int, default=5, help="sortol: shell refinement level")
+    ap.add_argument("--omega", type=float, default=1.4, help="sortol: SOR weight")
+    ap.add_argument("--chunk", type=int, default=2000, help="sortol: sweeps between residual checks")
+    ap.add_argument("--budget", type=float, default=900.0, help="sortol: seconds this process runs")
+    ap.add_argument("--state", default="", help="sortol: directory keeping the run across processes")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
@@ -1164,7 +1233,7 @@ def main():
         sys.exit(2)
     res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
            "serving": bench_serving, "copy": bench_copy, "restart": bench_restart,
-           "ghost": bench_ghost, "mesh": bench_mesh}[args.workload](args)
+           "ghost": bench_ghost, "mesh": bench_mesh, "sortol": bench_sortol}[args.workload](args)
     if args.workload == "sor" and not args.quick:
         # The rest of the metric: SOR to a tolerance, batched solves/s and
         # served solves/s with p50/p99 latency, each on a bounded sample.
