@@ -188,8 +188,8 @@ def sor_to_tolerance(level=3, n=32, omega=1.4, tol=1e-8, chunk=500, cap=400000):
                          "omega >= 1.5 on these meshes (relres 1e23 after 500 sweeps at 1.5, 1e47 at 1.6 on the "
                          "level-4 mesh); the compiled reference sor1 gives errorMax 5.8e31 at omega 1.6 and "
                          "0.334 at 1.4 on 64 x 16^3 after 400 sweeps",
-            "full_size_note": "on the level-5 mesh itself (3,816 x 32^3) omega 1.4 reaches relres 0.36 after "
-                              "60,000 sweeps (35 s): the tolerance needs several hundred thousand sweeps there"}
+            "full_size_note": "on the level-5 mesh itself (3,816 x 32^3) omega 1.4 reaches relres 9.4e-9 after "
+                              "475,000 sweeps in 272 s (bench.py --workload sortol), too long for this line"}
 
 
 def cpu_sor(blocks, n, vc, s, omega, vals, steps):

@@ -125,3 +125,28 @@ def test_comm_rejects_faces_outside_the_buffer():
     assert lib.bcm_comm_pack(fld.handle, None, 0, None, 0) == L.OK
     assert lib.bcm_comm_pack(fld.handle, ctypes.cast(None, ctypes.POINTER(L.CommFace)), 1, buf.ptr, 8) == L.ERR_ARG
     buf.free()
+
+
+def test_pack_unpack_equal_the_ghost_fill_on_the_config2_mesh():
+    """Every link of the 3,816 x 32^3 shell mesh (copy, F2C, C2F) through
+    the buffers equals the whole-mesh exchange, bit for bit."""
+    from bcmtools_amd import workloads as W
+    t = W.config2_tree()
+    blocks, n, vc = t.blocks, 32, 1
+    send, recv = comm_faces(blocks)
+    case(f"comm config 2: {len(blocks)} blocks, {len(send)} faces")
+    rng = np.random.default_rng(5)
+    data = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2))
+    mesh = Mesh(blocks, n, vc)
+    want = mesh.field(np.float64, data)
+    mesh.update_vc(want)
+    fld = mesh.field(np.float64, data)
+    lib = mesh.lib
+    sizes = [int(lib.bcm_comm_face_size(fld.handle, k)) for _, _, _, k in send]
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total = int(np.sum(sizes))
+    buf = DeviceArray(np.zeros(total))
+    assert lib.bcm_comm_pack(fld.handle, face_array(send, offsets), len(send), buf.ptr, total * 8) == L.OK
+    assert lib.bcm_comm_unpack(fld.handle, face_array(recv, offsets), len(recv), buf.ptr, total * 8) == L.OK
+    np.testing.assert_array_equal(fld.download(), want.download())
+    buf.free()
