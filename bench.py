@@ -877,6 +877,52 @@ def host_exchange(mesh, f, args):
     return out
 
 
+def comm_exchange(mesh, blocks, f, args):
+    """Every link of the mesh through face buffers, as a multi-GPU exchange
+    would move them (bcm_comm_pack into one device buffer, bcm_comm_unpack
+    from it): wall time of each call, its face list upload included."""
+    import ctypes
+
+    from bcmtools_amd import _lib as L
+
+    lib = L.load()
+    send, recv = [], []
+    for b in range(len(blocks)):
+        for fc in range(6):
+            ld = int(blocks["nb_level_diff"][b][fc])
+            ids = [int(v) for v in blocks["nb_id"][b][fc]]
+            if ld == 0 and ids[0] >= 0:
+                send.append((ids[0], fc ^ 1, 0, L.COMM_SAME))
+                recv.append((b, fc, 0, L.COMM_SAME))
+            elif ld == -1 and ids[0] >= 0:
+                send.append((ids[0], fc ^ 1, int(blocks["nb_subface"][b][fc]), L.COMM_C2F))
+                recv.append((b, fc, 0, L.COMM_C2F))
+            elif ld == 1:
+                for q in range(4):
+                    if ids[q] >= 0:
+                        send.append((ids[q], fc ^ 1, 0, L.COMM_F2C))
+                        recv.append((b, fc, q, L.COMM_F2C))
+    sizes = [int(lib.bcm_comm_face_size(f.handle, k)) for _, _, _, k in send]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    nbytes = int(np.sum(sizes)) * 8
+    stage = mesh.field(np.float64)  # device memory for the buffer
+    buf = lib.bcm_field_device_ptr(stage.handle)
+    assert lib.bcm_field_bytes(stage.handle) >= nbytes
+    fs = (L.CommFace * len(send))(*[L.CommFace(b, fc, q, k, int(o)) for (b, fc, q, k), o in zip(send, offs)])
+    fr = (L.CommFace * len(recv))(*[L.CommFace(b, fc, q, k, int(o)) for (b, fc, q, k), o in zip(recv, offs)])
+    tp, tu = [], []
+    for _ in range(max(1, args.warmup) + max(1, min(args.steps, 10))):
+        t0 = time.perf_counter()
+        L.check(lib.bcm_comm_pack(f.handle, fs, len(send), buf, nbytes), "bcm_comm_pack")
+        t1 = time.perf_counter()
+        L.check(lib.bcm_comm_unpack(f.handle, fr, len(recv), buf, nbytes), "bcm_comm_unpack")
+        tp.append(t1 - t0)
+        tu.append(time.perf_counter() - t1)
+    skip = max(1, args.warmup)
+    return {"faces": len(send), "buffer_bytes": nbytes, "pack_ms": 1e3 * min(tp[skip:]),
+            "unpack_ms": 1e3 * min(tu[skip:])}
+
+
 def bench_ghost(args):
     """The reference's updateVC on its own (include/BlockManager.h:311-726,
     include/Scalar3DUpdater.h:190-347): the complete ghost fill of an FP64
@@ -910,6 +956,7 @@ def bench_ghost(args):
         row["algorithmic_GBps_all"] = nbytes / (row["ms_all"] * 1e-3) / 1e9
         if n == 32:
             row["host_resident_exchange_ms"] = host_exchange(mesh, f, args)
+            row["pack_unpack_ms"] = comm_exchange(mesh, tree.blocks, f, args)
         print(f"[bench] ghost fill {name}: {row}", file=sys.stderr, flush=True)
         out[name] = row
         mesh.close()
