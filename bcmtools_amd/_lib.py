@@ -219,6 +219,7 @@ SIGNATURES = {
     "bcm_fold_face_bc": (_I, [ctypes.POINTER(_P), _P, _IP, _DP]),
     "bcm_rbsor_sweep": (_I, [_P, _P, _D]),
     "bcm_sor_iterate": (_I, [_P, _P, _D, _DP, _I, _I]),
+    "bcm_sor_iterate_inner": (_I, [_P, _P, _D, _DP, _I, _I, _I]),
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
     "bcm_ghost_fill_face": (_I, [_P, _I, _I, _I]),
     "bcm_comm_face_size": (ctypes.c_int64, [_P, _I]),

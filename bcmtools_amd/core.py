@@ -294,10 +294,17 @@ class Mesh:
         L.check(self.lib.bcm_rbsor_sweep(f.handle, source.handle if source is not None else None, float(omega)),
                 "bcm_rbsor_sweep")
 
-    def sor_iterate(self, f, source, omega, bvalues, iters):
+    def sor_iterate(self, f, source, omega, bvalues, iters, inner=1):
+        """`iters` outer iterations of the sor1/sor2 Solver::run, each with
+        `inner` (nLoopInner) sweeps per block between ghost exchanges."""
         v = np.ascontiguousarray(bvalues, dtype=np.float64)
-        L.check(self.lib.bcm_sor_iterate(f.handle, source.handle if source is not None else None, float(omega),
-                                         L.dptr(v), len(v), int(iters)), "bcm_sor_iterate")
+        src = source.handle if source is not None else None
+        if inner == 1:
+            L.check(self.lib.bcm_sor_iterate(f.handle, src, float(omega), L.dptr(v), len(v), int(iters)),
+                    "bcm_sor_iterate")
+        else:
+            L.check(self.lib.bcm_sor_iterate_inner(f.handle, src, float(omega), L.dptr(v), len(v), int(iters),
+                                                   int(inner)), "bcm_sor_iterate_inner")
 
     def set_sor_variant(self, variant):
         L.check(self.lib.bcm_sor_set_variant(self.handle, int(variant)), "bcm_sor_set_variant")

@@ -737,18 +737,25 @@ static bool variant_supported(const bcm_mesh* m, int v) {
   }
 }
 
+// `inner` sweeps + in-block BC per block between exchanges (nLoopInner).
+// The halo-gathering schedule reads the neighbours' latest iterate, which
+// only matches the exchange order for inner == 1; otherwise the fused
+// sweeps over f's own (frozen) ghosts run.
 template <typename T>
 static int sor_iterate_t(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
-                         int iters, int variant) {
+                         int iters, int variant, int inner = 1) {
   bcm_mesh* m = f->mesh;
   T* d = static_cast<T*>(f->data);
   const T* s = source ? static_cast<const T*>(source->data) : nullptr;
   if (variant == BCM_SOR_AUTO) variant = variant_supported(m, BCM_SOR_HALO) ? BCM_SOR_HALO : BCM_SOR_FUSED_TASKS;
+  if (variant == BCM_SOR_HALO && inner > 1) variant = BCM_SOR_FUSED_TASKS;
   if (variant == BCM_SOR_HALO && iters >= 2) return sor_iterate_halo<T>(f, source, omega, iters);
   for (int it = 0; it < iters; ++it) {
     d = static_cast<T*>(f->data);
-    BCM_TRY(rbsor_raw<T>(m, d, s, omega, variant == BCM_SOR_COLOUR_TASKS));
-    BCM_TRY(dirichlet_device(m, f));
+    for (int in = 0; in < inner; ++in) {
+      BCM_TRY(rbsor_raw<T>(m, d, s, omega, variant == BCM_SOR_COLOUR_TASKS));
+      BCM_TRY(dirichlet_device(m, f));
+    }
     if (variant == BCM_SOR_FUSED_GATHER) BCM_TRY(ghost_gather_raw<T>(m, d));
     else BCM_TRY(ghost_fill_raw<T>(m, d, BCM_GF_ALL));
     BCM_TRY(dirichlet_device(m, f));
@@ -756,14 +763,19 @@ static int sor_iterate_t(bcm_field_t f, bcm_field_t source, double omega, const 
   return BCM_OK;
 }
 
-extern "C" int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
-                               int iters) {
-  if (!f || iters < 0 || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
+extern "C" int bcm_sor_iterate_inner(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues,
+                                     int nbvalues, int iters, int inner) {
+  if (!f || iters < 0 || inner < 1 || (source && !same_mesh(f, source))) return BCM_ERR_ARG;
   BCM_TRY(bcm_dirichlet_mirror(f, bvalues, nbvalues));  // validates and uploads the table
   bcm_mesh* m = f->mesh;
   const int v = variant_supported(m, m->sor_variant) ? m->sor_variant : BCM_SOR_AUTO;
-  return f->dtype == BCM_F64 ? sor_iterate_t<double>(f, source, omega, bvalues, nbvalues, iters, v)
-                             : sor_iterate_t<float>(f, source, omega, bvalues, nbvalues, iters, v);
+  return f->dtype == BCM_F64 ? sor_iterate_t<double>(f, source, omega, bvalues, nbvalues, iters, v, inner)
+                             : sor_iterate_t<float>(f, source, omega, bvalues, nbvalues, iters, v, inner);
+}
+
+extern "C" int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega, const double* bvalues, int nbvalues,
+                               int iters) {
+  return bcm_sor_iterate_inner(f, source, omega, bvalues, nbvalues, iters, 1);
 }
 
 extern "C" int bcm_sor_set_variant(bcm_mesh_t m, int variant) {

@@ -382,6 +382,14 @@ int bcm_rbsor_sweep(bcm_field_t f, bcm_field_t source, double omega);
  * (examples/sor2/Solver.cpp:77-133 with nLoopInner == 1). */
 int bcm_sor_iterate(bcm_field_t f, bcm_field_t source, double omega,
                     const double* bvalues, int nbvalues, int iters);
+/* bcm_sor_iterate with nLoopInner = `inner` >= 1 (examples/sor1/Solver.cpp:
+ * 102-105, examples/sor2/Solver.cpp:106-109): per outer iteration, `inner`
+ * times SOR + in-block Dirichlet BC on every block over the ghosts of the
+ * last exchange, then ghost fill and Dirichlet BC.  inner == 1 is
+ * bcm_sor_iterate; for inner > 1 the HALO schedule is replaced by
+ * FUSED_TASKS (the halo sweeps read the neighbours' latest iterate). */
+int bcm_sor_iterate_inner(bcm_field_t f, bcm_field_t source, double omega,
+                          const double* bvalues, int nbvalues, int iters, int inner);
 /* Scheduling of bcm_sor_iterate's outer iteration (all give the same bits):
  *   AUTO          halo-gathering sweeps where the mesh allows, else FUSED_TASKS
  *   FUSED_TASKS   one-pass red-black sweep per block (2x2 cell columns per

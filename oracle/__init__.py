@@ -43,6 +43,7 @@ def load():
         "oracle_impose_bc_f": [_P, _I, _I, _I, _P, _P],
         "oracle_sor_sweep_d": [_P, _I, _I, _I, _P, _P, _D],
         "oracle_sor_iterate_d": [_P, _I, _I, _I, _P, _P, _D, _P, _I],
+        "oracle_sor_iterate_inner_d": [_P, _I, _I, _I, _P, _P, _D, _P, _I, _I],
         "oracle_jacobi7_d": [_I, _I, _I, _P, _P, _P, _P, _D],
         "oracle_jacobi7_f": [_I, _I, _I, _P, _P, _P, _P, _D],
         "oracle_calc_ax_d": [_I, _I, _I, _P, _P, _P],
@@ -118,10 +119,12 @@ def sor_sweep(blocks, n, vc, f, s, omega):
     load().oracle_sor_sweep_d(blocks.ctypes.data, len(blocks), n, vc, f.ctypes.data, _ptr(s), float(omega))
 
 
-def sor_iterate(blocks, n, vc, f, s, omega, values, iters):
+def sor_iterate(blocks, n, vc, f, s, omega, values, iters, inner=1):
+    """`iters` outer iterations of examples/sor{1,2}/Solver.cpp's run() with
+    nLoopInner = `inner` sweeps per block between exchanges."""
     v = np.ascontiguousarray(values, dtype=np.float64)
-    load().oracle_sor_iterate_d(blocks.ctypes.data, len(blocks), n, vc, f.ctypes.data, _ptr(s), float(omega),
-                                v.ctypes.data, int(iters))
+    load().oracle_sor_iterate_inner_d(blocks.ctypes.data, len(blocks), n, vc, f.ctypes.data, _ptr(s), float(omega),
+                                      v.ctypes.data, int(iters), int(inner))
 
 
 def sor_residual(blocks, n, vc, f, s):

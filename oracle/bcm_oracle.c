@@ -280,17 +280,29 @@ void oracle_sor_sweep_d(const bcm_block_info* blk, int nb, int n, int vc, double
   }
 }
 
-/* Solver::run with nLoopInner == 1 (examples/sor1/Solver.cpp:74-129,
- * examples/sor2/Solver.cpp:77-133): per outer iteration, SOR + in-block BC
- * on every block, beginUpdateVC, setBoundaryCondition, endUpdateVC. */
-void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, double* data, const double* src,
-                          double omega, const double* vals, int iters) {
+/* Solver::run (examples/sor1/Solver.cpp:74-129, examples/sor2/Solver.cpp:
+ * 77-133): per outer iteration, on every block `inner` times (nLoopInner)
+ * SOR + in-block BC, the ghosts of other blocks staying as the last exchange
+ * left them; then beginUpdateVC, setBoundaryCondition, endUpdateVC.  A sweep
+ * writes only its block's interior and the in-block BC only its block's
+ * Dirichlet ghosts, so all blocks' inner loops in turn equal the reference's
+ * block-by-block order. */
+void oracle_sor_iterate_inner_d(const bcm_block_info* blk, int nb, int n, int vc, double* data, const double* src,
+                                double omega, const double* vals, int iters, int inner) {
   for (int it = 0; it < iters; ++it) {
-    oracle_sor_sweep_d(blk, nb, n, vc, data, src, omega);
-    oracle_dirichlet_d(blk, nb, n, vc, data, vals);
+    for (int in = 0; in < inner; ++in) {
+      oracle_sor_sweep_d(blk, nb, n, vc, data, src, omega);
+      oracle_dirichlet_d(blk, nb, n, vc, data, vals);
+    }
     oracle_ghost_fill_d(blk, nb, n, vc, data, BCM_GF_ALL);
     oracle_dirichlet_d(blk, nb, n, vc, data, vals);
   }
+}
+
+/* nLoopInner == 1. */
+void oracle_sor_iterate_d(const bcm_block_info* blk, int nb, int n, int vc, double* data, const double* src,
+                          double omega, const double* vals, int iters) {
+  oracle_sor_iterate_inner_d(blk, nb, n, vc, data, src, omega, vals, iters, 1);
 }
 
 /* ------------------------------------------------------------------ */

@@ -21,9 +21,10 @@ def load_golden(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     out = {k: z[k] for k in z.files}
     out["blocks"] = out["blocks"].view(BLOCK_DTYPE).copy()
-    for k in ("n", "vc", "method", "iters"):
+    for k in ("n", "vc", "method", "iters", "inner"):
         if k in out:
             out[k] = int(out[k])
+    out.setdefault("inner", 1)  # nLoopInner of the sor1 fixtures written before it was stored
     return out
 
 

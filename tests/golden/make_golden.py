@@ -59,6 +59,10 @@ SOR1 = {
     "sor1_adaptive_y": ("minLevel = 1\nmaxLevel = 3\ntreeType = simple\nordering = Z\ntype = y\n"
                         "nLoopOuter = 25\nnLoopInner = 1\nomega = 1.5\nsize = 8\nvc = 1\nb0 = 0.25\nb1 = 2.0\n"
                         "output = out.dat\n", 1, "Z"),
+    # nLoopInner = 3: three sweeps + in-block BC per block between exchanges
+    "sor1_inner3_z": ("minLevel = 1\nmaxLevel = 3\ntreeType = simple\nordering = Hilbert\ntype = z\n"
+                      "nLoopOuter = 12\nnLoopInner = 3\nomega = 1.3\nsize = 4\nvc = 1\nb0 = -0.5\nb1 = 1.25\n"
+                      "output = out.dat\n", 1, "Hilbert"),
 }
 
 
@@ -150,7 +154,8 @@ def main(only=()):
                     keys[k.strip()] = v.strip()
             np.savez_compressed(os.path.join(HERE, name + ".npz"), blocks=blocks.view(np.uint8), n=n, vc=vc,
                                 fout=fout, error_max=err, omega=float(keys["omega"]),
-                                iters=int(keys["nLoopOuter"]), type=keys["type"],
+                                iters=int(keys["nLoopOuter"]), inner=int(keys.get("nLoopInner", 1)),
+                                type=keys["type"],
                                 b0=float(keys.get("b0", 0.0)), b1=float(keys.get("b1", 1.0)))
             print(name, len(blocks), "errorMax", err)
 

@@ -77,3 +77,23 @@ def test_reference_sor1_reproduces_log4_on_the_gpu(tmp_path, separate, ordering)
     m = re.search(r"errorMax = (\S+)", r.stdout)
     assert m, r.stdout[-2000:]
     assert m.group(1) == "0.00353051", m.group(1)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="the drop-in sor1 binary is built where the reference tree is")
+def test_reference_sor1_with_three_inner_sweeps_on_the_gpu(tmp_path):
+    """nLoopInner = 3 on a level 1-3 mesh of 4^3 blocks, z Dirichlet
+    (b0 = -0.5, b1 = 1.25): the errorMax the reference library itself
+    printed for this configuration (tests/golden/sor1_inner3_z.npz)."""
+    from conftest import load_golden
+    g = load_golden("sor1_inner3_z")
+    conf = tmp_path / "inner3.conf"
+    conf.write_text("minLevel = 1\nmaxLevel = 3\ntreeType = simple\nordering = Hilbert\ntype = z\n"
+                    "nLoopOuter = 12\nnLoopInner = 3\nomega = 1.3\nsize = 4\nvc = 1\nb0 = -0.5\nb1 = 1.25\n"
+                    "output = out.dat\n")
+    print("[gpu case] reference sor1 driver, nLoopInner=3", flush=True)
+    r = subprocess.run([DROPIN, str(conf)], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    m = re.search(r"errorMax = (\S+)", r.stdout)
+    assert m, r.stdout[-2000:]
+    assert float(m.group(1)) == pytest.approx(float(g["error_max"]), rel=1e-6), (m.group(1), g["error_max"])

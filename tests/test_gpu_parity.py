@@ -68,17 +68,17 @@ def test_ghost_fill_vs_reference_fixture(name):
     np.testing.assert_array_equal(fld.download(), g["fout"])
 
 
-@pytest.mark.parametrize("name", ["sor1_log4", "sor1_adaptive_y"])
+@pytest.mark.parametrize("name", ["sor1_log4", "sor1_adaptive_y", "sor1_inner3_z"])
 def test_sor1_driver_vs_reference_fixture(name):
     g = load_golden(name)
     blocks, n, vc = g["blocks"], g["n"], g["vc"]
-    case(f"sor1 {name} nb={len(blocks)} n={n} iters={g['iters']}")
+    case(f"sor1 {name} nb={len(blocks)} n={n} iters={g['iters']} inner={g['inner']}")
     mesh = Mesh(blocks, n, vc)
     f = mesh.field(np.float64)
     vals = [g["b0"], g["b1"]]
     mesh.update_vc(f)
     mesh.dirichlet_mirror(f, vals)
-    mesh.sor_iterate(f, None, g["omega"], vals, g["iters"])
+    mesh.sor_iterate(f, None, g["omega"], vals, g["iters"], g["inner"])
     out = f.download()
     mask = W.face_ghost_mask(n, vc)
     np.testing.assert_array_equal(out[:, mask], g["fout"][:, mask])
@@ -524,6 +524,38 @@ def test_halo_schedule_equals_classic_schedule(dtype, iters, n, level):
             mesh.sor_iterate(f, sf, 1.25, vals, iters)
             out.append(f.download())
         np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("variant", [L.SOR_AUTO, L.SOR_HALO, L.SOR_FUSED_TASKS, L.SOR_FUSED_GATHER,
+                                     L.SOR_COLOUR_TASKS])
+@pytest.mark.parametrize("inner", [2, 3])
+def test_sor2_inner_loop_vs_oracle(variant, inner):
+    """nLoopInner > 1 (examples/sor2/Solver.cpp:106-109): `inner` sweeps +
+    in-block BC per block between exchanges, sor2 form with a source, on
+    every schedule (the halo one hands over to the fused sweeps)."""
+    n, level = 16, 3
+    blocks = W.shell_tree(level).blocks
+    case(f"sor2 inner={inner} variant={variant} n={n} nb={len(blocks)}")
+    rng = np.random.default_rng(inner)
+    f0 = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2))
+    s = W.rhs_field(blocks, n, 1, seed=9)
+    vals = [0.75]
+    mesh = Mesh(blocks, n, 1)
+    mesh.set_sor_variant(variant)
+    f = mesh.field(np.float64, f0)
+    mesh.sor_iterate(f, mesh.field(np.float64, s), 1.4, vals, 4, inner)
+    ref = f0.copy()
+    oracle.dirichlet(blocks, n, 1, ref, vals)
+    oracle.sor_iterate(blocks, n, 1, ref, s, 1.4, vals, 4, inner)
+    np.testing.assert_array_equal(f.download(), ref)
+
+
+def test_sor_iterate_inner_rejects_a_loop_count_below_one():
+    blocks = W.shell_tree(2).blocks
+    mesh = Mesh(blocks, 16, 1)
+    f = mesh.field(np.float64)
+    v = np.array([0.0])
+    assert mesh.lib.bcm_sor_iterate_inner(f.handle, None, 1.2, L.dptr(v), 1, 1, 0) == L.ERR_ARG
 
 
 @pytest.mark.parametrize("variant", [L.SOR_HALO, L.SOR_FUSED_TASKS])

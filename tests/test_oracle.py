@@ -121,14 +121,27 @@ def sor1_start(g):
     return f, vals
 
 
-@pytest.mark.parametrize("name", ["sor1_log4", "sor1_adaptive_y"])
+@pytest.mark.parametrize("name", ["sor1_log4", "sor1_adaptive_y", "sor1_inner3_z"])
 def test_sor1_bit_exact_vs_reference(name):
     g = load_golden(name)
     blocks, n, vc = g["blocks"], g["n"], g["vc"]
     f, vals = sor1_start(g)
-    oracle.sor_iterate(blocks, n, vc, f, None, g["omega"], vals, g["iters"])
+    oracle.sor_iterate(blocks, n, vc, f, None, g["omega"], vals, g["iters"], g["inner"])
     mask = W.face_ghost_mask(n, vc)
     np.testing.assert_array_equal(f[:, mask], g["fout"][:, mask])
+
+
+def test_sor1_inner_loop_count_matters():
+    """The nLoopInner = 3 fixture pins the inner loop: one, two or four
+    sweeps per exchange each give other values."""
+    g = load_golden("sor1_inner3_z")
+    blocks, n, vc = g["blocks"], g["n"], g["vc"]
+    assert g["inner"] == 3
+    mask = W.face_ghost_mask(n, vc)
+    for inner in (1, 2, 4):
+        f, vals = sor1_start(g)
+        oracle.sor_iterate(blocks, n, vc, f, None, g["omega"], vals, g["iters"], inner)
+        assert not np.array_equal(f[:, mask], g["fout"][:, mask]), inner
 
 
 def test_sor1_known_answer_log4():
