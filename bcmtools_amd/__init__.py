@@ -9,9 +9,9 @@ BlockManager / Scalar3D / Scalar3DUpdater API over that ABI.
 """
 from . import _lib
 from ._lib import BCMError, BLOCK_DTYPE
-from .core import Field, Mesh, Server, Tree
+from .core import CommPlan, Field, Mesh, Server, Tree, partition
 
-__all__ = ["BCMError", "BLOCK_DTYPE", "Field", "Mesh", "Tree", "build"]
+__all__ = ["BCMError", "BLOCK_DTYPE", "CommPlan", "Field", "Mesh", "Tree", "build", "partition"]
 
 
 def build(quiet=True):

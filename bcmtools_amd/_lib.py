@@ -136,6 +136,18 @@ class CommFace(ctypes.Structure):
 
 
 COMM_SAME, COMM_F2C, COMM_C2F = 0, 1, 2
+COMM_SEND, COMM_RECV = 0, 1
+COMM_ANY_LEVEL = 2
+
+
+class CommPeer(ctypes.Structure):
+    _fields_ = [
+        ("rank", ctypes.c_int32),
+        ("nfaces", ctypes.c_int32),
+        ("first", ctypes.c_int64),
+        ("offset", ctypes.c_int64),
+        ("bytes", ctypes.c_int64),
+    ]
 
 
 class ServerParams(ctypes.Structure):
@@ -223,6 +235,10 @@ SIGNATURES = {
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
     "bcm_ghost_fill_face": (_I, [_P, _I, _I, _I]),
     "bcm_comm_face_size": (ctypes.c_int64, [_P, _I]),
+    "bcm_comm_plan": (_I, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32,
+                           ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CommPeer), ctypes.c_int32,
+                           ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(CommFace), ctypes.c_int64,
+                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "bcm_comm_pack": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),
     "bcm_comm_unpack": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),
     "bcm_comm_pack_host": (_I, [_P, ctypes.POINTER(CommFace), _I, _P, ctypes.c_size_t]),

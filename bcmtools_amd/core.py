@@ -512,6 +512,43 @@ class Mesh:
         return rc, outer.value, inner.value, rr.value
 
 
+def partition(nblocks, nprocs):
+    """Rank of every block under the reference's Partition (Utils/include/
+    Partition.h:38-52): contiguous ranges, the first nblocks % nprocs
+    ranks one block longer."""
+    m, r = divmod(int(nblocks), int(nprocs))
+    sizes = [m + 1 if i < r else m for i in range(nprocs)]
+    return np.repeat(np.arange(nprocs, dtype=np.int32), sizes)
+
+
+class CommPlan:
+    """One rank's per-peer buffers (bcm_comm_plan; the reference's
+    CommBuffer over BlockManager's face lists, src/CommBuffer.cpp:55-94,
+    src/BlockManager.cpp:471-711).  `peers`: list of (rank, first face,
+    number of faces, byte offset, bytes); `faces`: the bcm_comm_face array
+    for bcm_comm_pack / bcm_comm_unpack; `total_bytes`: the buffer size."""
+
+    def __init__(self, blocks, n, vc, dtype, block_rank, rank, direction, axis=-1, level_diff=L.COMM_ANY_LEVEL):
+        lib = L.load()
+        blocks = np.ascontiguousarray(blocks)
+        br = np.ascontiguousarray(block_rank, dtype=np.int32)
+        if len(br) != len(blocks):
+            raise ValueError("block_rank needs one entry per block")
+        dt = L.F64 if np.dtype(dtype) == np.float64 else L.F32
+        npeers, nfaces, total = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        args = (blocks.ctypes.data, len(blocks), int(n), int(vc), dt, br.ctypes.data, int(rank), int(direction),
+                int(axis), int(level_diff))
+        L.check(lib.bcm_comm_plan(*args, None, 0, ctypes.byref(npeers), None, 0, ctypes.byref(nfaces),
+                                  ctypes.byref(total)), "bcm_comm_plan")
+        peers = (L.CommPeer * max(1, npeers.value))()
+        self.faces = (L.CommFace * max(1, nfaces.value))()
+        L.check(lib.bcm_comm_plan(*args, peers, npeers.value, ctypes.byref(npeers), self.faces, nfaces.value,
+                                  ctypes.byref(nfaces), ctypes.byref(total)), "bcm_comm_plan")
+        self.nfaces = nfaces.value
+        self.total_bytes = total.value
+        self.peers = [(p.rank, p.first, p.nfaces, p.offset, p.bytes) for p in peers[:npeers.value]]
+
+
 class Server:
     """Request server: queued Poisson solves admitted by an HBM budget and
     run with continuous batching (bcm_server_*)."""

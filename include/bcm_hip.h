@@ -295,6 +295,44 @@ typedef struct bcm_comm_face {
 int64_t bcm_comm_face_size(bcm_field_t field, int kind);
 int bcm_comm_pack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes);
 int bcm_comm_unpack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, const void* buffer, size_t bytes);
+/* One rank's per-peer buffers for a partition of a mesh's blocks over
+ * ranks (block_rank[b] >= 0 for every block): the layout of the
+ * reference's CommBuffer (src/CommBuffer.cpp:55-94, one buffer per peer,
+ * each face's slot padded to 8 bytes) over the face lists of
+ * BlockManager::setFaceLists / setSeparateFaceLists /
+ * setSeparateLevelDiffFaceLists (src/BlockManager.cpp:471-650) as
+ * setSend/RecvBufferPointers (:671-711) fill it.  Replaces what
+ * prepareForVCUpdate (:63-114) builds for the faces between ranks.
+ *   direction   BCM_COMM_SEND: the faces `rank` packs for each peer, in the
+ *               order it visits them (block, face, subface); BCM_COMM_RECV:
+ *               the faces it unpacks, in each sender's order
+ *               (RecvFaceComp, include/BlockManager.h:134-148)
+ *   axis        -1: all faces (AtOnce); 0..2: the SeparateXYZ list
+ *   level_diff  BCM_COMM_ANY_LEVEL, or -1 / 0 / +1: the
+ *               SeparateXYZ_SeparateLevelDiff list of that level difference
+ *               (neighbour minus own level, as seen by `rank`'s block;
+ *               updateVC_*_F2C receives on +1 and sends on -1, C2F the
+ *               reverse, include/BlockManager.h:562-592)
+ * The peers come in ascending rank order, each with a contiguous run of
+ * faces and a byte range of one buffer of *total_bytes bytes; a face's
+ * `offset` is in elements of `dtype` (BCM_F64 / BCM_F32), ready for
+ * bcm_comm_pack / bcm_comm_unpack on a field of that mesh.  With
+ * peers == faces == NULL only *npeers, *nfaces and *total_bytes are
+ * written; otherwise both arrays must hold max_peers / max_faces entries
+ * at least. */
+enum { BCM_COMM_SEND = 0, BCM_COMM_RECV = 1 };
+enum { BCM_COMM_ANY_LEVEL = 2 };
+typedef struct bcm_comm_peer {
+  int32_t rank;     /* the peer */
+  int32_t nfaces;   /* its faces: faces[first .. first + nfaces) */
+  int64_t first;
+  int64_t offset;   /* byte offset of the peer's buffer */
+  int64_t bytes;    /* the peer's message size */
+} bcm_comm_peer;
+int bcm_comm_plan(const bcm_block_info* blocks, int32_t nblocks, int32_t n, int32_t vc, int32_t dtype,
+                  const int32_t* block_rank, int32_t rank, int32_t direction, int32_t axis, int32_t level_diff,
+                  bcm_comm_peer* peers, int32_t max_peers, int32_t* npeers, bcm_comm_face* faces,
+                  int64_t max_faces, int64_t* nfaces, int64_t* total_bytes);
 /* The same with `buffer` in host memory (staged through a device buffer):
  * what the C++ facade's Scalar3DUpdater uses for a driver's send and
  * receive buffers. */

@@ -15,6 +15,10 @@
 //   ref_harness vtk   <out> <step> <difflevel> <mesh args>
 //       (VtkWriter::writeScalar of a double field into ./VTK; <out> gets
 //        the block table and the field)
+//   mpiexec -n R ref_harness commrank <prefix> <method 0|1|2> <dtype d|f> <mesh args>
+//       (the exchange on R ranks; each rank writes <prefix>.<rank> with the
+//        messages it sent, recorded through the MPI profiling interface,
+//        and its blocks after the exchange)
 //   ref_harness meshtime <mesh args>
 //       (prints the wall time of the reference's mesh construction, one
 //        rank: the octree, the blocks with their neighbour information,
@@ -125,8 +129,9 @@ BCMOctree* g_tree = 0;
 RootGrid* g_root = 0;
 Partition* g_partition = 0;
 
-// Build the mesh as examples/sor1/main.cpp:76-123 does, one rank.
-void build_mesh(const MeshArgs& a) {
+// Build the mesh as examples/sor1/main.cpp:76-123 does: the whole octree
+// on every rank, the blocks of this rank's Partition range registered.
+void build_mesh(const MeshArgs& a, int nprocs = 1, int rank = 0) {
   RootGrid* rootGrid = new RootGrid(a.rn[0], a.rn[1], a.rn[2]);
   int axis = -1;
   if (a.bc == "x") axis = 0;
@@ -143,7 +148,7 @@ void build_mesh(const MeshArgs& a) {
   else divider = new SphereDivider(rootGrid, a.minLevel, a.maxLevel, a.sph[0], a.sph[1], a.sph[2], a.sph[3]);
   BCMOctree::Ordering ord = a.ordering == "Hilbert" ? BCMOctree::HILBERT : BCMOctree::Z;
   g_tree = new BCMOctree(rootGrid, divider, ord);
-  Partition* partition = new Partition(1, g_tree->getNumLeafNode());
+  Partition* partition = new Partition(nprocs, g_tree->getNumLeafNode());
   g_root = rootGrid;
   g_partition = partition;
   Vec3i size(a.n, a.n, a.n);
@@ -153,7 +158,8 @@ void build_mesh(const MeshArgs& a) {
   BlockFactory factory(g_tree, partition, setter, size);
   BlockManager& bm = BlockManager::getInstance();
   std::vector<Node*>& leaves = g_tree->getLeafNodeArray();
-  for (int id = 0; id < g_tree->getNumLeafNode(); id++) bm.registerBlock(factory.makeBlock(leaves[id]));
+  for (int id = partition->getStart(rank); id < partition->getEnd(rank); id++)
+    bm.registerBlock(factory.makeBlock(leaves[id]));
   bm.endRegisterBlock();
 }
 
@@ -330,7 +336,79 @@ int run_meshtime(const MeshArgs& a) {
   return 0;
 }
 
+// Messages this rank hands to MPI_Isend (SendBuffer::sendBegin,
+// src/CommBuffer.cpp:146-153): the per-peer buffers of CommBuffer.
+struct SentMessage {
+  int dest, tag;
+  std::vector<unsigned char> bytes;
+};
+std::vector<SentMessage> g_sent;
+
+// The ghost exchange on `nprocs` ranks (run under mpiexec): every rank
+// registers its Partition range, fills its blocks with cell_value of the
+// global block index, runs updateVC by `method` and writes <prefix>.<rank>:
+//   int32 {rank, nprocs, start, end, nmsg}
+//   nmsg x (int32 {dest, tag}, int64 nbytes, bytes)   in send order
+//   the rank's blocks after the exchange (T, ghosted, block-major)
+template <typename T>
+int run_commrank(const char* prefix, int method, const MeshArgs& a) {
+  const int nprocs = MPI::COMM_WORLD.Get_size(), rank = MPI::COMM_WORLD.Get_rank();
+  build_mesh(a, nprocs, rank);
+  BlockManager& bm = BlockManager::getInstance();
+  const int start = g_partition->getStart(rank), end = g_partition->getEnd(rank);
+  int id = bm.setDataClass<Scalar3D<T>, Scalar3DUpdater<T> >(a.vc);
+  VCUpdateMethod::Type mt = method == 0 ? VCUpdateMethod::AtOnce
+                          : method == 1 ? VCUpdateMethod::SeparateXYZ
+                                        : VCUpdateMethod::SeparateXYZ_SeparateLevelDiff;
+  bm.prepareForVCUpdate(id, 100, mt);
+  const int nx0 = a.n + 2 * a.vc;
+  const size_t bvol = size_t(nx0) * nx0 * nx0;
+  for (int b = 0; b < bm.getNumBlock(); ++b) {
+    T* d = dynamic_cast<Scalar3D<T>*>(bm.getBlock(b)->getDataClass(id))->getData();
+    for (size_t c = 0; c < bvol; ++c) d[c] = T(cell_value(uint64_t(start + b), c));
+  }
+  g_sent.clear();
+  if (method == 0) {
+    bm.updateVC(id);
+  } else if (method == 1) {
+    bm.updateVC_X(id);
+    bm.updateVC_Y(id);
+    bm.updateVC_Z(id);
+  } else {
+    bm.updateVC_X_F2C(id); bm.updateVC_Y_F2C(id); bm.updateVC_Z_F2C(id);
+    bm.updateVC_X_Flat(id); bm.updateVC_Y_Flat(id); bm.updateVC_Z_Flat(id);
+    bm.updateVC_X_C2F(id); bm.updateVC_Y_C2F(id); bm.updateVC_Z_C2F(id);
+  }
+  char path[4096];
+  snprintf(path, sizeof(path), "%s.%d", prefix, rank);
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return 2;
+  int32_t h[5] = {rank, nprocs, start, end, int32_t(g_sent.size())};
+  fwrite(h, sizeof(h), 1, fp);
+  for (const SentMessage& m : g_sent) {
+    int32_t dt[2] = {m.dest, m.tag};
+    int64_t nb = int64_t(m.bytes.size());
+    fwrite(dt, sizeof(dt), 1, fp);
+    fwrite(&nb, sizeof(nb), 1, fp);
+    fwrite(m.bytes.data(), 1, m.bytes.size(), fp);
+  }
+  for (int b = 0; b < bm.getNumBlock(); ++b)
+    fwrite(dynamic_cast<Scalar3D<T>*>(bm.getBlock(b)->getDataClass(id))->getData(), sizeof(T), bvol, fp);
+  fclose(fp);
+  return 0;
+}
+
 }  // namespace
+
+// Profiling-interface hook: record each message, then send it.
+extern "C" int MPI_Isend(const void* buf, int count, MPI_Datatype type, int dest, int tag, MPI_Comm comm,
+                         MPI_Request* request) {
+  int size = 0;
+  PMPI_Type_size(type, &size);
+  const unsigned char* p = static_cast<const unsigned char*>(buf);
+  g_sent.push_back(SentMessage{dest, tag, std::vector<unsigned char>(p, p + size_t(count) * size_t(size))});
+  return PMPI_Isend(buf, count, type, dest, tag, comm, request);
+}
 
 int main(int argc, char** argv) {
   MPI::Init(argc, argv);
@@ -352,6 +430,9 @@ int main(int argc, char** argv) {
       rc = run_sor1(argv[2], argv[3], atoi(argv[4]), argv[5]);
     } else if (cmd == "meshtime" && parse_mesh(argv + 2, argc - 2, a) == 0) {
       rc = run_meshtime(a);
+    } else if (cmd == "commrank" && argc >= 5 && parse_mesh(argv + 5, argc - 5, a) == 0) {
+      rc = argv[4][0] == 'f' ? run_commrank<float>(argv[2], atoi(argv[3]), a)
+                             : run_commrank<double>(argv[2], atoi(argv[3]), a);
     } else if (cmd == "vtk" && argc >= 5 && parse_mesh(argv + 5, argc - 5, a) == 0) {
       rc = run_vtk(argv[2], atoi(argv[3]), atoi(argv[4]), a);
     }

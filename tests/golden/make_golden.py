@@ -53,6 +53,21 @@ VTK = {
     "vtk_sphere": (7, 2, "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3"),
 }
 
+COMMRANK = {
+    # name: (ranks, method, dtype, mesh args, ghost fixture of the same mesh,
+    #        method and dtype, or None to store the mesh and fields here).
+    # The reference's exchange on that many MPI ranks (mpiexec); every
+    # message a rank sends is recorded through the MPI profiling interface.
+    "comm3_sphere_atonce_d": (3, 0, "d", "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3", "ghost_sphere_atonce_d"),
+    "comm2_sphere_xyz_d": (2, 1, "d", "sphere 1 3 Z 2 1 1 dir 4 1 1.0 0.5 0.5 0.3", "ghost_sphere_xyz_d"),
+    "comm4_sphere_vc2_leveldiff_f": (4, 2, "f", "sphere 1 3 Hilbert 1 1 1 dir 4 2 0.5 0.5 0.5 0.3",
+                                     "ghost_sphere_vc2_leveldiff_f"),
+    "comm5_periodic_x_d": (5, 0, "d", "simple 1 3 Z 1 1 1 x 4 1", "ghost_periodic_x_d"),
+    # n = 2 in FP32: 4-byte F2C slots, padded to 8 in the buffers
+    "comm3_simple_n2_f": (3, 0, "f", "simple 0 3 Z 1 1 1 x 2 1", None),
+}
+MPIEXEC = "/opt/conda/bin/mpiexec"
+
 SOR1 = {
     # name: (config text, roots, ordering)
     "sor1_log4": (None, 1, "Z"),
@@ -134,6 +149,46 @@ def main(only=()):
                                 args=margs,
                                 **vti)
             print(name, len(blocks), "vti kept", len(keep))
+        for name, (nprocs, method, dt, margs, ghost) in COMMRANK.items():
+            if not wanted(name):
+                continue
+            dtype = np.float64 if dt == "d" else np.float32
+            single = os.path.join(tmp, name + ".bin")
+            run(["ghost", single, str(method), dt] + margs.split(), tmp)
+            blocks, n, vc, (fin, fout) = read_dump(single, 2, dtype)
+            prefix = os.path.join(tmp, name + ".rank")
+            subprocess.run([MPIEXEC, "-n", str(nprocs), HARNESS, "commrank", prefix, str(method), dt] + margs.split(),
+                           cwd=tmp, check=True, capture_output=True, text=True)
+            src, dst, tag, start, end, lens, blobs, parts = [], [], [], [], [], [], [], []
+            nx0 = n + 2 * vc
+            for r in range(nprocs):
+                with open(f"{prefix}.{r}", "rb") as fp:
+                    rank, npr, st, en, nmsg = (int(v) for v in np.frombuffer(fp.read(20), dtype=np.int32))
+                    assert (rank, npr) == (r, nprocs)
+                    start.append(st)
+                    end.append(en)
+                    for _ in range(nmsg):
+                        d, t = (int(v) for v in np.frombuffer(fp.read(8), dtype=np.int32))
+                        nb = int(np.frombuffer(fp.read(8), dtype=np.int64)[0])
+                        src.append(r)
+                        dst.append(d)
+                        tag.append(t)
+                        lens.append(nb)
+                        blobs.append(fp.read(nb))
+                    parts.append(np.frombuffer(fp.read(), dtype=dtype).reshape(en - st, nx0, nx0, nx0))
+            # The exchange over ranks gives the one-rank result.
+            assert np.array_equal(np.concatenate(parts), fout), name
+            extra = {}
+            if ghost is None:
+                extra = dict(blocks=blocks.view(np.uint8), n=n, vc=vc, fin=fin, fout=fout, args=margs)
+            else:
+                g = np.load(os.path.join(HERE, ghost + ".npz"))
+                assert np.array_equal(g["fin"], fin) and np.array_equal(g["fout"], fout), (name, ghost)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), ranks=nprocs, method=method, dtype=dt,
+                                ghost=ghost or "", start=np.array(start), end=np.array(end), msg_src=np.array(src),
+                                msg_dst=np.array(dst), msg_tag=np.array(tag), msg_len=np.array(lens, dtype=np.int64),
+                                msg_bytes=np.frombuffer(b"".join(blobs), dtype=np.uint8), **extra)
+            print(name, nprocs, "ranks", len(src), "messages", sum(lens), "bytes")
         for name, (conf, rn, ordering) in SOR1.items():
             if not wanted(name):
                 continue
