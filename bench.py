@@ -923,6 +923,47 @@ def comm_exchange(mesh, blocks, f, args):
             "unpack_ms": 1e3 * min(tu[skip:])}
 
 
+def rank_buffers(mesh, blocks, f, args, ranks=8):
+    """The mesh split over `ranks` ranks by the reference's Partition: the
+    per-peer buffers of every rank (bcm_comm_plan: host time of all send
+    and receive plans), then, rank by rank, the pack of its sends and the
+    unpack of its receives on this GPU (wall time per call, face-list
+    upload included; the slowest rank is reported)."""
+    from bcmtools_amd import CommPlan, partition
+    from bcmtools_amd import _lib as L
+
+    lib = L.load()
+    rank_of = partition(len(blocks), ranks)
+    t0 = time.perf_counter()
+    sends = [CommPlan(blocks, mesh.n, mesh.vc, np.float64, rank_of, r, L.COMM_SEND) for r in range(ranks)]
+    recvs = [CommPlan(blocks, mesh.n, mesh.vc, np.float64, rank_of, r, L.COMM_RECV) for r in range(ranks)]
+    plan_ms = 1e3 * (time.perf_counter() - t0)
+    stage = mesh.field(np.float64)  # device memory for one rank's buffer
+    buf = lib.bcm_field_device_ptr(stage.handle)
+    assert lib.bcm_field_bytes(stage.handle) >= max(p.total_bytes for p in sends + recvs)
+    worst_p, worst_u = 0.0, 0.0
+    for r in range(ranks):
+        tp, tu = [], []
+        for _ in range(max(1, args.warmup) + max(1, min(args.steps, 10))):
+            t1 = time.perf_counter()
+            L.check(lib.bcm_comm_pack(f.handle, sends[r].faces, sends[r].nfaces, buf, sends[r].total_bytes),
+                    "bcm_comm_pack")
+            t2 = time.perf_counter()
+            L.check(lib.bcm_comm_unpack(f.handle, recvs[r].faces, recvs[r].nfaces, buf, recvs[r].total_bytes),
+                    "bcm_comm_unpack")
+            tp.append(t2 - t1)
+            tu.append(time.perf_counter() - t2)
+        skip = max(1, args.warmup)
+        worst_p = max(worst_p, 1e3 * min(tp[skip:]))
+        worst_u = max(worst_u, 1e3 * min(tu[skip:]))
+    del stage
+    return {"ranks": ranks, "plan_ms_all_ranks": plan_ms,
+            "peers_per_rank": [len(p.peers) for p in sends],
+            "send_bytes_per_rank": [p.total_bytes for p in sends],
+            "faces_between_ranks": sum(p.nfaces for p in sends),
+            "pack_ms_slowest_rank": worst_p, "unpack_ms_slowest_rank": worst_u}
+
+
 def bench_ghost(args):
     """The reference's updateVC on its own (include/BlockManager.h:311-726,
     include/Scalar3DUpdater.h:190-347): the complete ghost fill of an FP64
@@ -957,6 +998,7 @@ def bench_ghost(args):
         if n == 32:
             row["host_resident_exchange_ms"] = host_exchange(mesh, f, args)
             row["pack_unpack_ms"] = comm_exchange(mesh, tree.blocks, f, args)
+            row["rank_buffers"] = rank_buffers(mesh, tree.blocks, f, args)
         print(f"[bench] ghost fill {name}: {row}", file=sys.stderr, flush=True)
         out[name] = row
         mesh.close()
