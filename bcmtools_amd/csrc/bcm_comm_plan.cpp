@@ -72,7 +72,9 @@ extern "C" int bcm_comm_plan(const bcm_block_info* blocks, int32_t nblocks, int3
           if (nb < 0 || block_rank[nb] == rank) continue;  // no neighbour, or a local copy
           Slot sl;
           sl.peer = block_rank[nb];
-          sl.key = int64_t(nb) * (BCM_NUM_FACE * 4) + int64_t(f ^ 1) * 4 + bi.nb_subface[f];
+          // The neighbour subface is meaningful for a coarser neighbour
+          // only; the reference's NeighborInfo holds 0 otherwise.
+          sl.key = int64_t(nb) * (BCM_NUM_FACE * 4) + int64_t(f ^ 1) * 4 + (ld == -1 ? bi.nb_subface[f] : 0);
           sl.face.block = b;
           sl.face.face = f;
           sl.face.offset = 0;
