@@ -9,9 +9,9 @@ BlockManager / Scalar3D / Scalar3DUpdater API over that ABI.
 """
 from . import _lib
 from ._lib import BCMError, BLOCK_DTYPE
-from .core import CommPlan, Field, Mesh, Server, Tree, partition
+from .core import CommList, CommPlan, Field, Mesh, Server, Tree, partition
 
-__all__ = ["BCMError", "BLOCK_DTYPE", "CommPlan", "Field", "Mesh", "Tree", "build", "partition"]
+__all__ = ["BCMError", "BLOCK_DTYPE", "CommList", "CommPlan", "Field", "Mesh", "Tree", "build", "partition"]
 
 
 def build(quiet=True):

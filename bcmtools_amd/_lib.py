@@ -235,6 +235,10 @@ SIGNATURES = {
     "bcm_sor_residual": (_I, [_P, _P, _DP]),
     "bcm_ghost_fill_face": (_I, [_P, _I, _I, _I]),
     "bcm_comm_face_size": (ctypes.c_int64, [_P, _I]),
+    "bcm_comm_list_create": (_I, [_P, ctypes.POINTER(CommFace), _I, ctypes.POINTER(_P)]),
+    "bcm_comm_list_destroy": (_I, [_P]),
+    "bcm_comm_pack_list": (_I, [_P, _P, _P, ctypes.c_size_t]),
+    "bcm_comm_unpack_list": (_I, [_P, _P, _P, ctypes.c_size_t]),
     "bcm_comm_plan": (_I, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32,
                            ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CommPeer), ctypes.c_int32,
                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(CommFace), ctypes.c_int64,

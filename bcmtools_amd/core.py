@@ -549,6 +549,37 @@ class CommPlan:
         self.peers = [(p.rank, p.first, p.nfaces, p.offset, p.bytes) for p in peers[:npeers.value]]
 
 
+class CommList:
+    """A face list uploaded once (bcm_comm_list_create) for repeated packs
+    and unpacks on one mesh; pack / unpack enqueue on the mesh's stream and
+    return without waiting (Mesh.sync before reading a buffer)."""
+
+    def __init__(self, mesh, faces, nfaces):
+        self.lib = mesh.lib
+        h = ctypes.c_void_p()
+        L.check(self.lib.bcm_comm_list_create(mesh.handle, faces, int(nfaces), ctypes.byref(h)),
+                "bcm_comm_list_create")
+        self.handle = h
+
+    def pack(self, field, buffer, nbytes):
+        L.check(self.lib.bcm_comm_pack_list(field.handle, self.handle, buffer, int(nbytes)), "bcm_comm_pack_list")
+
+    def unpack(self, field, buffer, nbytes):
+        L.check(self.lib.bcm_comm_unpack_list(field.handle, self.handle, buffer, int(nbytes)),
+                "bcm_comm_unpack_list")
+
+    def close(self):
+        if self.handle:
+            self.lib.bcm_comm_list_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Request server: queued Poisson solves admitted by an HBM budget and
     run with continuous batching (bcm_server_*)."""

@@ -916,29 +916,43 @@ __global__ __launch_bounds__(256) void k_comm(T* __restrict__ d, T* __restrict__
   }
 }
 
-template <typename T, bool PACK>
-int comm_run(bcm_field* f, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes) {
-  bcm_mesh* m = f->mesh;
-  const long long cap = (long long)(bytes / sizeof(T));
+// Checks a face list against the mesh; *need: the elements the buffer must
+// hold (the largest offset + face size).
+int comm_check(const bcm_mesh* m, const bcm_comm_face* faces, int nfaces, long long* need) {
   const long long full = (long long)m->n * m->n * m->vc;
+  long long hi = 0;
   for (int i = 0; i < nfaces; ++i) {
     const bcm_comm_face& c = faces[i];
     if (c.block < 0 || c.block >= m->nblocks || c.face < 0 || c.face >= BCM_NUM_FACE || c.subface < 0 ||
-        c.subface > 3 || c.kind < BCM_COMM_SAME || c.kind > BCM_COMM_C2F || c.offset < 0)
+        c.subface > 3 || c.kind < BCM_COMM_SAME || c.kind > BCM_COMM_C2F || c.offset < 0 ||
+        c.offset > (1ll << 60))
       return BCM_ERR_ARG;
-    const long long need = c.kind == BCM_COMM_F2C ? full / 4 : full;
-    if (c.offset > cap - need) return BCM_ERR_ARG;
+    hi = std::max(hi, (long long)c.offset + (c.kind == BCM_COMM_F2C ? full / 4 : full));
   }
+  *need = hi;
+  return BCM_OK;
+}
+
+template <typename T, bool PACK>
+hipError_t comm_launch(bcm_field* f, const bcm_comm_face* d_faces, int nfaces, void* buffer) {
+  bcm_mesh* m = f->mesh;
+  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
+  hipLaunchKernelGGL((k_comm<T, PACK>), dim3(unsigned(nfaces)), dim3(256), 0, m->stream, static_cast<T*>(f->data),
+                     static_cast<T*>(buffer), d_faces, g);
+  return hipGetLastError();
+}
+
+template <typename T, bool PACK>
+int comm_run(bcm_field* f, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes) {
+  bcm_mesh* m = f->mesh;
+  long long need = 0;
+  BCM_TRY(comm_check(m, faces, nfaces, &need));
+  if (need > (long long)(bytes / sizeof(T))) return BCM_ERR_ARG;
   if (nfaces == 0) return BCM_OK;
   void* df = nullptr;
   BCM_TRY(dev_alloc(&df, sizeof(bcm_comm_face) * size_t(nfaces)));
-  Geo g{m->n, m->vc, m->nx0, (long long)m->bvol};
   hipError_t e = hipMemcpyAsync(df, faces, sizeof(bcm_comm_face) * size_t(nfaces), hipMemcpyHostToDevice, m->stream);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL((k_comm<T, PACK>), dim3(unsigned(nfaces)), dim3(256), 0, m->stream, static_cast<T*>(f->data),
-                       static_cast<T*>(buffer), static_cast<const bcm_comm_face*>(df), g);
-    e = hipGetLastError();
-  }
+  if (e == hipSuccess) e = comm_launch<T, PACK>(f, static_cast<const bcm_comm_face*>(df), nfaces, buffer);
   if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   dev_free(df);
   return e == hipSuccess ? BCM_OK : BCM_ERR_HIP;
@@ -946,6 +960,70 @@ int comm_run(bcm_field* f, const bcm_comm_face* faces, int nfaces, void* buffer,
 
 }  // namespace
 }  // namespace bcm
+
+// A face list uploaded once (what prepareForVCUpdate keeps between
+// exchanges): packs and unpacks over it allocate nothing, copy nothing
+// from the host and do not wait for the stream.
+struct bcm_comm_list {
+  bcm_mesh* mesh = nullptr;
+  bcm_comm_face* d_faces = nullptr;
+  int nfaces = 0;
+  long long need = 0;  // elements a buffer must hold
+};
+
+extern "C" int bcm_comm_list_create(bcm_mesh_t m, const bcm_comm_face* faces, int nfaces, bcm_comm_list_t* out) {
+  if (!out) return BCM_ERR_ARG;
+  *out = nullptr;
+  if (!m || nfaces < 0 || (nfaces > 0 && !faces)) return BCM_ERR_ARG;
+  long long need = 0;
+  BCM_TRY(bcm::comm_check(m, faces, nfaces, &need));
+  bcm_comm_list* l = new (std::nothrow) bcm_comm_list;
+  if (!l) return BCM_ERR_NOMEM;
+  l->mesh = m;
+  l->nfaces = nfaces;
+  l->need = need;
+  if (nfaces > 0) {
+    const size_t bytes = sizeof(bcm_comm_face) * size_t(nfaces);
+    int rc = bcm::dev_alloc(reinterpret_cast<void**>(&l->d_faces), bytes);
+    if (rc == BCM_OK && (hipMemcpyAsync(l->d_faces, faces, bytes, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+                         hipStreamSynchronize(m->stream) != hipSuccess))
+      rc = BCM_ERR_HIP;
+    if (rc != BCM_OK) {
+      bcm::dev_free(l->d_faces);
+      delete l;
+      return rc;
+    }
+  }
+  *out = l;
+  return BCM_OK;
+}
+
+extern "C" int bcm_comm_list_destroy(bcm_comm_list_t l) {
+  if (!l) return BCM_OK;
+  if (l->d_faces) (void)hipStreamSynchronize(l->mesh->stream);  // no launch may still read the list
+  bcm::dev_free(l->d_faces);
+  delete l;
+  return BCM_OK;
+}
+
+template <bool PACK>
+static int comm_list_run(bcm_field_t f, bcm_comm_list_t l, void* buffer, size_t bytes) {
+  if (!f || !f->mesh || !l || l->mesh != f->mesh || (l->nfaces > 0 && !buffer)) return BCM_ERR_ARG;
+  const size_t es = f->dtype == BCM_F64 ? sizeof(double) : sizeof(float);
+  if (l->need > (long long)(bytes / es)) return BCM_ERR_ARG;
+  if (l->nfaces == 0) return BCM_OK;
+  const hipError_t e = f->dtype == BCM_F64 ? bcm::comm_launch<double, PACK>(f, l->d_faces, l->nfaces, buffer)
+                                           : bcm::comm_launch<float, PACK>(f, l->d_faces, l->nfaces, buffer);
+  return e == hipSuccess ? BCM_OK : BCM_ERR_HIP;
+}
+
+extern "C" int bcm_comm_pack_list(bcm_field_t f, bcm_comm_list_t l, void* buffer, size_t bytes) {
+  return comm_list_run<true>(f, l, buffer, bytes);
+}
+
+extern "C" int bcm_comm_unpack_list(bcm_field_t f, bcm_comm_list_t l, const void* buffer, size_t bytes) {
+  return comm_list_run<false>(f, l, const_cast<void*>(buffer), bytes);
+}
 
 extern "C" int64_t bcm_comm_face_size(bcm_field_t f, int kind) {
   if (!f || !f->mesh || kind < BCM_COMM_SAME || kind > BCM_COMM_C2F) return -BCM_ERR_ARG;

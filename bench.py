@@ -941,7 +941,10 @@ def rank_buffers(mesh, blocks, f, args, ranks=8):
     stage = mesh.field(np.float64)  # device memory for one rank's buffer
     buf = lib.bcm_field_device_ptr(stage.handle)
     assert lib.bcm_field_bytes(stage.handle) >= max(p.total_bytes for p in sends + recvs)
-    worst_p, worst_u = 0.0, 0.0
+    from bcmtools_amd import CommList
+    lists = [(CommList(mesh, sends[r].faces, sends[r].nfaces), CommList(mesh, recvs[r].faces, recvs[r].nfaces))
+             for r in range(ranks)]
+    worst_p, worst_u, worst_pl, worst_ul = 0.0, 0.0, 0.0, 0.0
     for r in range(ranks):
         tp, tu = [], []
         for _ in range(max(1, args.warmup) + max(1, min(args.steps, 10))):
@@ -956,12 +959,29 @@ def rank_buffers(mesh, blocks, f, args, ranks=8):
         skip = max(1, args.warmup)
         worst_p = max(worst_p, 1e3 * min(tp[skip:]))
         worst_u = max(worst_u, 1e3 * min(tu[skip:]))
+        # The same over the uploaded lists: enqueue, then wait for the stream.
+        tp, tu = [], []
+        for _ in range(max(1, args.warmup) + max(1, min(args.steps, 10))):
+            t1 = time.perf_counter()
+            lists[r][0].pack(f, buf, sends[r].total_bytes)
+            mesh.sync()
+            t2 = time.perf_counter()
+            lists[r][1].unpack(f, buf, recvs[r].total_bytes)
+            mesh.sync()
+            tp.append(t2 - t1)
+            tu.append(time.perf_counter() - t2)
+        worst_pl = max(worst_pl, 1e3 * min(tp[skip:]))
+        worst_ul = max(worst_ul, 1e3 * min(tu[skip:]))
+    for ls, lr in lists:
+        ls.close()
+        lr.close()
     del stage
     return {"ranks": ranks, "plan_ms_all_ranks": plan_ms,
             "peers_per_rank": [len(p.peers) for p in sends],
             "send_bytes_per_rank": [p.total_bytes for p in sends],
             "faces_between_ranks": sum(p.nfaces for p in sends),
-            "pack_ms_slowest_rank": worst_p, "unpack_ms_slowest_rank": worst_u}
+            "pack_ms_slowest_rank": worst_p, "unpack_ms_slowest_rank": worst_u,
+            "uploaded_list_pack_ms_slowest_rank": worst_pl, "uploaded_list_unpack_ms_slowest_rank": worst_ul}
 
 
 def bench_ghost(args):

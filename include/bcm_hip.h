@@ -295,6 +295,19 @@ typedef struct bcm_comm_face {
 int64_t bcm_comm_face_size(bcm_field_t field, int kind);
 int bcm_comm_pack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, void* buffer, size_t bytes);
 int bcm_comm_unpack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, const void* buffer, size_t bytes);
+/* A face list uploaded to the device once, for exchanges repeated over
+ * the same faces (the setup prepareForVCUpdate does once,
+ * src/BlockManager.cpp:63-114; each later exchange only packs and unpacks).
+ * bcm_comm_pack_list / bcm_comm_unpack_list enqueue the kernel on the
+ * mesh's stream and return: no allocation, no host copy, no wait (use
+ * bcm_mesh_sync before reading the buffer on the host).  The list belongs
+ * to one mesh; fields of other meshes are rejected.  Destroying a list
+ * waits for the mesh's stream. */
+typedef struct bcm_comm_list* bcm_comm_list_t;
+int bcm_comm_list_create(bcm_mesh_t mesh, const bcm_comm_face* faces, int nfaces, bcm_comm_list_t* out);
+int bcm_comm_list_destroy(bcm_comm_list_t list);
+int bcm_comm_pack_list(bcm_field_t field, bcm_comm_list_t list, void* buffer, size_t bytes);
+int bcm_comm_unpack_list(bcm_field_t field, bcm_comm_list_t list, const void* buffer, size_t bytes);
 /* One rank's per-peer buffers for a partition of a mesh's blocks over
  * ranks (block_rank[b] >= 0 for every block): the layout of the
  * reference's CommBuffer (src/CommBuffer.cpp:55-94, one buffer per peer,

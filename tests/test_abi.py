@@ -67,7 +67,7 @@ def test_field_api_rejects_bad_arguments_without_gpu():
 _SETTINGS = {"bcm_tree_set_threads"}
 _COUNTS = {"bcm_tree_num_blocks", "bcm_sor_get_variant", "bcm_server_poll", "bcm_comm_face_size"}
 _DESTROYS = {"bcm_tree_destroy", "bcm_mesh_destroy", "bcm_field_destroy", "bcm_server_destroy", "bcm_vfield_destroy",
-             "bcm_host_free"}
+             "bcm_host_free", "bcm_comm_list_destroy"}
 
 
 def test_every_entry_point_rejects_null_handles_without_gpu():

@@ -17,7 +17,7 @@ import sys
 import numpy as np
 import pytest
 
-from bcmtools_amd import CommPlan, Mesh, partition
+from bcmtools_amd import CommList, CommPlan, Mesh, partition
 from bcmtools_amd import _lib as L
 
 from hipmem import DeviceArray
@@ -150,3 +150,55 @@ def test_eight_ranks_on_the_config2_mesh():
     got = out.download()
     np.testing.assert_array_equal(got[covered], want[covered])
     np.testing.assert_array_equal(got[~covered], data[~covered])
+
+
+def test_uploaded_face_lists_equal_the_per_call_lists():
+    """bcm_comm_pack_list / bcm_comm_unpack_list over lists uploaded once:
+    the buffers and fields of bcm_comm_pack / bcm_comm_unpack, bit for bit,
+    over three exchanges of changing data on the 8-rank config-2 plans; a
+    list is refused for another mesh's field and for a short buffer."""
+    from bcmtools_amd import workloads as W
+    blocks = W.config2_tree().blocks
+    n, vc, ranks = 32, 1, 8
+    rank_of = partition(len(blocks), ranks)
+    mesh = Mesh(blocks, n, vc)
+    case("comm lists: config 2, 8 ranks, 3 exchanges")
+    plans = [(CommPlan(blocks, n, vc, np.float64, rank_of, r, L.COMM_SEND),
+              CommPlan(blocks, n, vc, np.float64, rank_of, r, L.COMM_RECV)) for r in (0, 3, 7)]
+    lists = [(CommList(mesh, s.faces, s.nfaces), CommList(mesh, q.faces, q.nfaces)) for s, q in plans]
+    rng = np.random.default_rng(11)
+    for it in range(3):
+        data = rng.standard_normal((len(blocks), n + 2, n + 2, n + 2))
+        a = mesh.field(np.float64, data)
+        b = mesh.field(np.float64, data)
+        for (send, recv), (ls, lr) in zip(plans, lists):
+            nb = max(send.total_bytes, recv.total_bytes)
+            b1 = DeviceArray(np.zeros(nb, np.uint8))
+            b2 = DeviceArray(np.zeros(nb, np.uint8))
+            assert mesh.lib.bcm_comm_pack(a.handle, send.faces, send.nfaces, b1.ptr, send.total_bytes) == L.OK
+            ls.pack(b, b2.ptr, send.total_bytes)
+            mesh.sync()
+            np.testing.assert_array_equal(b1.get(), b2.get())
+            # Unpack the same received bytes both ways.
+            msg = rng.standard_normal(nb // 8).view(np.uint8)
+            b1.put(msg)
+            b2.put(msg)
+            assert mesh.lib.bcm_comm_unpack(a.handle, recv.faces, recv.nfaces, b1.ptr, recv.total_bytes) == L.OK
+            lr.unpack(b, b2.ptr, recv.total_bytes)
+            mesh.sync()
+            b1.free()
+            b2.free()
+        np.testing.assert_array_equal(a.download(), b.download())
+    other = Mesh(blocks, n, vc)
+    f = other.field(np.float64)
+    ls = lists[0][0]
+    buf = DeviceArray(np.zeros(plans[0][0].total_bytes, np.uint8))
+    assert mesh.lib.bcm_comm_pack_list(f.handle, ls.handle, buf.ptr, plans[0][0].total_bytes) == L.ERR_ARG
+    g = mesh.field(np.float64)
+    assert mesh.lib.bcm_comm_pack_list(g.handle, ls.handle, buf.ptr, plans[0][0].total_bytes - 8) == L.ERR_ARG
+    assert mesh.lib.bcm_comm_pack_list(g.handle, ls.handle, buf.ptr, plans[0][0].total_bytes) == L.OK
+    mesh.sync()
+    buf.free()
+    for ls, lr in lists:
+        ls.close()
+        lr.close()
