@@ -113,15 +113,21 @@ def test_plans_name_the_references_peers_and_message_sizes(name):
                     assert all(axis < 0 or f.face // 2 == axis for f in faces)
 
 
-def test_plans_cover_every_link_once():
+@pytest.mark.parametrize("name,split", [("comm5_periodic_x_d", "partition"), ("comm5_periodic_x_d", "scattered"),
+                                        ("comm4_sphere_vc2_leveldiff_f", "scattered")])
+def test_plans_cover_every_link_once(name, split):
     """Over all ranks, the send and receive plans of an AtOnce exchange
     list each link between ranks exactly once, the receiver's faces
-    matching the sender's, slot by slot."""
-    c = load_comm("comm5_periodic_x_d")
+    matching the sender's, slot by slot: for the reference's contiguous
+    partition and for blocks scattered over ranks at random."""
+    c = load_comm(name)
     g = c["mesh"]
     blocks = g["blocks"]
     ranks = int(c["ranks"])
-    rank_of = partition(len(blocks), ranks)
+    if split == "partition":
+        rank_of = partition(len(blocks), ranks)
+    else:
+        rank_of = np.random.default_rng(ranks).integers(0, ranks, len(blocks)).astype(np.int32)
     sends = [CommPlan(blocks, g["n"], g["vc"], np.float64, rank_of, r, L.COMM_SEND) for r in range(ranks)]
     recvs = [CommPlan(blocks, g["n"], g["vc"], np.float64, rank_of, r, L.COMM_RECV) for r in range(ranks)]
     links = 0
