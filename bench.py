@@ -984,6 +984,49 @@ def rank_buffers(mesh, blocks, f, args, ranks=8):
             "uploaded_list_pack_ms_slowest_rank": worst_pl, "uploaded_list_unpack_ms_slowest_rank": worst_ul}
 
 
+def bench_sorinner(args):
+    """The sor drivers with nLoopInner > 1 (examples/sor2/Solver.cpp:
+    106-109): `--inner` sweeps + in-block BC per block between exchanges,
+    on the 3,816 x 32^3 shell mesh, FP64 (bcm_sor_iterate_inner; the fused
+    sweeps over the stored ghosts, since the halo schedule cannot serve an
+    inner loop).  A cell update is one cell of one sweep."""
+    from bcmtools_amd import Mesh
+    from bcmtools_amd import _lib as L
+    from bcmtools_amd import workloads as W
+
+    omega, inner = 1.6, max(1, args.inner)
+    t = W.config2_tree()
+    blocks, n, vc = t.blocks, 32, 1
+    ncell = len(blocks) * n ** 3
+    s = -W.rhs_field(blocks, n, vc, seed=0)
+    mesh = Mesh(blocks, n, vc)
+    gf = mesh.field(np.float64)
+    gs = mesh.field(np.float64, s)
+    vals = [0.0]
+    mesh.dirichlet_mirror(gf, vals)
+    mesh.sor_iterate(gf, gs, omega, vals, max(1, args.warmup), inner)
+    mesh.sync()
+    mesh.reset_timing()
+    mesh.set_timing(True)
+    t0 = time.perf_counter()
+    mesh.sor_iterate(gf, gs, omega, vals, args.steps, inner)
+    mesh.sync()
+    elapsed = time.perf_counter() - t0
+    mesh.set_timing(False)
+    sw_ms, sw_n = mesh.kernel_time(L.K_SOR)
+    gh_ms, gh_n = mesh.kernel_time(L.K_GHOST)
+    sweep = sw_ms / max(sw_n, 1)
+    return {"metric": "stencil GCUPS (FP64 red-black SOR, nLoopInner sweeps per exchange)",
+            "value": ncell * args.steps * inner / elapsed / 1e9, "unit": "GCUPS", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"shell octree level 5, 3816 x 32^3 blocks, sor2 RB-SOR omega=1.6, nLoopInner "
+                                   f"{inner} (sweep + in-block Dirichlet) then ghost fill + Dirichlet per step",
+                       "inner": inner},
+            "sweep_ms": sweep, "sweep_algorithmic_GBps": 24.0 * ncell / (sweep * 1e-3) / 1e9,
+            "ghost_fill_ms": gh_ms / max(gh_n, 1)}
+
+
 def bench_ghost(args):
     """The reference's updateVC on its own (include/BlockManager.h:311-726,
     include/Scalar3DUpdater.h:190-347): the complete ghost fill of an FP64
@@ -1295,7 +1338,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sor",
                     choices=["sor", "jacobi", "bicgstab", "batched", "serving", "copy", "restart", "ghost", "mesh",
-                             "sortol"])
+                             "sortol", "sorinner"])
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--lam", type=float, default=10.0, help="serving: arrival rate (requests/s)")
     ap.add_argument("--lams", default="", help="serving: comma-separated arrival rates to sweep (same requests)")
@@ -1327,6 +1370,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=2000, help="sortol: sweeps between residual checks")
     ap.add_argument("--budget", type=float, default=900.0, help="sortol: seconds this process runs")
     ap.add_argument("--state", default="", help="sortol: directory keeping the run across processes")
+    ap.add_argument("--inner", type=int, default=3, help="sorinner: sweeps per exchange (nLoopInner)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
@@ -1342,7 +1386,8 @@ def main():
         sys.exit(2)
     res = {"sor": bench_sor, "jacobi": bench_jacobi, "bicgstab": bench_bicgstab, "batched": bench_batched,
            "serving": bench_serving, "copy": bench_copy, "restart": bench_restart,
-           "ghost": bench_ghost, "mesh": bench_mesh, "sortol": bench_sortol}[args.workload](args)
+           "ghost": bench_ghost, "mesh": bench_mesh, "sortol": bench_sortol,
+           "sorinner": bench_sorinner}[args.workload](args)
     if args.workload == "sor" and not args.quick:
         # The rest of the metric: SOR to a tolerance, batched solves/s and
         # served solves/s with p50/p99 latency, each on a bounded sample.
