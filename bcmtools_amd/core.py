@@ -556,6 +556,7 @@ class CommList:
 
     def __init__(self, mesh, faces, nfaces):
         self.lib = mesh.lib
+        self.mesh = mesh  # the list must not outlive its mesh
         h = ctypes.c_void_p()
         L.check(self.lib.bcm_comm_list_create(mesh.handle, faces, int(nfaces), ctypes.byref(h)),
                 "bcm_comm_list_create")

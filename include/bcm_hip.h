@@ -302,7 +302,7 @@ int bcm_comm_unpack(bcm_field_t field, const bcm_comm_face* faces, int nfaces, c
  * mesh's stream and return: no allocation, no host copy, no wait (use
  * bcm_mesh_sync before reading the buffer on the host).  The list belongs
  * to one mesh; fields of other meshes are rejected.  Destroying a list
- * waits for the mesh's stream. */
+ * waits for the mesh's stream; destroy every list before its mesh. */
 typedef struct bcm_comm_list* bcm_comm_list_t;
 int bcm_comm_list_create(bcm_mesh_t mesh, const bcm_comm_face* faces, int nfaces, bcm_comm_list_t* out);
 int bcm_comm_list_destroy(bcm_comm_list_t list);
