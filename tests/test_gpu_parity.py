@@ -558,14 +558,15 @@ def test_sor_iterate_inner_rejects_a_loop_count_below_one():
     assert mesh.lib.bcm_sor_iterate_inner(f.handle, None, 1.2, L.dptr(v), 1, 1, 0) == L.ERR_ARG
 
 
-@pytest.mark.parametrize("variant", [L.SOR_HALO, L.SOR_FUSED_TASKS])
-def test_sor_iterate_on_a_batch_equals_each_mesh_alone(variant):
+@pytest.mark.parametrize("variant,inner", [(L.SOR_HALO, 1), (L.SOR_FUSED_TASKS, 1), (L.SOR_HALO, 2)])
+def test_sor_iterate_on_a_batch_equals_each_mesh_alone(variant, inner):
     """Several independent meshes in one device mesh (segments): one
     sor_iterate over the batch gives every segment the bits of that mesh
-    iterated alone, with the halo schedule and the classic one."""
+    iterated alone, with the halo schedule and the classic one, and with
+    two inner sweeps per exchange."""
     n = 16
     tables = [W.random_tree(s, max_level=lv).blocks for s, lv in ((31, 3), (32, 2), (33, 3))]
-    case(f"sor batch variant={variant} segments={[len(t) for t in tables]}")
+    case(f"sor batch variant={variant} inner={inner} segments={[len(t) for t in tables]}")
     rng = np.random.default_rng(variant)
     f0 = [rng.standard_normal((len(t), n + 2, n + 2, n + 2)) for t in tables]
     s0 = [W.rhs_field(t, n, 1, seed=40 + i) for i, t in enumerate(tables)]
@@ -573,16 +574,16 @@ def test_sor_iterate_on_a_batch_equals_each_mesh_alone(variant):
     mesh.set_sor_variant(variant)
     f = mesh.field(np.float64, np.concatenate(f0))
     sf = mesh.field(np.float64, np.concatenate(s0))
-    mesh.sor_iterate(f, sf, 1.3, [0.5], 5)
+    mesh.sor_iterate(f, sf, 1.3, [0.5], 5, inner)
     out = f.download()
     for i, t in enumerate(tables):
         first, cnt = mesh.segment(i)
         alone = Mesh(t, n, 1)
         alone.set_sor_variant(variant)
         g = alone.field(np.float64, f0[i])
-        alone.sor_iterate(g, alone.field(np.float64, s0[i]), 1.3, [0.5], 5)
+        alone.sor_iterate(g, alone.field(np.float64, s0[i]), 1.3, [0.5], 5, inner)
         np.testing.assert_array_equal(out[first:first + cnt], g.download())
         ref = f0[i].copy()
         oracle.dirichlet(t, n, 1, ref, [0.5])
-        oracle.sor_iterate(t, n, 1, ref, s0[i], 1.3, [0.5], 5)
+        oracle.sor_iterate(t, n, 1, ref, s0[i], 1.3, [0.5], 5, inner)
         np.testing.assert_array_equal(g.download(), ref)
