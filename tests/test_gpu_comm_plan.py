@@ -67,6 +67,13 @@ def test_rank_buffers_carry_the_references_messages(name):
                 buf = DeviceArray(np.zeros(send.total_bytes, np.uint8))
                 assert mesh.lib.bcm_comm_pack(fld.handle, send.faces, send.nfaces, buf.ptr, send.total_bytes) == L.OK
                 host = buf.get()
+                # The same faces through a list uploaded once.
+                buf.put(np.zeros(send.total_bytes, np.uint8))
+                lst = CommList(mesh, send.faces, send.nfaces)
+                lst.pack(fld, buf.ptr, send.total_bytes)
+                mesh.sync()
+                np.testing.assert_array_equal(buf.get(), host)
+                lst.close()
                 buf.free()
                 for (peer, first, nf, off, nbytes), (dest, msg) in zip(send.peers, per_call[r][ci]):
                     assert peer == dest and nbytes == len(msg)
