@@ -514,7 +514,7 @@ class Mesh:
 
 def partition(nblocks, nprocs):
     """Rank of every block under the reference's Partition (Utils/include/
-    Partition.h:38-52): contiguous ranges, the first nblocks % nprocs
+    Partition.h:46-59): contiguous ranges, the first nblocks % nprocs
     ranks one block longer."""
     m, r = divmod(int(nblocks), int(nprocs))
     sizes = [m + 1 if i < r else m for i in range(nprocs)]
